@@ -1,0 +1,224 @@
+/*
+ * livre_amd.h — C ABI of the MI355X (gfx950) render hot path for Livre.
+ *
+ * The library replaces the GL ray caster behind livre::Renderer
+ * (livre/core/render/Renderer.h:38-109, livre/eq/render/RayCastRenderer.h:31-74)
+ * and the GPU brick cache behind TexturePool/TextureObject
+ * (livre/core/render/TexturePool.h:35-80, livre/lib/cache/TextureObject.cpp:76-120).
+ * Bricks are addressed by Livre's 64-bit NodeId (livre/data/NodeId.h:34-48).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative LVR_E* code on
+ *    failure; lvr_last_error() gives the message. No C++ exception crosses
+ *    the ABI (the reference throws std::runtime_error and logs per-brick
+ *    errors, RayCastRenderer.cpp:117-138, 517-522).
+ *  - Matrices are column-major float[16] (OpenGL / vmmlib layout).
+ *  - Bricks hold (block + 2*overlap)^3 voxels, x fastest, exactly what
+ *    DataSourcePlugin::getData returns (DataSourcePlugin.h:81) and what
+ *    glTexSubImage3D consumes (TextureObject.cpp:107-109).
+ *  - Images are RGBA8, 4 bytes per pixel, row 0 = bottom row (GL window
+ *    coordinates, what Equalizer reads back).
+ *  - A context belongs to one device and is not thread-safe: the host
+ *    serialises calls on it (the reference renders on one pipe thread).
+ */
+#ifndef LIVRE_AMD_H
+#define LIVRE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define LVR_OK 0
+#define LVR_EINVAL -1  /* invalid argument */
+#define LVR_EHIP -2    /* HIP runtime failure */
+#define LVR_ENOMEM -3  /* brick pool or device memory exhausted */
+#define LVR_ENOENT -4  /* brick not resident */
+#define LVR_ERANGE -5  /* output buffer too small */
+
+/* livre::DataType (livre/data/VolumeInformation.h:29-39) */
+#define LVR_DT_FLOAT 0
+#define LVR_DT_UINT8 1
+#define LVR_DT_UINT16 2
+#define LVR_DT_UINT32 3
+#define LVR_DT_INT8 4
+#define LVR_DT_INT16 5
+#define LVR_DT_INT32 6
+
+typedef struct lvr_ctx lvr_ctx;
+
+/* livre::VolumeInformation (livre/data/VolumeInformation.h:42-111), the
+ * fields the render path reads. The caller fills voxels, max_block, overlap
+ * and data_type; lvr_fill_regular_volume_info computes the rest. */
+typedef struct
+{
+    uint32_t voxels[3];
+    uint32_t max_block[3]; /* maximumBlockSize = block + 2 * overlap */
+    uint32_t overlap[3];
+    int32_t data_type;
+    float world_size[3];
+    float world_space_per_voxel;
+    uint32_t depth; /* RootNode::getDepth(): number of octree levels */
+    uint32_t root_blocks[3];
+} lvr_volume_info;
+
+/* Replaces fillRegularVolumeInfo (livre/data/DataSourcePlugin.cpp:73-95). */
+int lvr_fill_regular_volume_info(lvr_volume_info* info);
+
+/* Replaces DataSourcePlugin::internalNodeToLODNode's world box
+ * (livre/data/DataSourcePlugin.cpp:40-71): box = {min xyz, max xyz}. */
+int lvr_node_world_box(const lvr_volume_info* info, uint64_t node_id,
+                       float box[6]);
+
+/* livre::Frustum (livre/data/Frustum.h:37-111): the quantities the render
+ * path uses, computed from modelview and projection. */
+typedef struct
+{
+    float mv[16], proj[16];
+    float inv_mv[16], inv_proj[16];
+    float mvp[16];
+    float eye[3];       /* Frustum::getEyePos */
+    float view_dir[3];  /* Frustum::getViewDir */
+    float near_plane, far_plane, left, right, bottom, top;
+    float planes[6][4]; /* normalized culling planes: l, r, b, t, near, far */
+} lvr_frustum;
+
+/* Replaces Frustum::Frustum(mv, proj) (livre/data/Frustum.cpp:24-42). */
+int lvr_frustum_init(lvr_frustum* f, const float mv[16], const float proj[16]);
+
+/* ---------------------------------------------------------------- context */
+/* One context per volume and device: owns the HBM brick pool of
+ * pool_bytes, its device page table and the render scratch buffers.
+ * Replaces TexturePool(dataSource) + the texture Cache of gpu-cache-mem
+ * bytes (TexturePool.cpp:33-84, Window.cpp:54-72). */
+int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
+                   lvr_ctx** out);
+void lvr_ctx_destroy(lvr_ctx* ctx);
+const char* lvr_last_error(const lvr_ctx* ctx);
+/* Synchronise the context's stream. */
+int lvr_ctx_sync(lvr_ctx* ctx);
+
+/* Cache policy of the pool: 0 = LRU (touched on render, the default),
+ * 1 = load order (the reference LRUCachePolicy, which never refreshes on
+ * get, Cache.cpp:41-96, 158-170). Pinned bricks are never evicted. */
+int lvr_pool_set_policy(lvr_ctx* ctx, int policy);
+/* slots: capacity in bricks; used: resident bricks. */
+int lvr_pool_stats(const lvr_ctx* ctx, uint64_t* slots, uint64_t* used,
+                   uint64_t* slot_bytes);
+
+/* --------------------------------------------------------------- bricks */
+/* Upload one brick of getData() bytes into a pool slot, evicting by the
+ * policy when full. Replaces TextureObject::Impl::loadTextureToGPU
+ * (TextureObject.cpp:94-120) and TexturePool::generate (TexturePool.cpp:90-119).
+ * bytes must equal maximumBlockSize^3 * bytesPerVoxel. Synchronous with
+ * respect to the host buffer. */
+int lvr_brick_upload(lvr_ctx* ctx, uint64_t node_id, const void* voxels,
+                     size_t bytes);
+/* Same, for n bricks stored back to back in one host buffer; the copy is
+ * staged through pinned memory on the upload stream. */
+int lvr_brick_upload_many(lvr_ctx* ctx, const uint64_t* node_ids, int n,
+                          const void* voxels, size_t bytes_per_brick);
+/* Release a slot (TexturePool::release, TexturePool.cpp:121-126). */
+int lvr_brick_evict(lvr_ctx* ctx, uint64_t node_id);
+/* 1 if resident, 0 if not, negative on error. */
+int lvr_brick_resident(const lvr_ctx* ctx, uint64_t node_id);
+/* Copy a resident brick back to the host (for checks). */
+int lvr_brick_download(lvr_ctx* ctx, uint64_t node_id, void* out,
+                       size_t bytes);
+
+/* ------------------------------------------------------- procedural data */
+/* Procedural DataSource plugins for synthetic volumes: integer value noise
+ * (5 octaves by default) remapped through a 4096-entry uint16 table. Every
+ * voxel is a pure function of (seed, level, voxel coordinates), so a brick
+ * computed on the host equals the one computed on the device bit for bit.
+ * There is no reference counterpart: the reference's MemoryDataSource fills
+ * bricks with one constant (MemoryDataSource.cpp:38-61). */
+typedef struct
+{
+    uint64_t seed;
+    int32_t octaves;    /* number of octaves, 1..8 */
+    int32_t base_cells; /* lattice cells across the largest axis, octave 0 */
+    uint16_t remap[4097]; /* monotone table over the octave sum, 4096 segments */
+} lvr_noise_params;
+
+/* Fill the remap table: log-normal y = exp(gain * z) of the standardised
+ * octave sum, mapped linearly to the full range of the volume's type. */
+int lvr_noise_params_init(lvr_noise_params* p, uint64_t seed, int octaves,
+                          int base_cells, float gain);
+/* Host generation of one brick (DataSourcePlugin::getData). */
+int lvr_noise_brick_host(const lvr_volume_info* info,
+                         const lvr_noise_params* p, uint64_t node_id,
+                         void* out, size_t bytes);
+/* Device generation straight into pool slots (no PCIe traffic). */
+int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
+                            const uint64_t* node_ids, int n);
+
+/* ----------------------------------------------------- transfer function */
+/* FP32 RGBA transfer function of n entries over the data range
+ * [range0, range1] (RayCastRenderer.cpp:161-193, 280-285). Sampled with
+ * linear filtering and clamp-to-edge, as the GL 1D texture is. */
+int lvr_tf_set(lvr_ctx* ctx, const float* rgba, int n, float range0,
+               float range1);
+/* RGBA8 LUT as TransferFunction1D::getLUT produces (TransferFunction1D.cpp:
+ * 92-107); entries are read as unorm8, the GL_RGBA texture format. */
+int lvr_tf_set_rgba8(lvr_ctx* ctx, const uint8_t* lut, int n, float range0,
+                     float range1);
+
+/* ------------------------------------------------------------ selection */
+/* SelectVisibles + DFSTraversal (livre/data/SelectVisibles.cpp:45-131,
+ * livre/data/DFSTraversal.cpp:29-76) on the device, level by level, with
+ * the reference's FP32 expressions. Output in the reference's DFS order.
+ * n_out receives the count; LVR_ERANGE if cap is too small. */
+int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
+                        uint32_t window_height, float sse, uint32_t min_lod,
+                        uint32_t max_lod, uint64_t* out, int cap, int* n_out);
+
+/* RayCastRenderer::order (RayCastRenderer.cpp:57-81, 153-159): front to
+ * back by |MV * brick centre|, with libstdc++ std::sort on the given order. */
+int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
+              const uint64_t* ids, int n, uint64_t* out);
+
+/* ---------------------------------------------------------------- render */
+typedef struct
+{
+    int32_t n_samples_per_ray;   /* nSamplesPerRay (RayCastRenderer.cpp:216-234) */
+    int32_t max_samples_per_ray; /* opacity reference, 32 in the reference */
+    float early_exit;            /* EARLY_EXIT, 0.999 (fragRayCast.glsl:12) */
+    int32_t linear_filter;       /* 0: GL_NEAREST (reference default), 1: trilinear */
+    int32_t n_clip_planes;       /* 0..6 */
+    float clip_planes[24];       /* normal xyz, d */
+    int32_t tile_w, tile_h;      /* wavefront ray tile; 0 = autotuned default */
+} lvr_render_params;
+
+/* Render one frame of the ordered bricks (fragRayCast.glsl:113-207 per brick,
+ * RayCastRenderer.cpp:467-571) into the context framebuffer, followed by
+ * the RGBA32F -> RGBA8 copy (fragTexCopy.glsl:11-15). viewport = {x, y, w, h};
+ * only w and h are used (the frame covers the whole viewport). Bricks that
+ * are not resident are skipped as the reference skips invalid textures.
+ * Asynchronous on the context stream. */
+int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+               const uint64_t* ordered_ids, int n,
+               const lvr_render_params* params);
+/* Copy the last frame to the host (w * h * 4 bytes) and return the number of
+ * executed composite steps (fragRayCast.glsl:184-204 iterations). */
+int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes,
+                   uint64_t* samples);
+/* Device pointer of the framebuffer (uint32 RGBA8 per pixel). */
+int lvr_frame_device_ptr(lvr_ctx* ctx, void** ptr, int* width, int* height);
+
+/* ------------------------------------------------------------- histogram */
+/* Per-frame histogram of the given bricks (HistogramObject.cpp:140-224 +
+ * HistogramFilter.cpp:78-149): counts of interior voxels scaled by
+ * 8^(depth - level - 1). 8-bit data: 256 bins, bin = value; 16-bit data:
+ * 65536 bins, bin = value; float data: 256 bins over the TF range. The
+ * result stays on the device until lvr_histogram_read. */
+int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins);
+int lvr_histogram_read(lvr_ctx* ctx, uint64_t* out, int bins);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

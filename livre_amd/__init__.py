@@ -1,0 +1,357 @@
+"""Python binding of liblivre_amd.so, the MI355X render hot path of Livre.
+
+The library is C++/HIP behind a C ABI (include/livre_amd.h); this module
+binds it with ctypes for tests, benchmarks and scripting. There is no
+fallback: if the shared library is missing or no GPU is present, the calls
+that need it raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblivre_amd.so")
+
+DT_FLOAT, DT_UINT8, DT_UINT16, DT_UINT32, DT_INT8, DT_INT16, DT_INT32 = range(7)
+NP_DTYPE = {DT_FLOAT: np.float32, DT_UINT8: np.uint8, DT_UINT16: np.uint16,
+            DT_UINT32: np.uint32, DT_INT8: np.int8, DT_INT16: np.int16,
+            DT_INT32: np.int32}
+
+LVR_OK, LVR_EINVAL, LVR_EHIP, LVR_ENOMEM, LVR_ENOENT, LVR_ERANGE = 0, -1, -2, -3, -4, -5
+
+# every entry point declared in include/livre_amd.h
+EXPORTS = (
+    "lvr_fill_regular_volume_info", "lvr_node_world_box", "lvr_frustum_init",
+    "lvr_ctx_create", "lvr_ctx_destroy", "lvr_last_error", "lvr_ctx_sync",
+    "lvr_pool_set_policy", "lvr_pool_stats", "lvr_brick_upload",
+    "lvr_brick_upload_many", "lvr_brick_evict", "lvr_brick_resident",
+    "lvr_brick_download", "lvr_noise_params_init", "lvr_noise_brick_host",
+    "lvr_noise_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
+    "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
+    "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
+)
+
+
+class LivreError(RuntimeError):
+    pass
+
+
+class VolumeInfo(ctypes.Structure):
+    _fields_ = [("voxels", ctypes.c_uint32 * 3),
+                ("max_block", ctypes.c_uint32 * 3),
+                ("overlap", ctypes.c_uint32 * 3),
+                ("data_type", ctypes.c_int32),
+                ("world_size", ctypes.c_float * 3),
+                ("world_space_per_voxel", ctypes.c_float),
+                ("depth", ctypes.c_uint32),
+                ("root_blocks", ctypes.c_uint32 * 3)]
+
+    @property
+    def block(self):
+        return [self.max_block[i] - 2 * self.overlap[i] for i in range(3)]
+
+    @property
+    def dtype(self):
+        return NP_DTYPE[self.data_type]
+
+    @property
+    def brick_shape(self):
+        # numpy (z, y, x) shape of one brick, x fastest
+        return (self.max_block[2], self.max_block[1], self.max_block[0])
+
+
+class Frustum(ctypes.Structure):
+    _fields_ = [("mv", ctypes.c_float * 16), ("proj", ctypes.c_float * 16),
+                ("inv_mv", ctypes.c_float * 16), ("inv_proj", ctypes.c_float * 16),
+                ("mvp", ctypes.c_float * 16), ("eye", ctypes.c_float * 3),
+                ("view_dir", ctypes.c_float * 3), ("near_plane", ctypes.c_float),
+                ("far_plane", ctypes.c_float), ("left", ctypes.c_float),
+                ("right", ctypes.c_float), ("bottom", ctypes.c_float),
+                ("top", ctypes.c_float), ("planes", (ctypes.c_float * 4) * 6)]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [("n_samples_per_ray", ctypes.c_int32),
+                ("max_samples_per_ray", ctypes.c_int32),
+                ("early_exit", ctypes.c_float),
+                ("linear_filter", ctypes.c_int32),
+                ("n_clip_planes", ctypes.c_int32),
+                ("clip_planes", ctypes.c_float * 24),
+                ("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32)]
+
+
+class NoiseParams(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("octaves", ctypes.c_int32),
+                ("base_cells", ctypes.c_int32), ("remap", ctypes.c_uint16 * 4097)]
+
+
+_lib = None
+
+
+def lib():
+    """Load liblivre_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LivreError("liblivre_amd.so is not built: run python -m livre_amd.build")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    c = ctypes
+    f32p, u64p, u8p = P(c.c_float), P(c.c_uint64), P(c.c_uint8)
+    L.lvr_fill_regular_volume_info.argtypes = [P(VolumeInfo)]
+    L.lvr_node_world_box.argtypes = [P(VolumeInfo), c.c_uint64, f32p]
+    L.lvr_frustum_init.argtypes = [P(Frustum), f32p, f32p]
+    L.lvr_ctx_create.argtypes = [P(VolumeInfo), c.c_size_t, c.c_int, P(c.c_void_p)]
+    L.lvr_ctx_destroy.argtypes = [c.c_void_p]
+    L.lvr_ctx_destroy.restype = None
+    L.lvr_last_error.argtypes = [c.c_void_p]
+    L.lvr_last_error.restype = c.c_char_p
+    L.lvr_ctx_sync.argtypes = [c.c_void_p]
+    L.lvr_pool_set_policy.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_pool_stats.argtypes = [c.c_void_p, u64p, u64p, u64p]
+    L.lvr_brick_upload.argtypes = [c.c_void_p, c.c_uint64, c.c_void_p, c.c_size_t]
+    L.lvr_brick_upload_many.argtypes = [c.c_void_p, u64p, c.c_int, c.c_void_p,
+                                        c.c_size_t]
+    L.lvr_brick_evict.argtypes = [c.c_void_p, c.c_uint64]
+    L.lvr_brick_resident.argtypes = [c.c_void_p, c.c_uint64]
+    L.lvr_brick_download.argtypes = [c.c_void_p, c.c_uint64, c.c_void_p, c.c_size_t]
+    L.lvr_noise_params_init.argtypes = [P(NoiseParams), c.c_uint64, c.c_int, c.c_int,
+                                        c.c_float]
+    L.lvr_noise_brick_host.argtypes = [P(VolumeInfo), P(NoiseParams), c.c_uint64,
+                                       c.c_void_p, c.c_size_t]
+    L.lvr_noise_bricks_device.argtypes = [c.c_void_p, P(NoiseParams), u64p, c.c_int]
+    L.lvr_tf_set.argtypes = [c.c_void_p, f32p, c.c_int, c.c_float, c.c_float]
+    L.lvr_tf_set_rgba8.argtypes = [c.c_void_p, u8p, c.c_int, c.c_float, c.c_float]
+    L.lvr_select_visibles.argtypes = [c.c_void_p, P(Frustum), c.c_uint32, c.c_float,
+                                      c.c_uint32, c.c_uint32, u64p, c.c_int,
+                                      P(c.c_int)]
+    L.lvr_order.argtypes = [P(VolumeInfo), P(Frustum), u64p, c.c_int, u64p]
+    L.lvr_render.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
+                             P(RenderParams)]
+    L.lvr_frame_read.argtypes = [c.c_void_p, u8p, c.c_size_t, u64p]
+    L.lvr_frame_device_ptr.argtypes = [c.c_void_p, P(c.c_void_p), P(c.c_int),
+                                       P(c.c_int)]
+    L.lvr_histogram.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
+    L.lvr_histogram_read.argtypes = [c.c_void_p, u64p, c.c_int]
+    _lib = L
+    return L
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(-1)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def volume_info(voxels, block, overlap, data_type):
+    """VolumeInformation filled as fillRegularVolumeInfo does."""
+    vi = VolumeInfo()
+    if np.isscalar(block):
+        block = (block,) * 3
+    if np.isscalar(overlap):
+        overlap = (overlap,) * 3
+    if np.isscalar(voxels):
+        voxels = (voxels,) * 3
+    for i in range(3):
+        vi.voxels[i] = voxels[i]
+        vi.max_block[i] = block[i] + 2 * overlap[i]
+        vi.overlap[i] = overlap[i]
+    vi.data_type = data_type
+    rc = lib().lvr_fill_regular_volume_info(ctypes.byref(vi))
+    if rc != LVR_OK:
+        raise LivreError("invalid volume information (%d)" % rc)
+    return vi
+
+
+def world_box(vi, node_id):
+    out = np.zeros(6, np.float32)
+    rc = lib().lvr_node_world_box(ctypes.byref(vi), int(node_id),
+                                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    if rc != LVR_OK:
+        raise LivreError("invalid node id")
+    return out
+
+
+def frustum(mv, proj):
+    f = Frustum()
+    mv, pmv = _f32(mv)
+    proj, pproj = _f32(proj)
+    rc = lib().lvr_frustum_init(ctypes.byref(f), pmv, pproj)
+    if rc != LVR_OK:
+        raise LivreError("singular matrix")
+    return f
+
+
+def order(vi, fr, ids):
+    ids, pids = _u64(ids)
+    out = np.zeros(len(ids), np.uint64)
+    rc = lib().lvr_order(ctypes.byref(vi), ctypes.byref(fr), pids, len(ids),
+                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+    if rc != LVR_OK:
+        raise LivreError("order failed")
+    return out
+
+
+def noise_params(seed, octaves=5, base_cells=8, gain=1.5):
+    p = NoiseParams()
+    rc = lib().lvr_noise_params_init(ctypes.byref(p), seed, octaves, base_cells, gain)
+    if rc != LVR_OK:
+        raise LivreError("invalid noise parameters")
+    return p
+
+
+def noise_brick_host(vi, params, node_id):
+    out = np.zeros(vi.brick_shape, vi.dtype)
+    rc = lib().lvr_noise_brick_host(ctypes.byref(vi), ctypes.byref(params),
+                                    int(node_id), out.ctypes.data, out.nbytes)
+    if rc != LVR_OK:
+        raise LivreError("noise brick failed (%d)" % rc)
+    return out
+
+
+def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
+                  max_samples_per_ray=32, clip_planes=(), tile=(0, 0)):
+    p = RenderParams()
+    p.n_samples_per_ray = n_samples_per_ray
+    p.max_samples_per_ray = max_samples_per_ray
+    p.early_exit = early_exit
+    p.linear_filter = 1 if linear else 0
+    p.n_clip_planes = len(clip_planes)
+    for i, pl in enumerate(clip_planes):
+        for j in range(4):
+            p.clip_planes[4 * i + j] = pl[j]
+    p.tile_w, p.tile_h = tile
+    return p
+
+
+class Context:
+    """One volume's HBM brick pool and render state on one device."""
+
+    def __init__(self, vi, pool_bytes, device=0):
+        self.vi = vi
+        self._h = ctypes.c_void_p()
+        rc = lib().lvr_ctx_create(ctypes.byref(vi), int(pool_bytes), device,
+                                  ctypes.byref(self._h))
+        if rc != LVR_OK:
+            raise LivreError("lvr_ctx_create failed (%d)" % rc)
+        self.width = self.height = 0
+
+    def close(self):
+        if self._h:
+            lib().lvr_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != LVR_OK:
+            msg = lib().lvr_last_error(self._h).decode()
+            raise LivreError("%s failed (%d): %s" % (what, rc, msg))
+
+    def sync(self):
+        self._check(lib().lvr_ctx_sync(self._h), "sync")
+
+    def set_policy(self, policy):
+        self._check(lib().lvr_pool_set_policy(self._h, policy), "set_policy")
+
+    def stats(self):
+        s, u, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(lib().lvr_pool_stats(self._h, ctypes.byref(s), ctypes.byref(u),
+                                         ctypes.byref(b)), "stats")
+        return s.value, u.value, b.value
+
+    def upload(self, ids, bricks):
+        ids, pids = _u64(ids)
+        bricks = np.ascontiguousarray(bricks, dtype=self.vi.dtype)
+        per = int(np.prod(self.vi.brick_shape)) * bricks.itemsize
+        if bricks.nbytes != per * len(ids):
+            raise ValueError("brick array does not match ids")
+        self._check(lib().lvr_brick_upload_many(self._h, pids, len(ids),
+                                                bricks.ctypes.data, per), "upload")
+
+    def evict(self, node_id):
+        self._check(lib().lvr_brick_evict(self._h, int(node_id)), "evict")
+
+    def resident(self, node_id):
+        rc = lib().lvr_brick_resident(self._h, int(node_id))
+        if rc < 0:
+            self._check(rc, "resident")
+        return rc == 1
+
+    def download(self, node_id):
+        out = np.zeros(self.vi.brick_shape, self.vi.dtype)
+        self._check(lib().lvr_brick_download(self._h, int(node_id), out.ctypes.data,
+                                             out.nbytes), "download")
+        return out
+
+    def generate_noise(self, params, ids):
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_noise_bricks_device(self._h, ctypes.byref(params), pids,
+                                                  len(ids)), "generate_noise")
+
+    def set_tf(self, rgba, data_range):
+        rgba, p = _f32(rgba)
+        self._check(lib().lvr_tf_set(self._h, p, rgba.size // 4, data_range[0],
+                                     data_range[1]), "set_tf")
+
+    def set_tf_rgba8(self, lut, data_range):
+        lut = np.ascontiguousarray(lut, dtype=np.uint8).reshape(-1)
+        self._check(lib().lvr_tf_set_rgba8(
+            self._h, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), lut.size // 4,
+            data_range[0], data_range[1]), "set_tf_rgba8")
+
+    def select_visibles(self, fr, window_height, sse, min_lod, max_lod, cap=1 << 22):
+        out = np.zeros(cap, np.uint64)
+        n = ctypes.c_int(0)
+        self._check(lib().lvr_select_visibles(
+            self._h, ctypes.byref(fr), window_height, sse, min_lod, max_lod,
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap,
+            ctypes.byref(n)), "select_visibles")
+        return out[:n.value].copy()
+
+    def render(self, fr, width, height, ordered, params):
+        ordered, pord = _u64(ordered)
+        vp = (ctypes.c_int * 4)(0, 0, width, height)
+        self._check(lib().lvr_render(self._h, ctypes.byref(fr), vp, pord, len(ordered),
+                                     ctypes.byref(params)), "render")
+        self.width, self.height = width, height
+
+    def read_frame(self):
+        img = np.zeros((self.height, self.width, 4), np.uint8)
+        s = ctypes.c_uint64()
+        self._check(lib().lvr_frame_read(
+            self._h, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), img.nbytes,
+            ctypes.byref(s)), "read_frame")
+        return img, s.value
+
+    def samples(self):
+        s = ctypes.c_uint64()
+        self._check(lib().lvr_frame_read(self._h, None, 0, ctypes.byref(s)), "samples")
+        return s.value
+
+    def histogram(self, ids, bins, read=True):
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_histogram(self._h, pids, len(ids), bins), "histogram")
+        if not read:
+            return None
+        out = np.zeros(bins, np.uint64)
+        self._check(lib().lvr_histogram_read(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), bins),
+            "histogram_read")
+        return out

@@ -1,0 +1,187 @@
+// Internal declarations shared by the translation units of liblivre_amd.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/livre_amd.h"
+
+namespace lvr
+{
+// ------------------------------------------------------------------ NodeId
+// 64-bit brick key with GCC's bitfield layout from the LSB: level 4 bits,
+// x, y, z 14 bits each, time step 18 bits (livre/data/NodeId.h:34-48,
+// livre/data/types.h:80-93).
+struct NodeKey
+{
+    uint32_t level, x, y, z, t;
+};
+
+__host__ __device__ inline NodeKey decodeNode(uint64_t id)
+{
+    NodeKey k;
+    k.level = uint32_t(id & 0xFu);
+    k.x = uint32_t((id >> 4) & 0x3FFFu);
+    k.y = uint32_t((id >> 18) & 0x3FFFu);
+    k.z = uint32_t((id >> 32) & 0x3FFFu);
+    k.t = uint32_t(id >> 46);
+    return k;
+}
+
+__host__ __device__ inline uint64_t encodeNode(uint32_t level, uint32_t x,
+                                               uint32_t y, uint32_t z,
+                                               uint32_t t)
+{
+    return uint64_t(level & 0xFu) | (uint64_t(x & 0x3FFFu) << 4) |
+           (uint64_t(y & 0x3FFFu) << 18) | (uint64_t(z & 0x3FFFu) << 32) |
+           (uint64_t(t & 0x3FFFFu) << 46);
+}
+
+size_t bytesPerVoxel(int dataType);
+
+// World box of a node, as internalNodeToLODNode computes it
+// (livre/data/DataSourcePlugin.cpp:40-71), in FP32 with the same order of
+// operations. Usable on host and device.
+__host__ __device__ inline void nodeWorldBox(const lvr_volume_info& vi,
+                                             const NodeKey& k, float* mn,
+                                             float* mx)
+{
+    const uint32_t scale = 1u << k.level;
+    const uint32_t bx = vi.root_blocks[0] * scale;
+    const uint32_t by = vi.root_blocks[1] * scale;
+    const uint32_t bz = vi.root_blocks[2] * scale;
+    uint32_t bmax = bx;
+    if (by > bmax)
+        bmax = by;
+    if (bz > bmax)
+        bmax = bz;
+    const uint32_t pos[3] = {k.x, k.y, k.z};
+    const float denom = float(bmax);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const float cmin = float(pos[i]) / denom;
+        const float cmax = float(pos[i] + 1u) / denom;
+        const float half = vi.world_size[i] * 0.5f;
+        mn[i] = cmin * vi.world_size[i] - half;
+        mx[i] = cmax * vi.world_size[i] - half;
+    }
+}
+
+// Per-brick record of one frame, in the order of the ordered list. 48 bytes.
+struct BrickEntry
+{
+    float bmin[3];
+    float bmax[3];
+    float ext[3];   // bmax - bmin, rounded as the shader's (aabbMax - aabbMin)
+    int32_t slot;   // pool slot, -1 if not resident
+    uint32_t level;
+    uint32_t pad;
+};
+
+// Dense device page table over the octree: per level,
+// levelOffset[L] + x + nx * (y + ny * z) holds a slot index or -1.
+struct PageTable
+{
+    std::vector<uint64_t> levelOffset; // depth + 1 entries
+    std::vector<uint32_t> levelDim;    // 3 per level
+    int32_t* d_slots = nullptr;
+    std::vector<int32_t> h_slots;
+    uint64_t size = 0;
+    uint64_t dirtyLo = ~0ull, dirtyHi = 0; // host entries not yet on the device
+};
+
+struct PoolEntry
+{
+    int32_t slot;
+    int pinned;
+    std::list<uint64_t>::iterator lru;
+};
+} // namespace lvr
+
+struct lvr_ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipStream_t uploadStream = nullptr;
+    hipEvent_t uploadDone = nullptr;
+    lvr_volume_info vi{};
+    size_t voxelBytes = 0; // bytes per voxel
+    size_t brickBytes = 0; // maximumBlockSize^3 * bytesPerVoxel
+    size_t slotStride = 0; // brickBytes rounded to 256
+    uint64_t nSlots = 0;
+    uint8_t* d_pool = nullptr;
+    lvr::PageTable pt;
+    std::vector<int32_t> freeSlots;
+    std::unordered_map<uint64_t, lvr::PoolEntry> resident;
+    std::list<uint64_t> lruList; // front = next to evict
+    int policy = 0;
+
+    // pinned staging for uploads
+    uint8_t* h_staging = nullptr;
+    size_t stagingBytes = 0;
+
+    // transfer function
+    float* d_tf = nullptr; // float4 entries
+    int nTf = 0;
+    int tfCapacity = 0;
+    float range0 = 0.f, range1 = 0.f;
+    bool tfSet = false;
+
+    // per-frame buffers
+    uint32_t* d_frame = nullptr;
+    int fbW = 0, fbH = 0;
+    size_t fbCapacity = 0;
+    lvr::BrickEntry* d_bricks = nullptr;
+    lvr::BrickEntry* h_bricks = nullptr; // pinned
+    int brickCapacity = 0;
+    int32_t* d_tileRange = nullptr; // 4 ints per brick
+    int32_t* d_tileCount = nullptr;
+    int32_t* d_tileOffset = nullptr;
+    int32_t* d_tileList = nullptr;
+    size_t tileCapacity = 0;
+    size_t tileListCapacity = 0;
+    unsigned long long* d_counters = nullptr; // [0] samples, [1] list total
+    unsigned long long* h_counters = nullptr; // pinned
+
+    // histogram
+    unsigned long long* d_hist = nullptr;
+    int histBins = 0; // allocated bins
+    int histLast = 0; // bins of the last histogram
+    int32_t* d_histSlots = nullptr;
+    uint32_t* d_histScale = nullptr;
+    int histCapacity = 0;
+
+    // selection
+    uint64_t* d_selA = nullptr;
+    uint64_t* d_selB = nullptr;
+    uint64_t* d_selOut = nullptr;
+    int32_t* d_selCount = nullptr; // [0] next frontier, [1] visible
+    size_t selCapacity = 0;
+
+    std::string error;
+};
+
+namespace lvr
+{
+int fail(lvr_ctx* ctx, int code, const std::string& msg);
+int hipFail(lvr_ctx* ctx, hipError_t e, const char* what);
+int ensureDevice(lvr_ctx* ctx);
+int32_t slotOf(const lvr_ctx* ctx, uint64_t id);
+void touch(lvr_ctx* ctx, uint64_t id);
+int flushPageTable(lvr_ctx* ctx);
+int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
+} // namespace lvr
+
+#define LVR_HIP(ctx, call)                                                     \
+    do                                                                         \
+    {                                                                          \
+        hipError_t _e = (call);                                                \
+        if (_e != hipSuccess)                                                  \
+            return lvr::hipFail((ctx), _e, #call);                             \
+    } while (0)

@@ -1,0 +1,398 @@
+// Procedural value-noise DataSource: every voxel of every brick (any level,
+// halo included) is a pure integer function of (seed, level, voxel
+// coordinates), so bricks generated on the host and on the device are
+// bit-identical, and the device can fill its HBM pool without PCIe traffic.
+//
+// Field: sum over octaves o of 2^-o * trilinear value noise with a smoothstep
+// fade, lattice of base_cells * 2^o cells across the largest axis, in 16.16
+// fixed point. A coarse level samples the field at its voxel centres and
+// leaves out octaves finer than twice its voxel size (band limit). The sum
+// is remapped through a monotone 4096-segment uint16 table (log-normal
+// shaping, built on the host in double precision).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "lvr_internal.h"
+
+namespace lvr
+{
+namespace
+{
+__host__ __device__ inline uint64_t splitmix64(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__host__ __device__ inline int64_t lattice(uint64_t seed, int o, int64_t x,
+                                           int64_t y, int64_t z)
+{
+    const uint64_t key = (uint64_t(x & 0xFFFFF)) | (uint64_t(y & 0xFFFFF) << 20) |
+                         (uint64_t(z & 0xFFFFF) << 40) | (uint64_t(o) << 60);
+    return int64_t(splitmix64(seed ^ splitmix64(key)) >> 48); // 0..65535
+}
+
+__host__ __device__ inline int64_t fade(int64_t f)
+{
+    // f^2 (3 - 2f) in 16-bit fixed point
+    const uint64_t uf = uint64_t(f);
+    return int64_t((uf * uf * (3ull * 65536ull - 2ull * uf)) >> 32);
+}
+
+__host__ __device__ inline int64_t lerp16(int64_t a, int64_t b, int64_t s)
+{
+    return a + (((b - a) * s) >> 16);
+}
+
+struct NoiseShape
+{
+    uint32_t voxels[3];
+    uint32_t vmax;
+    uint32_t depth;
+    int32_t octaves;
+    int32_t baseCells;
+    uint64_t seed;
+    int64_t maxSum;
+    int64_t cellMult[8];   // floor(cells_o * 2^32 / vmax): cell coordinate scale
+    int64_t cellFinest[8]; // finest voxels per cell of octave o
+    int64_t segMult;       // floor(4096 * 2^32 / (maxSum + 1))
+};
+
+// Octave sum at a finest-voxel position in 16.16 fixed point.
+__host__ __device__ inline int64_t octaveSum(const NoiseShape& s,
+                                             const int64_t* ufp,
+                                             int octaves)
+{
+    int64_t sum = 0;
+    for (int o = 0; o < octaves; ++o)
+    {
+        int64_t ci[3], fr[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            const int64_t c = (ufp[a] * s.cellMult[o]) >> 32;
+            ci[a] = c >> 16;
+            fr[a] = fade(c & 0xFFFF);
+        }
+        int64_t v[8];
+        for (int k = 0; k < 8; ++k)
+            v[k] = lattice(s.seed, o, ci[0] + (k & 1), ci[1] + ((k >> 1) & 1),
+                           ci[2] + ((k >> 2) & 1));
+        const int64_t x00 = lerp16(v[0], v[1], fr[0]);
+        const int64_t x10 = lerp16(v[2], v[3], fr[0]);
+        const int64_t x01 = lerp16(v[4], v[5], fr[0]);
+        const int64_t x11 = lerp16(v[6], v[7], fr[0]);
+        const int64_t y0 = lerp16(x00, x10, fr[1]);
+        const int64_t y1 = lerp16(x01, x11, fr[1]);
+        sum += lerp16(y0, y1, fr[2]) >> o;
+    }
+    return sum;
+}
+
+// Voxel of a level-L brick at level-lattice coordinates g (may lie in the
+// halo, outside the volume: clamped to the volume), remapped to 0..65535.
+__host__ __device__ inline uint32_t noiseVoxel(const NoiseShape& s,
+                                               const uint16_t* remap,
+                                               uint32_t level, int64_t gx,
+                                               int64_t gy, int64_t gz)
+{
+    const uint32_t shift = s.depth - 1u - level;
+    const int64_t S = int64_t(1) << shift; // finest voxels per level voxel
+    const int64_t g[3] = {gx, gy, gz};
+    int64_t u[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        int64_t c = (2 * g[a] + 1) * S * 32768; // (g + 0.5) * S in 16.16
+        const int64_t lo = 32768;
+        const int64_t hi = (2 * int64_t(s.voxels[a]) - 1) * 32768;
+        c = c < lo ? lo : (c > hi ? hi : c);
+        u[a] = c;
+    }
+    // band limit: keep octaves whose cell spans at least 2 level voxels
+    int octaves = 0;
+    for (int o = 0; o < s.octaves; ++o)
+    {
+        if (s.cellFinest[o] >= 2 * S || o == 0)
+            octaves = o + 1;
+    }
+    const int64_t sum = octaveSum(s, u, octaves);
+    // 4096 segments over [0, maxSum]
+    const int64_t t = (sum * s.segMult) >> 16;
+    int64_t seg = t >> 16;
+    const int64_t fr = t & 0xFFFF;
+    if (seg > 4095)
+        seg = 4095;
+    const int64_t a = remap[seg], b = remap[seg + 1];
+    int64_t v = a + (((b - a) * fr) >> 16);
+    v = v < 0 ? 0 : (v > 65535 ? 65535 : v);
+    return uint32_t(v);
+}
+
+template <typename T>
+__host__ __device__ inline T convertNoise(uint32_t v);
+template <>
+__host__ __device__ inline uint8_t convertNoise<uint8_t>(uint32_t v)
+{
+    return uint8_t(v >> 8);
+}
+template <>
+__host__ __device__ inline uint16_t convertNoise<uint16_t>(uint32_t v)
+{
+    return uint16_t(v);
+}
+template <>
+__host__ __device__ inline float convertNoise<float>(uint32_t v)
+{
+    return float(v) * (2.0f / 65535.0f) - 1.0f;
+}
+
+void finishShape(NoiseShape& s)
+{
+    int64_t m = 0;
+    for (int o = 0; o < 8; ++o)
+    {
+        const int64_t cells = int64_t(s.baseCells) << o;
+        s.cellMult[o] = int64_t((uint64_t(cells) << 32) / s.vmax);
+        s.cellFinest[o] = int64_t(s.vmax) / cells;
+        if (o < s.octaves)
+            m += 65535 >> o;
+    }
+    s.maxSum = m;
+    // t = sum * 4096 / (maxSum + 1) in 16.16
+    s.segMult = int64_t((uint64_t(4096) << 32) / uint64_t(m + 1));
+}
+
+NoiseShape makeShape(const lvr_volume_info& vi, const lvr_noise_params& p)
+{
+    NoiseShape s;
+    for (int a = 0; a < 3; ++a)
+        s.voxels[a] = vi.voxels[a];
+    s.vmax = std::max(vi.voxels[0], std::max(vi.voxels[1], vi.voxels[2]));
+    s.depth = vi.depth;
+    s.octaves = p.octaves;
+    s.baseCells = p.base_cells;
+    s.seed = p.seed;
+    finishShape(s);
+    return s;
+}
+
+struct BrickGeo
+{
+    int64_t origin[3]; // level-lattice coordinate of the brick's first voxel
+    uint32_t dim[3];   // maximumBlockSize
+    uint32_t level;
+};
+
+BrickGeo brickGeo(const lvr_volume_info& vi, uint64_t id)
+{
+    const NodeKey k = decodeNode(id);
+    BrickGeo g;
+    const uint32_t pos[3] = {k.x, k.y, k.z};
+    for (int a = 0; a < 3; ++a)
+    {
+        const uint32_t block = vi.max_block[a] - 2 * vi.overlap[a];
+        g.origin[a] = int64_t(pos[a]) * block - int64_t(vi.overlap[a]);
+        g.dim[a] = vi.max_block[a];
+    }
+    g.level = k.level;
+    return g;
+}
+
+template <typename T>
+__global__ void noiseBricksKernel(NoiseShape s, const uint16_t* __restrict__ remap,
+                                  const BrickGeo* __restrict__ geos,
+                                  const int32_t* __restrict__ slots,
+                                  uint8_t* __restrict__ pool, size_t slotStride)
+{
+    const BrickGeo g = geos[blockIdx.y];
+    T* out = reinterpret_cast<T*>(pool + size_t(slots[blockIdx.y]) * slotStride);
+    const uint32_t n = g.dim[0] * g.dim[1] * g.dim[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+    {
+        const uint32_t x = i % g.dim[0];
+        const uint32_t y = (i / g.dim[0]) % g.dim[1];
+        const uint32_t z = i / (g.dim[0] * g.dim[1]);
+        out[i] = convertNoise<T>(noiseVoxel(s, remap, g.level, g.origin[0] + x,
+                                            g.origin[1] + y, g.origin[2] + z));
+    }
+}
+
+template <typename T>
+void noiseBrickHost(const NoiseShape& s, const uint16_t* remap,
+                    const BrickGeo& g, T* out)
+{
+    size_t i = 0;
+    for (uint32_t z = 0; z < g.dim[2]; ++z)
+        for (uint32_t y = 0; y < g.dim[1]; ++y)
+            for (uint32_t x = 0; x < g.dim[0]; ++x)
+                out[i++] = convertNoise<T>(
+                    noiseVoxel(s, remap, g.level, g.origin[0] + x,
+                               g.origin[1] + y, g.origin[2] + z));
+}
+} // namespace
+} // namespace lvr
+
+extern "C" {
+
+int lvr_noise_params_init(lvr_noise_params* p, uint64_t seed, int octaves,
+                          int base_cells, float gain)
+{
+    if (!p || octaves < 1 || octaves > 8 || base_cells < 1)
+        return LVR_EINVAL;
+    std::memset(p, 0, sizeof *p);
+    p->seed = seed;
+    p->octaves = octaves;
+    p->base_cells = base_cells;
+    // Statistics of the octave sum from 65536 lattice-independent sample
+    // points of a unit-shaped volume, then y = exp(gain * z) scaled to
+    // [0, 65535] between the 0.01% and 99.99% points.
+    lvr::NoiseShape s;
+    s.voxels[0] = s.voxels[1] = s.voxels[2] = 1u << 20;
+    s.vmax = 1u << 20;
+    s.depth = 1;
+    s.octaves = octaves;
+    s.baseCells = base_cells;
+    s.seed = seed;
+    lvr::finishShape(s);
+    const int64_t m = s.maxSum;
+    std::vector<double> samples;
+    samples.reserve(65536);
+    uint64_t r = seed ^ 0xA5A5A5A5ull;
+    for (int i = 0; i < 65536; ++i)
+    {
+        int64_t u[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            r = r * 6364136223846793005ull + 1442695040888963407ull;
+            u[a] = int64_t((r >> 24) % (uint64_t(s.vmax) << 16));
+        }
+        samples.push_back(double(lvr::octaveSum(s, u, octaves)));
+    }
+    double mean = 0, var = 0;
+    for (double v : samples)
+        mean += v;
+    mean /= samples.size();
+    for (double v : samples)
+        var += (v - mean) * (v - mean);
+    const double sd = std::sqrt(var / samples.size()) + 1e-9;
+    std::sort(samples.begin(), samples.end());
+    const double lo = samples[samples.size() / 10000];
+    const double hi = samples[samples.size() - 1 - samples.size() / 10000];
+    const double elo = std::exp(gain * (lo - mean) / sd);
+    const double ehi = std::exp(gain * (hi - mean) / sd);
+    for (int k = 0; k <= 4096; ++k)
+    {
+        const double x = double(k) * double(m + 1) / 4096.0;
+        const double xc = std::min(hi, std::max(lo, x));
+        const double y = (std::exp(gain * (xc - mean) / sd) - elo) / (ehi - elo);
+        p->remap[k] = uint16_t(std::lround(std::min(1.0, std::max(0.0, y)) * 65535.0));
+    }
+    return LVR_OK;
+}
+
+int lvr_noise_brick_host(const lvr_volume_info* info, const lvr_noise_params* p,
+                         uint64_t node_id, void* out, size_t bytes)
+{
+    if (!info || !p || !out)
+        return LVR_EINVAL;
+    lvr_volume_info vi = *info;
+    if (lvr_fill_regular_volume_info(&vi) != LVR_OK)
+        return LVR_EINVAL;
+    const size_t n = size_t(vi.max_block[0]) * vi.max_block[1] * vi.max_block[2];
+    if (bytes != n * lvr::bytesPerVoxel(vi.data_type))
+        return LVR_EINVAL;
+    if (lvr::decodeNode(node_id).level >= vi.depth)
+        return LVR_EINVAL;
+    const lvr::NoiseShape s = lvr::makeShape(vi, *p);
+    const lvr::BrickGeo g = lvr::brickGeo(vi, node_id);
+    switch (vi.data_type)
+    {
+    case LVR_DT_UINT8:
+        lvr::noiseBrickHost(s, p->remap, g, static_cast<uint8_t*>(out));
+        return LVR_OK;
+    case LVR_DT_UINT16:
+        lvr::noiseBrickHost(s, p->remap, g, static_cast<uint16_t*>(out));
+        return LVR_OK;
+    case LVR_DT_FLOAT:
+        lvr::noiseBrickHost(s, p->remap, g, static_cast<float*>(out));
+        return LVR_OK;
+    default:
+        return LVR_EINVAL;
+    }
+}
+
+int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
+                            const uint64_t* node_ids, int n)
+{
+    if (!ctx || !p || n < 0 || (n > 0 && !node_ids))
+        return LVR_EINVAL;
+    const int dt = ctx->vi.data_type;
+    if (dt != LVR_DT_UINT8 && dt != LVR_DT_UINT16 && dt != LVR_DT_FLOAT)
+        return lvr::fail(ctx, LVR_EINVAL, "noise source supports u8, u16, f32");
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    const lvr::NoiseShape s = lvr::makeShape(ctx->vi, *p);
+    uint16_t* d_remap = nullptr;
+    lvr::BrickGeo* d_geo = nullptr;
+    int32_t* d_slots = nullptr;
+    const int chunk = 4096;
+    LVR_HIP(ctx, hipMalloc((void**)&d_remap, sizeof p->remap));
+    LVR_HIP(ctx, hipMalloc((void**)&d_geo, sizeof(lvr::BrickGeo) * chunk));
+    LVR_HIP(ctx, hipMalloc((void**)&d_slots, sizeof(int32_t) * chunk));
+    LVR_HIP(ctx, hipMemcpy(d_remap, p->remap, sizeof p->remap,
+                           hipMemcpyHostToDevice));
+    std::vector<lvr::BrickGeo> geo;
+    std::vector<int32_t> slots;
+    for (int base = 0; base < n && rc == LVR_OK; base += chunk)
+    {
+        const int cnt = std::min(chunk, n - base);
+        geo.resize(cnt);
+        slots.resize(cnt);
+        for (int i = 0; i < cnt && rc == LVR_OK; ++i)
+        {
+            if (lvr::decodeNode(node_ids[base + i]).level >= ctx->vi.depth)
+                rc = lvr::fail(ctx, LVR_EINVAL, "node id outside the octree");
+            else
+            {
+                geo[i] = lvr::brickGeo(ctx->vi, node_ids[base + i]);
+                rc = lvr::acquireSlotFor(ctx, node_ids[base + i], &slots[i]);
+            }
+        }
+        if (rc)
+            break;
+        if (hipMemcpyAsync(d_geo, geo.data(), sizeof(lvr::BrickGeo) * cnt,
+                           hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt,
+                           hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+        {
+            rc = lvr::fail(ctx, LVR_EHIP, "noise upload");
+            break;
+        }
+        const dim3 grid(64, unsigned(cnt));
+        if (dt == LVR_DT_UINT8)
+            hipLaunchKernelGGL(lvr::noiseBricksKernel<uint8_t>, grid, dim3(256), 0,
+                               ctx->stream, s, d_remap, d_geo, d_slots,
+                               ctx->d_pool, ctx->slotStride);
+        else if (dt == LVR_DT_UINT16)
+            hipLaunchKernelGGL(lvr::noiseBricksKernel<uint16_t>, grid, dim3(256),
+                               0, ctx->stream, s, d_remap, d_geo, d_slots,
+                               ctx->d_pool, ctx->slotStride);
+        else
+            hipLaunchKernelGGL(lvr::noiseBricksKernel<float>, grid, dim3(256), 0,
+                               ctx->stream, s, d_remap, d_geo, d_slots,
+                               ctx->d_pool, ctx->slotStride);
+        if (hipGetLastError() != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");
+    }
+    (void)hipFree(d_remap);
+    (void)hipFree(d_geo);
+    (void)hipFree(d_slots);
+    return rc;
+}
+}

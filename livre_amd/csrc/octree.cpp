@@ -1,0 +1,218 @@
+// Octree shape, node boxes, frustum and brick ordering: the host part of the
+// render path that the reference computes on the CPU per frame.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "lvr_internal.h"
+
+namespace lvr
+{
+size_t bytesPerVoxel(int dataType)
+{
+    switch (dataType)
+    {
+    case LVR_DT_UINT8:
+    case LVR_DT_INT8:
+        return 1;
+    case LVR_DT_UINT16:
+    case LVR_DT_INT16:
+        return 2;
+    case LVR_DT_UINT32:
+    case LVR_DT_INT32:
+    case LVR_DT_FLOAT:
+        return 4;
+    default:
+        return 0;
+    }
+}
+
+namespace
+{
+inline float at(const float* m, int row, int col)
+{
+    return m[col * 4 + row];
+}
+
+// General 4x4 inverse by cofactors in double precision, rounded to float.
+bool inverse4(const float* src, float* dst)
+{
+    double m[16], inv[16];
+    for (int i = 0; i < 16; ++i)
+        m[i] = src[i];
+    inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] +
+             m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] +
+             m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] +
+             m[13] * m[3] * m[10];
+    inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] +
+             m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] -
+             m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] +
+             m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] +
+             m[12] * m[7] * m[10];
+    inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] +
+             m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] -
+             m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] +
+             m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] +
+             m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] -
+             m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] +
+              m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] -
+              m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] +
+              m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] +
+              m[12] * m[6] * m[9];
+    inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] +
+              m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] -
+              m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] +
+              m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    const double det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] +
+                       m[3] * inv[12];
+    if (det == 0.0 || !std::isfinite(det))
+        return false;
+    for (int i = 0; i < 16; ++i)
+        dst[i] = float(inv[i] / det);
+    return true;
+}
+} // namespace
+} // namespace lvr
+
+extern "C" {
+
+int lvr_fill_regular_volume_info(lvr_volume_info* info)
+{
+    if (!info || lvr::bytesPerVoxel(info->data_type) == 0)
+        return LVR_EINVAL;
+    uint32_t block[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        if (info->max_block[i] <= 2u * info->overlap[i] || info->voxels[i] == 0)
+            return LVR_EINVAL;
+        block[i] = info->max_block[i] - 2u * info->overlap[i];
+    }
+    // worldSpacePerVoxel = 1 / max(voxels); worldSize = voxels * that
+    const uint32_t vmax =
+        std::max(info->voxels[0], std::max(info->voxels[1], info->voxels[2]));
+    info->world_space_per_voxel = 1.0f / float(vmax);
+    uint32_t levels = ~0u;
+    for (int i = 0; i < 3; ++i)
+    {
+        info->world_size[i] = float(info->voxels[i]) * info->world_space_per_voxel;
+        const uint32_t nBlocks =
+            uint32_t(std::ceil(float(info->voxels[i]) / float(block[i])));
+        const uint32_t l = uint32_t(std::ceil(std::log2(double(nBlocks))));
+        levels = std::min(levels, l);
+    }
+    for (int i = 0; i < 3; ++i)
+        info->root_blocks[i] = uint32_t(
+            std::ceil(float(info->voxels[i] >> levels) / float(block[i])));
+    info->depth = levels + 1;
+    if (info->depth > 15)
+        return LVR_EINVAL; // NodeId holds 4 level bits, 15 = invalid
+    return LVR_OK;
+}
+
+int lvr_node_world_box(const lvr_volume_info* info, uint64_t node_id,
+                       float box[6])
+{
+    if (!info || !box)
+        return LVR_EINVAL;
+    const lvr::NodeKey k = lvr::decodeNode(node_id);
+    if (k.level >= info->depth)
+        return LVR_EINVAL;
+    lvr::nodeWorldBox(*info, k, box, box + 3);
+    return LVR_OK;
+}
+
+// Frustum::Impl (livre/data/Frustum.cpp:24-42) over vmmlib's Frustumf(P)
+// and FrustumCullerf(P * MV); vmmlib f4b3706 is not vendored, its published
+// formulas are used: near = P23 / (P22 - 1), far = P23 / (P22 + 1),
+// top/bottom = near * (P12 +/- 1) / P11, left/right = near * (P02 -/+ 1) /
+// P00, and normalized Gribb-Hartmann planes from the rows of P * MV.
+int lvr_frustum_init(lvr_frustum* f, const float mv[16], const float proj[16])
+{
+    using lvr::at;
+    if (!f || !mv || !proj)
+        return LVR_EINVAL;
+    std::memcpy(f->mv, mv, sizeof f->mv);
+    std::memcpy(f->proj, proj, sizeof f->proj);
+    if (!lvr::inverse4(mv, f->inv_mv) || !lvr::inverse4(proj, f->inv_proj))
+        return LVR_EINVAL;
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j)
+        {
+            float s = 0.f;
+            for (int k = 0; k < 4; ++k)
+                s += at(proj, i, k) * at(mv, k, j);
+            f->mvp[j * 4 + i] = s;
+        }
+    f->near_plane = at(proj, 2, 3) / (at(proj, 2, 2) - 1.0f);
+    f->far_plane = at(proj, 2, 3) / (at(proj, 2, 2) + 1.0f);
+    f->bottom = f->near_plane * (at(proj, 1, 2) - 1.0f) / at(proj, 1, 1);
+    f->top = f->near_plane * (at(proj, 1, 2) + 1.0f) / at(proj, 1, 1);
+    f->left = f->near_plane * (at(proj, 0, 2) - 1.0f) / at(proj, 0, 0);
+    f->right = f->near_plane * (at(proj, 0, 2) + 1.0f) / at(proj, 0, 0);
+    const float* m = f->mvp;
+    const int rowOf[6] = {0, 0, 1, 1, 2, 2};
+    const float sign[6] = {1.f, -1.f, 1.f, -1.f, 1.f, -1.f};
+    for (int p = 0; p < 6; ++p)
+    {
+        float* pl = f->planes[p];
+        for (int j = 0; j < 4; ++j)
+        {
+            const float r3 = at(m, 3, j), ri = at(m, rowOf[p], j);
+            pl[j] = sign[p] > 0.f ? r3 + ri : r3 - ri;
+        }
+        const float len = std::sqrt(pl[0] * pl[0] + pl[1] * pl[1] + pl[2] * pl[2]);
+        for (int j = 0; j < 4; ++j)
+            pl[j] /= len;
+    }
+    for (int i = 0; i < 3; ++i)
+    {
+        f->eye[i] = f->inv_mv[12 + i];
+        f->view_dir[i] = f->inv_mv[8 + i];
+    }
+    return LVR_OK;
+}
+
+// RayCastRenderer::order with DistanceOperator (RayCastRenderer.cpp:57-81,
+// 153-159): std::sort of the given order by |MV * centre|. The distances are
+// computed once; the comparisons, and so libstdc++'s tie handling, are the
+// same as recomputing them per comparison.
+int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
+              const uint64_t* ids, int n, uint64_t* out)
+{
+    if (!info || !f || (n > 0 && (!ids || !out)) || n < 0)
+        return LVR_EINVAL;
+    std::vector<std::pair<float, uint64_t>> v(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i)
+    {
+        float mn[3], mx[3];
+        lvr::nodeWorldBox(*info, lvr::decodeNode(ids[i]), mn, mx);
+        float c[3];
+        for (int a = 0; a < 3; ++a)
+            c[a] = (mn[a] + mx[a]) * 0.5f;
+        float r[4];
+        for (int a = 0; a < 4; ++a)
+            r[a] = lvr::at(f->mv, a, 0) * c[0] + lvr::at(f->mv, a, 1) * c[1] +
+                   lvr::at(f->mv, a, 2) * c[2] + lvr::at(f->mv, a, 3);
+        const float x = r[0] / r[3], y = r[1] / r[3], z = r[2] / r[3];
+        v[size_t(i)] = {std::sqrt(x * x + y * y + z * z), ids[i]};
+    }
+    std::sort(v.begin(), v.end(),
+              [](const std::pair<float, uint64_t>& a,
+                 const std::pair<float, uint64_t>& b) { return a.first < b.first; });
+    for (int i = 0; i < n; ++i)
+        out[i] = v[size_t(i)].second;
+    return LVR_OK;
+}
+}

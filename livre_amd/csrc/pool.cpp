@@ -1,0 +1,460 @@
+// Context, HBM brick pool, device page table, brick cache policy and
+// transfer-function upload.
+//
+// The pool is one hipMalloc arena of fixed-size slots (one slot per brick of
+// maximumBlockSize^3 voxels, 256-byte aligned), the HBM counterpart of
+// TexturePool's one-3D-texture-per-brick scheme (TexturePool.cpp:90-126).
+// The page table maps a NodeId to its slot on the device, so kernels can
+// resolve bricks without the host.
+#include <algorithm>
+#include <cstring>
+
+#include "lvr_internal.h"
+
+namespace lvr
+{
+int fail(lvr_ctx* ctx, int code, const std::string& msg)
+{
+    if (ctx)
+        ctx->error = msg;
+    return code;
+}
+
+int hipFail(lvr_ctx* ctx, hipError_t e, const char* what)
+{
+    return fail(ctx, LVR_EHIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+int ensureDevice(lvr_ctx* ctx)
+{
+    LVR_HIP(ctx, hipSetDevice(ctx->device));
+    return LVR_OK;
+}
+
+static bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx)
+{
+    const NodeKey k = decodeNode(id);
+    if (k.level >= ctx->vi.depth || k.t != 0)
+        return false;
+    const uint32_t* d = &ctx->pt.levelDim[3 * k.level];
+    if (k.x >= d[0] || k.y >= d[1] || k.z >= d[2])
+        return false;
+    *idx = ctx->pt.levelOffset[k.level] + k.x +
+           uint64_t(d[0]) * (k.y + uint64_t(d[1]) * k.z);
+    return true;
+}
+
+int32_t slotOf(const lvr_ctx* ctx, uint64_t id)
+{
+    auto it = ctx->resident.find(id);
+    return it == ctx->resident.end() ? -1 : it->second.slot;
+}
+
+void touch(lvr_ctx* ctx, uint64_t id)
+{
+    if (ctx->policy != 0)
+        return;
+    auto it = ctx->resident.find(id);
+    if (it == ctx->resident.end())
+        return;
+    ctx->lruList.splice(ctx->lruList.end(), ctx->lruList, it->second.lru);
+}
+
+static int setPage(lvr_ctx* ctx, uint64_t id, int32_t slot)
+{
+    uint64_t idx;
+    if (!pageIndex(ctx, id, &idx))
+        return fail(ctx, LVR_EINVAL, "node id outside the octree");
+    ctx->pt.h_slots[idx] = slot;
+    ctx->pt.dirtyLo = std::min(ctx->pt.dirtyLo, idx);
+    ctx->pt.dirtyHi = std::max(ctx->pt.dirtyHi, idx + 1);
+    return LVR_OK;
+}
+
+// Copy the dirty range of the host page table to the device on the render
+// stream, before any kernel that resolves bricks through it.
+int flushPageTable(lvr_ctx* ctx)
+{
+    if (ctx->pt.dirtyLo >= ctx->pt.dirtyHi)
+        return LVR_OK;
+    const uint64_t lo = ctx->pt.dirtyLo, hi = ctx->pt.dirtyHi;
+    LVR_HIP(ctx, hipMemcpyAsync(ctx->pt.d_slots + lo, &ctx->pt.h_slots[lo],
+                                (hi - lo) * sizeof(int32_t),
+                                hipMemcpyHostToDevice, ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pt.dirtyLo = ~0ull;
+    ctx->pt.dirtyHi = 0;
+    return LVR_OK;
+}
+
+// Evict the first unpinned brick in policy order (Cache.cpp:111-123 walks
+// the queue in insertion order and skips referenced objects).
+static int evictOne(lvr_ctx* ctx)
+{
+    for (auto it = ctx->lruList.begin(); it != ctx->lruList.end(); ++it)
+    {
+        auto r = ctx->resident.find(*it);
+        if (r->second.pinned)
+            continue;
+        const uint64_t id = *it;
+        const int32_t slot = r->second.slot;
+        ctx->lruList.erase(it);
+        ctx->resident.erase(r);
+        ctx->freeSlots.push_back(slot);
+        return setPage(ctx, id, -1);
+    }
+    return fail(ctx, LVR_ENOMEM, "brick pool full and every brick pinned");
+}
+
+// Find or allocate a slot for id; returns the slot index or a negative code.
+static int acquireSlot(lvr_ctx* ctx, uint64_t id, int32_t* slot)
+{
+    uint64_t idx;
+    if (!pageIndex(ctx, id, &idx))
+        return fail(ctx, LVR_EINVAL, "node id outside the octree");
+    auto it = ctx->resident.find(id);
+    if (it != ctx->resident.end())
+    {
+        *slot = it->second.slot;
+        touch(ctx, id);
+        return LVR_OK;
+    }
+    if (ctx->freeSlots.empty())
+    {
+        const int rc = evictOne(ctx);
+        if (rc != LVR_OK)
+            return rc;
+    }
+    *slot = ctx->freeSlots.back();
+    ctx->freeSlots.pop_back();
+    ctx->lruList.push_back(id);
+    PoolEntry e;
+    e.slot = *slot;
+    e.pinned = 0;
+    e.lru = std::prev(ctx->lruList.end());
+    ctx->resident.emplace(id, e);
+    return setPage(ctx, id, *slot);
+}
+
+int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot)
+{
+    return acquireSlot(ctx, id, slot);
+}
+
+static int ensureStaging(lvr_ctx* ctx, size_t bytes)
+{
+    if (ctx->stagingBytes >= bytes)
+        return LVR_OK;
+    if (ctx->h_staging)
+    {
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+        LVR_HIP(ctx, hipHostFree(ctx->h_staging));
+        ctx->h_staging = nullptr;
+    }
+    LVR_HIP(ctx, hipHostMalloc((void**)&ctx->h_staging, bytes,
+                               hipHostMallocDefault));
+    ctx->stagingBytes = bytes;
+    return LVR_OK;
+}
+} // namespace lvr
+
+extern "C" {
+
+int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
+                   lvr_ctx** out)
+{
+    if (!info || !out)
+        return LVR_EINVAL;
+    *out = nullptr;
+    lvr_volume_info vi = *info;
+    if (lvr_fill_regular_volume_info(&vi) != LVR_OK)
+        return LVR_EINVAL;
+    lvr_ctx* ctx = new lvr_ctx();
+    ctx->device = device;
+    ctx->vi = vi;
+    ctx->voxelBytes = lvr::bytesPerVoxel(vi.data_type);
+    ctx->brickBytes = size_t(vi.max_block[0]) * vi.max_block[1] *
+                      vi.max_block[2] * ctx->voxelBytes;
+    ctx->slotStride = (ctx->brickBytes + 255) & ~size_t(255);
+    ctx->nSlots = pool_bytes / ctx->slotStride;
+    auto bail = [&](int rc) {
+        std::string msg = ctx->error;
+        lvr_ctx_destroy(ctx);
+        (void)msg;
+        return rc;
+    };
+    if (ctx->nSlots == 0 || ctx->nSlots > (1ull << 31))
+        return bail(LVR_EINVAL);
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return bail(rc);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->uploadStream, hipStreamNonBlocking) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&ctx->uploadDone, hipEventDisableTiming) !=
+            hipSuccess)
+        return bail(LVR_EHIP);
+    if (hipMalloc((void**)&ctx->d_pool, ctx->nSlots * ctx->slotStride) !=
+        hipSuccess)
+        return bail(LVR_ENOMEM);
+
+    // page table: one entry per node of the full octree
+    ctx->pt.levelOffset.resize(vi.depth + 1);
+    ctx->pt.levelDim.resize(3 * vi.depth);
+    uint64_t total = 0;
+    for (uint32_t l = 0; l < vi.depth; ++l)
+    {
+        ctx->pt.levelOffset[l] = total;
+        uint64_t cnt = 1;
+        for (int a = 0; a < 3; ++a)
+        {
+            ctx->pt.levelDim[3 * l + a] = vi.root_blocks[a] << l;
+            cnt *= ctx->pt.levelDim[3 * l + a];
+        }
+        total += cnt;
+    }
+    ctx->pt.levelOffset[vi.depth] = total;
+    ctx->pt.size = total;
+    ctx->pt.h_slots.assign(total, -1);
+    if (hipMalloc((void**)&ctx->pt.d_slots, total * sizeof(int32_t)) !=
+        hipSuccess)
+        return bail(LVR_ENOMEM);
+    if (hipMemcpy(ctx->pt.d_slots, ctx->pt.h_slots.data(),
+                  total * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(LVR_EHIP);
+
+    ctx->freeSlots.reserve(ctx->nSlots);
+    for (int64_t s = int64_t(ctx->nSlots) - 1; s >= 0; --s)
+        ctx->freeSlots.push_back(int32_t(s));
+    if (hipHostMalloc((void**)&ctx->h_counters, 4 * sizeof(unsigned long long),
+                      hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_counters, 4 * sizeof(unsigned long long)) !=
+            hipSuccess)
+        return bail(LVR_ENOMEM);
+    *out = ctx;
+    return LVR_OK;
+}
+
+void lvr_ctx_destroy(lvr_ctx* ctx)
+{
+    if (!ctx)
+        return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream)
+        (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->uploadStream)
+        (void)hipStreamSynchronize(ctx->uploadStream);
+    (void)hipFree(ctx->d_pool);
+    (void)hipFree(ctx->pt.d_slots);
+    (void)hipFree(ctx->d_tf);
+    (void)hipFree(ctx->d_frame);
+    (void)hipFree(ctx->d_bricks);
+    (void)hipFree(ctx->d_tileRange);
+    (void)hipFree(ctx->d_tileCount);
+    (void)hipFree(ctx->d_tileOffset);
+    (void)hipFree(ctx->d_tileList);
+    (void)hipFree(ctx->d_counters);
+    (void)hipFree(ctx->d_hist);
+    (void)hipFree(ctx->d_histSlots);
+    (void)hipFree(ctx->d_histScale);
+    (void)hipFree(ctx->d_selA);
+    (void)hipFree(ctx->d_selB);
+    (void)hipFree(ctx->d_selOut);
+    (void)hipFree(ctx->d_selCount);
+    if (ctx->h_bricks)
+        (void)hipHostFree(ctx->h_bricks);
+    if (ctx->h_staging)
+        (void)hipHostFree(ctx->h_staging);
+    if (ctx->h_counters)
+        (void)hipHostFree(ctx->h_counters);
+    if (ctx->uploadDone)
+        (void)hipEventDestroy(ctx->uploadDone);
+    if (ctx->stream)
+        (void)hipStreamDestroy(ctx->stream);
+    if (ctx->uploadStream)
+        (void)hipStreamDestroy(ctx->uploadStream);
+    delete ctx;
+}
+
+const char* lvr_last_error(const lvr_ctx* ctx)
+{
+    return ctx ? ctx->error.c_str() : "null context";
+}
+
+int lvr_ctx_sync(lvr_ctx* ctx)
+{
+    if (!ctx)
+        return LVR_EINVAL;
+    LVR_HIP(ctx, hipSetDevice(ctx->device));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LVR_OK;
+}
+
+int lvr_pool_set_policy(lvr_ctx* ctx, int policy)
+{
+    if (!ctx || (policy != 0 && policy != 1))
+        return LVR_EINVAL;
+    ctx->policy = policy;
+    return LVR_OK;
+}
+
+int lvr_pool_stats(const lvr_ctx* ctx, uint64_t* slots, uint64_t* used,
+                   uint64_t* slot_bytes)
+{
+    if (!ctx)
+        return LVR_EINVAL;
+    if (slots)
+        *slots = ctx->nSlots;
+    if (used)
+        *used = ctx->resident.size();
+    if (slot_bytes)
+        *slot_bytes = ctx->slotStride;
+    return LVR_OK;
+}
+
+int lvr_brick_upload(lvr_ctx* ctx, uint64_t node_id, const void* voxels,
+                     size_t bytes)
+{
+    return lvr_brick_upload_many(ctx, &node_id, 1, voxels, bytes);
+}
+
+int lvr_brick_upload_many(lvr_ctx* ctx, const uint64_t* node_ids, int n,
+                          const void* voxels, size_t bytes_per_brick)
+{
+    if (!ctx || n < 0 || (n > 0 && (!node_ids || !voxels)))
+        return LVR_EINVAL;
+    if (bytes_per_brick != ctx->brickBytes)
+        return lvr::fail(ctx, LVR_EINVAL, "brick size does not match "
+                                          "maximumBlockSize^3 * bytesPerVoxel");
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    // stage in chunks of up to 64 MiB of pinned memory
+    const size_t chunkBricks =
+        std::max<size_t>(1, (64u << 20) / ctx->brickBytes);
+    rc = lvr::ensureStaging(ctx, std::min<size_t>(chunkBricks, size_t(n)) *
+                                     ctx->brickBytes);
+    if (rc)
+        return rc;
+    const uint8_t* src = static_cast<const uint8_t*>(voxels);
+    for (int base = 0; base < n; base += int(chunkBricks))
+    {
+        const int cnt = std::min<int>(int(chunkBricks), n - base);
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+        std::memcpy(ctx->h_staging, src + size_t(base) * ctx->brickBytes,
+                    size_t(cnt) * ctx->brickBytes);
+        for (int i = 0; i < cnt; ++i)
+        {
+            int32_t slot;
+            rc = lvr::acquireSlot(ctx, node_ids[base + i], &slot);
+            if (rc)
+                return rc;
+            LVR_HIP(ctx, hipMemcpyAsync(ctx->d_pool + size_t(slot) * ctx->slotStride,
+                                        ctx->h_staging + size_t(i) * ctx->brickBytes,
+                                        ctx->brickBytes, hipMemcpyHostToDevice,
+                                        ctx->uploadStream));
+        }
+    }
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+    return LVR_OK;
+}
+
+int lvr_brick_evict(lvr_ctx* ctx, uint64_t node_id)
+{
+    if (!ctx)
+        return LVR_EINVAL;
+    auto it = ctx->resident.find(node_id);
+    if (it == ctx->resident.end())
+        return lvr::fail(ctx, LVR_ENOENT, "brick not resident");
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    // kernels already queued may still read the slot
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->freeSlots.push_back(it->second.slot);
+    ctx->lruList.erase(it->second.lru);
+    ctx->resident.erase(it);
+    rc = lvr::setPage(ctx, node_id, -1);
+    if (rc)
+        return rc;
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+    return LVR_OK;
+}
+
+int lvr_brick_resident(const lvr_ctx* ctx, uint64_t node_id)
+{
+    if (!ctx)
+        return LVR_EINVAL;
+    return ctx->resident.count(node_id) ? 1 : 0;
+}
+
+int lvr_brick_download(lvr_ctx* ctx, uint64_t node_id, void* out, size_t bytes)
+{
+    if (!ctx || !out || bytes != ctx->brickBytes)
+        return LVR_EINVAL;
+    const int32_t slot = lvr::slotOf(ctx, node_id);
+    if (slot < 0)
+        return lvr::fail(ctx, LVR_ENOENT, "brick not resident");
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LVR_HIP(ctx, hipMemcpy(out, ctx->d_pool + size_t(slot) * ctx->slotStride,
+                           bytes, hipMemcpyDeviceToHost));
+    return LVR_OK;
+}
+
+static int tfUpload(lvr_ctx* ctx, const std::vector<float>& rgba, int n,
+                    float range0, float range1)
+{
+    if (n > 16384)
+        return lvr::fail(ctx, LVR_EINVAL, "transfer function above 16384 entries");
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    if (n > ctx->tfCapacity)
+    {
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        (void)hipFree(ctx->d_tf);
+        ctx->d_tf = nullptr;
+        LVR_HIP(ctx, hipMalloc((void**)&ctx->d_tf, size_t(n) * 4 * sizeof(float)));
+        ctx->tfCapacity = n;
+    }
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LVR_HIP(ctx, hipMemcpy(ctx->d_tf, rgba.data(), size_t(n) * 4 * sizeof(float),
+                           hipMemcpyHostToDevice));
+    ctx->nTf = n;
+    ctx->range0 = range0;
+    ctx->range1 = range1;
+    ctx->tfSet = true;
+    return LVR_OK;
+}
+
+int lvr_tf_set(lvr_ctx* ctx, const float* rgba, int n, float range0,
+               float range1)
+{
+    if (!ctx || !rgba || n < 1)
+        return LVR_EINVAL;
+    if (!(range1 > range0))
+        return lvr::fail(ctx, LVR_EINVAL, "empty data range");
+    std::vector<float> v(rgba, rgba + size_t(n) * 4);
+    return tfUpload(ctx, v, n, range0, range1);
+}
+
+int lvr_tf_set_rgba8(lvr_ctx* ctx, const uint8_t* lut, int n, float range0,
+                     float range1)
+{
+    if (!ctx || !lut || n < 1)
+        return LVR_EINVAL;
+    if (!(range1 > range0))
+        return lvr::fail(ctx, LVR_EINVAL, "empty data range");
+    std::vector<float> v(size_t(n) * 4);
+    for (size_t i = 0; i < v.size(); ++i)
+        v[i] = float(lut[i]) / 255.0f;
+    return tfUpload(ctx, v, n, range0, range1);
+}
+}

@@ -1,0 +1,239 @@
+"""GPU parity of the HIP render path against the oracle, through the C ABI.
+
+Tolerances (from the path's specification): RGBA8 frames within max 2/255
+and mean 0.5/255 per channel of the FP32 restatement of fragRayCast.glsl;
+executed composite steps within 1e-4 relative; visible-brick sets, their
+order and integer histograms bit-exact.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import livre_amd
+from livre_amd import scenes
+
+from scene_kit import Scene, diff_stats
+
+pytestmark = pytest.mark.gpu
+
+MAX_DIFF = 2
+MEAN_DIFF = 0.5
+
+
+def _check_frame(scene, ctx, eye, w, h, tf, n_samples, linear=False, exit_a=0.999,
+                 missing=(), rgba8_tf=True, fovy=60.0):
+    mv, proj, fr = scene.camera(eye, w, h, fovy)
+    ordered = scene.visible_ordered(mv, proj, h)
+    assert np.array_equal(livre_amd.order(scene.vi, fr, ordered), ordered)
+    if rgba8_tf:
+        lut = scenes.lut_rgba8(tf)
+        tf_used = lut.astype(np.float32) / np.float32(255.0)
+        ctx.set_tf_rgba8(lut, scene.range)
+    else:
+        tf_used = tf
+        ctx.set_tf(tf, scene.range)
+    for m in missing:
+        if ctx.resident(m):
+            ctx.evict(m)
+    ref, ref_samples = scene.oracle_render(fr, w, h, ordered, tf_used, n_samples,
+                                           early_exit=exit_a, linear=linear,
+                                           missing=missing)
+    params = livre_amd.render_params(n_samples, early_exit=exit_a, linear=linear)
+    ctx.render(fr, w, h, ordered, params)
+    img, samples = ctx.read_frame()
+    ref8 = oracle.to_rgba8(ref)
+    mx, mean = diff_stats(img, ref8)
+    print("eye", eye, "bricks", len(ordered), "max", mx, "mean", mean,
+          "samples", samples, ref_samples, flush=True)
+    assert mx <= MAX_DIFF and mean <= MEAN_DIFF
+    assert abs(int(samples) - int(ref_samples)) <= 1e-4 * max(1, ref_samples)
+    assert ref_samples > 0 or not ordered.size
+    return img
+
+
+@pytest.fixture(scope="module")
+def blobs_u8(lvr, orc):
+    s = Scene(livre_amd.DT_UINT8, n=64, block=16, overlap=1, source="blobs")
+    ctx = s.context()
+    yield s, ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_orbit_u8_nearest(blobs_u8, k):
+    s, ctx = blobs_u8
+    tf = scenes.piecewise_tf(256, seed=2)
+    _check_frame(s, ctx, scenes.orbit(k, 4, radius=2.0, height=0.3 * k - 0.4), 96, 80,
+                 tf, 64)
+
+
+def test_tile_shapes_agree(blobs_u8):
+    s, ctx = blobs_u8
+    tf = scenes.piecewise_tf(256, seed=3)
+    lut = scenes.lut_rgba8(tf)
+    ctx.set_tf_rgba8(lut, s.range)
+    mv, proj, fr = s.camera((1.2, 0.9, 1.4), 100, 70)
+    ordered = s.visible_ordered(mv, proj, 70)
+    imgs = []
+    for tile in ((8, 8), (16, 4), (4, 16), (32, 2), (64, 1)):
+        ctx.render(fr, 100, 70, ordered, livre_amd.render_params(64, tile=tile))
+        imgs.append(ctx.read_frame())
+    for img, smp in imgs[1:]:
+        assert np.array_equal(img, imgs[0][0]) and smp == imgs[0][1]
+
+
+def test_camera_inside_volume(blobs_u8):
+    s, ctx = blobs_u8
+    tf = scenes.piecewise_tf(256, seed=4, max_alpha=0.3)
+    mv, proj, fr = s.camera((0.1, 0.05, 0.2), 64, 64, 70.0, target=(0.4, 0.1, -0.3))
+    ordered = s.visible_ordered(mv, proj, 64)
+    lut = scenes.lut_rgba8(tf)
+    ctx.set_tf_rgba8(lut, s.range)
+    ref, ref_samples = s.oracle_render(fr, 64, 64, ordered,
+                                       lut.astype(np.float32) / np.float32(255), 64)
+    ctx.render(fr, 64, 64, ordered, livre_amd.render_params(64))
+    img, samples = ctx.read_frame()
+    mx, mean = diff_stats(img, oracle.to_rgba8(ref))
+    assert mx <= MAX_DIFF and mean <= MEAN_DIFF
+    assert abs(int(samples) - int(ref_samples)) <= 1e-4 * max(1, ref_samples)
+
+
+def test_exit_threshold_and_sampling_rate(blobs_u8):
+    s, ctx = blobs_u8
+    tf = scenes.piecewise_tf(256, seed=5, max_alpha=0.9)
+    _check_frame(s, ctx, (0.3, 1.6, 1.3), 64, 64, tf, 128, exit_a=0.99)
+    _check_frame(s, ctx, (0.3, 1.6, 1.3), 64, 64, tf, 32)
+
+
+def test_missing_bricks_are_skipped(lvr, orc):
+    s = Scene(livre_amd.DT_UINT8, n=64, block=16, overlap=1, source="blobs", seed=7)
+    ctx = s.context()
+    try:
+        mv, proj, fr = s.camera((1.7, 0.4, 1.0), 64, 64)
+        ordered = s.visible_ordered(mv, proj, 64)
+        missing = tuple(int(x) for x in ordered[::3])
+        _check_frame(s, ctx, (1.7, 0.4, 1.0), 64, 64, scenes.piecewise_tf(256, seed=8),
+                     64, missing=missing)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("dt", [livre_amd.DT_UINT16, livre_amd.DT_FLOAT])
+def test_noise_volumes_fp32_tf(lvr, orc, dt):
+    s = Scene(dt, n=64, block=16, overlap=1, source="noise", seed=11)
+    ctx = s.context()
+    try:
+        tf = scenes.piecewise_tf(4096 if dt == livre_amd.DT_UINT16 else 256, seed=12,
+                                 max_alpha=0.4)
+        _check_frame(s, ctx, (-1.4, 0.8, 1.5), 80, 64, tf, 64, rgba8_tf=False)
+        _check_frame(s, ctx, (0.2, -2.1, 0.6), 64, 64, tf, 128, rgba8_tf=False)
+    finally:
+        ctx.close()
+
+
+def test_trilinear_against_fp32_oracle(lvr, orc):
+    s = Scene(livre_amd.DT_UINT16, n=64, block=16, overlap=1, source="noise", seed=13)
+    ctx = s.context()
+    try:
+        tf = scenes.piecewise_tf(1024, seed=14, max_alpha=0.5)
+        _check_frame(s, ctx, (1.3, 1.1, -1.2), 72, 72, tf, 64, linear=True,
+                     rgba8_tf=False)
+    finally:
+        ctx.close()
+
+
+def test_device_noise_equals_host_noise(lvr):
+    vi = livre_amd.volume_info(128, 32, 1, livre_amd.DT_UINT16)
+    p = livre_amd.noise_params(3, octaves=5, base_cells=8)
+    ids = scenes.all_nodes(vi)
+    per = int(np.prod(vi.brick_shape)) * 2
+    with livre_amd.Context(vi, per * len(ids)) as ctx:
+        ctx.generate_noise(p, ids)
+        for nid in ids[:: max(1, len(ids) // 20)]:
+            assert np.array_equal(ctx.download(nid), livre_amd.noise_brick_host(vi, p, nid))
+
+
+@pytest.mark.parametrize("dt,bins", [(livre_amd.DT_UINT8, 256),
+                                     (livre_amd.DT_UINT16, 65536),
+                                     (livre_amd.DT_FLOAT, 256)])
+def test_histogram_bit_exact(lvr, orc, dt, bins):
+    s = Scene(dt, n=64, block=16, overlap=1, source="noise" if dt != livre_amd.DT_UINT8
+              else "blobs", seed=21)
+    ctx = s.context()
+    try:
+        if dt == livre_amd.DT_FLOAT:
+            ctx.set_tf(scenes.piecewise_tf(256, seed=1), s.range)
+        mv, proj, fr = s.camera((1.5, 0.5, 1.2), 128, 128)
+        ids = s.visible_ordered(mv, proj, 128)
+        got = ctx.histogram(ids, bins)
+        ref = oracle.histogram(s.ovi, ids, [s.bricks[int(i)] for i in ids], bins,
+                               s.range)
+        assert np.array_equal(got, ref)
+        # every brick of the tree: sum = interior voxels * scale
+        got = ctx.histogram(s.ids, bins)
+        ref = oracle.histogram(s.ovi, s.ids, [s.bricks[int(i)] for i in s.ids], bins,
+                               s.range)
+        assert np.array_equal(got, ref)
+        assert int(got.sum()) == s.vi.depth * 64 ** 3
+    finally:
+        ctx.close()
+
+
+def test_selection_bit_exact_against_goldens(lvr, orc):
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                     "lod_selection.json")))
+    vi = livre_amd.volume_info(fx["voxels"], fx["block"], fx["overlap"], livre_amd.DT_UINT8)
+    fr = livre_amd.frustum(fx["mv"], fx["proj"])
+    with livre_amd.Context(vi, 1 << 24) as ctx:
+        for case in fx["cases"]:
+            got = ctx.select_visibles(fr, case["window_height"], case["sse"],
+                                      case["min_lod"], case["max_lod"])
+            assert sorted(int(x) for x in got) == case["expected"]
+
+
+def test_selection_dfs_order_matches_oracle(lvr, orc):
+    vi = livre_amd.volume_info(2048, 64, 1, livre_amd.DT_UINT16)
+    ovi = orc.volume_info([2048] * 3, [64] * 3, [1] * 3, orc.DT_UINT16)
+    rng = np.random.default_rng(5)
+    with livre_amd.Context(vi, 1 << 24) as ctx:
+        for k in range(12):
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            h = int(rng.integers(256, 2160))
+            w = int(h * 16 / 9)
+            mv, proj = scenes.camera(tuple(d * rng.uniform(0.8, 3.0)), w, h)
+            fr = livre_amd.frustum(mv, proj)
+            sse = float(rng.choice([0.5, 1.0, 2.0, 4.0]))
+            got = ctx.select_visibles(fr, h, sse, 0, vi.depth - 1)
+            ref = orc.select_visibles(ovi, mv, proj, h, sse, 0, ovi.depth - 1)
+            assert np.array_equal(got, ref), k
+
+
+def test_pool_policy_and_eviction(lvr):
+    """A pool of 2 slots evicts in policy order (Cache.cpp:111-123) and the
+    load-order policy reproduces the reference cache test
+    (tests/core/cache.cpp:29-78)."""
+    vi = livre_amd.volume_info(64, 16, 1, livre_amd.DT_UINT8)
+    per = int(np.prod(vi.brick_shape))
+    slot = (per + 255) // 256 * 256
+    brick = np.zeros(vi.brick_shape, np.uint8)
+    a, b, c = (scenes.node_id(2, i, 0, 0) for i in range(3))
+    with livre_amd.Context(vi, 2 * slot + 100) as ctx:
+        assert ctx.stats()[0] == 2
+        ctx.set_policy(1)
+        ctx.upload([a], brick[None])
+        ctx.upload([b], brick[None])
+        ctx.upload([a], brick[None])  # already resident: no reload, no refresh
+        ctx.upload([c], brick[None])  # evicts a (first loaded)
+        assert not ctx.resident(a) and ctx.resident(b) and ctx.resident(c)
+        ctx.set_policy(0)
+        ctx.upload([b], (brick + 1)[None])  # LRU touch
+        ctx.upload([a], brick[None])        # evicts c
+        assert ctx.resident(a) and ctx.resident(b) and not ctx.resident(c)
+        assert np.array_equal(ctx.download(b), brick + 1)
+        ctx.evict(a)
+        assert ctx.stats()[1] == 1
+        with pytest.raises(livre_amd.LivreError):
+            ctx.evict(a)

@@ -1,0 +1,126 @@
+"""Host-side logic of liblivre_amd (no GPU needed): octree shape, node boxes,
+frustum, brick order and the procedural data source, against the oracle; and
+the C ABI exports every declared symbol."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from livre_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(lvr):
+    with open(os.path.join(ROOT, "include", "livre_amd.h")) as f:
+        header = f.read()
+    declared = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(lvr_\w+)\(", header,
+                              re.M))
+    assert declared == set(lvr.EXPORTS)
+    L = lvr.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+
+
+@pytest.mark.parametrize("voxels,block,overlap", [
+    ((256, 256, 256), 32, 1), ((2048, 2048, 2048), 64, 1), ((1536,) * 3, 48, 1),
+    ((2048, 2099, 2048), 64, 0), ((1024, 1024, 512), 32, 4), ((8192,) * 3, 64, 1),
+    ((100, 37, 250), 16, 2)])
+def test_volume_info_matches_oracle(lvr, orc, voxels, block, overlap):
+    a = lvr.volume_info(voxels, block, overlap, lvr.DT_UINT16)
+    b = orc.volume_info(voxels, [block] * 3, [overlap] * 3, orc.DT_UINT16)
+    assert a.depth == b.depth
+    assert list(a.root_blocks) == list(b.root_blocks)
+    assert list(a.world_size) == list(b.world_size)
+    assert a.world_space_per_voxel == b.world_space_per_voxel
+
+
+def test_world_boxes_match_oracle(lvr, orc):
+    rng = np.random.default_rng(0)
+    a = lvr.volume_info((2048, 2099, 1500), 64, 1, lvr.DT_UINT16)
+    b = orc.volume_info((2048, 2099, 1500), [64] * 3, [1] * 3, orc.DT_UINT16)
+    for _ in range(500):
+        lv = int(rng.integers(0, a.depth))
+        pos = [int(rng.integers(0, a.root_blocks[i] << lv)) for i in range(3)]
+        nid = scenes.node_id(lv, *pos)
+        assert np.array_equal(lvr.world_box(a, nid), orc.world_box(b, nid))
+
+
+def _random_cameras(n, seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        r = rng.uniform(0.9, 3.0)
+        w, h = int(rng.integers(64, 2048)), int(rng.integers(64, 2048))
+        yield scenes.camera(tuple(d * r), w, h, float(rng.uniform(30, 90))), h
+
+
+def test_frustum_matches_oracle(lvr, orc):
+    for (mv, proj), _h in _random_cameras(50, 1):
+        f = lvr.frustum(mv, proj)
+        g = orc.frustum_info(mv, proj)
+        assert f.near_plane == g.near_plane and f.far_plane == g.far_plane
+        assert f.top == g.top and f.bottom == g.bottom
+        assert list(f.planes[4]) == list(g.near_plane_eq)
+        assert list(f.eye) == pytest.approx(list(g.eye), abs=1e-6)
+        assert list(f.view_dir) == pytest.approx(list(g.view_dir), abs=1e-6)
+
+
+def test_order_matches_oracle(lvr, orc):
+    a = lvr.volume_info(512, 32, 1, lvr.DT_UINT8)
+    b = orc.volume_info([512] * 3, [32] * 3, [1] * 3, orc.DT_UINT8)
+    for (mv, proj), h in _random_cameras(10, 2):
+        vis = orc.select_visibles(b, mv, proj, h, 1.0, 0, b.depth - 1)
+        f = lvr.frustum(mv, proj)
+        assert np.array_equal(lvr.order(a, f, vis), orc.order(b, mv, vis))
+
+
+def test_order_is_front_to_back(lvr):
+    a = lvr.volume_info(256, 32, 1, lvr.DT_UINT8)
+    mv, proj = scenes.camera((1.5, 0.7, -1.1), 512, 512)
+    f = lvr.frustum(mv, proj)
+    ids = scenes.all_nodes(a)
+    o = lvr.order(a, f, ids)
+    m = np.asarray(mv, np.float64).reshape(4, 4).T
+    d = []
+    for nid in o:
+        box = lvr.world_box(a, nid).astype(np.float64)
+        c = np.append((box[:3] + box[3:]) / 2, 1.0)
+        d.append(np.linalg.norm((m @ c)[:3]))
+    assert np.all(np.diff(d) >= -1e-6)
+
+
+def test_noise_bricks_are_deterministic_and_consistent(lvr):
+    vi = lvr.volume_info(128, 32, 1, lvr.DT_UINT16)
+    p = lvr.noise_params(3, octaves=5, base_cells=8)
+    a = lvr.noise_brick_host(vi, p, scenes.node_id(2, 1, 2, 3))
+    b = lvr.noise_brick_host(vi, p, scenes.node_id(2, 1, 2, 3))
+    assert np.array_equal(a, b)
+    assert a.min() < a.max()
+    # the halo of a brick equals the interior of its neighbour at that level
+    c = lvr.noise_brick_host(vi, p, scenes.node_id(2, 2, 2, 3))
+    assert np.array_equal(a[:, :, -1], c[:, :, 1])
+    assert np.array_equal(a[:, :, -2], c[:, :, 0])
+    # the remap spans the type's range and is monotone
+    r = np.array(p.remap)
+    assert r[0] == 0 and r[-1] == 65535 and np.all(np.diff(r.astype(np.int64)) >= 0)
+
+
+def test_noise_types(lvr):
+    for dt, np_t in ((lvr.DT_UINT8, np.uint8), (lvr.DT_FLOAT, np.float32)):
+        vi = lvr.volume_info(64, 16, 1, dt)
+        p = lvr.noise_params(5, octaves=3, base_cells=4)
+        b = lvr.noise_brick_host(vi, p, scenes.node_id(1, 1, 0, 1))
+        assert b.dtype == np_t and b.shape == (18, 18, 18)
+        if dt == lvr.DT_FLOAT:
+            assert b.min() >= -1.0 and b.max() <= 1.0
+
+
+def test_invalid_arguments_are_rejected(lvr):
+    with pytest.raises(lvr.LivreError):
+        lvr.volume_info(64, 0, 0, lvr.DT_UINT8)
+    vi = lvr.volume_info(64, 16, 1, lvr.DT_UINT8)
+    with pytest.raises(lvr.LivreError):
+        lvr.world_box(vi, scenes.node_id(7, 0, 0, 0))

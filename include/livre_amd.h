@@ -209,6 +209,23 @@ int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes,
 /* Device pointer of the framebuffer (uint32 RGBA8 per pixel). */
 int lvr_frame_device_ptr(lvr_ctx* ctx, void** ptr, int* width, int* height);
 
+/* Statistics of the last frame. Kernel times come from HIP events recorded
+ * on the context's stream around each stage while timing is enabled (the
+ * reference has no timer on this path; it only counts cache statistics,
+ * CacheStatistics.cpp:57-73). lvr_frame_stats_get synchronises. */
+typedef struct
+{
+    uint64_t samples;      /* executed composite steps */
+    uint64_t bricks;       /* resident bricks that reached the screen */
+    uint64_t tile_entries; /* (tile, brick) pairs binned */
+    uint64_t brick_bytes;  /* maximumBlockSize^3 * bytesPerVoxel per brick */
+    float bin_ms;          /* binning kernels */
+    float raycast_ms;      /* ray-cast kernel */
+    float histogram_ms;    /* histogram kernels of the last lvr_histogram */
+} lvr_frame_stats;
+int lvr_set_timing(lvr_ctx* ctx, int enable);
+int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out);
+
 /* ------------------------------------------------------------- histogram */
 /* Per-frame histogram of the given bricks (HistogramObject.cpp:140-224 +
  * HistogramFilter.cpp:78-149): counts of interior voxels scaled by

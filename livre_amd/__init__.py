@@ -30,7 +30,15 @@ EXPORTS = (
     "lvr_noise_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
+    "lvr_set_timing", "lvr_frame_stats_get",
 )
+
+
+class FrameStats(ctypes.Structure):
+    _fields_ = [("samples", ctypes.c_uint64), ("bricks", ctypes.c_uint64),
+                ("tile_entries", ctypes.c_uint64), ("brick_bytes", ctypes.c_uint64),
+                ("bin_ms", ctypes.c_float), ("raycast_ms", ctypes.c_float),
+                ("histogram_ms", ctypes.c_float)]
 
 
 class LivreError(RuntimeError):
@@ -135,6 +143,8 @@ def lib():
                                        P(c.c_int)]
     L.lvr_histogram.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_histogram_read.argtypes = [c.c_void_p, u64p, c.c_int]
+    L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
     _lib = L
     return L
 
@@ -344,6 +354,14 @@ class Context:
         s = ctypes.c_uint64()
         self._check(lib().lvr_frame_read(self._h, None, 0, ctypes.byref(s)), "samples")
         return s.value
+
+    def set_timing(self, enable=True):
+        self._check(lib().lvr_set_timing(self._h, 1 if enable else 0), "set_timing")
+
+    def frame_stats(self):
+        s = FrameStats()
+        self._check(lib().lvr_frame_stats_get(self._h, ctypes.byref(s)), "frame_stats")
+        return s
 
     def histogram(self, ids, bins, read=True):
         ids, pids = _u64(ids)

@@ -152,6 +152,8 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         ctx->histBins = bins;
     }
     ctx->histLast = bins;
+    if (ctx->timing)
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
     hipLaunchKernelGGL(clearKernel, dim3((bins + 255) / 256), dim3(256), 0,
                        ctx->stream, ctx->d_hist, bins);
     if (!jobs.empty())
@@ -185,6 +187,12 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         }
         const float perBin = (r1 - r0) / float(bins - 1);
         const HistJob* dj = reinterpret_cast<const HistJob*>(ctx->d_histScale);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&histKernel<uint8_t>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&histKernel<uint16_t>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&histKernel<float>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
         if (dt == LVR_DT_UINT8)
             hipLaunchKernelGGL(histKernel<uint8_t>, dim3(jobs.size()), dim3(1024), lds,
                                ctx->stream, dj, ctx->d_histSlots, ctx->d_pool,
@@ -204,6 +212,11 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
                                ctx->slotStride, M, ov, binsPerJob, bins, r0, perBin,
                                ctx->d_hist);
         }
+    }
+    if (ctx->timing)
+    {
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        ctx->histTimed = true;
     }
     LVR_HIP(ctx, hipGetLastError());
     return LVR_OK;

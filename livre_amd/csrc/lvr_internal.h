@@ -164,6 +164,12 @@ struct lvr_ctx
     int32_t* d_selCount = nullptr; // [0] next frontier, [1] visible
     size_t selCapacity = 0;
 
+    // frame statistics and stage timing (events on `stream`)
+    bool timing = false;
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool histTimed = false;
+    uint64_t lastBricks = 0, lastEntries = 0;
+
     std::string error;
 };
 

@@ -271,6 +271,9 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipHostFree(ctx->h_counters);
     if (ctx->uploadDone)
         (void)hipEventDestroy(ctx->uploadDone);
+    for (auto& e : ctx->ev)
+        if (e)
+            (void)hipEventDestroy(e);
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     if (ctx->uploadStream)
@@ -411,8 +414,9 @@ int lvr_brick_download(lvr_ctx* ctx, uint64_t node_id, void* out, size_t bytes)
 static int tfUpload(lvr_ctx* ctx, const std::vector<float>& rgba, int n,
                     float range0, float range1)
 {
-    if (n > 16384)
-        return lvr::fail(ctx, LVR_EINVAL, "transfer function above 16384 entries");
+    // the render kernel stages the whole table in LDS as float4 (128 KiB max)
+    if (n > 8192)
+        return lvr::fail(ctx, LVR_EINVAL, "transfer function above 8192 entries");
     int rc = lvr::ensureDevice(ctx);
     if (rc)
         return rc;

@@ -357,7 +357,11 @@ __global__ __launch_bounds__(256) void raycastKernel(
         float pos[3] = {rs[0], rs[1], rs[2]};
         const T* vox = reinterpret_cast<const T*>(pool + size_t(b.slot) * U.slotStride);
 
-        for (float travel = segLen; travel > 0.0f; travel -= stepSize)
+        // A segment lies inside one brick, so it holds at most
+        // diagonal * nSamples steps; the guard only bounds the loop should
+        // `travel - stepSize` ever round back to `travel`.
+        int guard = 1 << 22;
+        for (float travel = segLen; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
         {
             float tp[3];
 #pragma unroll
@@ -423,6 +427,8 @@ void launchRaycast(int blocks, int threads, size_t lds, hipStream_t st,
                    const int32_t* lists, const uint8_t* pool, const float4* tf,
                    uint32_t* frame, unsigned long long* samples)
 {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     hipLaunchKernelGGL(raycastKernel<T>, dim3(blocks), dim3(threads), lds, st, U,
                        bricks, offsets, counts, lists, pool, tf, frame, samples);
 }
@@ -589,7 +595,7 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                 s += double(f->proj[k * 4 + i]) * double(f->mv[j * 4 + k]);
             mvp[j * 4 + i] = s;
         }
-    uint64_t listTotal = 0;
+    uint64_t listTotal = 0, onScreen = 0;
     for (int b = 0; b < n; ++b)
     {
         const NodeKey key = decodeNode(ordered_ids[b]);
@@ -655,6 +661,7 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         r[2] = x1 / tw;
         r[3] = y1 / th;
         listTotal += uint64_t(r[2] - r[0] + 1) * uint64_t(r[3] - r[1] + 1);
+        ++onScreen;
     }
     if (listTotal > (1ull << 31))
         return fail(ctx, LVR_ENOMEM, "tile lists exceed 2^31 entries");
@@ -697,6 +704,10 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                                 sizeof(unsigned long long), hipMemcpyHostToDevice,
                                 ctx->stream));
     const int binBlocks = (nTiles + 3) / 4;
+    ctx->lastBricks = onScreen;
+    ctx->lastEntries = listTotal;
+    if (ctx->timing)
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
     hipLaunchKernelGGL(binCountKernel, dim3(binBlocks), dim3(256), 0, ctx->stream,
                        reinterpret_cast<const int4*>(ctx->d_tileRange), n,
                        U.tilesX, nTiles, ctx->d_tileCount);
@@ -705,6 +716,8 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     hipLaunchKernelGGL(binFillKernel, dim3(binBlocks), dim3(256), 0, ctx->stream,
                        reinterpret_cast<const int4*>(ctx->d_tileRange), n,
                        U.tilesX, nTiles, ctx->d_tileOffset, ctx->d_tileList);
+    if (ctx->timing)
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     const int threads = 256;
     const int blocks = (nTiles + threads / 64 - 1) / (threads / 64);
     const size_t lds = size_t(U.nTf) * sizeof(float4);
@@ -747,6 +760,8 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                              ctx->d_pool, tf, ctx->d_frame, ctx->d_counters);
         break;
     }
+    if (ctx->timing)
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
     LVR_HIP(ctx, hipGetLastError());
     return LVR_OK;
 }
@@ -773,6 +788,47 @@ int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes,
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (samples)
         *samples = ctx->h_counters[0];
+    return LVR_OK;
+}
+
+int lvr_set_timing(lvr_ctx* ctx, int enable)
+{
+    if (!ctx)
+        return LVR_EINVAL;
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    if (enable && !ctx->ev[0])
+        for (auto& e : ctx->ev)
+            LVR_HIP(ctx, hipEventCreate(&e));
+    ctx->timing = enable != 0;
+    return LVR_OK;
+}
+
+int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out)
+{
+    if (!ctx || !out)
+        return LVR_EINVAL;
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    std::memset(out, 0, sizeof *out);
+    LVR_HIP(ctx, hipMemcpyAsync(ctx->h_counters, ctx->d_counters,
+                                sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    out->samples = ctx->h_counters[0];
+    out->bricks = ctx->lastBricks;
+    out->tile_entries = ctx->lastEntries;
+    out->brick_bytes = ctx->brickBytes;
+    if (ctx->timing)
+    {
+        LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, ctx->ev[0], ctx->ev[1]));
+        LVR_HIP(ctx, hipEventElapsedTime(&out->raycast_ms, ctx->ev[1], ctx->ev[2]));
+        if (ctx->histTimed)
+            LVR_HIP(ctx, hipEventElapsedTime(&out->histogram_ms, ctx->ev[3],
+                                             ctx->ev[4]));
+    }
     return LVR_OK;
 }
 

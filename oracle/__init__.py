@@ -48,7 +48,8 @@ class RenderParams(ctypes.Structure):
                 ("range1", ctypes.c_float),
                 ("linear_filter", ctypes.c_int32),
                 ("n_clip_planes", ctypes.c_int32),
-                ("clip_planes", ctypes.c_float * 24)]
+                ("clip_planes", ctypes.c_float * 24),
+                ("window", ctypes.c_int32 * 4)]
 
 
 class FrustumInfo(ctypes.Structure):
@@ -173,8 +174,11 @@ def camera(inv_proj, inv_mv, eye, near, far, width, height):
 
 
 def render_params(n_samples_per_ray, early_exit=0.999, data_range=(0.0, 255.0),
-                  linear=False, max_samples_per_ray=32, clip_planes=()):
+                  linear=False, max_samples_per_ray=32, clip_planes=(), window=None):
     p = RenderParams()
+    if window is not None:
+        for i in range(4):
+            p.window[i] = window[i]
     p.n_samples_per_ray = n_samples_per_ray
     p.max_samples_per_ray = max_samples_per_ray
     p.early_exit = early_exit

@@ -698,6 +698,14 @@ int render(const orc_volume_info* vi, const orc_camera* cam,
     }
     const int TS = 16;
     const int tx = (W + TS - 1) / TS, ty = (H + TS - 1) / TS;
+    int wx0 = 0, wy0 = 0, wx1 = W, wy1 = H;
+    if (p->window[2] > p->window[0] && p->window[3] > p->window[1])
+    {
+        wx0 = std::max(0, p->window[0]);
+        wy0 = std::max(0, p->window[1]);
+        wx1 = std::min(W, p->window[2]);
+        wy1 = std::min(H, p->window[3]);
+    }
     uint64_t total = 0;
 #ifdef _OPENMP
     if (n_threads <= 0)
@@ -708,6 +716,8 @@ int render(const orc_volume_info* vi, const orc_camera* cam,
     {
         const int x0 = (t % tx) * TS, y0 = (t / tx) * TS;
         const int x1 = std::min(W, x0 + TS) - 1, y1 = std::min(H, y0 + TS) - 1;
+        if (x1 < wx0 || x0 >= wx1 || y1 < wy0 || y0 >= wy1)
+            continue;
         std::vector<int> list;
         for (int i = 0; i < n; ++i)
         {
@@ -720,6 +730,8 @@ int render(const orc_volume_info* vi, const orc_camera* cam,
         for (int y = y0; y <= y1; ++y)
             for (int x = x0; x <= x1; ++x)
             {
+                if (x < wx0 || x >= wx1 || y < wy0 || y >= wy1)
+                    continue;
                 T rgba[4];
                 shade_pixel<T>(*vi, *cam, bricks, list, tf, ntf, *p, x, y, rgba,
                                s);

@@ -65,6 +65,7 @@ typedef struct
     int32_t linear_filter;       /* 0: GL_NEAREST (reference default), 1: trilinear */
     int32_t n_clip_planes;
     float clip_planes[24];
+    int32_t window[4]; /* x0, y0, x1, y1 (exclusive); all 0 = whole frame */
 } orc_render_params;
 
 int orc_fill_volume_info(orc_volume_info* vi);

@@ -49,7 +49,8 @@ class Scene:
 
     def context(self, pool_bytes=None):
         per = int(np.prod(self.vi.brick_shape)) * np.dtype(self.vi.dtype).itemsize
-        ctx = livre_amd.Context(self.vi, pool_bytes or per * (len(self.ids) + 8))
+        slot = (per + 255) // 256 * 256
+        ctx = livre_amd.Context(self.vi, pool_bytes or slot * len(self.ids))
         ctx.upload(self.ids, np.stack([self.bricks[int(i)] for i in self.ids]))
         return ctx
 

@@ -73,15 +73,16 @@ __host__ __device__ inline void nodeWorldBox(const lvr_volume_info& vi,
     }
 }
 
-// Per-brick record of one frame, in the order of the ordered list. 48 bytes.
+// Per-brick record of one frame, in the order of the ordered list. 64 bytes.
 struct BrickEntry
 {
     float bmin[3];
     float bmax[3];
     float ext[3];   // bmax - bmin, rounded as the shader's (aabbMax - aabbMin)
+    float rcp[3];   // 1 / ext, correctly rounded (host division)
     int32_t slot;   // pool slot, -1 if not resident
     uint32_t level;
-    uint32_t pad;
+    uint32_t pad[2];
 };
 
 // Dense device page table over the octree: per level,

@@ -229,8 +229,13 @@ __device__ inline int xcdRemap(int bid, int nBlocks)
     return base + idx;
 }
 
+// A workgroup is kSuper x kSuper wavefront tiles (16 wavefronts, 1024 rays)
+// sharing one LDS copy of the transfer function: a 4096-entry FP32 table is
+// 64 KiB, so two workgroups per CU still give 8 wavefronts per SIMD.
+constexpr int kSuper = 4;
+
 template <typename T>
-__global__ __launch_bounds__(256) void raycastKernel(
+__global__ __launch_bounds__(1024) void raycastKernel(
     const FrameUniforms U, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
     const int32_t* __restrict__ lists, const uint8_t* __restrict__ pool,
@@ -242,16 +247,17 @@ __global__ __launch_bounds__(256) void raycastKernel(
         sTf[i] = tf[i];
     __syncthreads();
 
-    const int wavesPerBlock = blockDim.x / kWave;
     const int block = xcdRemap(int(blockIdx.x), int(gridDim.x));
-    const int tile = __builtin_amdgcn_readfirstlane(
-        block * wavesPerBlock + int(threadIdx.x) / kWave);
-    const int nTiles = U.tilesX * U.tilesY;
-    if (tile >= nTiles)
+    const int superX = (U.tilesX + kSuper - 1) / kSuper;
+    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
+    const int tx = __builtin_amdgcn_readfirstlane((block % superX) * kSuper + wave % kSuper);
+    const int ty = __builtin_amdgcn_readfirstlane((block / superX) * kSuper + wave / kSuper);
+    if (tx >= U.tilesX || ty >= U.tilesY)
         return;
+    const int tile = __builtin_amdgcn_readfirstlane(ty * U.tilesX + tx);
     const int lane = threadIdx.x & (kWave - 1);
-    const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
-    const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
+    const int px = tx * U.tileW + lane % U.tileW;
+    const int py = ty * U.tileH + lane / U.tileW;
     const bool inside = px < U.width && py < U.height;
 
     // pixel ray (fragRayCast.glsl:121-140)
@@ -366,7 +372,16 @@ __global__ __launch_bounds__(256) void raycastKernel(
             float tp[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a)
-                tp[a] = (pos[a] - b.bmin[a]) / b.ext[a] * texExt[a] + U.texMin[a];
+            {
+                // (pos - aabbMin) / (aabbMax - aabbMin), correctly rounded:
+                // reciprocal of the divisor (host, IEEE) and one FMA
+                // correction step (Markstein) give the IEEE quotient.
+                const float x = pos[a] - b.bmin[a];
+                const float q = x * b.rcp[a];
+                const float r = __builtin_fmaf(-q, b.ext[a], x);
+                const float quot = __builtin_fmaf(r, b.rcp[a], q);
+                tp[a] = quot * texExt[a] + U.texMin[a];
+            }
             const float value = sampleBrick<T>(vox, U, tp);
             const float density = value * U.mult + U.add;
             // texture(transferFnTex, density): linear, clamp to edge
@@ -384,7 +399,11 @@ __global__ __launch_bounds__(256) void raycastKernel(
             const float sb = ca.z * (1.0f - w) + cb.z * w;
             const float sa = ca.w * (1.0f - w) + cb.w * w;
             // composite (fragRayCast.glsl:103-111)
-            const float alpha = 1.0f - powf(1.0f - fminf(sa, maxA), alphaCorrection);
+            // pow(x, c) for x in [1/256, 1] as exp2(c * log2 x) on the
+            // hardware transcendental units (1 ulp each)
+            const float base = 1.0f - fminf(sa, maxA);
+            const float alpha =
+                1.0f - __builtin_amdgcn_exp2f(alphaCorrection * __builtin_amdgcn_logf(base));
             const float rem = 1.0f - acc3;
             acc0 += sr * alpha * rem;
             acc1 += sg * alpha * rem;
@@ -610,7 +629,10 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
             return fail(ctx, LVR_EINVAL, "node id outside the octree");
         nodeWorldBox(vi, key, e.bmin, e.bmax);
         for (int a = 0; a < 3; ++a)
+        {
             e.ext[a] = e.bmax[a] - e.bmin[a];
+            e.rcp[a] = 1.0f / e.ext[a];
+        }
         e.level = key.level;
         e.slot = slotOf(ctx, ordered_ids[b]);
         if (e.slot < 0)
@@ -718,8 +740,8 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                        U.tilesX, nTiles, ctx->d_tileOffset, ctx->d_tileList);
     if (ctx->timing)
         LVR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    const int threads = 256;
-    const int blocks = (nTiles + threads / 64 - 1) / (threads / 64);
+    const int threads = kSuper * kSuper * 64;
+    const int blocks = ((U.tilesX + kSuper - 1) / kSuper) * ((U.tilesY + kSuper - 1) / kSuper);
     const size_t lds = size_t(U.nTf) * sizeof(float4);
     const float4* tf = reinterpret_cast<const float4*>(ctx->d_tf);
     switch (vi.data_type)

@@ -155,8 +155,17 @@ struct lvr_ctx
     int histBins = 0; // allocated bins
     int histLast = 0; // bins of the last histogram
     int32_t* d_histSlots = nullptr;
-    uint32_t* d_histScale = nullptr;
-    int histCapacity = 0;
+    uint32_t* d_histScale = nullptr; // job records
+    int histCapacity = 0;            // slot entries
+    int histJobCapacity = 0;         // 32-bit words of job records
+    // running histogram of the last set (d_hist holds its sum)
+    std::vector<uint64_t> histSet;   // sorted resident ids
+    bool histValid = false;
+    uint64_t histGeneration = 0;
+    float histRange0 = 0.f, histRange1 = 0.f;
+    uint64_t histDeltaBricks = 0;
+    // bumped whenever brick contents change (upload, generation, eviction)
+    uint64_t poolGeneration = 0;
 
     // selection
     uint64_t* d_selA = nullptr;

@@ -113,6 +113,7 @@ static int acquireSlot(lvr_ctx* ctx, uint64_t id, int32_t* slot)
     uint64_t idx;
     if (!pageIndex(ctx, id, &idx))
         return fail(ctx, LVR_EINVAL, "node id outside the octree");
+    ++ctx->poolGeneration; // the slot's contents are about to change
     auto it = ctx->resident.find(id);
     if (it != ctx->resident.end())
     {
@@ -377,6 +378,7 @@ int lvr_brick_evict(lvr_ctx* ctx, uint64_t node_id)
         return rc;
     // kernels already queued may still read the slot
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ++ctx->poolGeneration;
     ctx->freeSlots.push_back(it->second.slot);
     ctx->lruList.erase(it->second.lru);
     ctx->resident.erase(it);

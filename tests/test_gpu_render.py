@@ -182,6 +182,27 @@ def test_histogram_bit_exact(lvr, orc, dt, bins):
         ctx.close()
 
 
+def test_histogram_incremental_sequence(lvr, orc):
+    """Consecutive frames update the running histogram by the bricks that
+    entered and left the set; every frame equals a full recount."""
+    s = Scene(livre_amd.DT_UINT16, n=64, block=16, overlap=1, source="noise", seed=31)
+    ctx = s.context()
+    try:
+        for k in range(8):
+            if k == 5:  # new voxels for one brick: the running sum is rebuilt
+                nid = int(s.ids[-1])
+                s.bricks[nid] = (s.bricks[nid] // 2).astype(np.uint16)
+                ctx.upload([nid], s.bricks[nid][None])
+            eye = scenes.orbit(k, 16, radius=1.6 + 0.1 * k, height=0.2)
+            mv, proj, fr = s.camera(eye, 96, 96)
+            ids = s.visible_ordered(mv, proj, 96, sse=0.7)
+            got = ctx.histogram(ids, 65536)
+            ref = oracle.histogram(s.ovi, ids, [s.bricks[int(i)] for i in ids], 65536)
+            assert np.array_equal(got, ref), k
+    finally:
+        ctx.close()
+
+
 def test_selection_bit_exact_against_goldens(lvr, orc):
     import json
     import os

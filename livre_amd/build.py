@@ -50,7 +50,26 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cpp_driver()
     return LIB
+
+
+CPP_DRIVER = os.path.join(HERE, "render_frame")
+
+
+def build_cpp_driver():
+    """The C++ drop-in classes (livre_amd/cpp/livre/hip/Renderer.h) compiled
+    by the host compiler against the C ABI, as a Livre build would."""
+    root = os.path.dirname(HERE)
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra",
+           "-I", os.path.join(root, "include"), "-I", os.path.join(HERE, "cpp"),
+           os.path.join(HERE, "cpp", "render_frame.cpp"), "-o", CPP_DRIVER + ".tmp",
+           "-L", HERE, "-llivre_amd", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed on render_frame.cpp:\n" + res.stderr)
+    os.replace(CPP_DRIVER + ".tmp", CPP_DRIVER)
+    return CPP_DRIVER
 
 
 if __name__ == "__main__":

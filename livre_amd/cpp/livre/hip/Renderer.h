@@ -1,0 +1,411 @@
+/*
+ * C++ face of the MI355X render path, mirroring the reference's interfaces:
+ *   livre::NodeId            livre/data/NodeId.h:31-160
+ *   livre::VolumeInformation livre/data/VolumeInformation.h:42-111
+ *   livre::Frustum           livre/data/Frustum.h:37-111
+ *   livre::Renderer          livre/core/render/Renderer.h:38-109
+ *   livre::DataSourcePlugin  livre/data/DataSourcePlugin.h:55-109 (getData)
+ *   livre::RayCastRenderer   livre/eq/render/RayCastRenderer.h:31-74
+ *   RenderPipeline::renderSync livre/lib/pipeline/RenderPipeline.cpp:92-158
+ * Header-only over the C ABI (include/livre_amd.h); link with liblivre_amd.so.
+ * vmmlib types are replaced by plain arrays (column-major float[16]).
+ * Errors throw std::runtime_error, as the reference's constructors do
+ * (RayCastRenderer.cpp:117-138).
+ */
+#pragma once
+
+#include <livre_amd.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace livre
+{
+typedef uint64_t Identifier;
+
+/** 64-bit octree node key (level 4 bits, x/y/z 14 bits, time step 18). */
+class NodeId
+{
+public:
+    NodeId() : _id(~Identifier(0)) {}
+    explicit NodeId(Identifier id) : _id(id) {}
+    NodeId(uint32_t level, const std::array<uint32_t, 3>& pos, uint32_t timeStep = 0)
+        : _id(Identifier(level & 0xFu) | (Identifier(pos[0] & 0x3FFFu) << 4) |
+              (Identifier(pos[1] & 0x3FFFu) << 18) | (Identifier(pos[2] & 0x3FFFu) << 32) |
+              (Identifier(timeStep & 0x3FFFFu) << 46))
+    {
+    }
+    uint32_t getLevel() const { return uint32_t(_id & 0xFu); }
+    uint32_t getTimeStep() const { return uint32_t(_id >> 46); }
+    std::array<uint32_t, 3> getPosition() const
+    {
+        return {uint32_t((_id >> 4) & 0x3FFFu), uint32_t((_id >> 18) & 0x3FFFu),
+                uint32_t((_id >> 32) & 0x3FFFu)};
+    }
+    bool isValid() const { return getLevel() != 0xFu; }
+    bool isRoot() const { return getLevel() == 0; }
+    NodeId getParent() const
+    {
+        if (!isValid() || isRoot())
+            return NodeId();
+        const auto p = getPosition();
+        return NodeId(getLevel() - 1, {p[0] / 2, p[1] / 2, p[2] / 2}, getTimeStep());
+    }
+    /** Children in the reference order: x outer, y, z inner (NodeId.cpp:92-113). */
+    std::vector<NodeId> getChildren() const
+    {
+        std::vector<NodeId> out;
+        if (!isValid())
+            return out;
+        const auto p = getPosition();
+        for (uint32_t x = 0; x < 2; ++x)
+            for (uint32_t y = 0; y < 2; ++y)
+                for (uint32_t z = 0; z < 2; ++z)
+                    out.emplace_back(getLevel() + 1,
+                                     std::array<uint32_t, 3>{2 * p[0] + x, 2 * p[1] + y,
+                                                             2 * p[2] + z},
+                                     getTimeStep());
+        return out;
+    }
+    Identifier getId() const { return _id; }
+    bool operator==(const NodeId& o) const { return _id == o._id; }
+    bool operator!=(const NodeId& o) const { return _id != o._id; }
+    bool operator<(const NodeId& o) const { return _id < o._id; }
+
+private:
+    Identifier _id;
+};
+typedef std::vector<NodeId> NodeIds;
+
+enum DataType
+{
+    DT_FLOAT = LVR_DT_FLOAT,
+    DT_UINT8 = LVR_DT_UINT8,
+    DT_UINT16 = LVR_DT_UINT16,
+    DT_UINT32 = LVR_DT_UINT32,
+    DT_INT8 = LVR_DT_INT8,
+    DT_INT16 = LVR_DT_INT16,
+    DT_INT32 = LVR_DT_INT32
+};
+
+inline void check(int rc, const char* what, const lvr_ctx* ctx = nullptr)
+{
+    if (rc != LVR_OK)
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) +
+                                 ")" + (ctx ? std::string(": ") + lvr_last_error(ctx) : ""));
+}
+
+/** VolumeInformation with fillRegularVolumeInfo applied. */
+struct VolumeInformation
+{
+    lvr_volume_info info{};
+    VolumeInformation() = default;
+    VolumeInformation(const std::array<uint32_t, 3>& voxels, uint32_t block, uint32_t overlap,
+                      DataType type)
+    {
+        for (int i = 0; i < 3; ++i)
+        {
+            info.voxels[i] = voxels[i];
+            info.max_block[i] = block + 2 * overlap;
+            info.overlap[i] = overlap;
+        }
+        info.data_type = type;
+        check(lvr_fill_regular_volume_info(&info), "fillRegularVolumeInfo");
+    }
+    uint32_t depth() const { return info.depth; }
+    size_t brickBytes() const
+    {
+        static const size_t bpv[] = {4, 1, 2, 4, 1, 2, 4};
+        return size_t(info.max_block[0]) * info.max_block[1] * info.max_block[2] *
+               bpv[info.data_type];
+    }
+};
+
+struct PixelViewport
+{
+    int32_t x = 0, y = 0, w = 0, h = 0;
+};
+
+struct ClipPlanes
+{
+    std::vector<std::array<float, 4>> planes; // normal xyz, d
+};
+
+/** Frustum from modelview and projection (column-major). */
+class Frustum
+{
+public:
+    Frustum(const float* mv, const float* proj) { check(lvr_frustum_init(&_f, mv, proj), "Frustum"); }
+    const lvr_frustum& raw() const { return _f; }
+    const float* getMVMatrix() const { return _f.mv; }
+    const float* getProjMatrix() const { return _f.proj; }
+    const float* getInvMVMatrix() const { return _f.inv_mv; }
+    const float* getInvProjMatrix() const { return _f.inv_proj; }
+    std::array<float, 3> getEyePos() const { return {_f.eye[0], _f.eye[1], _f.eye[2]}; }
+    std::array<float, 3> getViewDir() const
+    {
+        return {_f.view_dir[0], _f.view_dir[1], _f.view_dir[2]};
+    }
+    float nearPlane() const { return _f.near_plane; }
+    float farPlane() const { return _f.far_plane; }
+    float top() const { return _f.top; }
+    float bottom() const { return _f.bottom; }
+
+private:
+    lvr_frustum _f;
+};
+
+/** DataSourcePlugin::getData counterpart: bricks of maximumBlockSize^3 voxels. */
+class DataSourcePlugin
+{
+public:
+    virtual ~DataSourcePlugin() {}
+    const VolumeInformation& getVolumeInfo() const { return _volumeInfo; }
+    /** Fill `out` (getVolumeInfo().brickBytes() bytes); false if unavailable. */
+    virtual bool getData(const NodeId& node, void* out) = 0;
+
+protected:
+    VolumeInformation _volumeInfo;
+};
+
+/** The HBM brick pool of one volume: TexturePool + the texture cache. */
+class BrickPool
+{
+public:
+    BrickPool(const VolumeInformation& vi, size_t bytes, int device = 0) : _vi(vi)
+    {
+        check(lvr_ctx_create(&vi.info, bytes, device, &_ctx), "BrickPool");
+    }
+    ~BrickPool() { lvr_ctx_destroy(_ctx); }
+    BrickPool(const BrickPool&) = delete;
+    BrickPool& operator=(const BrickPool&) = delete;
+    lvr_ctx* ctx() const { return _ctx; }
+    const VolumeInformation& volumeInfo() const { return _vi; }
+    bool isResident(const NodeId& n) const { return lvr_brick_resident(_ctx, n.getId()) == 1; }
+    void upload(const NodeId& n, const void* voxels)
+    {
+        check(lvr_brick_upload(_ctx, n.getId(), voxels, _vi.brickBytes()), "upload", _ctx);
+    }
+    void evict(const NodeId& n) { check(lvr_brick_evict(_ctx, n.getId()), "evict", _ctx); }
+    uint64_t capacity() const
+    {
+        uint64_t s = 0;
+        lvr_pool_stats(_ctx, &s, nullptr, nullptr);
+        return s;
+    }
+
+private:
+    VolumeInformation _vi;
+    lvr_ctx* _ctx = nullptr;
+};
+
+static const uint32_t RENDER_BEGIN = 1u;
+static const uint32_t RENDER_FRAME = 1u << 1;
+static const uint32_t RENDER_END = 1u << 2;
+static const uint32_t RENDER_ALL = RENDER_BEGIN | RENDER_FRAME | RENDER_END;
+
+/** livre::Renderer: template method order -> begin -> render -> end. */
+class Renderer
+{
+public:
+    void render(const Frustum& frustum, const ClipPlanes& planes, const PixelViewport& view,
+                const NodeIds& bricks, uint32_t renderStages = RENDER_ALL)
+    {
+        const NodeIds ordered = order(bricks, frustum);
+        if (renderStages & RENDER_BEGIN)
+            _onFrameStart(frustum, planes, view, ordered);
+        if (renderStages & RENDER_FRAME)
+            _onFrameRender(frustum, planes, view, ordered);
+        if (renderStages & RENDER_END)
+            _onFrameEnd(frustum, planes, view, ordered);
+    }
+    virtual NodeIds order(const NodeIds& bricks, const Frustum& frustum) const = 0;
+    virtual ~Renderer() {}
+
+protected:
+    virtual void _onFrameStart(const Frustum&, const ClipPlanes&, const PixelViewport&,
+                               const NodeIds&)
+    {
+    }
+    virtual void _onFrameRender(const Frustum&, const ClipPlanes&, const PixelViewport&,
+                                const NodeIds&)
+    {
+    }
+    virtual void _onFrameEnd(const Frustum&, const ClipPlanes&, const PixelViewport&,
+                             const NodeIds&)
+    {
+    }
+};
+
+/** Drop-in for RayCastRenderer: same constructor meaning (samples per ray,
+ * 0 = derived from the data as onFrameStart does), same order(), the frame
+ * rendered by the HIP kernels into an RGBA8 framebuffer on the device. */
+class HipRayCastRenderer : public Renderer
+{
+public:
+    HipRayCastRenderer(BrickPool& pool, uint32_t samplesPerRay)
+        : _pool(pool), _samplesPerRay(samplesPerRay)
+    {
+    }
+
+    /** RayCastRenderer::update: TF (RGBA float, n entries, data range),
+     * samples per ray, linear filtering. */
+    void update(const std::vector<float>& rgba, std::array<float, 2> range,
+                uint32_t samplesPerRay, bool linearFiltering)
+    {
+        check(lvr_tf_set(_pool.ctx(), rgba.data(), int(rgba.size() / 4), range[0], range[1]),
+              "tf", _pool.ctx());
+        _samplesPerRay = samplesPerRay;
+        _linear = linearFiltering;
+    }
+    void updateLUT(const std::vector<uint8_t>& lut, std::array<float, 2> range)
+    {
+        check(lvr_tf_set_rgba8(_pool.ctx(), lut.data(), int(lut.size() / 4), range[0],
+                               range[1]),
+              "tf", _pool.ctx());
+    }
+    void setEarlyExit(float a) { _earlyExit = a; }
+
+    NodeIds order(const NodeIds& bricks, const Frustum& frustum) const final
+    {
+        std::vector<uint64_t> in(bricks.size()), out(bricks.size());
+        for (size_t i = 0; i < bricks.size(); ++i)
+            in[i] = bricks[i].getId();
+        check(lvr_order(&_pool.volumeInfo().info, &frustum.raw(), in.data(), int(in.size()),
+                        out.data()),
+              "order");
+        NodeIds r;
+        r.reserve(out.size());
+        for (uint64_t id : out)
+            r.emplace_back(id);
+        return r;
+    }
+
+    /** Bricks rendered by the last frame (RayCastRenderer::getVisibleNodes). */
+    const NodeIds& getVisibleNodes() const { return _visible; }
+    /** The frame of the last render, RGBA8, row 0 = bottom. */
+    std::vector<uint8_t> readFrame(uint64_t* samples = nullptr) const
+    {
+        std::vector<uint8_t> img(size_t(_vp.w) * _vp.h * 4);
+        check(lvr_frame_read(_pool.ctx(), img.data(), img.size(), samples), "frame", _pool.ctx());
+        return img;
+    }
+
+protected:
+    void _onFrameStart(const Frustum&, const ClipPlanes& planes, const PixelViewport& view,
+                       const NodeIds& bricks) override
+    {
+        _vp = view;
+        _computed = _samplesPerRay;
+        if (_samplesPerRay == 0) // RayCastRenderer.cpp:216-234
+        {
+            uint32_t maxLOD = 0;
+            for (const NodeId& n : bricks)
+                maxLOD = std::max(maxLOD, n.getLevel());
+            const lvr_volume_info& vi = _pool.volumeInfo().info;
+            const float maxVoxelDim = float(std::max(vi.voxels[0], std::max(vi.voxels[1], vi.voxels[2])));
+            const float atLOD = maxVoxelDim / float(1u << (vi.depth - maxLOD - 1));
+            _computed = uint32_t(std::max(atLOD, 512.0f));
+        }
+        std::memset(&_params, 0, sizeof _params);
+        _params.n_samples_per_ray = int32_t(_computed);
+        _params.max_samples_per_ray = 32;
+        _params.early_exit = _earlyExit;
+        _params.linear_filter = _linear ? 1 : 0;
+        _params.n_clip_planes = int32_t(std::min<size_t>(6, planes.planes.size()));
+        for (int i = 0; i < _params.n_clip_planes; ++i)
+            for (int j = 0; j < 4; ++j)
+                _params.clip_planes[4 * i + j] = planes.planes[size_t(i)][size_t(j)];
+    }
+
+    void _onFrameRender(const Frustum& frustum, const ClipPlanes&, const PixelViewport& view,
+                        const NodeIds& ordered) final
+    {
+        std::vector<uint64_t> ids(ordered.size());
+        _visible.clear();
+        for (size_t i = 0; i < ordered.size(); ++i)
+        {
+            ids[i] = ordered[i].getId();
+            if (_pool.isResident(ordered[i]))
+                _visible.push_back(ordered[i]);
+        }
+        const int vp[4] = {view.x, view.y, view.w, view.h};
+        check(lvr_render(_pool.ctx(), &frustum.raw(), vp, ids.data(), int(ids.size()), &_params),
+              "render", _pool.ctx());
+    }
+
+    void _onFrameEnd(const Frustum&, const ClipPlanes&, const PixelViewport&,
+                     const NodeIds&) final
+    {
+        check(lvr_ctx_sync(_pool.ctx()), "sync", _pool.ctx());
+    }
+
+private:
+    BrickPool& _pool;
+    uint32_t _samplesPerRay;
+    uint32_t _computed = 0;
+    bool _linear = false;
+    float _earlyExit = 0.999f;
+    PixelViewport _vp;
+    lvr_render_params _params{};
+    NodeIds _visible;
+};
+
+/** RenderPipeline::renderSync for one pipe: visible set on the device, order,
+ * load every missing brick from the data source (DataUploadFilter, sync
+ * mode), render, then the histogram of the rendered set. */
+class RenderPipeline
+{
+public:
+    RenderPipeline(DataSourcePlugin& source, BrickPool& pool) : _source(source), _pool(pool) {}
+
+    struct Params
+    {
+        float sse = 4.0f; // volumeRendererParameters.fbs default
+        uint32_t minLod = 0, maxLod = 14;
+    };
+
+    void render(HipRayCastRenderer& renderer, const Frustum& frustum, const PixelViewport& view,
+                const Params& p, std::vector<uint64_t>* histogram = nullptr)
+    {
+        const lvr_volume_info& vi = _pool.volumeInfo().info;
+        std::vector<uint64_t> vis(1u << 20);
+        int n = 0;
+        check(lvr_select_visibles(_pool.ctx(), &frustum.raw(), uint32_t(view.h), p.sse, p.minLod,
+                                  std::min(p.maxLod, vi.depth - 1), vis.data(), int(vis.size()),
+                                  &n),
+              "select", _pool.ctx());
+        NodeIds nodes;
+        nodes.reserve(size_t(n));
+        std::vector<uint8_t> buf(_pool.volumeInfo().brickBytes());
+        for (int i = 0; i < n; ++i)
+        {
+            const NodeId node(vis[size_t(i)]);
+            if (!_pool.isResident(node) && _source.getData(node, buf.data()))
+                _pool.upload(node, buf.data());
+            nodes.push_back(node);
+        }
+        renderer.render(frustum, ClipPlanes(), view, nodes);
+        if (histogram)
+        {
+            const int bins = vi.data_type == LVR_DT_UINT16 ? 65536 : 256;
+            histogram->assign(size_t(bins), 0);
+            std::vector<uint64_t> ids(vis.begin(), vis.begin() + n);
+            check(lvr_histogram(_pool.ctx(), ids.data(), n, bins), "histogram", _pool.ctx());
+            check(lvr_histogram_read(_pool.ctx(), histogram->data(), bins), "histogram",
+                  _pool.ctx());
+        }
+    }
+
+private:
+    DataSourcePlugin& _source;
+    BrickPool& _pool;
+};
+} // namespace livre

@@ -1,0 +1,173 @@
+/*
+ * render_frame: renders one frame of a procedural volume through the C++
+ * drop-in classes (livre::HipRayCastRenderer, livre::RenderPipeline), the
+ * way Livre's Channel drives RayCastRenderer, and writes the RGBA8 frame and
+ * the histogram to files.
+ *
+ *   render_frame OUT_PREFIX [n voxels] [block] [width] [height]
+ *   render_frame --host-check     (host-side API only, no GPU needed)
+ *
+ * OUT_PREFIX.rgba (w*h*4 bytes, row 0 = bottom), OUT_PREFIX.hist (u64 bins),
+ * OUT_PREFIX.txt (bricks, samples, camera matrices).
+ */
+#include <livre/hip/Renderer.h>
+
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+
+namespace
+{
+class NoiseSource : public livre::DataSourcePlugin
+{
+public:
+    NoiseSource(uint32_t n, uint32_t block, uint64_t seed)
+    {
+        _volumeInfo = livre::VolumeInformation({n, n, n}, block, 1, livre::DT_UINT16);
+        livre::check(lvr_noise_params_init(&_params, seed, 4, 4, 1.5f), "noise");
+    }
+    bool getData(const livre::NodeId& node, void* out) override
+    {
+        return lvr_noise_brick_host(&_volumeInfo.info, &_params, node.getId(), out,
+                                    _volumeInfo.brickBytes()) == LVR_OK;
+    }
+
+private:
+    lvr_noise_params _params;
+};
+
+void lookAt(const float eye[3], float* mv)
+{
+    float f[3] = {-eye[0], -eye[1], -eye[2]};
+    float l = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+    for (float& v : f)
+        v /= l;
+    const float up[3] = {0, 1, 0};
+    float s[3] = {f[1] * up[2] - f[2] * up[1], f[2] * up[0] - f[0] * up[2],
+                  f[0] * up[1] - f[1] * up[0]};
+    l = std::sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (float& v : s)
+        v /= l;
+    const float u[3] = {s[1] * f[2] - s[2] * f[1], s[2] * f[0] - s[0] * f[2],
+                        s[0] * f[1] - s[1] * f[0]};
+    const float rows[3][3] = {{s[0], s[1], s[2]}, {u[0], u[1], u[2]}, {-f[0], -f[1], -f[2]}};
+    std::memset(mv, 0, 16 * sizeof(float));
+    for (int r = 0; r < 3; ++r)
+    {
+        for (int c = 0; c < 3; ++c)
+            mv[c * 4 + r] = rows[r][c];
+        mv[12 + r] = -(rows[r][0] * eye[0] + rows[r][1] * eye[1] + rows[r][2] * eye[2]);
+    }
+    mv[15] = 1;
+}
+
+void perspective(float fovy, float aspect, float n, float f, float* p)
+{
+    const float t = 1.0f / std::tan(fovy * 3.14159265358979f / 360.0f);
+    std::memset(p, 0, 16 * sizeof(float));
+    p[0] = t / aspect;
+    p[5] = t;
+    p[10] = (f + n) / (n - f);
+    p[11] = -1;
+    p[14] = 2 * f * n / (n - f);
+}
+
+int hostCheck()
+{
+    const livre::NodeId root(0, {0, 0, 0});
+    const auto kids = root.getChildren();
+    if (kids.size() != 8 || kids[1].getId() != livre::NodeId(1, {0, 0, 1}).getId() ||
+        kids[4].getId() != livre::NodeId(1, {1, 0, 0}).getId() ||
+        kids[7].getParent() != root || livre::NodeId().isValid())
+        return 1;
+    if (livre::NodeId(1, {1, 0, 0}).getId() != 17u) // lodSelection.cpp:78
+        return 2;
+    const livre::VolumeInformation vi({2048, 2048, 2048}, 64, 0, livre::DT_UINT8);
+    if (vi.depth() != 6) // volumeInformation.cpp:63
+        return 3;
+    float mv[16], proj[16];
+    const float eye[3] = {0.4f, 0.3f, 1.9f};
+    lookAt(eye, mv);
+    perspective(60.f, 1.f, 0.9f, 3.0f, proj);
+    const livre::Frustum fr(mv, proj);
+    if (std::fabs(fr.getEyePos()[2] - 1.9f) > 1e-5f)
+        return 4;
+    std::cout << "host check ok" << std::endl;
+    return 0;
+}
+} // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc > 1 && std::string(argv[1]) == "--host-check")
+        return hostCheck();
+    if (argc < 2)
+    {
+        std::cerr << "usage: render_frame OUT_PREFIX [n] [block] [w] [h]" << std::endl;
+        return 2;
+    }
+    const std::string out = argv[1];
+    const uint32_t n = argc > 2 ? uint32_t(std::stoul(argv[2])) : 64;
+    const uint32_t block = argc > 3 ? uint32_t(std::stoul(argv[3])) : 16;
+    const int w = argc > 4 ? std::stoi(argv[4]) : 96;
+    const int h = argc > 5 ? std::stoi(argv[5]) : 72;
+    try
+    {
+        NoiseSource source(n, block, 7);
+        const livre::VolumeInformation& vi = source.getVolumeInfo();
+        livre::BrickPool pool(vi, (vi.brickBytes() + 256) * 4096);
+        livre::HipRayCastRenderer renderer(pool, n);
+        std::vector<float> tf(256 * 4);
+        for (int i = 0; i < 256; ++i)
+        {
+            const float x = float(i) / 255.0f;
+            tf[size_t(4 * i + 0)] = x;
+            tf[size_t(4 * i + 1)] = 1.0f - x;
+            tf[size_t(4 * i + 2)] = 0.5f;
+            tf[size_t(4 * i + 3)] = 0.4f * x * x;
+        }
+        renderer.update(tf, {0.0f, 65535.0f}, n, false);
+        float mv[16], proj[16];
+        const float eye[3] = {1.3f, 0.6f, 1.2f};
+        lookAt(eye, mv);
+        const float d = std::sqrt(eye[0] * eye[0] + eye[1] * eye[1] + eye[2] * eye[2]);
+        perspective(60.f, float(w) / float(h), std::max(1e-4f, d - 1.f), d + 1.f, proj);
+        const livre::Frustum frustum(mv, proj);
+        livre::RenderPipeline pipeline(source, pool);
+        livre::RenderPipeline::Params params;
+        params.sse = 1.0f;
+        livre::PixelViewport vp;
+        vp.w = w;
+        vp.h = h;
+        std::vector<uint64_t> hist;
+        pipeline.render(renderer, frustum, vp, params, &hist);
+        uint64_t samples = 0;
+        const std::vector<uint8_t> img = renderer.readFrame(&samples);
+        std::ofstream(out + ".rgba", std::ios::binary)
+            .write(reinterpret_cast<const char*>(img.data()), std::streamsize(img.size()));
+        std::ofstream(out + ".hist", std::ios::binary)
+            .write(reinterpret_cast<const char*>(hist.data()),
+                   std::streamsize(hist.size() * sizeof(uint64_t)));
+        std::ofstream txt(out + ".txt");
+        txt.precision(9);
+        txt << "bricks " << renderer.getVisibleNodes().size() << "\nsamples " << samples
+            << "\nwidth " << w << "\nheight " << h << "\nmv";
+        for (float v : mv)
+            txt << ' ' << v;
+        txt << "\nproj";
+        for (float v : proj)
+            txt << ' ' << v;
+        txt << "\nvisible";
+        for (const auto& node : renderer.getVisibleNodes())
+            txt << ' ' << node.getId();
+        txt << "\n";
+        std::cout << "rendered " << renderer.getVisibleNodes().size() << " bricks, " << samples
+                  << " samples" << std::endl;
+    }
+    catch (const std::exception& e)
+    {
+        std::cerr << "render_frame: " << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

@@ -108,6 +108,7 @@ struct PoolEntry
 struct lvr_ctx
 {
     int device = 0;
+    int numCUs = 256;
     hipStream_t stream = nullptr;
     hipStream_t uploadStream = nullptr;
     hipEvent_t uploadDone = nullptr;

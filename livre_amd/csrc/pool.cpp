@@ -190,6 +190,10 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
     int rc = lvr::ensureDevice(ctx);
     if (rc)
         return bail(rc);
+    if (hipDeviceGetAttribute(&ctx->numCUs, hipDeviceAttributeMultiprocessorCount,
+                              device) != hipSuccess ||
+        ctx->numCUs <= 0)
+        return bail(LVR_EHIP);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
             hipSuccess ||
         hipStreamCreateWithFlags(&ctx->uploadStream, hipStreamNonBlocking) !=

@@ -78,6 +78,24 @@ __device__ inline void intersectBox(const float o[3], const float d0[3],
     t1 = fminf(fminf(tmax[0], tmax[1]), fminf(tmax[0], tmax[2]));
 }
 
+// intersectBox with the reciprocal direction already formed
+__device__ inline void intersectBoxInv(const float o[3], const float inv[3],
+                                       const float mn[3], const float mx[3],
+                                       float& t0, float& t1)
+{
+    float tmin[3], tmax[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const float tb = inv[i] * (mn[i] - o[i]);
+        const float tt = inv[i] * (mx[i] - o[i]);
+        tmin[i] = fminf(tt, tb);
+        tmax[i] = fmaxf(tt, tb);
+    }
+    t0 = fmaxf(fmaxf(tmin[0], tmin[1]), fmaxf(tmin[0], tmin[2]));
+    t1 = fminf(fminf(tmax[0], tmax[1]), fminf(tmax[0], tmax[2]));
+}
+
 template <typename T>
 __device__ inline float loadVoxel(const T* v, uint32_t idx)
 {
@@ -155,10 +173,15 @@ __global__ __launch_bounds__(256) void binCountKernel(
 }
 
 // Exclusive scan of the tile counts, one workgroup of 1024 threads.
+// Also clears the frame's counters (composite steps, ray-cast work queue) for
+// the kernels that follow on the stream.
 __global__ __launch_bounds__(1024) void scanKernel(const int32_t* __restrict__ counts,
-                                                   int n, int32_t* __restrict__ offsets)
+                                                   int n, int32_t* __restrict__ offsets,
+                                                   unsigned long long* __restrict__ counters)
 {
     __shared__ int32_t partial[1024];
+    if (threadIdx.x < 2)
+        counters[threadIdx.x] = 0;
     const int per = (n + 1023) / 1024;
     const int begin = threadIdx.x * per;
     const int end = min(n, begin + per);
@@ -211,53 +234,31 @@ __global__ __launch_bounds__(256) void binFillKernel(
     }
 }
 
-// XCD-aware block order: consecutive workgroups are dispatched round-robin
-// over the 8 XCDs of MI355X; give each XCD a contiguous band of tiles so that
-// neighbouring tiles, which share bricks, share that XCD's L2.
-__device__ inline int xcdRemap(int bid, int nBlocks)
-{
-    constexpr int kXcd = 8;
-    const int per = (nBlocks + kXcd - 1) / kXcd;
-    const int xcd = bid % kXcd;
-    const int idx = bid / kXcd;
-    const int full = nBlocks - (per - 1) * kXcd; // XCDs that get `per` blocks
-    int base;
-    if (xcd < full)
-        base = xcd * per;
-    else
-        base = full * per + (xcd - full) * (per - 1);
-    return base + idx;
-}
-
-// A workgroup is kSuper x kSuper wavefront tiles (16 wavefronts, 1024 rays)
-// sharing one LDS copy of the transfer function: a 4096-entry FP32 table is
-// 64 KiB, so two workgroups per CU still give 8 wavefronts per SIMD.
+// Work items are wavefront tiles in super-tile order: the 16 tiles of a
+// kSuper x kSuper block of tiles are consecutive items, so they run at the
+// same time on neighbouring wavefronts and share bricks in L1/L2.
 constexpr int kSuper = 4;
 
+__device__ inline int itemToTile(const FrameUniforms& U, int item)
+{
+    const int superX = (U.tilesX + kSuper - 1) / kSuper;
+    const int s = item / (kSuper * kSuper), w = item % (kSuper * kSuper);
+    const int tx = (s % superX) * kSuper + w % kSuper;
+    const int ty = (s / superX) * kSuper + w / kSuper;
+    return (tx < U.tilesX && ty < U.tilesY) ? ty * U.tilesX + tx : -1;
+}
+
+// One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
+// ordered list, composited in registers, then the RGBA8 store.
 template <typename T>
-__global__ __launch_bounds__(1024) void raycastKernel(
-    const FrameUniforms U, const BrickEntry* __restrict__ bricks,
+__device__ __forceinline__ void castTile(
+    const FrameUniforms& U, int tile, int lane, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
     const int32_t* __restrict__ lists, const uint8_t* __restrict__ pool,
-    const float4* __restrict__ tf, uint32_t* __restrict__ frame,
-    unsigned long long* __restrict__ sampleCounter)
+    const float4* sTf, uint32_t* __restrict__ frame, uint32_t& nsamp)
 {
-    extern __shared__ float4 sTf[];
-    for (int i = threadIdx.x; i < U.nTf; i += blockDim.x)
-        sTf[i] = tf[i];
-    __syncthreads();
-
-    const int block = xcdRemap(int(blockIdx.x), int(gridDim.x));
-    const int superX = (U.tilesX + kSuper - 1) / kSuper;
-    const int wave = __builtin_amdgcn_readfirstlane(int(threadIdx.x) / kWave);
-    const int tx = __builtin_amdgcn_readfirstlane((block % superX) * kSuper + wave % kSuper);
-    const int ty = __builtin_amdgcn_readfirstlane((block / superX) * kSuper + wave / kSuper);
-    if (tx >= U.tilesX || ty >= U.tilesY)
-        return;
-    const int tile = __builtin_amdgcn_readfirstlane(ty * U.tilesX + tx);
-    const int lane = threadIdx.x & (kWave - 1);
-    const int px = tx * U.tileW + lane % U.tileW;
-    const int py = ty * U.tileH + lane / U.tileW;
+    const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
+    const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
     const bool inside = px < U.width && py < U.height;
 
     // pixel ray (fragRayCast.glsl:121-140)
@@ -291,8 +292,14 @@ __global__ __launch_bounds__(1024) void raycastKernel(
     }
     const float tNearPlane = -U.nearPlane / nz;
     const float viewCos = -nz;
+    // 1 / r.Dir with zero components replaced by EPSILON, as intersectBox
+    // computes it for every brick of this ray (same value each time)
+    float inv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        inv[i] = 1.0f / (dir[i] == 0.f ? 0.0000000001f : dir[i]);
     float tnearGlobal, tfarGlobal;
-    intersectBox(U.eye, dir, U.gmin, U.gmax, tnearGlobal, tfarGlobal);
+    intersectBoxInv(U.eye, inv, U.gmin, U.gmax, tnearGlobal, tfarGlobal);
 
     const float stepSize = 1.0f / float(U.nSamples);
     const float alphaCorrection = float(U.maxSamples) / float(U.nSamples);
@@ -302,7 +309,6 @@ __global__ __launch_bounds__(1024) void raycastKernel(
     const float nTfF = float(U.nTf);
 
     float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
-    uint32_t nsamp = 0;
     bool alive = inside;
 
     const int beg = __builtin_amdgcn_readfirstlane(offsets[tile]);
@@ -321,7 +327,7 @@ __global__ __launch_bounds__(1024) void raycastKernel(
             continue;
         }
         float tnear, tfar;
-        intersectBox(U.eye, dir, b.bmin, b.bmax, tnear, tfar);
+        intersectBoxInv(U.eye, inv, b.bmin, b.bmax, tnear, tfar);
         const float exitDepth = tfar * viewCos;
         if (!(exitDepth >= U.nearPlane && exitDepth <= U.farPlane))
             continue;
@@ -430,13 +436,52 @@ __global__ __launch_bounds__(1024) void raycastKernel(
         }
         frame[size_t(py) * U.width + px] = packed;
     }
+}
+
+// Persistent ray-cast kernel: each workgroup (16 wavefronts) stages the
+// transfer function in LDS once; then every wavefront pulls tiles from a
+// frame-wide atomic counter until none are left. Tiles differ by orders of
+// magnitude in work (empty sky vs rays through the whole volume), so pulling
+// them dynamically keeps every CU busy until the end of the frame. The queue
+// counter is reset by scanKernel earlier on the same stream.
+template <typename T>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void raycastKernel(
+    const FrameUniforms U, const BrickEntry* __restrict__ bricks,
+    const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
+    const int32_t* __restrict__ lists, const uint8_t* __restrict__ pool,
+    const float4* __restrict__ tf, uint32_t* __restrict__ frame,
+    unsigned long long* __restrict__ counters)
+{
+    extern __shared__ float4 sTf[];
+    for (int i = threadIdx.x; i < U.nTf; i += blockDim.x)
+        sTf[i] = tf[i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int superX = (U.tilesX + kSuper - 1) / kSuper;
+    const int superY = (U.tilesY + kSuper - 1) / kSuper;
+    const int nItems = superX * superY * kSuper * kSuper;
+    uint32_t nsamp = 0;
+    for (;;)
+    {
+        unsigned long long got = 0;
+        if (lane == 0)
+            got = atomicAdd(&counters[1], 1ull);
+        const int item = __builtin_amdgcn_readfirstlane(int(__shfl(got, 0, kWave)));
+        if (item >= nItems)
+            break;
+        const int tile = itemToTile(U, item);
+        if (tile >= 0)
+            castTile<T>(U, tile, lane, bricks, offsets, counts, lists, pool, sTf, frame,
+                        nsamp);
+    }
     // wavefront sum of executed composite steps
     unsigned long long s = nsamp;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1)
         s += __shfl_xor(s, off, kWave);
     if (lane == 0 && s)
-        atomicAdd(sampleCounter, s);
+        atomicAdd(&counters[0], s);
 }
 
 template <typename T>
@@ -444,12 +489,12 @@ void launchRaycast(int blocks, int threads, size_t lds, hipStream_t st,
                    const FrameUniforms& U, const BrickEntry* bricks,
                    const int32_t* offsets, const int32_t* counts,
                    const int32_t* lists, const uint8_t* pool, const float4* tf,
-                   uint32_t* frame, unsigned long long* samples)
+                   uint32_t* frame, unsigned long long* counters)
 {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     hipLaunchKernelGGL(raycastKernel<T>, dim3(blocks), dim3(threads), lds, st, U,
-                       bricks, offsets, counts, lists, pool, tf, frame, samples);
+                       bricks, offsets, counts, lists, pool, tf, frame, counters);
 }
 } // namespace
 
@@ -720,11 +765,6 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         LVR_HIP(ctx, hipMemcpyAsync(ctx->d_tileRange, hr, sizeof(int32_t) * 4 * n,
                                     hipMemcpyHostToDevice, ctx->stream));
     }
-    // the sample counter is cleared by a tiny copy, not a memset node
-    ctx->h_counters[2] = 0;
-    LVR_HIP(ctx, hipMemcpyAsync(ctx->d_counters, &ctx->h_counters[2],
-                                sizeof(unsigned long long), hipMemcpyHostToDevice,
-                                ctx->stream));
     const int binBlocks = (nTiles + 3) / 4;
     ctx->lastBricks = onScreen;
     ctx->lastEntries = listTotal;
@@ -734,14 +774,18 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                        reinterpret_cast<const int4*>(ctx->d_tileRange), n,
                        U.tilesX, nTiles, ctx->d_tileCount);
     hipLaunchKernelGGL(scanKernel, dim3(1), dim3(1024), 0, ctx->stream,
-                       ctx->d_tileCount, nTiles, ctx->d_tileOffset);
+                       ctx->d_tileCount, nTiles, ctx->d_tileOffset, ctx->d_counters);
     hipLaunchKernelGGL(binFillKernel, dim3(binBlocks), dim3(256), 0, ctx->stream,
                        reinterpret_cast<const int4*>(ctx->d_tileRange), n,
                        U.tilesX, nTiles, ctx->d_tileOffset, ctx->d_tileList);
     if (ctx->timing)
         LVR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    // persistent: two 1024-ray workgroups per CU are resident (LDS and VGPR
+    // budget), more would only find the queue empty
     const int threads = kSuper * kSuper * 64;
-    const int blocks = ((U.tilesX + kSuper - 1) / kSuper) * ((U.tilesY + kSuper - 1) / kSuper);
+    const int superTiles =
+        ((U.tilesX + kSuper - 1) / kSuper) * ((U.tilesY + kSuper - 1) / kSuper);
+    const int blocks = std::max(1, std::min(superTiles, 2 * ctx->numCUs));
     const size_t lds = size_t(U.nTf) * sizeof(float4);
     const float4* tf = reinterpret_cast<const float4*>(ctx->d_tf);
     switch (vi.data_type)

@@ -41,8 +41,7 @@ def build_scene(livre_amd, scenes):
     vi = livre_amd.volume_info(2048, 64, 1, livre_amd.DT_UINT16)
     ids = scenes.all_nodes(vi)
     per = int(np.prod(vi.brick_shape)) * 2
-    slot = (per + 255) // 256 * 256
-    ctx = livre_amd.Context(vi, slot * (len(ids) + 64))
+    ctx = livre_amd.Context(vi, livre_amd.slot_bytes(vi) * (len(ids) + 64))
     noise = livre_amd.noise_params(3, octaves=5, base_cells=8, gain=1.5)
     t0 = time.perf_counter()
     ctx.generate_noise(noise, ids)

@@ -101,6 +101,11 @@ const char* lvr_last_error(const lvr_ctx* ctx);
 /* Synchronise the context's stream. */
 int lvr_ctx_sync(lvr_ctx* ctx);
 
+/* HBM bytes one brick occupies in the pool (the pool stores bricks in its own
+ * tiled layout, padded, 256-byte aligned): pool_bytes = n * this holds n
+ * bricks. Replaces the texture size accounting of TextureObject.cpp:36-41. */
+int lvr_slot_bytes(const lvr_volume_info* info, uint64_t* out);
+
 /* Cache policy of the pool: 0 = LRU (touched on render, the default),
  * 1 = load order (the reference LRUCachePolicy, which never refreshes on
  * get, Cache.cpp:41-96, 158-170). Pinned bricks are never evicted. */

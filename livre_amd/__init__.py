@@ -30,7 +30,7 @@ EXPORTS = (
     "lvr_noise_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
-    "lvr_set_timing", "lvr_frame_stats_get",
+    "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes",
 )
 
 
@@ -143,6 +143,7 @@ def lib():
                                        P(c.c_int)]
     L.lvr_histogram.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_histogram_read.argtypes = [c.c_void_p, u64p, c.c_int]
+    L.lvr_slot_bytes.argtypes = [P(VolumeInfo), u64p]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
     L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
     _lib = L
@@ -177,6 +178,14 @@ def volume_info(voxels, block, overlap, data_type):
     if rc != LVR_OK:
         raise LivreError("invalid volume information (%d)" % rc)
     return vi
+
+
+def slot_bytes(vi):
+    """HBM bytes per brick in the pool: a pool of n * slot_bytes holds n bricks."""
+    out = ctypes.c_uint64()
+    if lib().lvr_slot_bytes(ctypes.byref(vi), ctypes.byref(out)) != LVR_OK:
+        raise LivreError("invalid volume information")
+    return out.value
 
 
 def world_box(vi, node_id):

@@ -115,7 +115,9 @@ int main(int argc, char** argv)
     {
         NoiseSource source(n, block, 7);
         const livre::VolumeInformation& vi = source.getVolumeInfo();
-        livre::BrickPool pool(vi, (vi.brickBytes() + 256) * 4096);
+        uint64_t slot = 0;
+        livre::check(lvr_slot_bytes(&vi.info, &slot), "slot bytes");
+        livre::BrickPool pool(vi, size_t(slot) * 4096);
         livre::HipRayCastRenderer renderer(pool, n);
         std::vector<float> tf(256 * 4);
         for (int i = 0; i < 256; ++i)

@@ -55,7 +55,7 @@ template <typename T>
 __global__ __launch_bounds__(1024) void histKernel(
     const HistJob* __restrict__ jobs, const int32_t* __restrict__ slots,
     const uint8_t* __restrict__ pool, size_t slotStride, int M, int ov,
-    int binsPerJob, int bins, float r0, float perBin,
+    int binsPerJob, int bins, float r0, float perBin, BrickLayout L,
     unsigned long long* __restrict__ out)
 {
     extern __shared__ uint32_t sHist[];
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(1024) void histKernel(
             const int x = i % B;
             const int y = (i / B) % B;
             const int z = i / (B * B);
-            const T v = vox[((z + ov) * M + (y + ov)) * M + (x + ov)];
+            const T v = vox[tiledIndex(L, uint32_t(x + ov), uint32_t(y + ov), uint32_t(z + ov))];
             const int bin = binOf<T>(v, r0, perBin, bins) - job.binLo;
             if (bin >= 0 && bin < binsPerJob)
                 atomicAdd(&sHist[bin], 1u);
@@ -238,7 +238,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             hipLaunchKernelGGL(histKernel<uint8_t>, grid, block, lds, ctx->stream, dj,
                                ctx->d_histSlots, ctx->d_pool, ctx->slotStride, M, ov,
-                               binsPerJob, bins, r0, perBin, ctx->d_hist);
+                               binsPerJob, bins, r0, perBin, ctx->layout, ctx->d_hist);
         }
         else if (dt == LVR_DT_UINT16)
         {
@@ -246,7 +246,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             hipLaunchKernelGGL(histKernel<uint16_t>, grid, block, lds, ctx->stream, dj,
                                ctx->d_histSlots, ctx->d_pool, ctx->slotStride, M, ov,
-                               binsPerJob, bins, r0, perBin, ctx->d_hist);
+                               binsPerJob, bins, r0, perBin, ctx->layout, ctx->d_hist);
         }
         else
         {
@@ -254,7 +254,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             hipLaunchKernelGGL(histKernel<float>, grid, block, lds, ctx->stream, dj,
                                ctx->d_histSlots, ctx->d_pool, ctx->slotStride, M, ov,
-                               binsPerJob, bins, r0, perBin, ctx->d_hist);
+                               binsPerJob, bins, r0, perBin, ctx->layout, ctx->d_hist);
         }
     }
     if (ctx->timing)

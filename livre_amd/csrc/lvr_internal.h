@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/livre_amd.h"
+#include "lvr_layout.h"
 
 namespace lvr
 {
@@ -111,11 +112,12 @@ struct lvr_ctx
     int numCUs = 256;
     hipStream_t stream = nullptr;
     hipStream_t uploadStream = nullptr;
-    hipEvent_t uploadDone = nullptr;
+    hipEvent_t renderDone = nullptr; // recorded on `stream` before slot reuse
     lvr_volume_info vi{};
     size_t voxelBytes = 0; // bytes per voxel
     size_t brickBytes = 0; // maximumBlockSize^3 * bytesPerVoxel
-    size_t slotStride = 0; // brickBytes rounded to 256
+    size_t slotStride = 0; // tiled brick bytes rounded to 256
+    lvr::BrickLayout layout{};
     uint64_t nSlots = 0;
     uint8_t* d_pool = nullptr;
     lvr::PageTable pt;
@@ -124,9 +126,12 @@ struct lvr_ctx
     std::list<uint64_t> lruList; // front = next to evict
     int policy = 0;
 
-    // pinned staging for uploads
+    // pinned staging for uploads, device staging for layout conversion
     uint8_t* h_staging = nullptr;
     size_t stagingBytes = 0;
+    uint8_t* d_staging = nullptr;
+    int32_t* d_stagingSlots = nullptr;
+    size_t stagingBricks = 0;
 
     // transfer function
     float* d_tf = nullptr; // float4 entries
@@ -193,6 +198,9 @@ int32_t slotOf(const lvr_ctx* ctx, uint64_t id);
 void touch(lvr_ctx* ctx, uint64_t id);
 int flushPageTable(lvr_ctx* ctx);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
+int ensureDeviceStaging(lvr_ctx* ctx, size_t bricks);
+// d_staging (linear bricks) <-> pool slots d_stagingSlots, on stream st
+int convertLayout(lvr_ctx* ctx, int nBricks, bool toTiled, hipStream_t st);
 } // namespace lvr
 
 #define LVR_HIP(ctx, call)                                                     \

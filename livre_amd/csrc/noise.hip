@@ -205,7 +205,8 @@ template <typename T>
 __global__ void noiseBricksKernel(NoiseShape s, const uint16_t* __restrict__ remap,
                                   const BrickGeo* __restrict__ geos,
                                   const int32_t* __restrict__ slots,
-                                  uint8_t* __restrict__ pool, size_t slotStride)
+                                  uint8_t* __restrict__ pool, size_t slotStride,
+                                  BrickLayout L)
 {
     const BrickGeo g = geos[blockIdx.y];
     T* out = reinterpret_cast<T*>(pool + size_t(slots[blockIdx.y]) * slotStride);
@@ -216,7 +217,7 @@ __global__ void noiseBricksKernel(NoiseShape s, const uint16_t* __restrict__ rem
         const uint32_t x = i % g.dim[0];
         const uint32_t y = (i / g.dim[0]) % g.dim[1];
         const uint32_t z = i / (g.dim[0] * g.dim[1]);
-        out[i] = convertNoise<T>(noiseVoxel(s, remap, g.level, g.origin[0] + x,
+        out[tiledIndex(L, x, y, z)] = convertNoise<T>(noiseVoxel(s, remap, g.level, g.origin[0] + x,
                                             g.origin[1] + y, g.origin[2] + z));
     }
 }
@@ -377,15 +378,15 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
         if (dt == LVR_DT_UINT8)
             hipLaunchKernelGGL(lvr::noiseBricksKernel<uint8_t>, grid, dim3(256), 0,
                                ctx->stream, s, d_remap, d_geo, d_slots,
-                               ctx->d_pool, ctx->slotStride);
+                               ctx->d_pool, ctx->slotStride, ctx->layout);
         else if (dt == LVR_DT_UINT16)
             hipLaunchKernelGGL(lvr::noiseBricksKernel<uint16_t>, grid, dim3(256),
                                0, ctx->stream, s, d_remap, d_geo, d_slots,
-                               ctx->d_pool, ctx->slotStride);
+                               ctx->d_pool, ctx->slotStride, ctx->layout);
         else
             hipLaunchKernelGGL(lvr::noiseBricksKernel<float>, grid, dim3(256), 0,
                                ctx->stream, s, d_remap, d_geo, d_slots,
-                               ctx->d_pool, ctx->slotStride);
+                               ctx->d_pool, ctx->slotStride, ctx->layout);
         if (hipGetLastError() != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");

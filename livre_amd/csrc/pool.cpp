@@ -143,6 +143,75 @@ int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot)
     return acquireSlot(ctx, id, slot);
 }
 
+// Linear (x fastest, getData layout) <-> tiled pool layout, one brick per
+// grid row. `linear` holds the bricks back to back; slots[b] is the pool slot
+// of brick b.
+template <typename V>
+__global__ void convertLayoutKernel(V* __restrict__ linear,
+                                    const int32_t* __restrict__ slots,
+                                    uint8_t* __restrict__ pool, size_t slotStride,
+                                    BrickLayout L, int toTiled)
+{
+    const uint32_t n = L.dim[0] * L.dim[1] * L.dim[2];
+    V* lin = linear + size_t(blockIdx.y) * n;
+    V* tiled = reinterpret_cast<V*>(pool + size_t(slots[blockIdx.y]) * slotStride);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+    {
+        const uint32_t x = i % L.dim[0];
+        const uint32_t y = (i / L.dim[0]) % L.dim[1];
+        const uint32_t z = i / (L.dim[0] * L.dim[1]);
+        const uint32_t t = tiledIndex(L, x, y, z);
+        if (toTiled)
+            tiled[t] = lin[i];
+        else
+            lin[i] = tiled[t];
+    }
+}
+
+int convertLayout(lvr_ctx* ctx, int nBricks, bool toTiled, hipStream_t st)
+{
+    const dim3 grid(64, unsigned(nBricks)), block(256);
+    switch (ctx->voxelBytes)
+    {
+    case 1:
+        hipLaunchKernelGGL(convertLayoutKernel<uint8_t>, grid, block, 0, st,
+                           reinterpret_cast<uint8_t*>(ctx->d_staging), ctx->d_stagingSlots,
+                           ctx->d_pool, ctx->slotStride, ctx->layout, toTiled ? 1 : 0);
+        break;
+    case 2:
+        hipLaunchKernelGGL(convertLayoutKernel<uint16_t>, grid, block, 0, st,
+                           reinterpret_cast<uint16_t*>(ctx->d_staging), ctx->d_stagingSlots,
+                           ctx->d_pool, ctx->slotStride, ctx->layout, toTiled ? 1 : 0);
+        break;
+    default:
+        hipLaunchKernelGGL(convertLayoutKernel<uint32_t>, grid, block, 0, st,
+                           reinterpret_cast<uint32_t*>(ctx->d_staging), ctx->d_stagingSlots,
+                           ctx->d_pool, ctx->slotStride, ctx->layout, toTiled ? 1 : 0);
+        break;
+    }
+    LVR_HIP(ctx, hipGetLastError());
+    return LVR_OK;
+}
+
+// Device buffer for `bricks` linear bricks plus their slot list.
+int ensureDeviceStaging(lvr_ctx* ctx, size_t bricks)
+{
+    if (ctx->stagingBricks >= bricks)
+        return LVR_OK;
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    (void)hipFree(ctx->d_staging);
+    (void)hipFree(ctx->d_stagingSlots);
+    ctx->d_staging = nullptr;
+    ctx->d_stagingSlots = nullptr;
+    ctx->stagingBricks = 0;
+    LVR_HIP(ctx, hipMalloc((void**)&ctx->d_staging, bricks * ctx->brickBytes));
+    LVR_HIP(ctx, hipMalloc((void**)&ctx->d_stagingSlots, bricks * sizeof(int32_t)));
+    ctx->stagingBricks = bricks;
+    return LVR_OK;
+}
+
 static int ensureStaging(lvr_ctx* ctx, size_t bytes)
 {
     if (ctx->stagingBytes >= bytes)
@@ -177,7 +246,9 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
     ctx->voxelBytes = lvr::bytesPerVoxel(vi.data_type);
     ctx->brickBytes = size_t(vi.max_block[0]) * vi.max_block[1] *
                       vi.max_block[2] * ctx->voxelBytes;
-    ctx->slotStride = (ctx->brickBytes + 255) & ~size_t(255);
+    ctx->layout = lvr::makeLayout(vi.max_block);
+    ctx->slotStride =
+        (size_t(lvr::layoutVoxels(ctx->layout)) * ctx->voxelBytes + 255) & ~size_t(255);
     ctx->nSlots = pool_bytes / ctx->slotStride;
     auto bail = [&](int rc) {
         std::string msg = ctx->error;
@@ -198,7 +269,7 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
             hipSuccess ||
         hipStreamCreateWithFlags(&ctx->uploadStream, hipStreamNonBlocking) !=
             hipSuccess ||
-        hipEventCreateWithFlags(&ctx->uploadDone, hipEventDisableTiming) !=
+        hipEventCreateWithFlags(&ctx->renderDone, hipEventDisableTiming) !=
             hipSuccess)
         return bail(LVR_EHIP);
     if (hipMalloc((void**)&ctx->d_pool, ctx->nSlots * ctx->slotStride) !=
@@ -268,14 +339,16 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->d_selB);
     (void)hipFree(ctx->d_selOut);
     (void)hipFree(ctx->d_selCount);
+    (void)hipFree(ctx->d_staging);
+    (void)hipFree(ctx->d_stagingSlots);
     if (ctx->h_bricks)
         (void)hipHostFree(ctx->h_bricks);
     if (ctx->h_staging)
         (void)hipHostFree(ctx->h_staging);
     if (ctx->h_counters)
         (void)hipHostFree(ctx->h_counters);
-    if (ctx->uploadDone)
-        (void)hipEventDestroy(ctx->uploadDone);
+    if (ctx->renderDone)
+        (void)hipEventDestroy(ctx->renderDone);
     for (auto& e : ctx->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -298,6 +371,19 @@ int lvr_ctx_sync(lvr_ctx* ctx)
     LVR_HIP(ctx, hipSetDevice(ctx->device));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LVR_OK;
+}
+
+int lvr_slot_bytes(const lvr_volume_info* info, uint64_t* out)
+{
+    if (!info || !out)
+        return LVR_EINVAL;
+    lvr_volume_info vi = *info;
+    if (lvr_fill_regular_volume_info(&vi) != LVR_OK)
+        return LVR_EINVAL;
+    const lvr::BrickLayout L = lvr::makeLayout(vi.max_block);
+    *out = (uint64_t(lvr::layoutVoxels(L)) * lvr::bytesPerVoxel(vi.data_type) + 255) &
+           ~uint64_t(255);
     return LVR_OK;
 }
 
@@ -340,11 +426,18 @@ int lvr_brick_upload_many(lvr_ctx* ctx, const uint64_t* node_ids, int n,
     int rc = lvr::ensureDevice(ctx);
     if (rc)
         return rc;
-    // stage in chunks of up to 64 MiB of pinned memory
-    const size_t chunkBricks =
-        std::max<size_t>(1, (64u << 20) / ctx->brickBytes);
-    rc = lvr::ensureStaging(ctx, std::min<size_t>(chunkBricks, size_t(n)) *
-                                     ctx->brickBytes);
+    if (n == 0)
+        return LVR_OK;
+    // frames queued on the render stream may still read slots evicted here
+    LVR_HIP(ctx, hipEventRecord(ctx->renderDone, ctx->stream));
+    LVR_HIP(ctx, hipStreamWaitEvent(ctx->uploadStream, ctx->renderDone, 0));
+    // stage in chunks of up to 64 MiB: pinned host -> device linear -> tiled slots
+    const size_t chunkBricks = std::max<size_t>(1, (64u << 20) / ctx->brickBytes);
+    const size_t chunk = std::min<size_t>(chunkBricks, size_t(n));
+    rc = lvr::ensureStaging(ctx, chunk * (ctx->brickBytes + sizeof(int32_t)));
+    if (rc)
+        return rc;
+    rc = lvr::ensureDeviceStaging(ctx, chunk);
     if (rc)
         return rc;
     const uint8_t* src = static_cast<const uint8_t*>(voxels);
@@ -354,17 +447,21 @@ int lvr_brick_upload_many(lvr_ctx* ctx, const uint64_t* node_ids, int n,
         LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
         std::memcpy(ctx->h_staging, src + size_t(base) * ctx->brickBytes,
                     size_t(cnt) * ctx->brickBytes);
+        int32_t* slots = reinterpret_cast<int32_t*>(ctx->h_staging + chunk * ctx->brickBytes);
         for (int i = 0; i < cnt; ++i)
         {
-            int32_t slot;
-            rc = lvr::acquireSlot(ctx, node_ids[base + i], &slot);
+            rc = lvr::acquireSlot(ctx, node_ids[base + i], &slots[i]);
             if (rc)
                 return rc;
-            LVR_HIP(ctx, hipMemcpyAsync(ctx->d_pool + size_t(slot) * ctx->slotStride,
-                                        ctx->h_staging + size_t(i) * ctx->brickBytes,
-                                        ctx->brickBytes, hipMemcpyHostToDevice,
-                                        ctx->uploadStream));
         }
+        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_staging, ctx->h_staging,
+                                    size_t(cnt) * ctx->brickBytes, hipMemcpyHostToDevice,
+                                    ctx->uploadStream));
+        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_stagingSlots, slots, sizeof(int32_t) * cnt,
+                                    hipMemcpyHostToDevice, ctx->uploadStream));
+        rc = lvr::convertLayout(ctx, cnt, true, ctx->uploadStream);
+        if (rc)
+            return rc;
     }
     LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
     return LVR_OK;
@@ -412,8 +509,17 @@ int lvr_brick_download(lvr_ctx* ctx, uint64_t node_id, void* out, size_t bytes)
         return rc;
     LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    LVR_HIP(ctx, hipMemcpy(out, ctx->d_pool + size_t(slot) * ctx->slotStride,
-                           bytes, hipMemcpyDeviceToHost));
+    rc = lvr::ensureDeviceStaging(ctx, 1);
+    if (rc)
+        return rc;
+    LVR_HIP(ctx, hipMemcpyAsync(ctx->d_stagingSlots, &slot, sizeof(int32_t),
+                                hipMemcpyHostToDevice, ctx->stream));
+    rc = lvr::convertLayout(ctx, 1, false, ctx->stream);
+    if (rc)
+        return rc;
+    LVR_HIP(ctx, hipMemcpyAsync(out, ctx->d_staging, bytes, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return LVR_OK;
 }
 

@@ -46,6 +46,7 @@ struct FrameUniforms
     int32_t nClip;
     float clip[24];
     uint64_t slotStride;
+    BrickLayout layout; // tiled voxel order inside a pool slot
 };
 
 namespace
@@ -107,17 +108,17 @@ __device__ inline int clampi(int v, int lo, int hi)
     return v < lo ? lo : (v > hi ? hi : v);
 }
 
-template <typename T>
+template <typename T, bool LINEAR>
 __device__ inline float sampleBrick(const T* __restrict__ vox,
                                     const FrameUniforms& U, const float tp[3])
 {
     const int nx = U.texDim[0], ny = U.texDim[1], nz = U.texDim[2];
-    if (!U.linear)
+    if (!LINEAR)
     {
         const int ix = clampi(int(floorf(tp[0] * float(nx))), 0, nx - 1);
         const int iy = clampi(int(floorf(tp[1] * float(ny))), 0, ny - 1);
         const int iz = clampi(int(floorf(tp[2] * float(nz))), 0, nz - 1);
-        return loadVoxel(vox, uint32_t((iz * ny + iy) * nx + ix));
+        return loadVoxel(vox, tiledIndex(U.layout, uint32_t(ix), uint32_t(iy), uint32_t(iz)));
     }
     const float u[3] = {tp[0] * float(nx) - 0.5f, tp[1] * float(ny) - 0.5f,
                         tp[2] * float(nz) - 0.5f};
@@ -134,7 +135,7 @@ __device__ inline float sampleBrick(const T* __restrict__ vox,
         i1[a] = clampi(k + 1, 0, n[a] - 1);
     }
     auto at = [&](int x, int y, int z) {
-        return loadVoxel(vox, uint32_t((z * ny + y) * nx + x));
+        return loadVoxel(vox, tiledIndex(U.layout, uint32_t(x), uint32_t(y), uint32_t(z)));
     };
     const float c00 = at(i0[0], i0[1], i0[2]) * (1.f - w[0]) + at(i1[0], i0[1], i0[2]) * w[0];
     const float c10 = at(i0[0], i1[1], i0[2]) * (1.f - w[0]) + at(i1[0], i1[1], i0[2]) * w[0];
@@ -250,7 +251,7 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
 
 // One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
 // ordered list, composited in registers, then the RGBA8 store.
-template <typename T>
+template <typename T, bool LINEAR>
 __device__ __forceinline__ void castTile(
     const FrameUniforms& U, int tile, int lane, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
@@ -388,7 +389,7 @@ __device__ __forceinline__ void castTile(
                 const float quot = __builtin_fmaf(r, b.rcp[a], q);
                 tp[a] = quot * texExt[a] + U.texMin[a];
             }
-            const float value = sampleBrick<T>(vox, U, tp);
+            const float value = sampleBrick<T, LINEAR>(vox, U, tp);
             const float density = value * U.mult + U.add;
             // texture(transferFnTex, density): linear, clamp to edge
             const float u = density * nTfF - 0.5f;
@@ -444,7 +445,7 @@ __device__ __forceinline__ void castTile(
 // magnitude in work (empty sky vs rays through the whole volume), so pulling
 // them dynamically keeps every CU busy until the end of the frame. The queue
 // counter is reset by scanKernel earlier on the same stream.
-template <typename T>
+template <typename T, bool LINEAR>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void raycastKernel(
     const FrameUniforms U, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
@@ -472,8 +473,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
             break;
         const int tile = itemToTile(U, item);
         if (tile >= 0)
-            castTile<T>(U, tile, lane, bricks, offsets, counts, lists, pool, sTf, frame,
-                        nsamp);
+            castTile<T, LINEAR>(U, tile, lane, bricks, offsets, counts, lists, pool, sTf,
+                                frame, nsamp);
     }
     // wavefront sum of executed composite steps
     unsigned long long s = nsamp;
@@ -491,10 +492,20 @@ void launchRaycast(int blocks, int threads, size_t lds, hipStream_t st,
                    const int32_t* lists, const uint8_t* pool, const float4* tf,
                    uint32_t* frame, unsigned long long* counters)
 {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    hipLaunchKernelGGL(raycastKernel<T>, dim3(blocks), dim3(threads), lds, st, U,
-                       bricks, offsets, counts, lists, pool, tf, frame, counters);
+    if (U.linear)
+    {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, true>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL((raycastKernel<T, true>), dim3(blocks), dim3(threads), lds, st,
+                           U, bricks, offsets, counts, lists, pool, tf, frame, counters);
+    }
+    else
+    {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, false>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL((raycastKernel<T, false>), dim3(blocks), dim3(threads), lds, st,
+                           U, bricks, offsets, counts, lists, pool, tf, frame, counters);
+    }
 }
 } // namespace
 
@@ -626,6 +637,7 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     U.nClip = params->n_clip_planes;
     std::memcpy(U.clip, params->clip_planes, sizeof U.clip);
     U.slotStride = ctx->slotStride;
+    U.layout = ctx->layout;
     const int nTiles = U.tilesX * U.tilesY;
 
     // brick records and conservative tile ranges

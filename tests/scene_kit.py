@@ -48,9 +48,9 @@ class Scene:
         return img, samples
 
     def context(self, pool_bytes=None):
-        per = int(np.prod(self.vi.brick_shape)) * np.dtype(self.vi.dtype).itemsize
-        slot = (per + 255) // 256 * 256
+        slot = livre_amd.slot_bytes(self.vi)
         ctx = livre_amd.Context(self.vi, pool_bytes or slot * len(self.ids))
+        assert ctx.stats()[0] >= len(self.ids)
         ctx.upload(self.ids, np.stack([self.bricks[int(i)] for i in self.ids]))
         return ctx
 

@@ -148,8 +148,7 @@ def test_device_noise_equals_host_noise(lvr):
     vi = livre_amd.volume_info(128, 32, 1, livre_amd.DT_UINT16)
     p = livre_amd.noise_params(3, octaves=5, base_cells=8)
     ids = scenes.all_nodes(vi)
-    slot = (int(np.prod(vi.brick_shape)) * 2 + 255) // 256 * 256
-    with livre_amd.Context(vi, slot * len(ids)) as ctx:
+    with livre_amd.Context(vi, livre_amd.slot_bytes(vi) * len(ids)) as ctx:
         assert ctx.stats()[0] == len(ids)
         ctx.generate_noise(p, ids)
         for nid in ids[:: max(1, len(ids) // 20)]:
@@ -240,8 +239,7 @@ def test_pool_policy_and_eviction(lvr):
     load-order policy reproduces the reference cache test
     (tests/core/cache.cpp:29-78)."""
     vi = livre_amd.volume_info(64, 16, 1, livre_amd.DT_UINT8)
-    per = int(np.prod(vi.brick_shape))
-    slot = (per + 255) // 256 * 256
+    slot = livre_amd.slot_bytes(vi)
     brick = np.zeros(vi.brick_shape, np.uint8)
     a, b, c = (scenes.node_id(2, i, 0, 0) for i in range(3))
     with livre_amd.Context(vi, 2 * slot + 100) as ctx:

@@ -231,6 +231,48 @@ typedef struct
 int lvr_set_timing(lvr_ctx* ctx, int enable);
 int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out);
 
+/* --------------------------------------------------------------- batches */
+/* Many frames (requests) of different volumes, viewports, cameras and
+ * transfer functions rendered by one persistent launch: binning of every
+ * request in one pass, then one work queue over all requests' ray tiles.
+ * The launch sequence is captured as a HIP graph and replayed while the
+ * batch keeps its shape. No reference counterpart: Livre renders one frame
+ * per channel per call (Channel.cpp:283-336). Requests use nearest
+ * filtering; u8, u16 and f32 volumes; transfer functions up to 256 entries
+ * are staged in LDS, larger ones (up to 8192) are read from memory. */
+typedef struct lvr_batch lvr_batch;
+typedef struct
+{
+    uint64_t requests;
+    uint64_t samples;      /* composite steps of all requests */
+    uint64_t bricks;       /* rendered bricks of all requests */
+    uint64_t tile_entries; /* (tile, brick) pairs binned */
+    float gpu_ms;          /* binning + ray casting on the device */
+    uint64_t graph_captures, graph_launches;
+} lvr_batch_stats;
+
+int lvr_batch_create(int device, lvr_batch** out);
+void lvr_batch_destroy(lvr_batch* batch);
+const char* lvr_batch_last_error(const lvr_batch* batch);
+/* 1 (default): capture and replay the launch sequence as a HIP graph. */
+int lvr_batch_set_graph(lvr_batch* batch, int enable);
+/* Queue one frame of ctx's volume; request receives its index. The TF is
+ * n_tf FP32 RGBA entries over [range0, range1] (as lvr_tf_set). Adding after
+ * a run starts a new batch. */
+int lvr_batch_add(lvr_batch* batch, lvr_ctx* ctx, const lvr_frustum* f,
+                  const int viewport[4], const uint64_t* ordered_ids, int n,
+                  const lvr_render_params* params, const float* tf_rgba,
+                  int n_tf, float range0, float range1, int* request);
+int lvr_batch_clear(lvr_batch* batch);
+/* Launch every queued request (asynchronous). */
+int lvr_batch_run(lvr_batch* batch);
+/* Wait for the launch; fills stats (may be NULL). */
+int lvr_batch_wait(lvr_batch* batch, lvr_batch_stats* stats);
+/* RGBA8 frame (w * h * 4 bytes, row 0 = bottom) and composite steps of one
+ * request, after lvr_batch_wait. */
+int lvr_batch_read(lvr_batch* batch, int request, uint8_t* out_rgba8,
+                   size_t bytes, uint64_t* samples);
+
 /* ------------------------------------------------------------- histogram */
 /* Per-frame histogram of the given bricks (HistogramObject.cpp:140-224 +
  * HistogramFilter.cpp:78-149): counts of interior voxels scaled by

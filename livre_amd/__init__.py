@@ -31,7 +31,17 @@ EXPORTS = (
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes",
+    "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
+    "lvr_batch_set_graph", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
+    "lvr_batch_wait", "lvr_batch_read",
 )
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("samples", ctypes.c_uint64),
+                ("bricks", ctypes.c_uint64), ("tile_entries", ctypes.c_uint64),
+                ("gpu_ms", ctypes.c_float), ("graph_captures", ctypes.c_uint64),
+                ("graph_launches", ctypes.c_uint64)]
 
 
 class FrameStats(ctypes.Structure):
@@ -144,6 +154,19 @@ def lib():
     L.lvr_histogram.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_histogram_read.argtypes = [c.c_void_p, u64p, c.c_int]
     L.lvr_slot_bytes.argtypes = [P(VolumeInfo), u64p]
+    L.lvr_batch_create.argtypes = [c.c_int, P(c.c_void_p)]
+    L.lvr_batch_destroy.argtypes = [c.c_void_p]
+    L.lvr_batch_destroy.restype = None
+    L.lvr_batch_last_error.argtypes = [c.c_void_p]
+    L.lvr_batch_last_error.restype = c.c_char_p
+    L.lvr_batch_set_graph.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_batch_add.argtypes = [c.c_void_p, c.c_void_p, P(Frustum), P(c.c_int), u64p,
+                                c.c_int, P(RenderParams), f32p, c.c_int, c.c_float,
+                                c.c_float, P(c.c_int)]
+    L.lvr_batch_clear.argtypes = [c.c_void_p]
+    L.lvr_batch_run.argtypes = [c.c_void_p]
+    L.lvr_batch_wait.argtypes = [c.c_void_p, P(BatchStats)]
+    L.lvr_batch_read.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, u64p]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
     L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
     _lib = L
@@ -382,3 +405,74 @@ class Context:
             self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), bins),
             "histogram_read")
         return out
+
+
+class Batch:
+    """Many frames of different volumes / viewports / TFs in one persistent
+    launch (lvr_batch_*)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        if lib().lvr_batch_create(device, ctypes.byref(self._h)) != LVR_OK:
+            raise LivreError("lvr_batch_create failed")
+        self.sizes = []
+
+    def close(self):
+        if self._h:
+            lib().lvr_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != LVR_OK:
+            msg = lib().lvr_batch_last_error(self._h).decode()
+            raise LivreError("%s failed (%d): %s" % (what, rc, msg))
+
+    def set_graph(self, enable):
+        self._check(lib().lvr_batch_set_graph(self._h, 1 if enable else 0), "set_graph")
+
+    def clear(self):
+        self._check(lib().lvr_batch_clear(self._h), "clear")
+        self.sizes = []
+
+    def add(self, ctx, fr, width, height, ordered, params, tf, data_range):
+        ordered, pord = _u64(ordered)
+        tf, ptf = _f32(tf)
+        vp = (ctypes.c_int * 4)(0, 0, width, height)
+        req = ctypes.c_int()
+        self._check(lib().lvr_batch_add(self._h, ctx._h, ctypes.byref(fr), vp, pord,
+                                        len(ordered), ctypes.byref(params), ptf,
+                                        tf.size // 4, data_range[0], data_range[1],
+                                        ctypes.byref(req)), "add")
+        if req.value == 0:
+            self.sizes = []
+        self.sizes.append((width, height))
+        return req.value
+
+    def run(self):
+        self._check(lib().lvr_batch_run(self._h), "run")
+
+    def wait(self):
+        s = BatchStats()
+        self._check(lib().lvr_batch_wait(self._h, ctypes.byref(s)), "wait")
+        return s
+
+    def read(self, request):
+        w, h = self.sizes[request]
+        img = np.zeros((h, w, 4), np.uint8)
+        s = ctypes.c_uint64()
+        self._check(lib().lvr_batch_read(
+            self._h, request, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+            img.nbytes, ctypes.byref(s)), "read")
+        return img, s.value

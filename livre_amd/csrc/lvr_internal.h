@@ -104,6 +104,40 @@ struct PoolEntry
     int pinned;
     std::list<uint64_t>::iterator lru;
 };
+
+// Everything one frame (one request of a batch) needs on the device.
+struct FrameUniforms
+{
+    float invProj[16];
+    float invMV[16];
+    float eye[3];
+    float nearPlane, farPlane;
+    int32_t width, height;
+    float gmin[3], gmax[3];
+    float texMin[3], texMax[3];
+    int32_t texDim[3];
+    int32_t nSamples, maxSamples;
+    float exitA;
+    float mult, add;
+    int32_t nTf;
+    int32_t tileW, tileH, tilesX, tilesY;
+    int32_t nBricks;
+    int32_t linear;
+    int32_t nClip;
+    float clip[24];
+    uint64_t slotStride;
+    BrickLayout layout; // tiled voxel order inside a pool slot
+    // position of this frame in a batch
+    int32_t brickBase; // first brick record / tile range
+    int32_t tileBase;  // first tile of the per-tile arrays
+    int32_t itemBase;  // first work item
+    int32_t nItems;    // work items (16 per 4x4 super-tile)
+    int32_t dataType;
+    int32_t request;   // index in the batch
+    const uint8_t* pool;
+    const float4* tf;  // nTf RGBA entries
+    uint32_t* frame;   // RGBA8 output, row 0 = bottom
+};
 } // namespace lvr
 
 struct lvr_ctx
@@ -145,15 +179,21 @@ struct lvr_ctx
     int fbW = 0, fbH = 0;
     size_t fbCapacity = 0;
     lvr::BrickEntry* d_bricks = nullptr;
-    lvr::BrickEntry* h_bricks = nullptr; // pinned
     int brickCapacity = 0;
-    int32_t* d_tileRange = nullptr; // 4 ints per brick
+    int4* d_ranges = nullptr; // conservative tile range per brick
+    size_t rangeCapacity = 0;
     int32_t* d_tileCount = nullptr;
     int32_t* d_tileOffset = nullptr;
     int32_t* d_tileList = nullptr;
     size_t tileCapacity = 0;
+    size_t tileOffsetCapacity = 0;
     size_t tileListCapacity = 0;
-    unsigned long long* d_counters = nullptr; // [0] samples, [1] list total
+    lvr::FrameUniforms* d_req = nullptr; // the frame as a one-request table
+    size_t reqCapacity = 0;
+    lvr::FrameUniforms hostReq{};
+    std::vector<lvr::BrickEntry> hostBricks;
+    std::vector<int4> hostRanges;
+    unsigned long long* d_counters = nullptr; // [0] samples, [1] work queue
     unsigned long long* h_counters = nullptr; // pinned
 
     // histogram

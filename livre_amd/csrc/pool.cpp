@@ -327,7 +327,8 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->d_tf);
     (void)hipFree(ctx->d_frame);
     (void)hipFree(ctx->d_bricks);
-    (void)hipFree(ctx->d_tileRange);
+    (void)hipFree(ctx->d_ranges);
+    (void)hipFree(ctx->d_req);
     (void)hipFree(ctx->d_tileCount);
     (void)hipFree(ctx->d_tileOffset);
     (void)hipFree(ctx->d_tileList);
@@ -341,8 +342,6 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->d_selCount);
     (void)hipFree(ctx->d_staging);
     (void)hipFree(ctx->d_stagingSlots);
-    if (ctx->h_bricks)
-        (void)hipHostFree(ctx->h_bricks);
     if (ctx->h_staging)
         (void)hipHostFree(ctx->h_staging);
     if (ctx->h_counters)

@@ -1,0 +1,326 @@
+// Device code of the ray caster shared by the single-frame kernel and the
+// batched (many-request) kernel: per-frame uniforms, the per-tile march that
+// restates fragRayCast.glsl:113-207 for every brick of a tile's ordered list,
+// and the tile/work-item numbering.
+#pragma once
+
+#include "lvr_internal.h"
+
+namespace lvr
+{
+constexpr int kWave = 64;
+// work items are wavefront tiles in kSuper x kSuper super-tile order
+constexpr int kSuper = 4;
+
+__device__ inline void intersectBoxInv(const float o[3], const float inv[3],
+                                       const float mn[3], const float mx[3], float& t0,
+                                       float& t1)
+{
+    float tmin[3], tmax[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+    {
+        const float tb = inv[i] * (mn[i] - o[i]);
+        const float tt = inv[i] * (mx[i] - o[i]);
+        tmin[i] = fminf(tt, tb);
+        tmax[i] = fmaxf(tt, tb);
+    }
+    t0 = fmaxf(fmaxf(tmin[0], tmin[1]), fmaxf(tmin[0], tmin[2]));
+    t1 = fminf(fminf(tmax[0], tmax[1]), fminf(tmax[0], tmax[2]));
+}
+
+__device__ inline int clampi(int v, int lo, int hi)
+{
+    return v < lo ? lo : (v > hi ? hi : v);
+}
+
+template <typename T>
+__device__ inline float loadVoxel(const T* v, uint32_t idx)
+{
+    return float(v[idx]);
+}
+
+// texture() on the brick: GL_NEAREST picks floor(u * N) clamped; GL_LINEAR
+// blends the 8 texels around u * N - 0.5 with clamp-to-edge.
+template <typename T, bool LINEAR>
+__device__ inline float sampleBrick(const T* __restrict__ vox, const FrameUniforms& U,
+                                    const float tp[3])
+{
+    const int nx = U.texDim[0], ny = U.texDim[1], nz = U.texDim[2];
+    if (!LINEAR)
+    {
+        const int ix = clampi(int(floorf(tp[0] * float(nx))), 0, nx - 1);
+        const int iy = clampi(int(floorf(tp[1] * float(ny))), 0, ny - 1);
+        const int iz = clampi(int(floorf(tp[2] * float(nz))), 0, nz - 1);
+        return loadVoxel(vox, tiledIndex(U.layout, uint32_t(ix), uint32_t(iy), uint32_t(iz)));
+    }
+    const float u[3] = {tp[0] * float(nx) - 0.5f, tp[1] * float(ny) - 0.5f,
+                        tp[2] * float(nz) - 0.5f};
+    const int n[3] = {nx, ny, nz};
+    int i0[3], i1[3];
+    float w[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const float fl = floorf(u[a]);
+        w[a] = u[a] - fl;
+        const int k = int(fl);
+        i0[a] = clampi(k, 0, n[a] - 1);
+        i1[a] = clampi(k + 1, 0, n[a] - 1);
+    }
+    auto at = [&](int x, int y, int z) {
+        return loadVoxel(vox, tiledIndex(U.layout, uint32_t(x), uint32_t(y), uint32_t(z)));
+    };
+    const float c00 = at(i0[0], i0[1], i0[2]) * (1.f - w[0]) + at(i1[0], i0[1], i0[2]) * w[0];
+    const float c10 = at(i0[0], i1[1], i0[2]) * (1.f - w[0]) + at(i1[0], i1[1], i0[2]) * w[0];
+    const float c01 = at(i0[0], i0[1], i1[2]) * (1.f - w[0]) + at(i1[0], i0[1], i1[2]) * w[0];
+    const float c11 = at(i0[0], i1[1], i1[2]) * (1.f - w[0]) + at(i1[0], i1[1], i1[2]) * w[0];
+    const float c0 = c00 * (1.f - w[1]) + c10 * w[1];
+    const float c1 = c01 * (1.f - w[1]) + c11 * w[1];
+    return c0 * (1.f - w[2]) + c1 * w[2];
+}
+
+__device__ inline int itemToTile(const FrameUniforms& U, int item)
+{
+    const int superX = (U.tilesX + kSuper - 1) / kSuper;
+    const int s = item / (kSuper * kSuper), w = item % (kSuper * kSuper);
+    const int tx = (s % superX) * kSuper + w % kSuper;
+    const int ty = (s / superX) * kSuper + w / kSuper;
+    return (tx < U.tilesX && ty < U.tilesY) ? ty * U.tilesX + tx : -1;
+}
+
+// One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
+// ordered list, composited in registers, then the RGBA8 store. `tf` is the
+// transfer function (LDS or global, resolved at the call site).
+template <typename T, bool LINEAR>
+__device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int lane,
+                                         const BrickEntry* __restrict__ bricks,
+                                         const int32_t* __restrict__ offsets,
+                                         const int32_t* __restrict__ counts,
+                                         const int32_t* __restrict__ lists,
+                                         const float4* tf, uint32_t& nsamp)
+{
+    const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
+    const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
+    const bool inside = px < U.width && py < U.height;
+
+    // pixel ray (fragRayCast.glsl:121-140)
+    const float ndcx = (float(px) + 0.5f) / float(U.width) * 2.0f - 1.0f;
+    const float ndcy = (float(py) + 0.5f) / float(U.height) * 2.0f - 1.0f;
+    const float* P = U.invProj;
+    float ep[3];
+    ep[0] = P[0] * ndcx + P[4] * ndcy + P[8] * -1.0f + P[12] * 1.0f;
+    ep[1] = P[1] * ndcx + P[5] * ndcy + P[9] * -1.0f + P[13] * 1.0f;
+    ep[2] = P[2] * ndcx + P[6] * ndcy + P[10] * -1.0f + P[14] * 1.0f;
+    const float ew = P[3] * ndcx + P[7] * ndcy + P[11] * -1.0f + P[15] * 1.0f;
+    ep[0] = ep[0] / ew;
+    ep[1] = ep[1] / ew;
+    ep[2] = ep[2] / ew;
+    const float* M = U.invMV;
+    float wp[3];
+    wp[0] = M[0] * ep[0] + M[4] * ep[1] + M[8] * ep[2] + M[12] * 1.0f;
+    wp[1] = M[1] * ep[0] + M[5] * ep[1] + M[9] * ep[2] + M[13] * 1.0f;
+    wp[2] = M[2] * ep[0] + M[6] * ep[1] + M[10] * ep[2] + M[14] * 1.0f;
+    float dir[3] = {wp[0] - U.eye[0], wp[1] - U.eye[1], wp[2] - U.eye[2]};
+    {
+        const float l = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        dir[0] = dir[0] / l;
+        dir[1] = dir[1] / l;
+        dir[2] = dir[2] / l;
+    }
+    float nz;
+    {
+        const float l = sqrtf(ep[0] * ep[0] + ep[1] * ep[1] + ep[2] * ep[2]);
+        nz = ep[2] / l;
+    }
+    const float tNearPlane = -U.nearPlane / nz;
+    const float viewCos = -nz;
+    // 1 / r.Dir with zero components replaced by EPSILON, as intersectBox
+    // computes it for every brick of this ray (same value each time)
+    float inv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        inv[i] = 1.0f / (dir[i] == 0.f ? 0.0000000001f : dir[i]);
+    float tnearGlobal, tfarGlobal;
+    intersectBoxInv(U.eye, inv, U.gmin, U.gmax, tnearGlobal, tfarGlobal);
+
+    const float stepSize = 1.0f / float(U.nSamples);
+    const float alphaCorrection = float(U.maxSamples) / float(U.nSamples);
+    const float maxA = 1.0f - 1.0f / 256.0f;
+    const float texExt[3] = {U.texMax[0] - U.texMin[0], U.texMax[1] - U.texMin[1],
+                             U.texMax[2] - U.texMin[2]};
+    const float nTfF = float(U.nTf);
+
+    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+    bool alive = inside;
+
+    const int gtile = U.tileBase + tile;
+    const int beg = __builtin_amdgcn_readfirstlane(offsets[gtile]);
+    const int cnt = __builtin_amdgcn_readfirstlane(counts[gtile]);
+    for (int k = 0; k < cnt; ++k)
+    {
+        if (__ballot(alive) == 0)
+            break;
+        const int bi = __builtin_amdgcn_readfirstlane(lists[beg + k]);
+        const BrickEntry b = bricks[bi];
+        if (!alive)
+            continue;
+        if (acc3 > U.exitA) // imageLoad + discard (fragRayCast.glsl:115-117)
+        {
+            alive = false;
+            continue;
+        }
+        float tnear, tfar;
+        intersectBoxInv(U.eye, inv, b.bmin, b.bmax, tnear, tfar);
+        // a back-face fragment exists where the exit point lies between the
+        // near and far clip planes
+        const float exitDepth = tfar * viewCos;
+        if (!(exitDepth >= U.nearPlane && exitDepth <= U.farPlane))
+            continue;
+        if (tnear < tNearPlane)
+            tnear = tNearPlane;
+        {
+            // GLSL mod(x, y) = x - y * floor(x / y)
+            const float x = tnear - tnearGlobal;
+            const float residu = x - stepSize * floorf(x / stepSize);
+            if (residu > 0.0f)
+                tnear += stepSize - residu;
+        }
+        if (tnear > tfar)
+            continue;
+        for (int c = 0; c < U.nClip; ++c)
+        {
+            const float* cp = U.clip + 4 * c;
+            float rn = dir[0] * cp[0] + dir[1] * cp[1] + dir[2] * cp[2];
+            if (rn == 0.f)
+                rn = 0.0000000001f;
+            const float t =
+                -((cp[0] * U.eye[0] + cp[1] * U.eye[1] + cp[2] * U.eye[2]) + cp[3]) / rn;
+            if (rn > 0.0f)
+                tnear = fmaxf(tnear, t);
+            else
+                tfar = fminf(tfar, t);
+        }
+        if (tnear > tfar)
+            continue;
+
+        const float rs[3] = {U.eye[0] + dir[0] * tnear, U.eye[1] + dir[1] * tnear,
+                             U.eye[2] + dir[2] * tnear};
+        const float re[3] = {U.eye[0] + dir[0] * tfar, U.eye[1] + dir[1] * tfar,
+                             U.eye[2] + dir[2] * tfar};
+        const float seg[3] = {re[0] - rs[0], re[1] - rs[1], re[2] - rs[2]};
+        const float segLen = sqrtf(seg[0] * seg[0] + seg[1] * seg[1] + seg[2] * seg[2]);
+        const float step[3] = {seg[0] / segLen * stepSize, seg[1] / segLen * stepSize,
+                               seg[2] / segLen * stepSize};
+        float pos[3] = {rs[0], rs[1], rs[2]};
+        const T* vox = reinterpret_cast<const T*>(U.pool + size_t(b.slot) * U.slotStride);
+
+        // A segment lies inside one brick, so it holds at most
+        // diagonal * nSamples steps; the guard only bounds the loop should
+        // `travel - stepSize` ever round back to `travel`.
+        int guard = 1 << 22;
+        for (float travel = segLen; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
+        {
+            float tp[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                // calcTexturePositionFromAABBPos (fragRayCast.glsl:72-76):
+                // (pos - aabbMin) / (aabbMax - aabbMin) correctly rounded, from
+                // the host's IEEE reciprocal of the divisor and one FMA
+                // correction step (Markstein), then * (texMax - texMin) + texMin
+                const float x = pos[a] - b.bmin[a];
+                const float q = x * b.rcp[a];
+                const float r = __builtin_fmaf(-q, b.ext[a], x);
+                const float quot = __builtin_fmaf(r, b.rcp[a], q);
+                tp[a] = quot * texExt[a] + U.texMin[a];
+            }
+            const float value = sampleBrick<T, LINEAR>(vox, U, tp);
+            const float density = value * U.mult + U.add;
+            // texture(transferFnTex, density): linear, clamp to edge
+            const float u = density * nTfF - 0.5f;
+            const float fl = floorf(u);
+            const float w = u - fl;
+            const float flc = fminf(fmaxf(fl, -1.0f), nTfF);
+            const int kk = int(flc);
+            const int ia = clampi(kk, 0, U.nTf - 1);
+            const int ib = clampi(kk + 1, 0, U.nTf - 1);
+            const float4 ca = tf[ia];
+            const float4 cb = tf[ib];
+            const float sr = ca.x * (1.0f - w) + cb.x * w;
+            const float sg = ca.y * (1.0f - w) + cb.y * w;
+            const float sb = ca.z * (1.0f - w) + cb.z * w;
+            const float sa = ca.w * (1.0f - w) + cb.w * w;
+            // composite (fragRayCast.glsl:103-111); pow(x, c) for x in
+            // [1/256, 1] as exp2(c * log2 x) on the hardware transcendental
+            // units (1 ulp each)
+            const float base = 1.0f - fminf(sa, maxA);
+            const float alpha =
+                1.0f - __builtin_amdgcn_exp2f(alphaCorrection * __builtin_amdgcn_logf(base));
+            const float rem = 1.0f - acc3;
+            acc0 += sr * alpha * rem;
+            acc1 += sg * alpha * rem;
+            acc2 += sb * alpha * rem;
+            acc3 += alpha * rem;
+            ++nsamp;
+            if (acc3 > U.exitA)
+                break;
+            pos[0] = pos[0] + step[0];
+            pos[1] = pos[1] + step[1];
+            pos[2] = pos[2] + step[2];
+        }
+    }
+
+    if (inside)
+    {
+        // fragTexCopy.glsl into the RGBA8 framebuffer: round(clamp(c) * 255)
+        const float c[4] = {acc0, acc1, acc2, acc3};
+        uint32_t packed = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+        {
+            const float v = fminf(1.0f, fmaxf(0.0f, c[i]));
+            packed |= uint32_t(floorf(v * 255.0f + 0.5f)) << (8 * i);
+        }
+        U.frame[size_t(py) * U.width + px] = packed;
+    }
+}
+
+__device__ inline unsigned long long waveSum(unsigned long long s)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        s += __shfl_xor(s, off, kWave);
+    return s;
+}
+
+// Request of a global tile / work item: requests are sorted by tileBase and
+// itemBase; binary search over the (wave-uniform) request table.
+__device__ inline int findByTile(const FrameUniforms* __restrict__ reqs, int nReq, int gtile)
+{
+    int lo = 0, hi = nReq - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].tileBase <= gtile)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ inline int findByItem(const FrameUniforms* __restrict__ reqs, int nReq, int item)
+{
+    int lo = 0, hi = nReq - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].itemBase <= item)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+} // namespace lvr

@@ -1,0 +1,108 @@
+"""Batched rendering: every request of one persistent launch (mixed volumes,
+data types, viewports, cameras and transfer functions) equals the same frame
+rendered alone by lvr_render, bit for bit, with the same composite-step
+count; graph replays reproduce the first launch; and the frames stay within
+the image tolerance of the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+import livre_amd
+from livre_amd import scenes
+
+from scene_kit import Scene, diff_stats
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def volumes(lvr, orc):
+    out = [Scene(livre_amd.DT_UINT8, n=64, block=16, overlap=1, source="blobs", seed=3),
+           Scene(livre_amd.DT_UINT16, n=64, block=16, overlap=1, source="noise", seed=4),
+           Scene(livre_amd.DT_FLOAT, n=48, block=16, overlap=1, source="noise", seed=5)]
+    ctxs = [s.context() for s in out]
+    yield list(zip(out, ctxs))
+    for c in ctxs:
+        c.close()
+
+
+def _requests(volumes, seed):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for k in range(7):
+        s, ctx = volumes[k % 3]
+        w, h = int(rng.integers(24, 120)), int(rng.integers(24, 120))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = tuple(d * rng.uniform(1.4, 2.6))
+        mv, proj, fr = s.camera(eye, w, h)
+        ordered = s.visible_ordered(mv, proj, h)
+        n_tf = 1024 if k == 4 else 256  # k == 4 reads its TF from global memory
+        tf = scenes.piecewise_tf(n_tf, seed=100 + k, max_alpha=0.5)
+        nsamp = int(rng.choice([32, 64, 128]))
+        reqs.append((s, ctx, fr, w, h, ordered, tf, nsamp))
+    return reqs
+
+
+def _alone(req):
+    s, ctx, fr, w, h, ordered, tf, nsamp = req
+    ctx.set_tf(tf, s.range)
+    ctx.render(fr, w, h, ordered, livre_amd.render_params(nsamp))
+    return ctx.read_frame()
+
+
+def test_batch_equals_single_frames(volumes):
+    reqs = _requests(volumes, 1)
+    with livre_amd.Batch() as b:
+        for s, ctx, fr, w, h, ordered, tf, nsamp in reqs:
+            b.add(ctx, fr, w, h, ordered, livre_amd.render_params(nsamp), tf, s.range)
+        b.run()
+        st = b.wait()
+        assert st.requests == len(reqs) and st.graph_captures == 1
+        total = 0
+        for i, req in enumerate(reqs):
+            img, smp = b.read(i)
+            ref, ref_smp = _alone(req)
+            print("request", i, req[3], req[4], "samples", smp, ref_smp, flush=True)
+            assert np.array_equal(img, ref), i
+            assert smp == ref_smp
+            total += smp
+        assert st.samples == total
+        # replay of the captured graph: same shape, same frames
+        first = [b.read(i)[0] for i in range(len(reqs))]
+        for i, (s, ctx, fr, w, h, ordered, tf, nsamp) in enumerate(reqs):
+            b.add(ctx, fr, w, h, ordered, livre_amd.render_params(nsamp), tf, s.range)
+        b.run()
+        st = b.wait()
+        assert st.graph_captures == 1 and st.graph_launches == 2
+        for i in range(len(reqs)):
+            assert np.array_equal(b.read(i)[0], first[i])
+
+
+def test_batch_without_graph_and_oracle_tolerance(volumes, orc):
+    reqs = _requests(volumes, 2)
+    with livre_amd.Batch() as b:
+        b.set_graph(False)
+        for s, ctx, fr, w, h, ordered, tf, nsamp in reqs:
+            b.add(ctx, fr, w, h, ordered, livre_amd.render_params(nsamp), tf, s.range)
+        b.run()
+        st = b.wait()
+        assert st.graph_launches == 0
+        for i, (s, ctx, fr, w, h, ordered, tf, nsamp) in enumerate(reqs):
+            img, smp = b.read(i)
+            ref, ref_smp = s.oracle_render(fr, w, h, ordered, tf, nsamp)
+            mx, mean = diff_stats(img, oracle.to_rgba8(ref))
+            assert mx <= 2 and mean <= 0.5, (i, mx, mean)
+            assert smp == ref_smp
+
+
+def test_batch_rejects_what_it_does_not_render(volumes):
+    s, ctx = volumes[0]
+    mv, proj, fr = s.camera((1.5, 0.2, 1.1), 32, 32)
+    with livre_amd.Batch() as b:
+        with pytest.raises(livre_amd.LivreError):
+            b.add(ctx, fr, 32, 32, s.ids[:4], livre_amd.render_params(32, linear=True),
+                  scenes.piecewise_tf(256, 1), s.range)
+        with pytest.raises(livre_amd.LivreError):
+            b.add(ctx, fr, 32, 32, s.ids[:4], livre_amd.render_params(32, tile=(5, 5)),
+                  scenes.piecewise_tf(256, 1), s.range)

@@ -161,6 +161,29 @@ int lvr_noise_brick_host(const lvr_volume_info* info,
 int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
                             const uint64_t* node_ids, int n);
 
+/* Two more procedural sources, host/device bit-identical like the noise:
+ * LVR_FIELD_BLOBS sums `count` anisotropic rotated bumps plus uniform noise
+ * of amplitude `noise` (scaled to the type's range; floats in [0, 1]);
+ * LVR_FIELD_SINES is the mean of `count` low-frequency plane waves, in
+ * [-1, 1]. Supported types: u8, u16, float. No reference counterpart (same
+ * reason as the noise source). */
+#define LVR_FIELD_BLOBS 0
+#define LVR_FIELD_SINES 1
+typedef struct
+{
+    int32_t kind;  /* LVR_FIELD_BLOBS or LVR_FIELD_SINES */
+    int32_t count; /* blobs or waves, 1..4096 */
+    uint64_t seed;
+    float noise;   /* blobs: uniform noise amplitude, relative to one bump */
+    int32_t pad;
+} lvr_field_params;
+
+int lvr_field_brick_host(const lvr_volume_info* info,
+                         const lvr_field_params* p, uint64_t node_id,
+                         void* out, size_t bytes);
+int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p,
+                            const uint64_t* node_ids, int n);
+
 /* ----------------------------------------------------- transfer function */
 /* FP32 RGBA transfer function of n entries over the data range
  * [range0, range1] (RayCastRenderer.cpp:161-193, 280-285). Sampled with

@@ -27,7 +27,8 @@ EXPORTS = (
     "lvr_pool_set_policy", "lvr_pool_stats", "lvr_brick_upload",
     "lvr_brick_upload_many", "lvr_brick_evict", "lvr_brick_resident",
     "lvr_brick_download", "lvr_noise_params_init", "lvr_noise_brick_host",
-    "lvr_noise_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
+    "lvr_noise_bricks_device", "lvr_field_brick_host",
+    "lvr_field_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes",
@@ -104,6 +105,12 @@ class NoiseParams(ctypes.Structure):
                 ("base_cells", ctypes.c_int32), ("remap", ctypes.c_uint16 * 4097)]
 
 
+class FieldParams(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("count", ctypes.c_int32),
+                ("seed", ctypes.c_uint64), ("noise", ctypes.c_float),
+                ("pad", ctypes.c_int32)]
+
+
 _lib = None
 
 
@@ -140,6 +147,9 @@ def lib():
     L.lvr_noise_brick_host.argtypes = [P(VolumeInfo), P(NoiseParams), c.c_uint64,
                                        c.c_void_p, c.c_size_t]
     L.lvr_noise_bricks_device.argtypes = [c.c_void_p, P(NoiseParams), u64p, c.c_int]
+    L.lvr_field_brick_host.argtypes = [P(VolumeInfo), P(FieldParams), c.c_uint64,
+                                       c.c_void_p, c.c_size_t]
+    L.lvr_field_bricks_device.argtypes = [c.c_void_p, P(FieldParams), u64p, c.c_int]
     L.lvr_tf_set.argtypes = [c.c_void_p, f32p, c.c_int, c.c_float, c.c_float]
     L.lvr_tf_set_rgba8.argtypes = [c.c_void_p, u8p, c.c_int, c.c_float, c.c_float]
     L.lvr_select_visibles.argtypes = [c.c_void_p, P(Frustum), c.c_uint32, c.c_float,
@@ -257,6 +267,23 @@ def noise_brick_host(vi, params, node_id):
     return out
 
 
+FIELD_BLOBS, FIELD_SINES = 0, 1
+
+
+def field_params(kind, count, seed, noise=0.0):
+    """Procedural blob (kind FIELD_BLOBS) or sum-of-sines (FIELD_SINES) source."""
+    return FieldParams(kind, count, seed, noise, 0)
+
+
+def field_brick_host(vi, params, node_id):
+    out = np.zeros(vi.brick_shape, vi.dtype)
+    rc = lib().lvr_field_brick_host(ctypes.byref(vi), ctypes.byref(params),
+                                    int(node_id), out.ctypes.data, out.nbytes)
+    if rc != LVR_OK:
+        raise LivreError("field brick failed (%d)" % rc)
+    return out
+
+
 def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
                   max_samples_per_ray=32, clip_planes=(), tile=(0, 0)):
     p = RenderParams()
@@ -346,6 +373,11 @@ class Context:
         ids, pids = _u64(ids)
         self._check(lib().lvr_noise_bricks_device(self._h, ctypes.byref(params), pids,
                                                   len(ids)), "generate_noise")
+
+    def generate_field(self, params, ids):
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_field_bricks_device(self._h, ctypes.byref(params), pids,
+                                                  len(ids)), "generate_field")
 
     def set_tf(self, rgba, data_range):
         rgba, p = _f32(rgba)

@@ -155,6 +155,19 @@ def test_device_noise_equals_host_noise(lvr):
             assert np.array_equal(ctx.download(nid), livre_amd.noise_brick_host(vi, p, nid))
 
 
+@pytest.mark.parametrize("kind,dt", [(livre_amd.FIELD_BLOBS, livre_amd.DT_UINT8),
+                                     (livre_amd.FIELD_BLOBS, livre_amd.DT_UINT16),
+                                     (livre_amd.FIELD_SINES, livre_amd.DT_FLOAT)])
+def test_device_field_equals_host_field(lvr, kind, dt):
+    vi = livre_amd.volume_info(96, 32, 1, dt)
+    p = livre_amd.field_params(kind, 24, 8, noise=0.1 if kind == livre_amd.FIELD_BLOBS else 0)
+    ids = scenes.all_nodes(vi)
+    with livre_amd.Context(vi, livre_amd.slot_bytes(vi) * len(ids)) as ctx:
+        ctx.generate_field(p, ids)
+        for nid in ids:
+            assert np.array_equal(ctx.download(nid), livre_amd.field_brick_host(vi, p, nid)), nid
+
+
 @pytest.mark.parametrize("dt,bins", [(livre_amd.DT_UINT8, 256),
                                      (livre_amd.DT_UINT16, 65536),
                                      (livre_amd.DT_FLOAT, 256)])

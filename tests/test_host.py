@@ -118,6 +118,29 @@ def test_noise_types(lvr):
             assert b.min() >= -1.0 and b.max() <= 1.0
 
 
+def test_field_sources_are_deterministic_and_consistent(lvr):
+    for kind, dt, count in ((lvr.FIELD_BLOBS, lvr.DT_UINT8, 64),
+                            (lvr.FIELD_BLOBS, lvr.DT_UINT16, 16),
+                            (lvr.FIELD_SINES, lvr.DT_FLOAT, 8)):
+        vi = lvr.volume_info(128, 32, 1, dt)
+        p = lvr.field_params(kind, count, 21, noise=0.1 if kind == lvr.FIELD_BLOBS else 0.0)
+        a = lvr.field_brick_host(vi, p, scenes.node_id(2, 1, 2, 1))
+        assert np.array_equal(a, lvr.field_brick_host(vi, p, scenes.node_id(2, 1, 2, 1)))
+        assert a.min() < a.max()
+        c = lvr.field_brick_host(vi, p, scenes.node_id(2, 2, 2, 1))
+        assert np.array_equal(a[:, :, -1], c[:, :, 1])
+        assert np.array_equal(a[:, :, -2], c[:, :, 0])
+        if kind == lvr.FIELD_SINES:
+            assert a.min() >= -1.0 and a.max() <= 1.0
+    # another seed, another volume
+    vi = lvr.volume_info(64, 16, 1, lvr.DT_UINT8)
+    x = lvr.field_brick_host(vi, lvr.field_params(lvr.FIELD_BLOBS, 32, 1, 0.1), 0)
+    y = lvr.field_brick_host(vi, lvr.field_params(lvr.FIELD_BLOBS, 32, 2, 0.1), 0)
+    assert not np.array_equal(x, y)
+    with pytest.raises(lvr.LivreError):
+        lvr.field_brick_host(vi, lvr.field_params(7, 32, 1), 0)
+
+
 def test_invalid_arguments_are_rejected(lvr):
     with pytest.raises(lvr.LivreError):
         lvr.volume_info(64, 0, 0, lvr.DT_UINT8)

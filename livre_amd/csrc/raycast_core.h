@@ -40,6 +40,25 @@ __device__ inline float loadVoxel(const T* v, uint32_t idx)
     return float(v[idx]);
 }
 
+// Texel index along one axis for GL_NEAREST: floor(u * N) clamped to
+// [0, N-1]. Clamping the float to [0, N-1] first and truncating gives the same
+// integer for every input (truncation equals floor on non-negative values, and
+// everything below 0 clamps to 0 either way), in two instructions.
+__device__ inline uint32_t nearestTexel(float tpN, float nMinus1)
+{
+    return uint32_t(int(__builtin_amdgcn_fmed3f(tpN, 0.0f, nMinus1)));
+}
+
+// tiledIndex() with 24-bit multiplies (coordinates and tile counts are far
+// below 2^24), so the address takes no 64-bit multiply-add
+__device__ inline uint32_t tiledIndexDev(const BrickLayout& L, uint32_t x, uint32_t y,
+                                         uint32_t z)
+{
+    const uint32_t tile = __umul24(z >> 2, L.tiles[0] * L.tiles[1]) +
+                          __umul24(y >> 2, L.tiles[0]) + (x >> 2);
+    return (tile << 6) | ((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u);
+}
+
 // texture() on the brick: GL_NEAREST picks floor(u * N) clamped; GL_LINEAR
 // blends the 8 texels around u * N - 0.5 with clamp-to-edge.
 template <typename T, bool LINEAR>
@@ -49,10 +68,10 @@ __device__ inline float sampleBrick(const T* __restrict__ vox, const FrameUnifor
     const int nx = U.texDim[0], ny = U.texDim[1], nz = U.texDim[2];
     if (!LINEAR)
     {
-        const int ix = clampi(int(floorf(tp[0] * float(nx))), 0, nx - 1);
-        const int iy = clampi(int(floorf(tp[1] * float(ny))), 0, ny - 1);
-        const int iz = clampi(int(floorf(tp[2] * float(nz))), 0, nz - 1);
-        return loadVoxel(vox, tiledIndex(U.layout, uint32_t(ix), uint32_t(iy), uint32_t(iz)));
+        const uint32_t ix = nearestTexel(tp[0] * float(nx), float(nx - 1));
+        const uint32_t iy = nearestTexel(tp[1] * float(ny), float(ny - 1));
+        const uint32_t iz = nearestTexel(tp[2] * float(nz), float(nz - 1));
+        return loadVoxel(vox, tiledIndexDev(U.layout, ix, iy, iz));
     }
     const float u[3] = {tp[0] * float(nx) - 0.5f, tp[1] * float(ny) - 0.5f,
                         tp[2] * float(nz) - 0.5f};
@@ -69,7 +88,7 @@ __device__ inline float sampleBrick(const T* __restrict__ vox, const FrameUnifor
         i1[a] = clampi(k + 1, 0, n[a] - 1);
     }
     auto at = [&](int x, int y, int z) {
-        return loadVoxel(vox, tiledIndex(U.layout, uint32_t(x), uint32_t(y), uint32_t(z)));
+        return loadVoxel(vox, tiledIndexDev(U.layout, uint32_t(x), uint32_t(y), uint32_t(z)));
     };
     const float c00 = at(i0[0], i0[1], i0[2]) * (1.f - w[0]) + at(i1[0], i0[1], i0[2]) * w[0];
     const float c10 = at(i0[0], i1[1], i0[2]) * (1.f - w[0]) + at(i1[0], i1[1], i0[2]) * w[0];
@@ -79,6 +98,80 @@ __device__ inline float sampleBrick(const T* __restrict__ vox, const FrameUnifor
     const float c1 = c01 * (1.f - w[1]) + c11 * w[1];
     return c0 * (1.f - w[2]) + c1 * w[2];
 }
+
+// calcTexturePositionFromAABBPos (fragRayCast.glsl:72-76) for a sample
+// position: (pos - aabbMin) / (aabbMax - aabbMin) correctly rounded, from the
+// host's IEEE reciprocal of the divisor and one FMA correction step
+// (Markstein), then * (texMax - texMin) + texMin.
+__device__ __forceinline__ void texturePosition(const FrameUniforms& U, const BrickEntry& b,
+                                                const float texExt[3], const float pos[3],
+                                                float tp[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const float x = pos[a] - b.bmin[a];
+        const float q = x * b.rcp[a];
+        const float r = __builtin_fmaf(-q, b.ext[a], x);
+        const float quot = __builtin_fmaf(r, b.rcp[a], q);
+        tp[a] = quot * texExt[a] + U.texMin[a];
+    }
+}
+
+// GL_NEAREST fetch returning the stored voxel unconverted, so that the
+// conversion (the first use) can sit after other work while the load is in
+// flight.
+template <typename T>
+__device__ __forceinline__ T fetchNearestRaw(const T* __restrict__ vox, const FrameUniforms& U,
+                                             const BrickEntry& b, const float texExt[3],
+                                             const float pos[3])
+{
+    float tp[3];
+    texturePosition(U, b, texExt, pos, tp);
+    const uint32_t ix = nearestTexel(tp[0] * float(U.texDim[0]), float(U.texDim[0] - 1));
+    const uint32_t iy = nearestTexel(tp[1] * float(U.texDim[1]), float(U.texDim[1] - 1));
+    const uint32_t iz = nearestTexel(tp[2] * float(U.texDim[2]), float(U.texDim[2] - 1));
+    return vox[tiledIndexDev(U.layout, ix, iy, iz)];
+}
+
+// Classification and compositing of one sample (fragRayCast.glsl:184-204):
+// texture(transferFnTex, density) with linear filtering and clamp to edge,
+// opacity correction, front-to-back "over". Returns true when the ray
+// reached the early-exit opacity.
+struct Composite
+{
+    float nTfF, nTfMax, alphaCorrection, maxA;
+    float acc0, acc1, acc2, acc3;
+
+    __device__ __forceinline__ bool add(const FrameUniforms& U, const float4* tf, float value)
+    {
+        const float density = value * U.mult + U.add;
+        // the two entries are floor(u) and floor(u) + 1 clamped to
+        // [0, nTf - 1] (fmed3 + truncation, as for the voxel)
+        const float u = density * nTfF - 0.5f;
+        const float fl = floorf(u);
+        const float w = u - fl;
+        const int ia = int(__builtin_amdgcn_fmed3f(fl, 0.0f, nTfMax));
+        const int ib = int(__builtin_amdgcn_fmed3f(fl + 1.0f, 0.0f, nTfMax));
+        const float4 ca = tf[ia];
+        const float4 cb = tf[ib];
+        const float sr = ca.x * (1.0f - w) + cb.x * w;
+        const float sg = ca.y * (1.0f - w) + cb.y * w;
+        const float sb = ca.z * (1.0f - w) + cb.z * w;
+        const float sa = ca.w * (1.0f - w) + cb.w * w;
+        // pow(x, c) for x in [1/256, 1] as exp2(c * log2 x) on the hardware
+        // transcendental units (1 ulp each)
+        const float base = 1.0f - fminf(sa, maxA);
+        const float alpha =
+            1.0f - __builtin_amdgcn_exp2f(alphaCorrection * __builtin_amdgcn_logf(base));
+        const float rem = 1.0f - acc3;
+        acc0 += sr * alpha * rem;
+        acc1 += sg * alpha * rem;
+        acc2 += sb * alpha * rem;
+        acc3 += alpha * rem;
+        return acc3 > U.exitA;
+    }
+};
 
 __device__ inline int itemToTile(const FrameUniforms& U, int item)
 {
@@ -145,13 +238,14 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
     intersectBoxInv(U.eye, inv, U.gmin, U.gmax, tnearGlobal, tfarGlobal);
 
     const float stepSize = 1.0f / float(U.nSamples);
-    const float alphaCorrection = float(U.maxSamples) / float(U.nSamples);
-    const float maxA = 1.0f - 1.0f / 256.0f;
     const float texExt[3] = {U.texMax[0] - U.texMin[0], U.texMax[1] - U.texMin[1],
                              U.texMax[2] - U.texMin[2]};
-    const float nTfF = float(U.nTf);
-
-    float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
+    Composite C;
+    C.nTfF = float(U.nTf);
+    C.nTfMax = float(U.nTf - 1);
+    C.alphaCorrection = float(U.maxSamples) / float(U.nSamples);
+    C.maxA = 1.0f - 1.0f / 256.0f;
+    C.acc0 = C.acc1 = C.acc2 = C.acc3 = 0.f;
     bool alive = inside;
 
     const int gtile = U.tileBase + tile;
@@ -165,7 +259,7 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         const BrickEntry b = bricks[bi];
         if (!alive)
             continue;
-        if (acc3 > U.exitA) // imageLoad + discard (fragRayCast.glsl:115-117)
+        if (C.acc3 > U.exitA) // imageLoad + discard (fragRayCast.glsl:115-117)
         {
             alive = false;
             continue;
@@ -218,63 +312,56 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         // A segment lies inside one brick, so it holds at most
         // diagonal * nSamples steps; the guard only bounds the loop should
         // `travel - stepSize` ever round back to `travel`.
+        float travel = segLen;
         int guard = 1 << 22;
-        for (float travel = segLen; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
+        if (!LINEAR)
         {
-            float tp[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
+            // Software pipeline: the voxel of sample i+1 is fetched before
+            // sample i is classified and composited, so the fetch latency
+            // overlaps that arithmetic. Positions advance exactly as the
+            // shader's `pos += step` does; the fetch after the last sample of
+            // a segment (or after an early exit) is discarded, and its texel
+            // index is clamped into the brick like every other.
+            if (travel > 0.0f)
             {
-                // calcTexturePositionFromAABBPos (fragRayCast.glsl:72-76):
-                // (pos - aabbMin) / (aabbMax - aabbMin) correctly rounded, from
-                // the host's IEEE reciprocal of the divisor and one FMA
-                // correction step (Markstein), then * (texMax - texMin) + texMin
-                const float x = pos[a] - b.bmin[a];
-                const float q = x * b.rcp[a];
-                const float r = __builtin_fmaf(-q, b.ext[a], x);
-                const float quot = __builtin_fmaf(r, b.rcp[a], q);
-                tp[a] = quot * texExt[a] + U.texMin[a];
+                T raw = fetchNearestRaw<T>(vox, U, b, texExt, pos);
+                for (;;)
+                {
+                    pos[0] = pos[0] + step[0];
+                    pos[1] = pos[1] + step[1];
+                    pos[2] = pos[2] + step[2];
+                    travel -= stepSize;
+                    --guard;
+                    const T rawNext = fetchNearestRaw<T>(vox, U, b, texExt, pos);
+                    // keep the load above the compositing of this sample
+                    __builtin_amdgcn_sched_barrier(0);
+                    ++nsamp;
+                    if (C.add(U, tf, float(raw)) || !(travel > 0.0f && guard > 0))
+                        break;
+                    raw = rawNext;
+                }
             }
-            const float value = sampleBrick<T, LINEAR>(vox, U, tp);
-            const float density = value * U.mult + U.add;
-            // texture(transferFnTex, density): linear, clamp to edge
-            const float u = density * nTfF - 0.5f;
-            const float fl = floorf(u);
-            const float w = u - fl;
-            const float flc = fminf(fmaxf(fl, -1.0f), nTfF);
-            const int kk = int(flc);
-            const int ia = clampi(kk, 0, U.nTf - 1);
-            const int ib = clampi(kk + 1, 0, U.nTf - 1);
-            const float4 ca = tf[ia];
-            const float4 cb = tf[ib];
-            const float sr = ca.x * (1.0f - w) + cb.x * w;
-            const float sg = ca.y * (1.0f - w) + cb.y * w;
-            const float sb = ca.z * (1.0f - w) + cb.z * w;
-            const float sa = ca.w * (1.0f - w) + cb.w * w;
-            // composite (fragRayCast.glsl:103-111); pow(x, c) for x in
-            // [1/256, 1] as exp2(c * log2 x) on the hardware transcendental
-            // units (1 ulp each)
-            const float base = 1.0f - fminf(sa, maxA);
-            const float alpha =
-                1.0f - __builtin_amdgcn_exp2f(alphaCorrection * __builtin_amdgcn_logf(base));
-            const float rem = 1.0f - acc3;
-            acc0 += sr * alpha * rem;
-            acc1 += sg * alpha * rem;
-            acc2 += sb * alpha * rem;
-            acc3 += alpha * rem;
-            ++nsamp;
-            if (acc3 > U.exitA)
-                break;
-            pos[0] = pos[0] + step[0];
-            pos[1] = pos[1] + step[1];
-            pos[2] = pos[2] + step[2];
+        }
+        else
+        {
+            for (; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
+            {
+                float tp[3];
+                texturePosition(U, b, texExt, pos, tp);
+                ++nsamp;
+                if (C.add(U, tf, sampleBrick<T, true>(vox, U, tp)))
+                    break;
+                pos[0] = pos[0] + step[0];
+                pos[1] = pos[1] + step[1];
+                pos[2] = pos[2] + step[2];
+            }
         }
     }
 
     if (inside)
     {
         // fragTexCopy.glsl into the RGBA8 framebuffer: round(clamp(c) * 255)
-        const float c[4] = {acc0, acc1, acc2, acc3};
+        const float c[4] = {C.acc0, C.acc1, C.acc2, C.acc3};
         uint32_t packed = 0;
 #pragma unroll
         for (int i = 0; i < 4; ++i)

@@ -218,7 +218,9 @@ typedef struct
     int32_t linear_filter;       /* 0: GL_NEAREST (reference default), 1: trilinear */
     int32_t n_clip_planes;       /* 0..6 */
     float clip_planes[24];       /* normal xyz, d */
-    int32_t tile_w, tile_h;      /* wavefront ray tile; 0 = autotuned default */
+    int32_t tile_w, tile_h;      /* wavefront ray tile (w * h = 64); 0, 0 = 8 x 8 */
+    int32_t samples_per_thread;  /* voxel fetches in flight per ray, 1..3 (NEAREST);
+                                    0 = 2 */
 } lvr_render_params;
 
 /* Render one frame of the ordered bricks (fragRayCast.glsl:113-207 per brick,

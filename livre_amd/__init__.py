@@ -97,7 +97,8 @@ class RenderParams(ctypes.Structure):
                 ("linear_filter", ctypes.c_int32),
                 ("n_clip_planes", ctypes.c_int32),
                 ("clip_planes", ctypes.c_float * 24),
-                ("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32)]
+                ("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
+                ("samples_per_thread", ctypes.c_int32)]
 
 
 class NoiseParams(ctypes.Structure):
@@ -285,7 +286,8 @@ def field_brick_host(vi, params, node_id):
 
 
 def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
-                  max_samples_per_ray=32, clip_planes=(), tile=(0, 0)):
+                  max_samples_per_ray=32, clip_planes=(), tile=(0, 0),
+                  samples_per_thread=0):
     p = RenderParams()
     p.n_samples_per_ray = n_samples_per_ray
     p.max_samples_per_ray = max_samples_per_ray
@@ -296,6 +298,7 @@ def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
         for j in range(4):
             p.clip_planes[4 * i + j] = pl[j]
     p.tile_w, p.tile_h = tile
+    p.samples_per_thread = samples_per_thread
     return p
 
 

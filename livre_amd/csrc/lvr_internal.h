@@ -123,6 +123,7 @@ struct FrameUniforms
     int32_t tileW, tileH, tilesX, tilesY;
     int32_t nBricks;
     int32_t linear;
+    int32_t depth; // voxel fetches in flight per ray (NEAREST march)
     int32_t nClip;
     float clip[24];
     uint64_t slotStride;

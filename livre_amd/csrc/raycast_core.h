@@ -118,20 +118,16 @@ __device__ __forceinline__ void texturePosition(const FrameUniforms& U, const Br
     }
 }
 
-// GL_NEAREST fetch returning the stored voxel unconverted, so that the
-// conversion (the first use) can sit after other work while the load is in
-// flight.
-template <typename T>
-__device__ __forceinline__ T fetchNearestRaw(const T* __restrict__ vox, const FrameUniforms& U,
-                                             const BrickEntry& b, const float texExt[3],
-                                             const float pos[3])
+// Pool index of the GL_NEAREST texel of a sample position.
+__device__ __forceinline__ uint32_t nearestIndex(const FrameUniforms& U, const BrickEntry& b,
+                                                 const float texExt[3], const float pos[3])
 {
     float tp[3];
     texturePosition(U, b, texExt, pos, tp);
     const uint32_t ix = nearestTexel(tp[0] * float(U.texDim[0]), float(U.texDim[0] - 1));
     const uint32_t iy = nearestTexel(tp[1] * float(U.texDim[1]), float(U.texDim[1] - 1));
     const uint32_t iz = nearestTexel(tp[2] * float(U.texDim[2]), float(U.texDim[2] - 1));
-    return vox[tiledIndexDev(U.layout, ix, iy, iz)];
+    return tiledIndexDev(U.layout, ix, iy, iz);
 }
 
 // Classification and compositing of one sample (fragRayCast.glsl:184-204):
@@ -182,10 +178,87 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
     return (tx < U.tilesX && ty < U.tilesY) ? ty * U.tilesX + tx : -1;
 }
 
+// The march of one ray through one brick segment with GL_NEAREST fetches
+// (fragRayCast.glsl:184-204): `travel` starts at the segment length and
+// drops by stepSize per composite step; a segment lies inside one brick, so it
+// holds at most diagonal * nSamples steps, and the guard only bounds the loop
+// should `travel - stepSize` ever round back to `travel`.
+//
+// D samples per thread are in the pipeline: while sample k is classified and
+// composited, the voxels of samples k+1 .. k+D-1 are being fetched. The fetch
+// position runs D - 1 steps ahead of the composited sample and advances by
+// exactly the shader's `pos += step`, so sample k reads the voxel of the k-th
+// position. Raw voxels sit in a ring of D - 1 registers: the oldest is
+// converted, then its register takes the next load, so no register copy waits
+// on a load. Fetches past the last composited sample (end of segment or early
+// exit) are discarded; their texel indices are clamped into the brick like
+// every other.
+template <typename T, int D>
+__device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const FrameUniforms& U,
+                                             const BrickEntry& b, const float texExt[3],
+                                             float pos[3], const float step[3], float travel,
+                                             float stepSize, Composite& C, const float4* tf,
+                                             uint32_t& nsamp)
+{
+    constexpr int S = D > 1 ? D - 1 : 1;
+    if (!(travel > 0.0f))
+        return;
+    int guard = 1 << 22;
+    T raw[S];
+    raw[0] = vox[nearestIndex(U, b, texExt, pos)];
+#pragma unroll
+    for (int i = 1; i < S; ++i)
+    {
+        pos[0] = pos[0] + step[0];
+        pos[1] = pos[1] + step[1];
+        pos[2] = pos[2] + step[2];
+        raw[i] = vox[nearestIndex(U, b, texExt, pos)];
+    }
+    for (bool more = true; more;)
+    {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+        {
+            travel -= stepSize;
+            --guard;
+            uint32_t next = 0;
+            if (D > 1)
+            {
+                pos[0] = pos[0] + step[0];
+                pos[1] = pos[1] + step[1];
+                pos[2] = pos[2] + step[2];
+                next = nearestIndex(U, b, texExt, pos);
+            }
+            const float value = float(raw[s]);
+            if (D > 1)
+            {
+                // the oldest voxel is converted before its register takes the
+                // next load; the load stays above this sample's compositing
+                __builtin_amdgcn_sched_barrier(0);
+                raw[s] = vox[next];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            ++nsamp;
+            if (C.add(U, tf, value) || !(travel > 0.0f && guard > 0))
+            {
+                more = false;
+                break;
+            }
+            if (D == 1)
+            {
+                pos[0] = pos[0] + step[0];
+                pos[1] = pos[1] + step[1];
+                pos[2] = pos[2] + step[2];
+                raw[0] = vox[nearestIndex(U, b, texExt, pos)];
+            }
+        }
+    }
+}
+
 // One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
 // ordered list, composited in registers, then the RGBA8 store. `tf` is the
 // transfer function (LDS or global, resolved at the call site).
-template <typename T, bool LINEAR>
+template <typename T, bool LINEAR, int D>
 __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int lane,
                                          const BrickEntry* __restrict__ bricks,
                                          const int32_t* __restrict__ offsets,
@@ -309,42 +382,15 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         float pos[3] = {rs[0], rs[1], rs[2]};
         const T* vox = reinterpret_cast<const T*>(U.pool + size_t(b.slot) * U.slotStride);
 
-        // A segment lies inside one brick, so it holds at most
-        // diagonal * nSamples steps; the guard only bounds the loop should
-        // `travel - stepSize` ever round back to `travel`.
-        float travel = segLen;
-        int guard = 1 << 22;
         if (!LINEAR)
-        {
-            // Software pipeline: the voxel of sample i+1 is fetched before
-            // sample i is classified and composited, so the fetch latency
-            // overlaps that arithmetic. Positions advance exactly as the
-            // shader's `pos += step` does; the fetch after the last sample of
-            // a segment (or after an early exit) is discarded, and its texel
-            // index is clamped into the brick like every other.
-            if (travel > 0.0f)
-            {
-                T raw = fetchNearestRaw<T>(vox, U, b, texExt, pos);
-                for (;;)
-                {
-                    pos[0] = pos[0] + step[0];
-                    pos[1] = pos[1] + step[1];
-                    pos[2] = pos[2] + step[2];
-                    travel -= stepSize;
-                    --guard;
-                    const T rawNext = fetchNearestRaw<T>(vox, U, b, texExt, pos);
-                    // keep the load above the compositing of this sample
-                    __builtin_amdgcn_sched_barrier(0);
-                    ++nsamp;
-                    if (C.add(U, tf, float(raw)) || !(travel > 0.0f && guard > 0))
-                        break;
-                    raw = rawNext;
-                }
-            }
-        }
+            marchNearest<T, D>(vox, U, b, texExt, pos, step, segLen, stepSize, C, tf, nsamp);
         else
         {
-            for (; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
+            // A segment lies inside one brick, so it holds at most
+            // diagonal * nSamples steps; the guard only bounds the loop should
+            // `travel - stepSize` ever round back to `travel`.
+            int guard = 1 << 22;
+            for (float travel = segLen; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
             {
                 float tp[3];
                 texturePosition(U, b, texExt, pos, tp);

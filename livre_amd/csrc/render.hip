@@ -144,7 +144,7 @@ __device__ inline int nextItem(unsigned long long* counter, int lane)
 // Tiles differ by orders of magnitude in work (empty sky vs rays through the
 // whole volume), so pulling them dynamically keeps every CU busy to the end.
 // Two 1024-ray workgroups per CU need <= 64 VGPRs (8 wavefronts per SIMD).
-template <typename T, bool LINEAR>
+template <typename T, bool LINEAR, int D>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void raycastKernel(
     const FrameUniforms U, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
             break;
         const int tile = itemToTile(U, item);
         if (tile >= 0)
-            castTile<T, LINEAR>(U, tile, lane, bricks, offsets, counts, lists, sTf, nsamp);
+            castTile<T, LINEAR, D>(U, tile, lane, bricks, offsets, counts, lists, sTf, nsamp);
     }
     const unsigned long long s = waveSum(nsamp);
     if (lane == 0 && s)
@@ -179,6 +179,7 @@ constexpr int kBatchTf = 256;
 // frame changes (tables above kBatchTf entries are read from global memory),
 // and dispatches on the frame's voxel type. Composite steps are counted per
 // frame.
+template <int D>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void batchKernel(
     const FrameUniforms* __restrict__ reqs, int nReq, int totalItems,
     const BrickEntry* __restrict__ bricks, const int32_t* __restrict__ offsets,
@@ -229,21 +230,21 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         {
         case LVR_DT_UINT8:
             if (lds)
-                castTile<uint8_t, false>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                castTile<uint8_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
-                castTile<uint8_t, false>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                castTile<uint8_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
             break;
         case LVR_DT_UINT16:
             if (lds)
-                castTile<uint16_t, false>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                castTile<uint16_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
-                castTile<uint16_t, false>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                castTile<uint16_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
             break;
         default:
             if (lds)
-                castTile<float, false>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                castTile<float, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
-                castTile<float, false>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                castTile<float, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
             break;
         }
     }
@@ -258,26 +259,31 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     }
 }
 
+template <typename T, bool LINEAR, int D>
+void launchRaycastAs(int blocks, size_t lds, hipStream_t st, const FrameUniforms& U,
+                     const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
+                     const int32_t* lists, unsigned long long* counters)
+{
+    const int threads = kSuper * kSuper * kWave;
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, LINEAR, D>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+    hipLaunchKernelGGL((raycastKernel<T, LINEAR, D>), dim3(blocks), dim3(threads), lds, st, U,
+                       bricks, offsets, counts, lists, counters);
+}
+
 template <typename T>
 void launchRaycast(int blocks, size_t lds, hipStream_t st, const FrameUniforms& U,
                    const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
                    const int32_t* lists, unsigned long long* counters)
 {
-    const int threads = kSuper * kSuper * kWave;
     if (U.linear)
-    {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        hipLaunchKernelGGL((raycastKernel<T, true>), dim3(blocks), dim3(threads), lds, st, U,
-                           bricks, offsets, counts, lists, counters);
-    }
+        launchRaycastAs<T, true, 1>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
+    else if (U.depth == 1)
+        launchRaycastAs<T, false, 1>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
+    else if (U.depth == 3)
+        launchRaycastAs<T, false, 3>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
     else
-    {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-        hipLaunchKernelGGL((raycastKernel<T, false>), dim3(blocks), dim3(threads), lds, st, U,
-                           bricks, offsets, counts, lists, counters);
-    }
+        launchRaycastAs<T, false, 2>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
 }
 } // namespace
 
@@ -351,6 +357,9 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     }
     if (tw <= 0 || th <= 0 || tw * th != kWave)
         return fail(ctx, LVR_EINVAL, "ray tile must hold 64 rays");
+    const int depth = params->samples_per_thread == 0 ? 2 : params->samples_per_thread;
+    if (depth < 1 || depth > 3)
+        return fail(ctx, LVR_EINVAL, "samples per thread must be 1, 2 or 3");
 
     std::memset(U, 0, sizeof *U);
     std::memcpy(U->invProj, f->inv_proj, sizeof U->invProj);
@@ -388,6 +397,7 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     U->tilesY = (H + th - 1) / th;
     U->nBricks = n;
     U->linear = params->linear_filter ? 1 : 0;
+    U->depth = depth;
     U->nClip = params->n_clip_planes;
     std::memcpy(U->clip, params->clip_planes, sizeof U->clip);
     U->slotStride = ctx->slotStride;
@@ -879,6 +889,12 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
         b->ranges.resize(nb);
         return rc;
     }
+    if (!b->reqs.empty() && U.depth != b->reqs.front().depth)
+    {
+        b->bricks.resize(nb);
+        b->ranges.resize(nb);
+        return bfail(b, LVR_EINVAL, "samples per thread differs from the batch's");
+    }
     U.request = int32_t(b->reqs.size());
     b->tfOffset.push_back(b->tf.size());
     for (int i = 0; i < n_tf; ++i)
@@ -936,7 +952,9 @@ int lvr_batch_run(lvr_batch* b)
     }
 
     // the launch sequence depends on these sizes and buffers only
+    const int depth = b->reqs.front().depth;
     std::vector<int64_t> key = {nReq,
+                                depth,
                                 int64_t(b->bricks.size()),
                                 int64_t(b->tf.size()),
                                 tiles,
@@ -973,9 +991,16 @@ int lvr_batch_run(lvr_batch* b)
         hipLaunchKernelGGL(binFillKernel, dim3(binBlocks), dim3(256), 0, st, b->d_reqs, nReq,
                            b->d_ranges, tiles, b->d_offsets, b->d_lists);
         const int blocks = std::max(1, std::min(items / (kSuper * kSuper), 2 * b->numCUs));
-        hipLaunchKernelGGL(batchKernel, dim3(blocks), dim3(kSuper * kSuper * kWave), 0, st,
-                           b->d_reqs, nReq, items, b->d_bricks, b->d_offsets, b->d_counts,
-                           b->d_lists, b->d_counters);
+        const dim3 grid(blocks), block(kSuper * kSuper * kWave);
+        if (depth == 1)
+            hipLaunchKernelGGL(batchKernel<1>, grid, block, 0, st, b->d_reqs, nReq, items,
+                               b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);
+        else if (depth == 3)
+            hipLaunchKernelGGL(batchKernel<3>, grid, block, 0, st, b->d_reqs, nReq, items,
+                               b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);
+        else
+            hipLaunchKernelGGL(batchKernel<2>, grid, block, 0, st, b->d_reqs, nReq, items,
+                               b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);
         return hipGetLastError();
     };
 

@@ -102,18 +102,144 @@ def cpu_baseline(livre_amd, scenes, vi, ctx, tf, target_s):
                          samples, elapsed)}
 
 
+def batch_workload(args):
+    """64 uneven requests over four volumes: the plain renderer (one
+    lvr_render + wait per request, default launch shape) against one
+    persistent batched launch (autotuned launch shape, graph-replayed), then
+    16 concurrent 1080p sessions of the 2048^3 volume the same two ways.
+    Selection and ordering are done before either timed region."""
+    import livre_amd
+    from livre_amd import autotune, scenes, workloads
+    livre_amd.lib()
+    vols = {}
+    for name in workloads.BATCH_VOLUMES:
+        vols[name] = workloads.Volume(name)
+        log("%s: %d bricks generated in %.1f s" % (name, len(vols[name].ids),
+                                                    vols[name].gen_s))
+
+    def plain_pass(items):
+        total = 0
+        for ctx, fr, w, h, ordered, params, tf, rng in items:
+            ctx.set_tf(tf, rng)
+            ctx.render(fr, w, h, ordered, params)
+            total += ctx.samples()  # waits for the frame
+        return total
+
+    def batch_pass(batch, items, tuned):
+        for (ctx, fr, w, h, ordered, _p, tf, rng), tp in zip(items, tuned):
+            batch.add(ctx, fr, w, h, ordered, tp, tf, rng)
+        batch.run()
+        return batch.wait()
+
+    def compare(items, reps, label):
+        plain_pass(items)  # warm-up
+        t0 = time.perf_counter()
+        plain_samples = sum(plain_pass(items) for _ in range(reps))
+        t_plain = time.perf_counter() - t0
+        tuner = autotune.Tuner()
+        t0 = time.perf_counter()
+        tuned = tuner.tune_batch([(c, f, w, h, o, p) for c, f, w, h, o, p, _t, _r in items])
+        t_tune = time.perf_counter() - t0
+        with livre_amd.Batch() as batch:
+            batch_pass(batch, items, tuned)  # warm-up and graph capture
+            gpu_ms, batch_samples = [], 0
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                st = batch_pass(batch, items, tuned)
+                gpu_ms.append(st.gpu_ms)
+                batch_samples += st.samples
+            t_batch = time.perf_counter() - t0
+            # every batched frame equals the plain frame of the same request
+            same = True
+            for i in range(0, len(items), max(1, len(items) // 8)):
+                ctx, fr, w, h, ordered, params, tf, rng = items[i]
+                ctx.set_tf(tf, rng)
+                ctx.render(fr, w, h, ordered, params)
+                img, smp = ctx.read_frame()
+                bimg, bsmp = batch.read(i)
+                same = same and smp == bsmp and bool(np.array_equal(img, bimg))
+            graph = (st.graph_captures, st.graph_launches)
+        plain = plain_samples / t_plain / 1e9
+        batched = batch_samples / t_batch / 1e9
+        log("%s: plain %.1f Gsamples/s, batched %.1f Gsamples/s (x%.2f), tuning %.1f s"
+            % (label, plain, batched, batched / plain, t_tune))
+        return {"requests": len(items), "reps": reps,
+                "samples_per_pass": batch_samples // reps,
+                "plain_gsamples_per_s": plain, "plain_ms_per_pass": t_plain / reps * 1e3,
+                "batched_gsamples_per_s": batched, "batched_ms_per_pass": t_batch / reps * 1e3,
+                "batch_gpu_ms_mean": float(np.mean(gpu_ms)), "speedup": batched / plain,
+                "frames_identical": same, "graph_captures_launches": list(graph),
+                "tuned_classes": tuner.tuned, "tuning_s": t_tune,
+                "launch_shapes": sorted({(t.tile_w, t.tile_h, t.samples_per_thread)
+                                         for t in tuned})}
+
+    reqs = workloads.batch_requests(64, seed=11)
+    items = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        vol = vols[r["volume"]]
+        fr, ordered = workloads.prepare(r, vol)
+        items.append((vol.ctx, fr, r["width"], r["height"], ordered,
+                      livre_amd.render_params(workloads.samples_per_ray(r, vol)), r["tf"],
+                      vol.range))
+    t_prep = time.perf_counter() - t0
+    uneven = compare(items, args.steps, "64 uneven requests")
+    uneven["selection_order_ms_per_pass"] = t_prep * 1e3
+    uneven["pixels"] = int(sum(r["width"] * r["height"] for r in reqs))
+
+    big = vols["noise2048_u16"]
+    sess = []
+    tf = scenes.piecewise_tf(TF_ENTRIES, seed=4, points=8, max_alpha=0.35)
+    for k in range(16):
+        mv, proj, fr = camera(livre_amd, scenes, k * (FRAMES // 16))
+        vis = big.ctx.select_visibles(fr, H, 1.0, 0, big.vi.depth - 1)
+        sess.append((big.ctx, fr, W, H, livre_amd.order(big.vi, fr, vis),
+                     livre_amd.render_params(N_SAMPLES), tf, (0.0, 65535.0)))
+    sessions = compare(sess, max(2, args.steps // 2), "16 sessions at 1080p")
+    for v in vols.values():
+        v.close()
+    result = {
+        "metric": "Gsamples/s, batched uneven requests in one persistent launch",
+        "value": uneven["batched_gsamples_per_s"], "unit": "Gsamples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "u8/u16/f32", "data": "synthetic: procedural volumes generated on the device",
+        "config": {"workload": "64 requests, square viewports log-uniform in [256^2, 2048^2], "
+                               "volumes 512^3 u8 blobs / 1024^3 u16 noise / 1536^3 f32 sines "
+                               "(48^3 bricks) / 2048^3 u16 noise, 256-entry TF each, rates "
+                               "{0.5,1,2} samples/voxel, sse 1; plus 16 x 1080p sessions of "
+                               "the 2048^3 volume (4096-entry TF)",
+                   "plain": "lvr_render + wait per request, default 8x8 tile, 2 samples "
+                            "per thread",
+                   "batched": "one lvr_batch launch (graph replay), autotuned tiles and "
+                              "samples per thread"},
+        "uneven_requests": uneven,
+        "sessions_1080p": sessions,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=FRAMES)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("render", "batch"), default="render",
+                    help="render: the 1080p single-volume frame loop (default); "
+                         "batch: uneven requests, plain vs one batched launch")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the brick pool, kernels and measurements are "
               "single-device; multi-GPU sharding is a design only", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        if args.steps is None:
+            args.steps = 4
+        batch_workload(args)
+        return
+    if args.steps is None:
+        args.steps = FRAMES
 
     import livre_amd
     from livre_amd import scenes

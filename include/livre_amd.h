@@ -256,6 +256,24 @@ typedef struct
 int lvr_set_timing(lvr_ctx* ctx, int enable);
 int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out);
 
+/* Autotuner of the launch shape (no reference counterpart: the GL path has
+ * no tunable launch). Renders the given frame with every ray tile in
+ * {8x8, 16x4, 4x16, 32x2, 2x32} and, for NEAREST, every samples_per_thread in
+ * {1, 2, 3}; times the ray-cast kernel with HIP events (median of reps) and
+ * writes the fastest tile_w, tile_h and samples_per_thread into *params.
+ * All candidates produce the same image. The timings of the candidates are
+ * copied to results (up to cap entries); *n_results = number of candidates.
+ * Leaves the last candidate's frame in the framebuffer. */
+typedef struct
+{
+    int32_t tile_w, tile_h, samples_per_thread;
+    float raycast_ms;
+} lvr_tune_entry;
+int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                 const uint64_t* ordered_ids, int n, int reps,
+                 lvr_render_params* params, lvr_tune_entry* results, int cap,
+                 int* n_results);
+
 /* --------------------------------------------------------------- batches */
 /* Many frames (requests) of different volumes, viewports, cameras and
  * transfer functions rendered by one persistent launch: binning of every

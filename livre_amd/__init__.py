@@ -31,7 +31,7 @@ EXPORTS = (
     "lvr_field_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
-    "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes",
+    "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes", "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
     "lvr_batch_set_graph", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
     "lvr_batch_wait", "lvr_batch_read",
@@ -101,6 +101,11 @@ class RenderParams(ctypes.Structure):
                 ("samples_per_thread", ctypes.c_int32)]
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
+                ("samples_per_thread", ctypes.c_int32), ("raycast_ms", ctypes.c_float)]
+
+
 class NoiseParams(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("octaves", ctypes.c_int32),
                 ("base_cells", ctypes.c_int32), ("remap", ctypes.c_uint16 * 4097)]
@@ -143,6 +148,8 @@ def lib():
     L.lvr_brick_evict.argtypes = [c.c_void_p, c.c_uint64]
     L.lvr_brick_resident.argtypes = [c.c_void_p, c.c_uint64]
     L.lvr_brick_download.argtypes = [c.c_void_p, c.c_uint64, c.c_void_p, c.c_size_t]
+    L.lvr_autotune.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int, c.c_int,
+                               P(RenderParams), P(TuneEntry), c.c_int, P(c.c_int)]
     L.lvr_noise_params_init.argtypes = [P(NoiseParams), c.c_uint64, c.c_int, c.c_int,
                                         c.c_float]
     L.lvr_noise_brick_host.argtypes = [P(VolumeInfo), P(NoiseParams), c.c_uint64,
@@ -408,6 +415,22 @@ class Context:
         self._check(lib().lvr_render(self._h, ctypes.byref(fr), vp, pord, len(ordered),
                                      ctypes.byref(params)), "render")
         self.width, self.height = width, height
+
+    def autotune(self, fr, w, h, ordered, params, reps=3):
+        """Fastest ray tile and samples per thread for this frame: returns a
+        copy of params with them set, and the list of (tile_w, tile_h,
+        samples_per_thread, raycast_ms) candidates."""
+        ordered, pids = _u64(ordered)
+        vp = (ctypes.c_int * 4)(0, 0, w, h)
+        out = RenderParams.from_buffer_copy(params)
+        res = (TuneEntry * 32)()
+        n = ctypes.c_int()
+        self._check(lib().lvr_autotune(self._h, ctypes.byref(fr), vp, pids, len(ordered), reps,
+                                       ctypes.byref(out), res, 32, ctypes.byref(n)), "autotune")
+        self.width, self.height = w, h
+        cands = [(r.tile_w, r.tile_h, r.samples_per_thread, r.raycast_ms)
+                 for r in res[:min(n.value, 32)]]
+        return out, cands
 
     def read_frame(self):
         img = np.zeros((self.height, self.width, 4), np.uint8)

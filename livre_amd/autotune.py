@@ -1,0 +1,103 @@
+"""Launch-shape autotuner with a persistent cache.
+
+The ray tile (how a wavefront's 64 rays cover the screen) and the samples per
+thread (voxel fetches in flight per ray) change only the speed of a frame,
+never its pixels. `Tuner` picks them per request class -- voxel type,
+viewport size (log2 of the pixel count), sampling rate (samples per voxel of
+the finest level, log2) and filter -- by timing every candidate on the first
+frame of that class (`lvr_autotune`), and keeps the choices in a JSON file so
+that later runs start tuned. The package ships the table measured on MI355X
+(tune_gfx950.json beside this module); the user cache (LIVRE_AMD_TUNE_CACHE,
+default ~/.cache/livre_amd/tune_gfx950.json) takes precedence over it and
+receives every class tuned later.
+
+A batch renders with one samples-per-thread value for all of its requests
+(it selects the kernel), so `tune_batch` gives every request its class's
+ray tile and the batch the samples-per-thread value that the classes chose
+most often, weighted by pixels.
+"""
+import json
+import math
+import os
+
+from . import RenderParams, lib
+
+ARCH = "gfx950"
+SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_%s.json" % ARCH)
+
+
+def default_cache_path():
+    return os.environ.get("LIVRE_AMD_TUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "livre_amd",
+                                       "tune_%s.json" % ARCH))
+
+
+def request_class(vi, width, height, params):
+    """Key of the class a frame belongs to."""
+    vmax = max(vi.voxels[0], vi.voxels[1], vi.voxels[2])
+    rate = params.n_samples_per_ray / float(vmax)
+    return "%s/dt%d/px%d/rate%+d/%s" % (
+        ARCH, vi.data_type, int(round(math.log2(max(1, width * height)))),
+        int(round(math.log2(max(rate, 1e-6)))), "linear" if params.linear_filter else "nearest")
+
+
+class Tuner:
+    def __init__(self, path=None, reps=3):
+        self.path = path or default_cache_path()
+        self.reps = reps
+        self.tuned = 0  # classes tuned by this instance (cache misses)
+        self.table = self._load(SHIPPED)
+        self.table.update(self._load(self.path))
+
+    @staticmethod
+    def _load(path):
+        try:
+            with open(path) as f:
+                table = json.load(f)
+            return table if isinstance(table, dict) else {}
+        except (OSError, ValueError):
+            return {}
+
+    def save(self):
+        os.makedirs(os.path.dirname(self.path) or ".", exist_ok=True)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.table, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
+
+    def choice(self, key):
+        return self.table.get(key)
+
+    def params_for(self, ctx, fr, width, height, ordered, params):
+        """params with the tuned tile and samples per thread of the frame's
+        class; tunes the class on this frame if it is not cached."""
+        lib()
+        key = request_class(ctx.vi, width, height, params)
+        c = self.table.get(key)
+        if c is None:
+            tuned, cands = ctx.autotune(fr, width, height, ordered, params, self.reps)
+            c = {"tile": [tuned.tile_w, tuned.tile_h],
+                 "samples_per_thread": tuned.samples_per_thread,
+                 "candidates": [list(x) for x in cands]}
+            self.table[key] = c
+            self.tuned += 1
+            self.save()
+        out = RenderParams.from_buffer_copy(params)
+        out.tile_w, out.tile_h = c["tile"]
+        out.samples_per_thread = c["samples_per_thread"]
+        return out
+
+    def tune_batch(self, requests):
+        """requests: list of (ctx, fr, width, height, ordered, params).
+        Returns one RenderParams per request: the class's tile, and the
+        batch-wide samples per thread."""
+        tuned = [self.params_for(*r) for r in requests]
+        votes = {}
+        for (_ctx, _fr, w, h, _o, _p), t in zip(requests, tuned):
+            if not t.linear_filter:
+                votes[t.samples_per_thread] = votes.get(t.samples_per_thread, 0) + w * h
+        if votes:
+            depth = max(votes.items(), key=lambda kv: kv[1])[0]
+            for t in tuned:
+                t.samples_per_thread = depth
+        return tuned

@@ -1,0 +1,77 @@
+// Autotuner of the ray-cast launch shape: the ray tile (how a wavefront's 64
+// rays cover the screen) and the samples per thread (voxel fetches in flight
+// per ray). Every candidate renders the caller's frame; the ray-cast kernel is
+// timed with HIP events on the context stream, median of `reps` launches, and
+// the fastest candidate is written back into the parameters. Results are
+// bit-identical for every candidate, so the choice only affects speed.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "lvr_internal.h"
+
+namespace
+{
+const int kTiles[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}};
+} // namespace
+
+extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                            const uint64_t* ordered_ids, int n, int reps,
+                            lvr_render_params* params, lvr_tune_entry* results, int cap,
+                            int* n_results)
+{
+    if (!ctx || !f || !viewport || !params || reps < 1 || reps > 64 || cap < 0 ||
+        (cap > 0 && !results))
+        return LVR_EINVAL;
+    const bool wasTiming = ctx->timing;
+    int rc = lvr_set_timing(ctx, 1);
+    if (rc)
+        return rc;
+    const int depths = params->linear_filter ? 1 : 3; // depth applies to NEAREST
+    std::vector<lvr_tune_entry> all;
+    for (const auto& t : kTiles)
+        for (int d = 1; d <= depths; ++d)
+        {
+            lvr_render_params p = *params;
+            p.tile_w = t[0];
+            p.tile_h = t[1];
+            p.samples_per_thread = params->linear_filter ? 0 : d;
+            std::vector<float> ms;
+            for (int r = 0; r < reps && rc == LVR_OK; ++r)
+            {
+                rc = lvr_render(ctx, f, viewport, ordered_ids, n, &p);
+                lvr_frame_stats st;
+                if (rc == LVR_OK)
+                    rc = lvr_frame_stats_get(ctx, &st);
+                if (rc == LVR_OK)
+                    ms.push_back(st.raycast_ms);
+            }
+            if (rc)
+                break;
+            std::nth_element(ms.begin(), ms.begin() + ms.size() / 2, ms.end());
+            lvr_tune_entry e;
+            std::memset(&e, 0, sizeof e);
+            e.tile_w = p.tile_w;
+            e.tile_h = p.tile_h;
+            e.samples_per_thread = p.samples_per_thread;
+            e.raycast_ms = ms[ms.size() / 2];
+            all.push_back(e);
+        }
+    if (!wasTiming)
+        (void)lvr_set_timing(ctx, 0);
+    if (rc)
+        return rc;
+    const auto best = std::min_element(all.begin(), all.end(),
+                                       [](const lvr_tune_entry& a, const lvr_tune_entry& b) {
+                                           return a.raycast_ms < b.raycast_ms;
+                                       });
+    params->tile_w = best->tile_w;
+    params->tile_h = best->tile_h;
+    params->samples_per_thread = best->samples_per_thread;
+    const int k = std::min(cap, int(all.size()));
+    for (int i = 0; i < k; ++i)
+        results[i] = all[size_t(i)];
+    if (n_results)
+        *n_results = int(all.size());
+    return LVR_OK;
+}

@@ -1,0 +1,99 @@
+"""Synthetic workloads of the measured configurations, shared by bench.py and
+the tests: the volumes (generated on the device straight into the brick
+pool by the procedural data sources), seeded request lists and camera paths.
+
+Volumes
+  blobs512_u8    512^3 uint8, 64 anisotropic Gaussian blobs + 10% noise, seed 5
+  noise1024_u16  1024^3 uint16, 5-octave value noise, log-normal remap, seed 6
+  sines1536_f32  1536^3 float32, mean of 8 low-frequency sines in [-1, 1],
+                 seed 7, 48^3 bricks
+  noise2048_u16  2048^3 uint16, 5-octave value noise, seed 3 (the 1080p
+                 single-volume workload)
+All use a 1-voxel halo and 64^3 bricks unless stated.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import (DT_FLOAT, DT_UINT8, DT_UINT16, FIELD_BLOBS, FIELD_SINES, Context,
+               field_params, frustum, noise_params, slot_bytes, volume_info)
+from . import scenes
+
+VOLUMES = {
+    "blobs512_u8": dict(voxels=512, block=64, dtype=DT_UINT8, source=("blobs", 64, 5, 0.1)),
+    "noise1024_u16": dict(voxels=1024, block=64, dtype=DT_UINT16, source=("noise", 6)),
+    "sines1536_f32": dict(voxels=1536, block=48, dtype=DT_FLOAT, source=("sines", 8, 7)),
+    "noise2048_u16": dict(voxels=2048, block=64, dtype=DT_UINT16, source=("noise", 3)),
+}
+BATCH_VOLUMES = ("blobs512_u8", "noise1024_u16", "sines1536_f32", "noise2048_u16")
+
+
+class Volume:
+    """A volume resident in its own brick pool (every brick of every level)."""
+
+    def __init__(self, name, extra_slots=64, pool_bricks=None):
+        spec = VOLUMES[name]
+        self.name = name
+        self.vi = volume_info(spec["voxels"], spec["block"], 1, spec["dtype"])
+        self.ids = scenes.all_nodes(self.vi)
+        self.range = scenes.type_range(spec["dtype"])
+        self.vmax = spec["voxels"]
+        n = pool_bricks if pool_bricks is not None else len(self.ids) + extra_slots
+        self.ctx = Context(self.vi, slot_bytes(self.vi) * n)
+        t0 = time.perf_counter()
+        src = spec["source"]
+        if src[0] == "noise":
+            self.ctx.generate_noise(noise_params(src[1], octaves=5, base_cells=8, gain=1.5),
+                                    self.ids)
+        elif src[0] == "blobs":
+            self.ctx.generate_field(field_params(FIELD_BLOBS, src[1], src[2], src[3]),
+                                    self.ids)
+        else:
+            self.ctx.generate_field(field_params(FIELD_SINES, src[1], src[2]), self.ids)
+        self.ctx.sync()
+        self.gen_s = time.perf_counter() - t0
+
+    def close(self):
+        self.ctx.close()
+
+
+def look_from(eye, width, height, fovy=60.0):
+    """Camera at eye looking at the volume centre (up = +y unless the view is
+    nearly vertical)."""
+    d = np.asarray(eye, np.float64)
+    up = (0.0, 1.0, 0.0) if abs(d[1]) < 0.98 * np.linalg.norm(d) else (0.0, 0.0, 1.0)
+    mv, proj = scenes.camera(tuple(d), width, height, fovy, up=up)
+    return mv, proj, frustum(mv, proj)
+
+
+def batch_requests(n=64, seed=11, volumes=BATCH_VOLUMES):
+    """Uneven requests: square viewports with log-uniform area in
+    [256^2, 2048^2], a volume drawn uniformly, a random view direction at
+    radius U[1.5, 3], a seeded 256-entry transfer function and a sampling rate
+    from {0.5, 1, 2} samples per voxel of the finest level."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        area = math.exp(rng.uniform(math.log(256.0 ** 2), math.log(2048.0 ** 2)))
+        side = int(round(math.sqrt(area)))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        out.append(dict(volume=volumes[int(rng.integers(len(volumes)))], width=side,
+                        height=side, eye=tuple(d * rng.uniform(1.5, 3.0)),
+                        tf=scenes.piecewise_tf(256, seed=1000 + i, points=6, max_alpha=0.5),
+                        rate=float(rng.choice([0.5, 1.0, 2.0]))))
+    return out
+
+
+def prepare(request, vol, sse=1.0):
+    """Camera, visible set (device SSE selection) and front-to-back order of
+    one request."""
+    from . import order
+    mv, proj, fr = look_from(request["eye"], request["width"], request["height"])
+    vis = vol.ctx.select_visibles(fr, request["height"], sse, 0, vol.vi.depth - 1)
+    return fr, order(vol.vi, fr, vis)
+
+
+def samples_per_ray(request, vol):
+    return max(1, int(round(request["rate"] * vol.vmax)))

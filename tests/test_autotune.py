@@ -1,0 +1,91 @@
+"""Autotuner: request classes, the cache (shipped table + user file), the
+batch-wide samples-per-thread vote (CPU, with a stand-in context), and on the
+GPU that lvr_autotune times every candidate and that every candidate renders
+the same frame."""
+import json
+
+import numpy as np
+import pytest
+
+import livre_amd
+from livre_amd import autotune
+
+
+class FakeCtx:
+    """Stands in for Context.autotune: returns a fixed choice per dtype."""
+
+    def __init__(self, vi, choice):
+        self.vi = vi
+        self.choice = choice
+        self.calls = 0
+
+    def autotune(self, fr, w, h, ordered, params, reps):
+        self.calls += 1
+        out = livre_amd.RenderParams.from_buffer_copy(params)
+        out.tile_w, out.tile_h, out.samples_per_thread = self.choice
+        return out, [(8, 8, 2, 1.0), (self.choice[0], self.choice[1], self.choice[2], 0.5)]
+
+
+def test_request_class_buckets(lvr):
+    vi = lvr.volume_info(512, 64, 1, lvr.DT_UINT8)
+    p = lvr.render_params(512)
+    assert autotune.request_class(vi, 1024, 1024, p) == "gfx950/dt1/px20/rate+0/nearest"
+    assert autotune.request_class(vi, 256, 256, lvr.render_params(256)) == \
+        "gfx950/dt1/px16/rate-1/nearest"
+    assert autotune.request_class(vi, 1024, 1024, lvr.render_params(1024, linear=True)) \
+        .endswith("rate+1/linear")
+
+
+def test_cache_hits_persist_and_vote(lvr, tmp_path):
+    path = str(tmp_path / "tune.json")
+    t = autotune.Tuner(path=path)
+    t.table = {}  # ignore the shipped table for this test
+    u8 = FakeCtx(lvr.volume_info(256, 32, 1, lvr.DT_UINT8), (16, 4, 3))
+    f32 = FakeCtx(lvr.volume_info(256, 32, 1, lvr.DT_FLOAT), (4, 16, 1))
+    p = lvr.render_params(256)
+    a = t.params_for(u8, None, 512, 512, [], p)
+    assert (a.tile_w, a.tile_h, a.samples_per_thread) == (16, 4, 3)
+    t.params_for(u8, None, 500, 520, [], p)  # same class: cached
+    assert u8.calls == 1 and t.tuned == 1
+    with open(path) as f:
+        saved = json.load(f)
+    assert saved[autotune.request_class(u8.vi, 512, 512, p)]["tile"] == [16, 4]
+    # a new tuner reads the user cache back
+    t2 = autotune.Tuner(path=path)
+    b = t2.params_for(u8, None, 512, 512, [], p)
+    assert u8.calls == 1 and (b.tile_w, b.tile_h) == (16, 4)
+    # batch: tiles per class, samples per thread by pixel-weighted vote
+    reqs = [(u8, None, 512, 512, [], p), (f32, None, 2048, 2048, [], p),
+            (u8, None, 512, 512, [], p)]
+    tuned = t2.tune_batch(reqs)
+    assert [(x.tile_w, x.tile_h) for x in tuned] == [(16, 4), (4, 16), (16, 4)]
+    assert {x.samples_per_thread for x in tuned} == {1}
+    assert p.samples_per_thread == 0  # inputs untouched
+
+
+@pytest.mark.gpu
+def test_autotune_candidates_render_the_same_frame(lvr):
+    from scene_kit import Scene
+    s = Scene(lvr.DT_UINT16, n=64, block=16, overlap=1, source="noise", seed=21)
+    ctx = s.context()
+    try:
+        mv, proj, fr = s.camera((1.6, 0.9, 1.2), 96, 80)
+        ordered = s.visible_ordered(mv, proj, 80)
+        from livre_amd import scenes
+        ctx.set_tf(scenes.piecewise_tf(256, seed=3, max_alpha=0.5), s.range)
+        base = lvr.render_params(128)
+        ctx.render(fr, 96, 80, ordered, base)
+        ref, ref_smp = ctx.read_frame()
+        tuned, cands = ctx.autotune(fr, 96, 80, ordered, base, reps=2)
+        assert len(cands) == 15
+        assert {(c[0], c[1]) for c in cands} == {(8, 8), (16, 4), (4, 16), (32, 2), (2, 32)}
+        assert {c[2] for c in cands} == {1, 2, 3}
+        best = min(cands, key=lambda c: c[3])
+        assert (tuned.tile_w, tuned.tile_h, tuned.samples_per_thread) == best[:3]
+        for tw, th, d, _ms in cands:
+            ctx.render(fr, 96, 80, ordered,
+                       lvr.render_params(128, tile=(tw, th), samples_per_thread=d))
+            img, smp = ctx.read_frame()
+            assert smp == ref_smp and np.array_equal(img, ref), (tw, th, d)
+    finally:
+        ctx.close()

@@ -315,6 +315,72 @@ int lvr_batch_wait(lvr_batch* batch, lvr_batch_stats* stats);
  * request, after lvr_batch_wait. */
 int lvr_batch_read(lvr_batch* batch, int request, uint8_t* out_rgba8,
                    size_t bytes, uint64_t* samples);
+/* 1 when the last launch has finished, 0 while it runs (never blocks). */
+int lvr_batch_query(lvr_batch* batch);
+
+/* --------------------------------------------------------------- serving */
+/* Request loop for concurrent camera sessions on one device, by continuous
+ * batching (the reference serves one session per process through
+ * Client::run / Config::frame, Client.cpp:89-122, Config.cpp:298-342, with
+ * no batching). Sessions are admitted against a byte budget of device memory
+ * (the volumes' brick pools + 3 frames of w*h*4 per session); later ones wait
+ * until a session closes. Submitted frame requests queue; lvr_server_step
+ * retires finished launches and puts every queued request of an admitted
+ * session (up to max_batch) into the next persistent batched launch while at
+ * most one other launch is on the GPU. A newer request of a session
+ * supersedes its older one still queued (status 1). Missing bricks are
+ * produced on the device by the volume's procedural source, evicting
+ * least-recently-used bricks; bricks read by queued or running launches are
+ * pinned. Finished frames go to per-session device framebuffers. Not
+ * thread-safe: one host thread drives a server. While a context belongs to a
+ * server, other batches must not read its pool. */
+typedef struct lvr_server lvr_server;
+typedef struct
+{
+    int64_t request;
+    int32_t session;
+    int32_t status;      /* 0 rendered, 1 superseded, < 0 error code */
+    uint64_t samples;    /* executed composite steps */
+    double arrive_ms, launch_ms, done_ms; /* lvr_server_now_ms clock */
+} lvr_server_frame;
+typedef struct
+{
+    uint64_t admitted, frames, batches, samples, superseded, failed;
+    uint64_t bricks_generated;
+    uint64_t budget_bytes, used_bytes, pending, waiting_sessions, in_flight;
+} lvr_server_stats;
+double lvr_server_now_ms(void);
+int lvr_server_create(int device, size_t budget_bytes, int max_batch,
+                      lvr_server** out);
+void lvr_server_destroy(lvr_server* server);
+const char* lvr_server_last_error(const lvr_server* server);
+int lvr_server_set_coalesce(lvr_server* server, int enable);
+/* Serve a volume from ctx's pool. noise or field (at most one) names the
+ * procedural source of missing bricks; with neither, the caller keeps the
+ * needed bricks resident. The pool's bytes count against the budget. */
+int lvr_server_add_volume(lvr_server* server, lvr_ctx* ctx,
+                          const lvr_noise_params* noise,
+                          const lvr_field_params* field, int* volume);
+/* Open a session of width x height frames with its own transfer function and
+ * render parameters (NEAREST). *admitted = 1 if it fits the budget now. */
+int lvr_server_open(lvr_server* server, int volume, int width, int height,
+                    const float* tf_rgba, int n_tf, float range0, float range1,
+                    const lvr_render_params* params, int* session,
+                    int* admitted);
+int lvr_server_close(lvr_server* server, int session);
+/* Queue a frame request of a session; arrive_ms stamps it (server clock). */
+int lvr_server_submit(lvr_server* server, int session, const lvr_frustum* f,
+                      double arrive_ms, int64_t* request);
+/* Non-blocking scheduler step: retire finished launches, launch the queue. */
+int lvr_server_step(lvr_server* server, int* launched, int* retired);
+/* Step until nothing admitted is queued or running (blocks). */
+int lvr_server_drain(lvr_server* server);
+/* Take up to cap finished (or superseded / failed) requests. */
+int lvr_server_poll(lvr_server* server, lvr_server_frame* out, int cap, int* n);
+/* The session's last finished frame (RGBA8, row 0 = bottom). */
+int lvr_server_read(lvr_server* server, int session, uint8_t* out_rgba8,
+                    size_t bytes, uint64_t* samples);
+int lvr_server_get_stats(lvr_server* server, lvr_server_stats* out);
 
 /* ------------------------------------------------------------- histogram */
 /* Per-frame histogram of the given bricks (HistogramObject.cpp:140-224 +

@@ -34,7 +34,11 @@ EXPORTS = (
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes", "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
     "lvr_batch_set_graph", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
-    "lvr_batch_wait", "lvr_batch_read",
+    "lvr_batch_wait", "lvr_batch_read", "lvr_batch_query",
+    "lvr_server_now_ms", "lvr_server_create", "lvr_server_destroy", "lvr_server_last_error",
+    "lvr_server_set_coalesce", "lvr_server_add_volume", "lvr_server_open", "lvr_server_close",
+    "lvr_server_submit", "lvr_server_step", "lvr_server_drain", "lvr_server_poll",
+    "lvr_server_read", "lvr_server_get_stats",
 )
 
 
@@ -43,6 +47,20 @@ class BatchStats(ctypes.Structure):
                 ("bricks", ctypes.c_uint64), ("tile_entries", ctypes.c_uint64),
                 ("gpu_ms", ctypes.c_float), ("graph_captures", ctypes.c_uint64),
                 ("graph_launches", ctypes.c_uint64)]
+
+
+class ServerFrame(ctypes.Structure):
+    _fields_ = [("request", ctypes.c_int64), ("session", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("samples", ctypes.c_uint64),
+                ("arrive_ms", ctypes.c_double), ("launch_ms", ctypes.c_double),
+                ("done_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "admitted", "frames", "batches", "samples", "superseded", "failed",
+        "bricks_generated", "budget_bytes", "used_bytes", "pending", "waiting_sessions",
+        "in_flight")]
 
 
 class FrameStats(ctypes.Structure):
@@ -185,6 +203,28 @@ def lib():
     L.lvr_batch_run.argtypes = [c.c_void_p]
     L.lvr_batch_wait.argtypes = [c.c_void_p, P(BatchStats)]
     L.lvr_batch_read.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, u64p]
+    L.lvr_batch_query.argtypes = [c.c_void_p]
+    L.lvr_server_now_ms.argtypes = []
+    L.lvr_server_now_ms.restype = c.c_double
+    L.lvr_server_create.argtypes = [c.c_int, c.c_size_t, c.c_int, P(c.c_void_p)]
+    L.lvr_server_destroy.argtypes = [c.c_void_p]
+    L.lvr_server_destroy.restype = None
+    L.lvr_server_last_error.argtypes = [c.c_void_p]
+    L.lvr_server_last_error.restype = c.c_char_p
+    L.lvr_server_set_coalesce.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_server_add_volume.argtypes = [c.c_void_p, c.c_void_p, P(NoiseParams),
+                                        P(FieldParams), P(c.c_int)]
+    L.lvr_server_open.argtypes = [c.c_void_p, c.c_int, c.c_int, c.c_int, f32p, c.c_int,
+                                  c.c_float, c.c_float, P(RenderParams), P(c.c_int),
+                                  P(c.c_int)]
+    L.lvr_server_close.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_server_submit.argtypes = [c.c_void_p, c.c_int, P(Frustum), c.c_double,
+                                    P(c.c_int64)]
+    L.lvr_server_step.argtypes = [c.c_void_p, P(c.c_int), P(c.c_int)]
+    L.lvr_server_drain.argtypes = [c.c_void_p]
+    L.lvr_server_poll.argtypes = [c.c_void_p, P(ServerFrame), c.c_int, P(c.c_int)]
+    L.lvr_server_read.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, u64p]
+    L.lvr_server_get_stats.argtypes = [c.c_void_p, P(ServerStats)]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
     L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
     _lib = L
@@ -534,3 +574,104 @@ class Batch:
             self._h, request, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
             img.nbytes, ctypes.byref(s)), "read")
         return img, s.value
+
+
+class Server:
+    """Continuous-batching request loop over concurrent sessions
+    (lvr_server_*)."""
+
+    def __init__(self, budget_bytes, max_batch=64, device=0):
+        self._h = ctypes.c_void_p()
+        if lib().lvr_server_create(device, int(budget_bytes), max_batch,
+                                   ctypes.byref(self._h)) != LVR_OK:
+            raise LivreError("lvr_server_create failed")
+        self.sizes = {}
+        self._keep = []  # contexts and sources must outlive the server
+
+    @staticmethod
+    def now_ms():
+        return lib().lvr_server_now_ms()
+
+    def close_server(self):
+        if self._h:
+            lib().lvr_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close_server()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close_server()
+
+    def _check(self, rc, what):
+        if rc != LVR_OK:
+            msg = lib().lvr_server_last_error(self._h).decode()
+            raise LivreError("%s failed (%d): %s" % (what, rc, msg))
+
+    def set_coalesce(self, enable):
+        self._check(lib().lvr_server_set_coalesce(self._h, 1 if enable else 0), "set_coalesce")
+
+    def add_volume(self, ctx, noise=None, field=None):
+        v = ctypes.c_int()
+        self._check(lib().lvr_server_add_volume(
+            self._h, ctx._h, ctypes.byref(noise) if noise is not None else None,
+            ctypes.byref(field) if field is not None else None, ctypes.byref(v)), "add_volume")
+        self._keep.append((ctx, noise, field))
+        return v.value
+
+    def open(self, volume, width, height, tf, data_range, params):
+        tf, ptf = _f32(tf)
+        sess, adm = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().lvr_server_open(self._h, volume, width, height, ptf, tf.size // 4,
+                                          data_range[0], data_range[1], ctypes.byref(params),
+                                          ctypes.byref(sess), ctypes.byref(adm)), "open")
+        self.sizes[sess.value] = (width, height)
+        return sess.value, bool(adm.value)
+
+    def close(self, session):
+        self._check(lib().lvr_server_close(self._h, session), "close")
+
+    def submit(self, session, fr, arrive_ms=None):
+        if arrive_ms is None:
+            arrive_ms = self.now_ms()
+        rid = ctypes.c_int64()
+        self._check(lib().lvr_server_submit(self._h, session, ctypes.byref(fr), arrive_ms,
+                                            ctypes.byref(rid)), "submit")
+        return rid.value
+
+    def step(self):
+        launched, retired = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().lvr_server_step(self._h, ctypes.byref(launched),
+                                          ctypes.byref(retired)), "step")
+        return launched.value, retired.value
+
+    def drain(self):
+        self._check(lib().lvr_server_drain(self._h), "drain")
+
+    def poll(self, cap=4096):
+        buf = (ServerFrame * cap)()
+        n = ctypes.c_int()
+        self._check(lib().lvr_server_poll(self._h, buf, cap, ctypes.byref(n)), "poll")
+        return [dict(request=f.request, session=f.session, status=f.status,
+                     samples=f.samples, arrive_ms=f.arrive_ms, launch_ms=f.launch_ms,
+                     done_ms=f.done_ms) for f in buf[:n.value]]
+
+    def read(self, session):
+        w, h = self.sizes[session]
+        img = np.zeros((h, w, 4), np.uint8)
+        s = ctypes.c_uint64()
+        self._check(lib().lvr_server_read(
+            self._h, session, img.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), img.nbytes,
+            ctypes.byref(s)), "read")
+        return img, s.value
+
+    def stats(self):
+        st = ServerStats()
+        self._check(lib().lvr_server_get_stats(self._h, ctypes.byref(st)), "stats")
+        return st

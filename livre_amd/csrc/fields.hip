@@ -353,39 +353,30 @@ int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p, const uint6
     if (rc)
         return rc;
     const lvr::FieldSetup f = lvr::makeField(ctx->vi, *p);
-    lvr::Blob* d_blobs = nullptr;
-    lvr::Wave* d_waves = nullptr;
-    lvr::Geo* d_geo = nullptr;
-    int32_t* d_slots = nullptr;
     const int chunk = 4096;
-    auto cleanup = [&]() {
-        (void)hipFree(d_blobs);
-        (void)hipFree(d_waves);
-        (void)hipFree(d_geo);
-        (void)hipFree(d_slots);
-    };
-    if (hipMalloc((void**)&d_blobs, sizeof(lvr::Blob) * (f.blobs.size() + 1)) != hipSuccess ||
-        hipMalloc((void**)&d_waves, sizeof(lvr::Wave) * (f.waves.size() + 1)) != hipSuccess ||
-        hipMalloc((void**)&d_geo, sizeof(lvr::Geo) * chunk) != hipSuccess ||
-        hipMalloc((void**)&d_slots, sizeof(int32_t) * chunk) != hipSuccess)
-    {
-        cleanup();
-        return lvr::fail(ctx, LVR_ENOMEM, "field source buffers");
-    }
-    if ((!f.blobs.empty() && hipMemcpy(d_blobs, f.blobs.data(), sizeof(lvr::Blob) * f.blobs.size(),
-                                       hipMemcpyHostToDevice) != hipSuccess) ||
-        (!f.waves.empty() && hipMemcpy(d_waves, f.waves.data(), sizeof(lvr::Wave) * f.waves.size(),
-                                       hipMemcpyHostToDevice) != hipSuccess))
-    {
-        cleanup();
+    auto up256 = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t blobBytes = up256(sizeof(lvr::Blob) * (f.blobs.size() + 1));
+    const size_t waveBytes = up256(sizeof(lvr::Wave) * (f.waves.size() + 1));
+    const size_t geoBytes = up256(sizeof(lvr::Geo) * chunk);
+    uint8_t* scratch = static_cast<uint8_t*>(
+        lvr::genScratch(ctx, blobBytes + waveBytes + geoBytes + sizeof(int32_t) * chunk));
+    if (!scratch)
+        return LVR_ENOMEM;
+    lvr::Blob* d_blobs = reinterpret_cast<lvr::Blob*>(scratch);
+    lvr::Wave* d_waves = reinterpret_cast<lvr::Wave*>(scratch + blobBytes);
+    lvr::Geo* d_geo = reinterpret_cast<lvr::Geo*>(scratch + blobBytes + waveBytes);
+    int32_t* d_slots = reinterpret_cast<int32_t*>(scratch + blobBytes + waveBytes + geoBytes);
+    // slots taken here may still be read by batches on other streams
+    rc = lvr::waitPoolReaders(ctx, ctx->stream);
+    if (rc)
+        return rc;
+    if ((!f.blobs.empty() &&
+         hipMemcpyAsync(d_blobs, f.blobs.data(), sizeof(lvr::Blob) * f.blobs.size(),
+                        hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (!f.waves.empty() &&
+         hipMemcpyAsync(d_waves, f.waves.data(), sizeof(lvr::Wave) * f.waves.size(),
+                        hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
         return lvr::fail(ctx, LVR_EHIP, "field source upload");
-    }
-    // frames queued on the render stream may still read slots evicted here
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
-    {
-        cleanup();
-        return lvr::fail(ctx, LVR_EHIP, "field source sync");
-    }
     std::vector<lvr::Geo> geo;
     std::vector<int32_t> slots;
     for (int base = 0; base < n && rc == LVR_OK; base += chunk)
@@ -393,6 +384,7 @@ int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p, const uint6
         const int cnt = std::min(chunk, n - base);
         geo.resize(size_t(cnt));
         slots.resize(size_t(cnt));
+        int taken = 0;
         for (int i = 0; i < cnt && rc == LVR_OK; ++i)
         {
             if (lvr::decodeNode(node_ids[base + i]).level >= ctx->vi.depth)
@@ -401,16 +393,22 @@ int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p, const uint6
             {
                 geo[size_t(i)] = lvr::brickOrigin(ctx->vi, node_ids[base + i]);
                 rc = lvr::acquireSlotFor(ctx, node_ids[base + i], &slots[size_t(i)]);
+                taken += rc == LVR_OK;
             }
         }
-        if (rc)
-            break;
-        if (hipMemcpyAsync(d_geo, geo.data(), sizeof(lvr::Geo) * cnt, hipMemcpyHostToDevice,
-                           ctx->stream) != hipSuccess ||
-            hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt, hipMemcpyHostToDevice,
-                           ctx->stream) != hipSuccess)
-        {
+        if (rc == LVR_OK &&
+            (hipMemcpyAsync(d_geo, geo.data(), sizeof(lvr::Geo) * cnt, hipMemcpyHostToDevice,
+                            ctx->stream) != hipSuccess ||
+             hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt, hipMemcpyHostToDevice,
+                            ctx->stream) != hipSuccess))
             rc = lvr::fail(ctx, LVR_EHIP, "field source upload");
+        if (rc)
+        {
+            // bricks of this chunk never received their voxels
+            const std::string msg = ctx->error;
+            for (int i = 0; i < taken; ++i)
+                (void)lvr::releaseSlot(ctx, node_ids[base + i]);
+            ctx->error = msg;
             break;
         }
         const dim3 grid(64, unsigned(cnt));
@@ -426,10 +424,10 @@ int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p, const uint6
             hipLaunchKernelGGL(lvr::fieldBricksKernel<float>, grid, dim3(256), 0, ctx->stream,
                                f.shape, d_blobs, d_waves, d_geo, d_slots, ctx->d_pool,
                                ctx->slotStride, ctx->layout);
+        // the host vectors and the scratch are reused by the next chunk
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = lvr::fail(ctx, LVR_EHIP, "field source kernel");
     }
-    cleanup();
     return rc;
 }
 }

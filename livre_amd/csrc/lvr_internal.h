@@ -227,6 +227,16 @@ struct lvr_ctx
     bool histTimed = false;
     uint64_t lastBricks = 0, lastEntries = 0;
 
+    // batches launched on their own streams that read this pool; slot reuse
+    // waits for them unless the owner pins every brick a launch reads
+    std::vector<lvr_batch*> readers;
+    bool pinsGuardReaders = false;
+    uint64_t pinnedBricks = 0; // resident bricks with a pin count above 0
+
+    // device scratch of the procedural sources (grow-only)
+    void* d_gen = nullptr;
+    size_t genCapacity = 0;
+
     std::string error;
 };
 
@@ -239,6 +249,22 @@ int32_t slotOf(const lvr_ctx* ctx, uint64_t id);
 void touch(lvr_ctx* ctx, uint64_t id);
 int flushPageTable(lvr_ctx* ctx);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
+// Undo acquireSlotFor for a brick whose contents were never written.
+int releaseSlot(lvr_ctx* ctx, uint64_t id);
+// Change a resident brick's pin count (pinned bricks are never evicted).
+int pinBrick(lvr_ctx* ctx, uint64_t id, int delta);
+// Slots that can take a new brick without touching pinned ones.
+uint64_t evictableSlots(const lvr_ctx* ctx);
+// Order slot reuse after every batch still reading the pool: stream `st`
+// waits for them, or the host does when st is null.
+int waitPoolReaders(lvr_ctx* ctx, hipStream_t st);
+// Device scratch of at least `bytes` owned by the context (nullptr + error).
+void* genScratch(lvr_ctx* ctx, size_t bytes);
+// Batch internals used by the pool and the server.
+hipEvent_t batchDoneEvent(lvr_batch* b);
+void batchForgetCtx(lvr_batch* b, lvr_ctx* ctx);
+hipStream_t batchStream(lvr_batch* b);
+uint32_t* batchFrame(lvr_batch* b, int request);
 int ensureDeviceStaging(lvr_ctx* ctx, size_t bricks);
 // d_staging (linear bricks) <-> pool slots d_stagingSlots, on stream st
 int convertLayout(lvr_ctx* ctx, int nBricks, bool toTiled, hipStream_t st);

@@ -338,15 +338,22 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
     if (rc)
         return rc;
     const lvr::NoiseShape s = lvr::makeShape(ctx->vi, *p);
-    uint16_t* d_remap = nullptr;
-    lvr::BrickGeo* d_geo = nullptr;
-    int32_t* d_slots = nullptr;
     const int chunk = 4096;
-    LVR_HIP(ctx, hipMalloc((void**)&d_remap, sizeof p->remap));
-    LVR_HIP(ctx, hipMalloc((void**)&d_geo, sizeof(lvr::BrickGeo) * chunk));
-    LVR_HIP(ctx, hipMalloc((void**)&d_slots, sizeof(int32_t) * chunk));
-    LVR_HIP(ctx, hipMemcpy(d_remap, p->remap, sizeof p->remap,
-                           hipMemcpyHostToDevice));
+    const size_t remapBytes = (sizeof p->remap + 255) & ~size_t(255);
+    const size_t geoBytes = (sizeof(lvr::BrickGeo) * chunk + 255) & ~size_t(255);
+    uint8_t* scratch = static_cast<uint8_t*>(
+        lvr::genScratch(ctx, remapBytes + geoBytes + sizeof(int32_t) * chunk));
+    if (!scratch)
+        return LVR_ENOMEM;
+    uint16_t* d_remap = reinterpret_cast<uint16_t*>(scratch);
+    lvr::BrickGeo* d_geo = reinterpret_cast<lvr::BrickGeo*>(scratch + remapBytes);
+    int32_t* d_slots = reinterpret_cast<int32_t*>(scratch + remapBytes + geoBytes);
+    // slots taken here may still be read by batches on other streams
+    rc = lvr::waitPoolReaders(ctx, ctx->stream);
+    if (rc)
+        return rc;
+    LVR_HIP(ctx, hipMemcpyAsync(d_remap, p->remap, sizeof p->remap, hipMemcpyHostToDevice,
+                                ctx->stream));
     std::vector<lvr::BrickGeo> geo;
     std::vector<int32_t> slots;
     for (int base = 0; base < n && rc == LVR_OK; base += chunk)
@@ -354,6 +361,7 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
         const int cnt = std::min(chunk, n - base);
         geo.resize(cnt);
         slots.resize(cnt);
+        int taken = 0;
         for (int i = 0; i < cnt && rc == LVR_OK; ++i)
         {
             if (lvr::decodeNode(node_ids[base + i]).level >= ctx->vi.depth)
@@ -362,16 +370,22 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
             {
                 geo[i] = lvr::brickGeo(ctx->vi, node_ids[base + i]);
                 rc = lvr::acquireSlotFor(ctx, node_ids[base + i], &slots[i]);
+                taken += rc == LVR_OK;
             }
         }
-        if (rc)
-            break;
-        if (hipMemcpyAsync(d_geo, geo.data(), sizeof(lvr::BrickGeo) * cnt,
-                           hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-            hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt,
-                           hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
-        {
+        if (rc == LVR_OK &&
+            (hipMemcpyAsync(d_geo, geo.data(), sizeof(lvr::BrickGeo) * cnt,
+                            hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+             hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt,
+                            hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
             rc = lvr::fail(ctx, LVR_EHIP, "noise upload");
+        if (rc)
+        {
+            // bricks of this chunk never received their voxels
+            const std::string msg = ctx->error;
+            for (int i = 0; i < taken; ++i)
+                (void)lvr::releaseSlot(ctx, node_ids[base + i]);
+            ctx->error = msg;
             break;
         }
         const dim3 grid(64, unsigned(cnt));
@@ -387,13 +401,11 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
             hipLaunchKernelGGL(lvr::noiseBricksKernel<float>, grid, dim3(256), 0,
                                ctx->stream, s, d_remap, d_geo, d_slots,
                                ctx->d_pool, ctx->slotStride, ctx->layout);
+        // the host vectors and the scratch are reused by the next chunk
         if (hipGetLastError() != hipSuccess ||
             hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");
     }
-    (void)hipFree(d_remap);
-    (void)hipFree(d_geo);
-    (void)hipFree(d_slots);
     return rc;
 }
 }

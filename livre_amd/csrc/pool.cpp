@@ -88,6 +88,75 @@ int flushPageTable(lvr_ctx* ctx)
     return LVR_OK;
 }
 
+int pinBrick(lvr_ctx* ctx, uint64_t id, int delta)
+{
+    auto it = ctx->resident.find(id);
+    if (it == ctx->resident.end())
+        return fail(ctx, LVR_ENOENT, "pin of a brick that is not resident");
+    const int before = it->second.pinned;
+    const int after = before + delta;
+    if (after < 0)
+        return fail(ctx, LVR_EINVAL, "unbalanced brick unpin");
+    it->second.pinned = after;
+    if (before == 0 && after > 0)
+        ++ctx->pinnedBricks;
+    else if (before > 0 && after == 0)
+        --ctx->pinnedBricks;
+    return LVR_OK;
+}
+
+uint64_t evictableSlots(const lvr_ctx* ctx)
+{
+    return ctx->freeSlots.size() + (ctx->resident.size() - ctx->pinnedBricks);
+}
+
+int waitPoolReaders(lvr_ctx* ctx, hipStream_t st)
+{
+    if (ctx->pinsGuardReaders || ctx->readers.empty())
+        return LVR_OK;
+    std::vector<lvr_batch*> still;
+    for (lvr_batch* b : ctx->readers)
+    {
+        hipEvent_t e = batchDoneEvent(b);
+        const hipError_t q = hipEventQuery(e);
+        if (q == hipSuccess)
+            continue; // finished: nothing to wait for
+        if (q != hipErrorNotReady)
+            return hipFail(ctx, q, "hipEventQuery(batch)");
+        if (st)
+            LVR_HIP(ctx, hipStreamWaitEvent(st, e, 0));
+        else
+            LVR_HIP(ctx, hipEventSynchronize(e));
+        if (st)
+            still.push_back(b);
+    }
+    ctx->readers.swap(still);
+    return LVR_OK;
+}
+
+void* genScratch(lvr_ctx* ctx, size_t bytes)
+{
+    if (ctx->genCapacity >= bytes)
+        return ctx->d_gen;
+    // the previous scratch may still be read by queued source kernels
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    {
+        fail(ctx, LVR_EHIP, "source scratch sync");
+        return nullptr;
+    }
+    (void)hipFree(ctx->d_gen);
+    ctx->d_gen = nullptr;
+    ctx->genCapacity = 0;
+    const size_t cap = std::max<size_t>(bytes, size_t(1) << 20);
+    if (hipMalloc(&ctx->d_gen, cap) != hipSuccess)
+    {
+        fail(ctx, LVR_ENOMEM, "source scratch");
+        return nullptr;
+    }
+    ctx->genCapacity = cap;
+    return ctx->d_gen;
+}
+
 // Evict the first unpinned brick in policy order (Cache.cpp:111-123 walks
 // the queue in insertion order and skips referenced objects).
 static int evictOne(lvr_ctx* ctx)
@@ -141,6 +210,20 @@ static int acquireSlot(lvr_ctx* ctx, uint64_t id, int32_t* slot)
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot)
 {
     return acquireSlot(ctx, id, slot);
+}
+
+int releaseSlot(lvr_ctx* ctx, uint64_t id)
+{
+    auto it = ctx->resident.find(id);
+    if (it == ctx->resident.end())
+        return LVR_OK;
+    if (it->second.pinned)
+        --ctx->pinnedBricks;
+    ctx->freeSlots.push_back(it->second.slot);
+    ctx->lruList.erase(it->second.lru);
+    ctx->resident.erase(it);
+    ++ctx->poolGeneration;
+    return setPage(ctx, id, -1);
 }
 
 // Linear (x fastest, getData layout) <-> tiled pool layout, one brick per
@@ -318,11 +401,17 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     if (!ctx)
         return;
     (void)hipSetDevice(ctx->device);
+    for (lvr_batch* b : ctx->readers)
+    {
+        (void)hipEventSynchronize(lvr::batchDoneEvent(b));
+        lvr::batchForgetCtx(b, ctx);
+    }
     if (ctx->stream)
         (void)hipStreamSynchronize(ctx->stream);
     if (ctx->uploadStream)
         (void)hipStreamSynchronize(ctx->uploadStream);
     (void)hipFree(ctx->d_pool);
+    (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->pt.d_slots);
     (void)hipFree(ctx->d_tf);
     (void)hipFree(ctx->d_frame);
@@ -427,9 +516,13 @@ int lvr_brick_upload_many(lvr_ctx* ctx, const uint64_t* node_ids, int n,
         return rc;
     if (n == 0)
         return LVR_OK;
-    // frames queued on the render stream may still read slots evicted here
+    // frames queued on the render stream, and batches on their own streams,
+    // may still read slots evicted here
     LVR_HIP(ctx, hipEventRecord(ctx->renderDone, ctx->stream));
     LVR_HIP(ctx, hipStreamWaitEvent(ctx->uploadStream, ctx->renderDone, 0));
+    rc = lvr::waitPoolReaders(ctx, ctx->uploadStream);
+    if (rc)
+        return rc;
     // stage in chunks of up to 64 MiB: pinned host -> device linear -> tiled slots
     const size_t chunkBricks = std::max<size_t>(1, (64u << 20) / ctx->brickBytes);
     const size_t chunk = std::min<size_t>(chunkBricks, size_t(n));
@@ -478,6 +571,11 @@ int lvr_brick_evict(lvr_ctx* ctx, uint64_t node_id)
         return rc;
     // kernels already queued may still read the slot
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    rc = lvr::waitPoolReaders(ctx, nullptr);
+    if (rc)
+        return rc;
+    if (it->second.pinned)
+        return lvr::fail(ctx, LVR_EINVAL, "brick is pinned");
     ++ctx->poolGeneration;
     ctx->freeSlots.push_back(it->second.slot);
     ctx->lruList.erase(it->second.lru);

@@ -543,8 +543,42 @@ struct lvr_batch
     hipGraphExec_t exec = nullptr;
     std::vector<int64_t> graphKey;
     uint64_t graphLaunches = 0, graphCaptures = 0;
+
+    // contexts whose pools the current requests read, and the event that
+    // marks the end of the last launch
+    std::vector<lvr_ctx*> ctxs;
+    hipEvent_t done = nullptr;
     float kernelMs = 0.f;
 };
+
+// The batch no longer reads these pools (its launch finished or it is going
+// away): take it off their reader lists.
+static void detachContexts(lvr_batch* b)
+{
+    for (lvr_ctx* c : b->ctxs)
+        c->readers.erase(std::remove(c->readers.begin(), c->readers.end(), b), c->readers.end());
+    b->ctxs.clear();
+}
+
+namespace lvr
+{
+hipEvent_t batchDoneEvent(lvr_batch* b)
+{
+    return b->done;
+}
+void batchForgetCtx(lvr_batch* b, lvr_ctx* ctx)
+{
+    b->ctxs.erase(std::remove(b->ctxs.begin(), b->ctxs.end(), ctx), b->ctxs.end());
+}
+hipStream_t batchStream(lvr_batch* b)
+{
+    return b->stream;
+}
+uint32_t* batchFrame(lvr_batch* b, int request)
+{
+    return b->d_frames + b->frameOffset[size_t(request)];
+}
+} // namespace lvr
 
 namespace lvr
 {
@@ -793,7 +827,8 @@ int lvr_batch_create(int device, lvr_batch** out)
         hipDeviceGetAttribute(&b->numCUs, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess)
+        hipEventCreate(&b->ev[0]) != hipSuccess || hipEventCreate(&b->ev[1]) != hipSuccess ||
+        hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess)
     {
         lvr_batch_destroy(b);
         return LVR_EHIP;
@@ -809,6 +844,9 @@ void lvr_batch_destroy(lvr_batch* b)
     (void)hipSetDevice(b->device);
     if (b->stream)
         (void)hipStreamSynchronize(b->stream);
+    detachContexts(b);
+    if (b->done)
+        (void)hipEventDestroy(b->done);
     if (b->exec)
         (void)hipGraphExecDestroy(b->exec);
     (void)hipFree(b->d_reqs);
@@ -846,6 +884,7 @@ int lvr_batch_clear(lvr_batch* b)
     if (!b)
         return LVR_EINVAL;
     LVR_BHIP(b, hipStreamSynchronize(b->stream));
+    detachContexts(b);
     b->reqs.clear();
     b->bricks.clear();
     b->ranges.clear();
@@ -896,6 +935,8 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
         return bfail(b, LVR_EINVAL, "samples per thread differs from the batch's");
     }
     U.request = int32_t(b->reqs.size());
+    if (std::find(b->ctxs.begin(), b->ctxs.end(), ctx) == b->ctxs.end())
+        b->ctxs.push_back(ctx);
     b->tfOffset.push_back(b->tf.size());
     for (int i = 0; i < n_tf; ++i)
         b->tf.push_back(make_float4(tf_rgba[4 * i], tf_rgba[4 * i + 1], tf_rgba[4 * i + 2],
@@ -1039,8 +1080,24 @@ int lvr_batch_run(lvr_batch* b)
     else
         LVR_BHIP(b, enqueue(b->stream));
     LVR_BHIP(b, hipEventRecord(b->ev[1], b->stream));
+    LVR_BHIP(b, hipEventRecord(b->done, b->stream));
+    for (lvr_ctx* c : b->ctxs)
+        if (std::find(c->readers.begin(), c->readers.end(), b) == c->readers.end())
+            c->readers.push_back(b);
     b->ran = true;
     return LVR_OK;
+}
+
+int lvr_batch_query(lvr_batch* b)
+{
+    if (!b)
+        return LVR_EINVAL;
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess)
+        return 1;
+    if (e == hipErrorNotReady)
+        return 0;
+    return lvr::bfail(b, LVR_EHIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
 }
 
 int lvr_batch_wait(lvr_batch* b, lvr_batch_stats* stats)

@@ -15,8 +15,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol(lvr):
     with open(os.path.join(ROOT, "include", "livre_amd.h")) as f:
         header = f.read()
-    declared = set(re.findall(r"^\s*(?:int|void|const char\*)\s+(lvr_\w+)\(", header,
-                              re.M))
+    declared = set(re.findall(r"^\s*(?:int|void|double|const char\*)\s+(lvr_\w+)\(",
+                              header, re.M))
     assert declared == set(lvr.EXPORTS)
     L = lvr.lib()
     for name in declared:

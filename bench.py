@@ -218,6 +218,140 @@ def batch_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def serve_workload(args):
+    """32 concurrent sessions over the five volumes (720p..4K viewports,
+    random-walk cameras, own TFs, 1 sample per voxel) served by the
+    continuous-batching loop; Poisson arrivals at each rate for a fixed
+    duration. Latency = completion - arrival of every rendered frame."""
+    import livre_amd
+    from livre_amd import workloads
+    livre_amd.lib()
+    # pools of pool_frac of each pyramid, filled on demand by the server's
+    # device sources (LRU eviction when a pool is smaller than the working
+    # sets of its sessions)
+    vols = {}
+    for name in workloads.SERVE_VOLUMES:
+        spec = workloads.VOLUMES[name]
+        vi = livre_amd.volume_info(spec["voxels"], spec["block"], 1, spec["dtype"])
+        n_ids = len(livre_amd.scenes.all_nodes(vi))
+        vols[name] = workloads.Volume(name, pool_bricks=int(n_ids * args.pool_frac) + 64,
+                                      generate=False)
+    sessions = workloads.serve_sessions(32, seed=9)
+    pools = sum(v.pool_bytes for v in vols.values())
+    frames_bytes = sum(3 * 4 * s["width"] * s["height"] for s in sessions)
+    budget = pools + frames_bytes
+    result_rates = []
+    with livre_amd.Server(budget, max_batch=args.max_batch) as srv:
+        vid = {n: srv.add_volume(v.ctx, noise=v.noise, field=v.field) for n, v in vols.items()}
+        sid, last_fr, req_fr, shown = [], {}, {}, {}
+        for s in sessions:
+            v = vols[s["volume"]]
+            params = livre_amd.render_params(v.vmax)
+            k, admitted = srv.open(vid[s["volume"]], s["width"], s["height"], s["tf"], v.range,
+                                   params)
+            sid.append(k)
+        # warm-up: one frame per session (fills the pools)
+        t0 = time.perf_counter()
+        for i, s in enumerate(sessions):
+            last_fr[i] = s["walk"].next()
+            req_fr[srv.submit(sid[i], last_fr[i])] = (i, last_fr[i])
+        srv.drain()
+
+        def collect(frames):
+            # the frustum of the newest rendered frame of every session
+            for d in frames:
+                k, fr = req_fr.pop(d["request"])
+                if d["status"] == 0:
+                    shown[k] = fr
+            return frames
+
+        collect(srv.poll())
+        log("warm-up: 32 frames in %.2f s, %d bricks generated"
+            % (time.perf_counter() - t0, srv.stats().bricks_generated))
+        for rate in args.rates:
+            arrivals = workloads.poisson_arrivals(rate, args.duration, len(sessions), seed=rate)
+            before = srv.stats()
+            done = []
+            t0 = srv.now_ms()
+            i = 0
+            while i < len(arrivals):
+                if srv.now_ms() - t0 >= arrivals[i][0]:
+                    k = arrivals[i][1]
+                    last_fr[k] = sessions[k]["walk"].next()
+                    req_fr[srv.submit(sid[k], last_fr[k], t0 + arrivals[i][0])] = \
+                        (k, last_fr[k])
+                    i += 1
+                    continue
+                if srv.step()[1]:
+                    done += collect(srv.poll())
+            srv.drain()
+            done += collect(srv.poll())
+            t_end = max([d["done_ms"] for d in done] + [srv.now_ms()])
+            after = srv.stats()
+            ok = [d for d in done if d["status"] == 0]
+            lat = np.array([d["done_ms"] - d["arrive_ms"] for d in ok])
+            wall = (t_end - t0) / 1e3
+            samples = sum(d["samples"] for d in ok)
+            row = {"rate_req_per_s": rate, "requests": len(arrivals), "rendered": len(ok),
+                   "superseded": sum(1 for d in done if d["status"] == 1),
+                   "failed": sum(1 for d in done if d["status"] < 0),
+                   "failure_codes": sorted({d["status"] for d in done if d["status"] < 0}),
+                   "fps": len(ok) / wall, "gsamples_per_s": samples / wall / 1e9,
+                   "p50_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+                   "p99_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+                   "max_ms": float(lat.max()) if len(lat) else None,
+                   "batches": after.batches - before.batches,
+                   "host_ms": {k: round(getattr(after, k) - getattr(before, k), 1) for k in (
+                       "select_ms", "order_ms", "residency_ms", "add_ms", "run_ms",
+                       "retire_ms", "gpu_ms")},
+                   "bricks_generated": after.bricks_generated - before.bricks_generated}
+            log("rate %d/s: %d rendered, %d superseded, %.1f fps, %.1f Gsamples/s, p50 %.1f ms, "
+                "p99 %.1f ms, host %s" % (rate, row["rendered"], row["superseded"], row["fps"],
+                                          row["gsamples_per_s"], row["p50_ms"] or -1,
+                                          row["p99_ms"] or -1, row["host_ms"]))
+            result_rates.append(row)
+        # the last frame of a few sessions equals the same frame rendered alone
+        same = True
+        for k in (0, 7, 15, 31):
+            s = sessions[k]
+            v = vols[s["volume"]]
+            img, smp = srv.read(sid[k])
+            fr = shown[k]
+            vis = v.ctx.select_visibles(fr, s["height"], 1.0, 0, v.vi.depth - 1)
+            ordered = livre_amd.order(v.vi, fr, vis)
+            missing = [x for x in ordered if not v.ctx.resident(x)]
+            if missing:
+                if v.noise is not None:
+                    v.ctx.generate_noise(v.noise, missing)
+                else:
+                    v.ctx.generate_field(v.field, missing)
+            v.ctx.set_tf(s["tf"], v.range)
+            v.ctx.render(fr, s["width"], s["height"], ordered,
+                         livre_amd.render_params(v.vmax))
+            ref, ref_smp = v.ctx.read_frame()
+            same = same and smp == ref_smp and bool(np.array_equal(img, ref))
+        st = srv.stats()
+    for v in vols.values():
+        v.close()
+    top = result_rates[-1]
+    result = {
+        "metric": "served Gsamples/s and p50/p99 frame latency, 32 concurrent sessions",
+        "value": top["gsamples_per_s"], "unit": "Gsamples/s", "n_gpus": 1,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u8/u16/f32",
+        "data": "synthetic: procedural volumes, bricks generated on the device on demand",
+        "config": {"workload": "32 sessions, viewports {720p,1080p,1440p,4K}, volumes "
+                               "256^3 u8 / 512^3 u8 / 1024^3 u16 / 1536^3 f32 / 2048^3 u16, "
+                               "random-walk cameras, 256-entry TFs, 1 sample/voxel, sse 1; "
+                               "Poisson arrivals, %.0f s per rate" % args.duration,
+                   "server": "continuous batching, 2 launches in flight, max %d requests "
+                             "per launch, newest request per session wins" % args.max_batch,
+                   "budget_bytes": budget, "pool_bytes": pools,
+                   "pool_fraction_of_pyramid": args.pool_frac},
+        "rates": result_rates, "served_equals_alone": same, "admitted_sessions": st.admitted,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,9 +359,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("render", "batch"), default="render",
+    ap.add_argument("--workload", choices=("render", "batch", "serve"), default="render",
                     help="render: the 1080p single-volume frame loop (default); "
-                         "batch: uneven requests, plain vs one batched launch")
+                         "batch: uneven requests, plain vs one batched launch; "
+                         "serve: 32 sessions under Poisson arrivals")
+    ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
+    ap.add_argument("--duration", type=float, default=3.0)
+    ap.add_argument("--max-batch", type=int, default=16)
+    ap.add_argument("--pool-frac", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the brick pool, kernels and measurements are "
@@ -237,6 +376,9 @@ def main():
         if args.steps is None:
             args.steps = 4
         batch_workload(args)
+        return
+    if args.workload == "serve":
+        serve_workload(args)
         return
     if args.steps is None:
         args.steps = FRAMES

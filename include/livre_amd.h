@@ -317,6 +317,12 @@ int lvr_batch_read(lvr_batch* batch, int request, uint8_t* out_rgba8,
                    size_t bytes, uint64_t* samples);
 /* 1 when the last launch has finished, 0 while it runs (never blocks). */
 int lvr_batch_query(lvr_batch* batch);
+/* Render request `request` (added since the last run) into caller device
+ * memory of width * height * 4 bytes instead of the batch's own frames. */
+int lvr_batch_set_output(lvr_batch* batch, int request, void* device_rgba8);
+/* Keep n CUs free of this batch's launches (a CU mask on its stream), for
+ * work other streams run while a launch is in flight. 0 = use every CU. */
+int lvr_batch_reserve_cus(lvr_batch* batch, int n);
 
 /* --------------------------------------------------------------- serving */
 /* Request loop for concurrent camera sessions on one device, by continuous
@@ -331,7 +337,9 @@ int lvr_batch_query(lvr_batch* batch);
  * supersedes its older one still queued (status 1). Missing bricks are
  * produced on the device by the volume's procedural source, evicting
  * least-recently-used bricks; bricks read by queued or running launches are
- * pinned. Finished frames go to per-session device framebuffers. Not
+ * pinned. Frames render straight into per-session device framebuffers
+ * (three per session, so the finished one is never the one being written);
+ * launches leave a few CUs to the selection and source kernels. Not
  * thread-safe: one host thread drives a server. While a context belongs to a
  * server, other batches must not read its pool. */
 typedef struct lvr_server lvr_server;
@@ -348,6 +356,11 @@ typedef struct
     uint64_t admitted, frames, batches, samples, superseded, failed;
     uint64_t bricks_generated;
     uint64_t budget_bytes, used_bytes, pending, waiting_sessions, in_flight;
+    /* host time spent in the scheduler, ms: visible-set selection, ordering,
+     * brick residency (misses + pins), batch assembly, launch, retirement */
+    double select_ms, order_ms, residency_ms, add_ms, run_ms, retire_ms;
+    /* GPU time of the launches (binning + ray casting), summed, ms */
+    double gpu_ms;
 } lvr_server_stats;
 double lvr_server_now_ms(void);
 int lvr_server_create(int device, size_t budget_bytes, int max_batch,

@@ -34,7 +34,8 @@ EXPORTS = (
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes", "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
     "lvr_batch_set_graph", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
-    "lvr_batch_wait", "lvr_batch_read", "lvr_batch_query",
+    "lvr_batch_wait", "lvr_batch_read", "lvr_batch_query", "lvr_batch_set_output",
+    "lvr_batch_reserve_cus",
     "lvr_server_now_ms", "lvr_server_create", "lvr_server_destroy", "lvr_server_last_error",
     "lvr_server_set_coalesce", "lvr_server_add_volume", "lvr_server_open", "lvr_server_close",
     "lvr_server_submit", "lvr_server_step", "lvr_server_drain", "lvr_server_poll",
@@ -60,7 +61,8 @@ class ServerStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in (
         "admitted", "frames", "batches", "samples", "superseded", "failed",
         "bricks_generated", "budget_bytes", "used_bytes", "pending", "waiting_sessions",
-        "in_flight")]
+        "in_flight")] + [(n, ctypes.c_double) for n in (
+        "select_ms", "order_ms", "residency_ms", "add_ms", "run_ms", "retire_ms", "gpu_ms")]
 
 
 class FrameStats(ctypes.Structure):
@@ -204,6 +206,8 @@ def lib():
     L.lvr_batch_wait.argtypes = [c.c_void_p, P(BatchStats)]
     L.lvr_batch_read.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, u64p]
     L.lvr_batch_query.argtypes = [c.c_void_p]
+    L.lvr_batch_set_output.argtypes = [c.c_void_p, c.c_int, c.c_void_p]
+    L.lvr_batch_reserve_cus.argtypes = [c.c_void_p, c.c_int]
     L.lvr_server_now_ms.argtypes = []
     L.lvr_server_now_ms.restype = c.c_double
     L.lvr_server_create.argtypes = [c.c_int, c.c_size_t, c.c_int, P(c.c_void_p)]
@@ -540,6 +544,15 @@ class Batch:
     def set_graph(self, enable):
         self._check(lib().lvr_batch_set_graph(self._h, 1 if enable else 0), "set_graph")
 
+    def reserve_cus(self, n):
+        self._check(lib().lvr_batch_reserve_cus(self._h, n), "reserve_cus")
+
+    def query(self):
+        rc = lib().lvr_batch_query(self._h)
+        if rc < 0:
+            self._check(rc, "query")
+        return rc == 1
+
     def clear(self):
         self._check(lib().lvr_batch_clear(self._h), "clear")
         self.sizes = []
@@ -654,13 +667,24 @@ class Server:
     def drain(self):
         self._check(lib().lvr_server_drain(self._h), "drain")
 
-    def poll(self, cap=4096):
-        buf = (ServerFrame * cap)()
+    def poll(self, cap=256):
+        buf = self._poll_buf if getattr(self, "_poll_cap", 0) == cap else None
+        if buf is None:
+            buf = self._poll_buf = (ServerFrame * cap)()
+            self._poll_cap = cap
         n = ctypes.c_int()
-        self._check(lib().lvr_server_poll(self._h, buf, cap, ctypes.byref(n)), "poll")
+        out = []
+        while True:
+            self._check(lib().lvr_server_poll(self._h, buf, cap, ctypes.byref(n)), "poll")
+            out += self._frames(buf, n.value)
+            if n.value < cap:
+                return out
+
+    @staticmethod
+    def _frames(buf, n):
         return [dict(request=f.request, session=f.session, status=f.status,
                      samples=f.samples, arrive_ms=f.arrive_ms, launch_ms=f.launch_ms,
-                     done_ms=f.done_ms) for f in buf[:n.value]]
+                     done_ms=f.done_ms) for f in buf[:n]]
 
     def read(self, session):
         w, h = self.sizes[session]

@@ -222,10 +222,14 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
             ctx->histJobCapacity = int(cj);
         }
         LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-        LVR_HIP(ctx, hipMemcpy(ctx->d_histSlots, slots.data(),
-                               sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice));
-        LVR_HIP(ctx, hipMemcpy(ctx->d_histScale, jobs.data(),
-                               sizeof(HistJob) * jobs.size(), hipMemcpyHostToDevice));
+        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_histSlots, slots.data(),
+                                    sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_histScale, jobs.data(),
+                                    sizeof(HistJob) * jobs.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        // the host vectors go out of scope at the end of this call
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         const int M = int(ctx->vi.max_block[0]);
         const int ov = int(ctx->vi.overlap[0]);
         const size_t lds = sizeof(uint32_t) * binsPerJob;

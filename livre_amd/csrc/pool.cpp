@@ -638,8 +638,9 @@ static int tfUpload(lvr_ctx* ctx, const std::vector<float>& rgba, int n,
         ctx->tfCapacity = n;
     }
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    LVR_HIP(ctx, hipMemcpy(ctx->d_tf, rgba.data(), size_t(n) * 4 * sizeof(float),
-                           hipMemcpyHostToDevice));
+    LVR_HIP(ctx, hipMemcpyAsync(ctx->d_tf, rgba.data(), size_t(n) * 4 * sizeof(float),
+                                hipMemcpyHostToDevice, ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->nTf = n;
     ctx->range0 = range0;
     ctx->range1 = range1;

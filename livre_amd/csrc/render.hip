@@ -173,6 +173,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 // Largest transfer function a batched request stages in LDS (per wavefront).
 constexpr int kBatchTf = 256;
 
+
 // Persistent batch kernel: the work items of all frames of a batch form one
 // queue. A wavefront looks up the frame of each item in the request table,
 // restages that frame's transfer function in its own LDS slice when the
@@ -549,6 +550,14 @@ struct lvr_batch
     std::vector<lvr_ctx*> ctxs;
     hipEvent_t done = nullptr;
     float kernelMs = 0.f;
+
+    // per-request output frames in caller memory (null: the batch's own)
+    std::vector<uint32_t*> outputs;
+    // counters land in pinned host memory at the end of each launch
+    unsigned long long* h_pinned = nullptr;
+    size_t pinnedCap = 0;
+    // CUs kept free of the batch kernel (stream CU mask)
+    int reservedCUs = 0;
 };
 
 // The batch no longer reads these pools (its launch finished or it is going
@@ -858,6 +867,8 @@ void lvr_batch_destroy(lvr_batch* b)
     (void)hipFree(b->d_offsets);
     (void)hipFree(b->d_lists);
     (void)hipFree(b->d_counters);
+    if (b->h_pinned)
+        (void)hipHostFree(b->h_pinned);
     for (auto& e : b->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -891,6 +902,7 @@ int lvr_batch_clear(lvr_batch* b)
     b->tf.clear();
     b->tfOffset.clear();
     b->frameOffset.clear();
+    b->outputs.clear();
     b->listTotal = b->onScreen = b->pixels = 0;
     b->ran = false;
     return LVR_OK;
@@ -942,6 +954,7 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
         b->tf.push_back(make_float4(tf_rgba[4 * i], tf_rgba[4 * i + 1], tf_rgba[4 * i + 2],
                                     tf_rgba[4 * i + 3]));
     b->frameOffset.push_back(b->pixels);
+    b->outputs.push_back(nullptr);
     b->pixels += size_t(U.width) * U.height;
     b->listTotal += listTotal;
     b->onScreen += onScreen;
@@ -969,6 +982,17 @@ int lvr_batch_run(lvr_batch* b)
         (rc = bgrow(b, &b->d_lists, &b->listCap, std::max<uint64_t>(1, b->listTotal), &grew)) ||
         (rc = bgrow(b, &b->d_counters, &b->counterCap, size_t(nReq) + 2, &grew)))
         return rc;
+    if (b->pinnedCap < size_t(nReq) + 2)
+    {
+        if (b->h_pinned)
+            (void)hipHostFree(b->h_pinned);
+        b->h_pinned = nullptr;
+        b->pinnedCap = 0;
+        LVR_BHIP(b, hipHostMalloc((void**)&b->h_pinned,
+                                  sizeof(unsigned long long) * (size_t(nReq) + 2),
+                                  hipHostMallocDefault));
+        b->pinnedCap = size_t(nReq) + 2;
+    }
     // request table: positions in the batch-wide arrays
     int tiles = 0, items = 0;
     for (int r = 0; r < nReq; ++r)
@@ -979,7 +1003,8 @@ int lvr_batch_run(lvr_batch* b)
         tiles += U.tilesX * U.tilesY;
         items += U.nItems;
         U.tf = b->d_tf + b->tfOffset[size_t(r)];
-        U.frame = b->d_frames + b->frameOffset[size_t(r)];
+        U.frame = b->outputs[size_t(r)] ? b->outputs[size_t(r)]
+                                        : b->d_frames + b->frameOffset[size_t(r)];
     }
     if ((rc = bgrow(b, &b->d_counts, &b->tileCap, size_t(tiles), &grew)))
         return rc;
@@ -1031,7 +1056,9 @@ int lvr_batch_run(lvr_batch* b)
                            b->d_offsets, b->d_counters, nReq + 2);
         hipLaunchKernelGGL(binFillKernel, dim3(binBlocks), dim3(256), 0, st, b->d_reqs, nReq,
                            b->d_ranges, tiles, b->d_offsets, b->d_lists);
-        const int blocks = std::max(1, std::min(items / (kSuper * kSuper), 2 * b->numCUs));
+        // two persistent workgroups per CU the stream may use
+        const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
+        const int blocks = std::max(1, std::min(items / (kSuper * kSuper), slots));
         const dim3 grid(blocks), block(kSuper * kSuper * kWave);
         if (depth == 1)
             hipLaunchKernelGGL(batchKernel<1>, grid, block, 0, st, b->d_reqs, nReq, items,
@@ -1080,11 +1107,58 @@ int lvr_batch_run(lvr_batch* b)
     else
         LVR_BHIP(b, enqueue(b->stream));
     LVR_BHIP(b, hipEventRecord(b->ev[1], b->stream));
+    LVR_BHIP(b, hipMemcpyAsync(b->h_pinned, b->d_counters,
+                               sizeof(unsigned long long) * (size_t(nReq) + 2),
+                               hipMemcpyDeviceToHost, b->stream));
     LVR_BHIP(b, hipEventRecord(b->done, b->stream));
     for (lvr_ctx* c : b->ctxs)
         if (std::find(c->readers.begin(), c->readers.end(), b) == c->readers.end())
             c->readers.push_back(b);
     b->ran = true;
+    return LVR_OK;
+}
+
+int lvr_batch_set_output(lvr_batch* b, int request, void* device_rgba8)
+{
+    if (!b || request < 0 || size_t(request) >= b->reqs.size() || b->ran)
+        return LVR_EINVAL;
+    if (reinterpret_cast<uintptr_t>(device_rgba8) % 4u)
+        return lvr::bfail(b, LVR_EINVAL, "output frame must be 4-byte aligned");
+    b->outputs[size_t(request)] = static_cast<uint32_t*>(device_rgba8);
+    return LVR_OK;
+}
+
+int lvr_batch_reserve_cus(lvr_batch* b, int n)
+{
+    if (!b || n < 0 || n >= b->numCUs)
+        return LVR_EINVAL;
+    LVR_BHIP(b, hipSetDevice(b->device));
+    LVR_BHIP(b, hipStreamSynchronize(b->stream));
+    hipStream_t st = nullptr;
+    if (n == 0)
+        LVR_BHIP(b, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    else
+    {
+        // n CUs spread evenly over the CU index space stay off the mask
+        std::vector<uint32_t> mask(size_t((b->numCUs + 31) / 32), 0u);
+        for (int c = 0; c < b->numCUs; ++c)
+            mask[size_t(c / 32)] |= 1u << (c % 32);
+        for (int j = 0; j < n; ++j)
+        {
+            const int c = int((int64_t(j) * b->numCUs) / n);
+            mask[size_t(c / 32)] &= ~(1u << (c % 32));
+        }
+        LVR_BHIP(b, hipExtStreamCreateWithCUMask(&st, uint32_t(mask.size()), mask.data()));
+    }
+    if (b->exec)
+    {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+        b->graphKey.clear();
+    }
+    (void)hipStreamDestroy(b->stream);
+    b->stream = st;
+    b->reservedCUs = n;
     return LVR_OK;
 }
 
@@ -1108,10 +1182,7 @@ int lvr_batch_wait(lvr_batch* b, lvr_batch_stats* stats)
     LVR_BHIP(b, hipStreamSynchronize(b->stream));
     if (!b->ran)
         return lvr::bfail(b, LVR_EINVAL, "batch has not run");
-    b->h_counters.resize(b->reqs.size() + 2);
-    LVR_BHIP(b, hipMemcpy(b->h_counters.data(), b->d_counters,
-                          sizeof(unsigned long long) * b->h_counters.size(),
-                          hipMemcpyDeviceToHost));
+    b->h_counters.assign(b->h_pinned, b->h_pinned + b->reqs.size() + 2);
     LVR_BHIP(b, hipEventElapsedTime(&b->kernelMs, b->ev[0], b->ev[1]));
     if (stats)
     {
@@ -1139,8 +1210,11 @@ int lvr_batch_read(lvr_batch* b, int request, uint8_t* out_rgba8, size_t bytes,
     {
         if (bytes < need)
             return lvr::bfail(b, LVR_ERANGE, "frame buffer too small");
-        LVR_BHIP(b, hipMemcpy(out_rgba8, b->d_frames + b->frameOffset[size_t(request)], need,
-                              hipMemcpyDeviceToHost));
+        const uint32_t* src = b->outputs[size_t(request)]
+                                  ? b->outputs[size_t(request)]
+                                  : b->d_frames + b->frameOffset[size_t(request)];
+        LVR_BHIP(b, hipMemcpyAsync(out_rgba8, src, need, hipMemcpyDeviceToHost, b->stream));
+        LVR_BHIP(b, hipStreamSynchronize(b->stream));
     }
     if (samples)
         *samples = b->h_counters[2 + size_t(request)];

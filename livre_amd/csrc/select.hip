@@ -261,8 +261,13 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
         return fail(ctx, LVR_ERANGE, "visible set larger than the output buffer");
     std::vector<uint64_t> ids(static_cast<size_t>(n));
     if (n > 0)
-        LVR_HIP(ctx, hipMemcpy(ids.data(), ctx->d_selOut, sizeof(uint64_t) * n,
-                               hipMemcpyDeviceToHost));
+    {
+        // on the context's own stream: a null-stream copy would also wait
+        // for every blocking stream of the process
+        LVR_HIP(ctx, hipMemcpyAsync(ids.data(), ctx->d_selOut, sizeof(uint64_t) * n,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    }
     std::vector<std::pair<uint64_t, uint64_t>> keyed(static_cast<size_t>(n));
     for (int i = 0; i < n; ++i)
         keyed[size_t(i)] = {dfsKey(ctx->vi, ids[size_t(i)]), ids[size_t(i)]};

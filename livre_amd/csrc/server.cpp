@@ -51,7 +51,9 @@ struct Session
     float r0 = 0.f, r1 = 0.f;
     lvr_render_params params{};
     size_t bytes = 0;
-    uint32_t* d_frame = nullptr;
+    uint32_t* d_frames = nullptr; // three frames: finished + up to two in flight
+    int nextBuf = 0, shownBuf = -1;
+    int64_t shownRequest = -1; // launches on two streams may finish out of order
     bool hasFrame = false;
     uint64_t lastSamples = 0;
     uint64_t frames = 0;
@@ -61,6 +63,7 @@ struct Request
 {
     int64_t id = 0;
     int session = 0;
+    int buf = 0; // session framebuffer the launch writes
     lvr_frustum f{};
     double arrive = 0.0;
 };
@@ -93,6 +96,7 @@ struct lvr_server
     std::deque<int> waiting;
     std::deque<Request> pending;
     Flight flights[2];
+    hipStream_t copyStream = nullptr;
     std::vector<lvr_server_frame> done;
     std::vector<uint64_t> vis, ord;
     int64_t nextId = 0;
@@ -113,11 +117,19 @@ int ctxFail(lvr_server* s, lvr_ctx* ctx, int code)
     return code;
 }
 
+size_t frameBytes(int w, int h)
+{
+    return size_t(w) * size_t(h) * 4u;
+}
+
 size_t sessionBytes(int w, int h)
 {
-    // the session framebuffer + its frame in each of the two batch slots
-    return size_t(w) * size_t(h) * 4u * 3u;
+    // the finished frame + one per batch slot in flight
+    return frameBytes(w, h) * 3u;
 }
+
+// Kept free of the batch kernel for selection and brick-source kernels.
+constexpr int kServerReservedCUs = 8;
 
 void admitWaiting(lvr_server* s)
 {
@@ -126,7 +138,7 @@ void admitWaiting(lvr_server* s)
         Session& ss = s->sessions[size_t(s->waiting.front())];
         if (s->used + ss.bytes > s->budget)
             return;
-        if (hipMalloc((void**)&ss.d_frame, size_t(ss.width) * ss.height * 4u) != hipSuccess)
+        if (hipMalloc((void**)&ss.d_frames, sessionBytes(ss.width, ss.height)) != hipSuccess)
             return; // stays waiting; retried at the next close
         s->used += ss.bytes;
         ss.state = 2;
@@ -160,11 +172,18 @@ void unpin(lvr_server* s, Flight& fl)
 // Retire a finished launch: frames into the sessions' framebuffers, pins off.
 int retire(lvr_server* s, Flight& fl)
 {
+    const double t0 = nowMs();
+    struct Acc
+    {
+        double t0;
+        double* acc;
+        ~Acc() { *acc += nowMs() - t0; }
+    } acc{t0, &s->stats.retire_ms};
     lvr_batch_stats st;
     int rc = lvr_batch_wait(fl.batch, &st);
     if (rc)
         return sfail(s, rc, lvr_batch_last_error(fl.batch));
-    const hipStream_t bs = lvr::batchStream(fl.batch);
+    s->stats.gpu_ms += st.gpu_ms;
     const double end = nowMs();
     for (size_t i = 0; i < fl.reqs.size(); ++i)
     {
@@ -174,21 +193,18 @@ int retire(lvr_server* s, Flight& fl)
         if (rc)
             return sfail(s, rc, lvr_batch_last_error(fl.batch));
         Session& ss = s->sessions[size_t(r.session)];
-        if (ss.state == 2 && ss.d_frame)
+        if (ss.state == 2 && r.id > ss.shownRequest)
         {
-            if (hipMemcpyAsync(ss.d_frame, lvr::batchFrame(fl.batch, int(i)),
-                               size_t(ss.width) * ss.height * 4u, hipMemcpyDeviceToDevice,
-                               bs) != hipSuccess)
-                return sfail(s, LVR_EHIP, "session frame copy");
+            ss.shownBuf = r.buf;
+            ss.shownRequest = r.id;
             ss.hasFrame = true;
             ss.lastSamples = samples;
-            ++ss.frames;
         }
+        if (ss.state == 2)
+            ++ss.frames;
         s->stats.samples += samples;
         report(s, r, 0, samples, fl.launch, end);
     }
-    if (hipStreamSynchronize(bs) != hipSuccess)
-        return sfail(s, LVR_EHIP, "session frame copy");
     s->stats.frames += fl.reqs.size();
     unpin(s, fl);
     fl.reqs.clear();
@@ -200,7 +216,7 @@ int retire(lvr_server* s, Flight& fl)
 // flight and adds it to the batch. Returns LVR_OK, LVR_ENOMEM when the pool
 // cannot hold the request next to what is pinned (try again later), or an
 // error.
-int stage(lvr_server* s, Flight& fl, const Request& r)
+int stage(lvr_server* s, Flight& fl, Request& r)
 {
     Session& ss = s->sessions[size_t(r.session)];
     Volume& v = s->volumes[size_t(ss.volume)];
@@ -212,13 +228,24 @@ int stage(lvr_server* s, Flight& fl, const Request& r)
         s->ord.resize(cap);
     }
     int n = 0;
+    double t = nowMs();
     int rc = lvr_select_visibles(ctx, &r.f, uint32_t(ss.height), 1.0f, 0, ctx->vi.depth - 1,
                                  s->vis.data(), int(cap), &n);
+    s->stats.select_ms += nowMs() - t;
     if (rc)
         return ctxFail(s, ctx, rc);
+    t = nowMs();
     rc = lvr_order(&ctx->vi, &r.f, s->vis.data(), n, s->ord.data());
+    s->stats.order_ms += nowMs() - t;
     if (rc)
         return sfail(s, rc, "order");
+    t = nowMs();
+    struct Acc
+    {
+        double t0;
+        double* acc;
+        ~Acc() { *acc += nowMs() - t0; }
+    };
     std::vector<uint64_t> missing;
     for (int i = 0; i < n; ++i)
         if (!ctx->resident.count(s->ord[size_t(i)]))
@@ -262,10 +289,18 @@ int stage(lvr_server* s, Flight& fl, const Request& r)
             (void)lvr::pinBrick(ctx, s->ord[size_t(i)], 1);
             fl.pins.emplace_back(ss.volume, s->ord[size_t(i)]);
         }
+    s->stats.residency_ms += nowMs() - t;
+    Acc addTime{nowMs(), &s->stats.add_ms};
     const int vp[4] = {0, 0, ss.width, ss.height};
     int req = 0;
     rc = lvr_batch_add(fl.batch, ctx, &r.f, vp, s->ord.data(), n, &ss.params, ss.tf.data(),
                        int(ss.tf.size() / 4), ss.r0, ss.r1, &req);
+    if (rc)
+        return sfail(s, rc, lvr_batch_last_error(fl.batch));
+    // the next of the three framebuffers that is neither shown nor in flight
+    r.buf = ss.nextBuf;
+    ss.nextBuf = (ss.nextBuf + 1) % 3;
+    rc = lvr_batch_set_output(fl.batch, req, ss.d_frames + size_t(r.buf) * ss.width * ss.height);
     if (rc)
         return sfail(s, rc, lvr_batch_last_error(fl.batch));
     return LVR_OK;
@@ -288,12 +323,17 @@ int launch(lvr_server* s, int* launched)
     if (rc)
         return sfail(s, rc, lvr_batch_last_error(fl->batch));
     std::deque<Request> keep;
+    std::vector<int> inFlight; // sessions with a request in this launch
     while (!s->pending.empty())
     {
         Request r = s->pending.front();
         s->pending.pop_front();
         const Session& ss = s->sessions[size_t(r.session)];
-        if (ss.state != 2 || int(fl->reqs.size()) >= s->maxBatch)
+        // one request per session per launch keeps at most two of its
+        // frames in flight, next to the finished one
+        const bool again =
+            std::find(inFlight.begin(), inFlight.end(), r.session) != inFlight.end();
+        if (ss.state != 2 || int(fl->reqs.size()) >= s->maxBatch || again)
         {
             if (ss.state == 0)
                 report(s, r, LVR_EINVAL, 0, 0.0, nowMs()); // session closed
@@ -306,6 +346,7 @@ int launch(lvr_server* s, int* launched)
         if (rc == LVR_OK)
         {
             fl->reqs.push_back(r);
+            inFlight.push_back(r.session);
             continue;
         }
         // undo this request's pins
@@ -324,7 +365,9 @@ int launch(lvr_server* s, int* launched)
     s->pending.swap(keep);
     if (fl->reqs.empty())
         return LVR_OK;
+    const double tRun = nowMs();
     rc = lvr_batch_run(fl->batch);
+    s->stats.run_ms += nowMs() - tRun;
     if (rc)
     {
         unpin(s, *fl);
@@ -358,9 +401,16 @@ int lvr_server_create(int device, size_t budget_bytes, int max_batch, lvr_server
     s->device = device;
     s->budget = budget_bytes;
     s->maxBatch = max_batch;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->copyStream, hipStreamNonBlocking) != hipSuccess)
+    {
+        lvr_server_destroy(s);
+        return LVR_EHIP;
+    }
     for (Flight& f : s->flights)
         if (lvr_batch_create(device, &f.batch) != LVR_OK ||
-            lvr_batch_set_graph(f.batch, 0) != LVR_OK)
+            lvr_batch_set_graph(f.batch, 0) != LVR_OK ||
+            lvr_batch_reserve_cus(f.batch, kServerReservedCUs) != LVR_OK)
         {
             lvr_server_destroy(s);
             return LVR_EHIP;
@@ -387,7 +437,9 @@ void lvr_server_destroy(lvr_server* s)
         if (v.ctx)
             v.ctx->pinsGuardReaders = false;
     for (Session& ss : s->sessions)
-        (void)hipFree(ss.d_frame);
+        (void)hipFree(ss.d_frames);
+    if (s->copyStream)
+        (void)hipStreamDestroy(s->copyStream);
     delete s;
 }
 
@@ -485,8 +537,8 @@ int lvr_server_close(lvr_server* s, int session)
     if (ss.state == 2)
     {
         s->used -= ss.bytes;
-        (void)hipFree(ss.d_frame);
-        ss.d_frame = nullptr;
+        (void)hipFree(ss.d_frames);
+        ss.d_frames = nullptr;
     }
     else
         s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), session),
@@ -605,7 +657,11 @@ int lvr_server_read(lvr_server* s, int session, uint8_t* out_rgba8, size_t bytes
         const size_t need = size_t(ss.width) * ss.height * 4u;
         if (bytes < need)
             return sfail(s, LVR_ERANGE, "frame buffer too small");
-        if (hipMemcpy(out_rgba8, ss.d_frame, need, hipMemcpyDeviceToHost) != hipSuccess)
+        // on the server's own stream (a null-stream copy would wait for the
+        // launches in flight)
+        if (hipMemcpyAsync(out_rgba8, ss.d_frames + size_t(ss.shownBuf) * ss.width * ss.height,
+                           need, hipMemcpyDeviceToHost, s->copyStream) != hipSuccess ||
+            hipStreamSynchronize(s->copyStream) != hipSuccess)
             return sfail(s, LVR_EHIP, "session frame read");
     }
     if (samples)

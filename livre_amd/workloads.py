@@ -3,6 +3,8 @@ the tests: the volumes (generated on the device straight into the brick
 pool by the procedural data sources), seeded request lists and camera paths.
 
 Volumes
+  blobs256_u8    256^3 uint8, 64 anisotropic Gaussian blobs + 10% noise, seed 1,
+                 32^3 bricks (the CPU-runnable configuration)
   blobs512_u8    512^3 uint8, 64 anisotropic Gaussian blobs + 10% noise, seed 5
   noise1024_u16  1024^3 uint16, 5-octave value noise, log-normal remap, seed 6
   sines1536_f32  1536^3 float32, mean of 8 low-frequency sines in [-1, 1],
@@ -21,6 +23,7 @@ from . import (DT_FLOAT, DT_UINT8, DT_UINT16, FIELD_BLOBS, FIELD_SINES, Context,
 from . import scenes
 
 VOLUMES = {
+    "blobs256_u8": dict(voxels=256, block=32, dtype=DT_UINT8, source=("blobs", 64, 1, 0.1)),
     "blobs512_u8": dict(voxels=512, block=64, dtype=DT_UINT8, source=("blobs", 64, 5, 0.1)),
     "noise1024_u16": dict(voxels=1024, block=64, dtype=DT_UINT16, source=("noise", 6)),
     "sines1536_f32": dict(voxels=1536, block=48, dtype=DT_FLOAT, source=("sines", 8, 7)),
@@ -29,10 +32,27 @@ VOLUMES = {
 BATCH_VOLUMES = ("blobs512_u8", "noise1024_u16", "sines1536_f32", "noise2048_u16")
 
 
-class Volume:
-    """A volume resident in its own brick pool (every brick of every level)."""
+SERVE_VOLUMES = ("blobs256_u8", "noise2048_u16", "blobs512_u8", "noise1024_u16",
+                 "sines1536_f32")
+VIEWPORTS = ((1280, 720), (1920, 1080), (2560, 1440), (3840, 2160))
 
-    def __init__(self, name, extra_slots=64, pool_bricks=None):
+
+def source_params(name):
+    """(noise, field) parameters of a volume's procedural source."""
+    src = VOLUMES[name]["source"]
+    if src[0] == "noise":
+        return noise_params(src[1], octaves=5, base_cells=8, gain=1.5), None
+    if src[0] == "blobs":
+        return None, field_params(FIELD_BLOBS, src[1], src[2], src[3])
+    return None, field_params(FIELD_SINES, src[1], src[2])
+
+
+class Volume:
+    """A volume in its own brick pool: every brick of every level generated
+    on the device (generate=True), or an empty pool of pool_bricks slots that
+    a server fills on demand."""
+
+    def __init__(self, name, extra_slots=64, pool_bricks=None, generate=True):
         spec = VOLUMES[name]
         self.name = name
         self.vi = volume_info(spec["voxels"], spec["block"], 1, spec["dtype"])
@@ -40,18 +60,17 @@ class Volume:
         self.range = scenes.type_range(spec["dtype"])
         self.vmax = spec["voxels"]
         n = pool_bricks if pool_bricks is not None else len(self.ids) + extra_slots
-        self.ctx = Context(self.vi, slot_bytes(self.vi) * n)
+        self.pool_bricks = n
+        self.pool_bytes = slot_bytes(self.vi) * n
+        self.ctx = Context(self.vi, self.pool_bytes)
+        self.noise, self.field = source_params(name)
         t0 = time.perf_counter()
-        src = spec["source"]
-        if src[0] == "noise":
-            self.ctx.generate_noise(noise_params(src[1], octaves=5, base_cells=8, gain=1.5),
-                                    self.ids)
-        elif src[0] == "blobs":
-            self.ctx.generate_field(field_params(FIELD_BLOBS, src[1], src[2], src[3]),
-                                    self.ids)
-        else:
-            self.ctx.generate_field(field_params(FIELD_SINES, src[1], src[2]), self.ids)
-        self.ctx.sync()
+        if generate:
+            if self.noise is not None:
+                self.ctx.generate_noise(self.noise, self.ids)
+            else:
+                self.ctx.generate_field(self.field, self.ids)
+            self.ctx.sync()
         self.gen_s = time.perf_counter() - t0
 
     def close(self):
@@ -97,3 +116,47 @@ def prepare(request, vol, sse=1.0):
 
 def samples_per_ray(request, vol):
     return max(1, int(round(request["rate"] * vol.vmax)))
+
+
+class RandomWalk:
+    """Seeded random-walk camera around the volume: the view direction turns
+    by a few degrees and the distance drifts within [1.5, 3] every frame."""
+
+    def __init__(self, seed, width, height):
+        self.rng = np.random.default_rng(seed)
+        d = self.rng.normal(size=3)
+        self.dir = d / np.linalg.norm(d)
+        self.r = self.rng.uniform(1.5, 3.0)
+        self.width, self.height = width, height
+
+    def next(self):
+        step = self.rng.normal(size=3) * 0.04
+        d = self.dir + step - np.dot(step, self.dir) * self.dir
+        self.dir = d / np.linalg.norm(d)
+        self.r = float(np.clip(self.r + self.rng.normal() * 0.03, 1.5, 3.0))
+        return look_from(tuple(self.dir * self.r), self.width, self.height)[2]
+
+
+def serve_sessions(n=32, seed=9, volumes=SERVE_VOLUMES):
+    """Sessions: a volume, a viewport between 720p and 4K, a 256-entry TF and
+    a random-walk camera each (seed + session)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        w, h = VIEWPORTS[int(rng.integers(len(VIEWPORTS)))]
+        out.append(dict(volume=volumes[int(rng.integers(len(volumes)))], width=w, height=h,
+                        tf=scenes.piecewise_tf(256, seed=2000 + k, points=6, max_alpha=0.5),
+                        walk=RandomWalk(seed + k, w, h)))
+    return out
+
+
+def poisson_arrivals(rate, duration_s, n_sessions, seed):
+    """(time_ms, session) of a Poisson process of `rate` requests per second,
+    each for a session drawn uniformly."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1000.0 / rate)
+        if t > duration_s * 1000.0:
+            return out
+        out.append((t, int(rng.integers(n_sessions))))

@@ -352,6 +352,106 @@ def serve_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def ooc_workload(args):
+    """8192^3 uint8 (512 GB logical) from the host value-noise DataSource,
+    64^3 bricks, depth 8, pool capped at --pool-gb; bricks stream through
+    pinned memory on the upload stream while frames render the coarser-LOD
+    rendering set; 3840x2160, 300-frame fly-through."""
+    import livre_amd
+    from livre_amd import scenes, workloads
+    livre_amd.lib()
+    W4, H4 = 3840, 2160
+    vi = livre_amd.volume_info(8192, 64, 1, livre_amd.DT_UINT8)
+    slot = livre_amd.slot_bytes(vi)
+    n_slots = int(args.pool_gb * 1e9) // slot
+    ctx = livre_amd.Context(vi, slot * n_slots)
+    noise = livre_amd.noise_params(8, octaves=5, base_cells=8, gain=1.5)
+    tf = scenes.piecewise_tf(256, seed=8, points=6, max_alpha=0.3)
+    ctx.set_tf(tf, (0.0, 255.0))
+    ctx.set_timing(True)
+    threads = max(1, min(14, (os.cpu_count() or 2) - 2))
+    rows = []
+    with livre_amd.Stream(ctx, noise, threads=threads, staging_bricks=256) as st:
+        t0 = time.perf_counter()
+        # levels 0..3 (585 bricks) before the first frame: every region has a
+        # fallback from then on
+        lv = []
+        for level in range(4):
+            n = 1 << level
+            lv += [scenes.node_id(level, x, y, z) for z in range(n) for y in range(n)
+                   for x in range(n)]
+        st.request(lv)
+        st.wait()
+        log("preloaded %d coarse bricks in %.2f s (%d source threads)"
+            % (len(lv), time.perf_counter() - t0, threads))
+        start = st.stats()
+        t_start = time.perf_counter()
+        for t in range(args.frames):
+            f0 = time.perf_counter()
+            pos, ahead = workloads.flythrough(t, args.frames)
+            up = (0.0, 1.0, 0.0)
+            mv = scenes.look_at(tuple(pos), tuple(ahead), up)
+            proj = scenes.perspective(60.0, W4 / H4, 0.01, 6.0)
+            fr = livre_amd.frustum(mv, proj)
+            vis = ctx.select_visibles(fr, H4, 1.0, 0, vi.depth - 1)
+            st.request(livre_amd.order(vi, fr, vis), ancestors=True)
+            st.pump()
+            rset, avail = st.rendering_set(vis)
+            ordered = livre_amd.order(vi, fr, rset)
+            # samples per ray from the finest rendered level, as
+            # RayCastRenderer::onFrameStart does when none is given
+            # (RayCastRenderer.cpp:216-234)
+            max_lod = max(int(i) & 0xF for i in ordered) if len(ordered) else 0
+            n_samples = max(8192 >> (vi.depth - max_lod - 1), 512)
+            ctx.pin(ordered, 1)
+            ctx.render(fr, W4, H4, ordered, livre_amd.render_params(n_samples))
+            # transfers overlap the ray casting of this frame
+            while not ctx.query():
+                st.pump()
+                time.sleep(0.0002)
+            fs = ctx.frame_stats()
+            ctx.pin(ordered, -1)
+            rows.append((time.perf_counter() - f0, fs.raycast_ms, fs.samples, len(vis), avail,
+                         len(rset), fs.bricks, max_lod))
+            if t % 30 == 0:
+                s = st.stats()
+                log("frame %d: visible %d, at full LOD %d, rendered %d (finest level %d), "
+                    "raycast %.1f ms, frame %.1f ms, loaded %d"
+                    % (t, len(vis), avail, len(rset), max_lod, fs.raycast_ms,
+                       rows[-1][0] * 1e3, s.bricks_loaded))
+        total = time.perf_counter() - t_start
+        end = st.stats()
+    pool_used = ctx.stats()[1]
+    ctx.close()
+    a = np.array(rows, dtype=np.float64)
+    loaded = end.bricks_loaded - start.bricks_loaded
+    result = {
+        "metric": "out-of-core fly-through: fps, Gsamples/s, streamed GB/s, full-LOD fraction",
+        "value": float(a[:, 2].sum() / total / 1e9), "unit": "Gsamples/s", "n_gpus": 1,
+        "higher_is_better": True, "vs_baseline": None, "dtype": "u8",
+        "data": "synthetic: 8192^3 u8 value noise (seed 8) from the host DataSource",
+        "config": {"workload": "8192^3 uint8 (512 GB logical), 64^3 bricks + 1 halo, depth 8, "
+                               "pool %.0f GB (%d slots) with LRU, 3840x2160, %d-frame "
+                               "fly-through, samples/unit from the finest rendered level "
+                               "(RayCastRenderer rule), sse 1"
+                               % (args.pool_gb, n_slots, args.frames),
+                   "streaming": "%d host source threads, 256 pinned staging bricks, "
+                                "hipMemcpyAsync + tiling on the upload stream" % threads},
+        "frames": len(rows), "fps": len(rows) / total,
+        "p50_ms": float(np.percentile(a[:, 0], 50) * 1e3),
+        "p99_ms": float(np.percentile(a[:, 0], 99) * 1e3),
+        "raycast_ms_mean": float(a[:, 1].mean()),
+        "visible_mean": float(a[:, 3].mean()),
+        "full_lod_fraction_mean": float(np.mean(a[:, 4] / np.maximum(a[:, 3], 1))),
+        "full_lod_fraction_last": float(a[-1, 4] / max(a[-1, 3], 1)),
+        "finest_rendered_level_mean": float(a[:, 7].mean()),
+        "bricks_streamed": int(loaded),
+        "streamed_gb_per_s": loaded * vi.max_block[0] ** 3 / total / 1e9,
+        "pool_bricks_resident": int(pool_used),
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,14 +459,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("render", "batch", "serve"), default="render",
+    ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc"),
+                    default="render",
                     help="render: the 1080p single-volume frame loop (default); "
                          "batch: uneven requests, plain vs one batched launch; "
-                         "serve: 32 sessions under Poisson arrivals")
+                         "serve: 32 sessions under Poisson arrivals; "
+                         "ooc: out-of-core 8192^3 fly-through")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--max-batch", type=int, default=16)
     ap.add_argument("--pool-frac", type=float, default=1.0)
+    ap.add_argument("--pool-gb", type=float, default=200.0)
+    ap.add_argument("--frames", type=int, default=300)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the brick pool, kernels and measurements are "
@@ -379,6 +483,9 @@ def main():
         return
     if args.workload == "serve":
         serve_workload(args)
+        return
+    if args.workload == "ooc":
+        ooc_workload(args)
         return
     if args.steps is None:
         args.steps = FRAMES

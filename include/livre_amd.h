@@ -100,6 +100,9 @@ void lvr_ctx_destroy(lvr_ctx* ctx);
 const char* lvr_last_error(const lvr_ctx* ctx);
 /* Synchronise the context's stream. */
 int lvr_ctx_sync(lvr_ctx* ctx);
+/* 1 when the work queued on the context's render stream has finished, 0
+ * while it runs (never blocks). */
+int lvr_ctx_query(lvr_ctx* ctx);
 
 /* HBM bytes one brick occupies in the pool (the pool stores bricks in its own
  * tiled layout, padded, 256-byte aligned): pool_bytes = n * this holds n
@@ -394,6 +397,70 @@ int lvr_server_poll(lvr_server* server, lvr_server_frame* out, int cap, int* n);
 int lvr_server_read(lvr_server* server, int session, uint8_t* out_rgba8,
                     size_t bytes, uint64_t* samples);
 int lvr_server_get_stats(lvr_server* server, lvr_server_stats* out);
+
+/* --------------------------------------------------------- out-of-core */
+/* DataSourcePlugin::getData as a callback (DataSourcePlugin.h:81): write the
+ * brick of node_id, bytes = maximumBlockSize^3 * bytesPerVoxel, x fastest;
+ * return 0 or an error code. Called from the stream's worker threads. */
+typedef int (*lvr_brick_source_fn)(void* user, uint64_t node_id, void* out,
+                                   size_t bytes);
+/* The value-noise source as such a callback; user = lvr_noise_source_data. */
+typedef struct
+{
+    lvr_volume_info info;
+    lvr_noise_params params;
+} lvr_noise_source_data;
+int lvr_noise_source(void* user, uint64_t node_id, void* out, size_t bytes);
+
+/* Asynchronous brick streaming into a context's pool (the reference's async
+ * RenderPipeline + DataUploadFilter, RenderPipeline.cpp:160-225,
+ * DataUploadFilter.cpp:104-120): `threads` host threads call the source into
+ * `staging_bricks` pinned buffers; lvr_stream_pump (caller's thread) takes
+ * pool slots (LRU, never a pinned brick) and issues hipMemcpyAsync + the
+ * layout conversion on the context's upload stream; bricks count as
+ * resident once that stream has passed them. While a stream exists, a frame
+ * must pin the bricks it renders (lvr_brick_pin) until it has finished. */
+typedef struct lvr_stream lvr_stream;
+typedef struct
+{
+    uint64_t requested, bricks_loaded, bytes_loaded, transfers;
+    uint64_t source_errors, pool_full;
+    uint64_t queued, generating, ready, in_flight;
+} lvr_stream_stats;
+int lvr_stream_create(lvr_ctx* ctx, lvr_brick_source_fn source, void* user,
+                      int threads, int staging_bricks, lvr_stream** out);
+void lvr_stream_destroy(lvr_stream* stream);
+const char* lvr_stream_last_error(const lvr_stream* stream);
+/* Replace the wish list: the given bricks that are neither resident nor on
+ * their way, coarse levels first, then in the given order. Resident bricks
+ * among the given ones become the most recently used (evicted last). */
+int lvr_stream_request(lvr_stream* stream, const uint64_t* ids, int n);
+/* Same; with ancestors != 0 the wish list also holds every ancestor of the
+ * given bricks (coarse first), so fallbacks refine level by level while the
+ * full-resolution bricks stream in. */
+int lvr_stream_request_ex(lvr_stream* stream, const uint64_t* ids, int n,
+                          int ancestors);
+/* Non-blocking: retire finished transfers, start the generated ones.
+ * *completed = bricks that became resident. */
+int lvr_stream_pump(lvr_stream* stream, int* completed);
+/* Pump until every wished brick is resident (blocks). */
+int lvr_stream_wait(lvr_stream* stream);
+/* RenderingSetGenerator::generateRenderingSet (RenderingSetGeneratorFilter.cpp:
+ * 39-108): every visible brick, or its nearest resident ancestor; then,
+ * when that set's size differs from n, the bricks with an ancestor in the
+ * set are dropped. Sorted by id (the caller orders front to back).
+ * *n_available = visible bricks resident themselves. */
+int lvr_stream_rendering_set(lvr_stream* stream, const uint64_t* visible,
+                             int n, uint64_t* out, int cap, int* n_out,
+                             int* n_available);
+/* The same rule against an explicit resident set (no device). */
+int lvr_rendering_set(const uint64_t* visible, int n, const uint64_t* resident,
+                      int n_resident, uint64_t* out, int cap, int* n_out,
+                      int* n_available);
+int lvr_stream_get_stats(lvr_stream* stream, lvr_stream_stats* out);
+/* Pin (delta = +1) or unpin (-1) resident bricks: pinned bricks are never
+ * evicted (a texture held by a renderer, Cache.cpp:148-149). */
+int lvr_brick_pin(lvr_ctx* ctx, const uint64_t* ids, int n, int delta);
 
 /* ------------------------------------------------------------- histogram */
 /* Per-frame histogram of the given bricks (HistogramObject.cpp:140-224 +

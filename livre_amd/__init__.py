@@ -23,7 +23,7 @@ LVR_OK, LVR_EINVAL, LVR_EHIP, LVR_ENOMEM, LVR_ENOENT, LVR_ERANGE = 0, -1, -2, -3
 # every entry point declared in include/livre_amd.h
 EXPORTS = (
     "lvr_fill_regular_volume_info", "lvr_node_world_box", "lvr_frustum_init",
-    "lvr_ctx_create", "lvr_ctx_destroy", "lvr_last_error", "lvr_ctx_sync",
+    "lvr_ctx_create", "lvr_ctx_destroy", "lvr_last_error", "lvr_ctx_sync", "lvr_ctx_query",
     "lvr_pool_set_policy", "lvr_pool_stats", "lvr_brick_upload",
     "lvr_brick_upload_many", "lvr_brick_evict", "lvr_brick_resident",
     "lvr_brick_download", "lvr_noise_params_init", "lvr_noise_brick_host",
@@ -40,6 +40,9 @@ EXPORTS = (
     "lvr_server_set_coalesce", "lvr_server_add_volume", "lvr_server_open", "lvr_server_close",
     "lvr_server_submit", "lvr_server_step", "lvr_server_drain", "lvr_server_poll",
     "lvr_server_read", "lvr_server_get_stats",
+    "lvr_noise_source", "lvr_stream_create", "lvr_stream_destroy", "lvr_stream_last_error",
+    "lvr_stream_request", "lvr_stream_request_ex", "lvr_stream_pump", "lvr_stream_wait", "lvr_stream_rendering_set",
+    "lvr_rendering_set", "lvr_stream_get_stats", "lvr_brick_pin",
 )
 
 
@@ -131,10 +134,20 @@ class NoiseParams(ctypes.Structure):
                 ("base_cells", ctypes.c_int32), ("remap", ctypes.c_uint16 * 4097)]
 
 
+class StreamStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in (
+        "requested", "bricks_loaded", "bytes_loaded", "transfers", "source_errors",
+        "pool_full", "queued", "generating", "ready", "in_flight")]
+
+
 class FieldParams(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("count", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("noise", ctypes.c_float),
                 ("pad", ctypes.c_int32)]
+
+
+class NoiseSourceData(ctypes.Structure):
+    _fields_ = [("info", VolumeInfo), ("params", NoiseParams)]
 
 
 _lib = None
@@ -160,6 +173,7 @@ def lib():
     L.lvr_last_error.argtypes = [c.c_void_p]
     L.lvr_last_error.restype = c.c_char_p
     L.lvr_ctx_sync.argtypes = [c.c_void_p]
+    L.lvr_ctx_query.argtypes = [c.c_void_p]
     L.lvr_pool_set_policy.argtypes = [c.c_void_p, c.c_int]
     L.lvr_pool_stats.argtypes = [c.c_void_p, u64p, u64p, u64p]
     L.lvr_brick_upload.argtypes = [c.c_void_p, c.c_uint64, c.c_void_p, c.c_size_t]
@@ -229,6 +243,22 @@ def lib():
     L.lvr_server_poll.argtypes = [c.c_void_p, P(ServerFrame), c.c_int, P(c.c_int)]
     L.lvr_server_read.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, u64p]
     L.lvr_server_get_stats.argtypes = [c.c_void_p, P(ServerStats)]
+    L.lvr_stream_create.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int, c.c_int,
+                                    P(c.c_void_p)]
+    L.lvr_stream_destroy.argtypes = [c.c_void_p]
+    L.lvr_stream_destroy.restype = None
+    L.lvr_stream_last_error.argtypes = [c.c_void_p]
+    L.lvr_stream_last_error.restype = c.c_char_p
+    L.lvr_stream_request.argtypes = [c.c_void_p, u64p, c.c_int]
+    L.lvr_stream_request_ex.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
+    L.lvr_stream_pump.argtypes = [c.c_void_p, P(c.c_int)]
+    L.lvr_stream_wait.argtypes = [c.c_void_p]
+    L.lvr_stream_rendering_set.argtypes = [c.c_void_p, u64p, c.c_int, u64p, c.c_int,
+                                           P(c.c_int), P(c.c_int)]
+    L.lvr_rendering_set.argtypes = [u64p, c.c_int, u64p, c.c_int, u64p, c.c_int, P(c.c_int),
+                                    P(c.c_int)]
+    L.lvr_stream_get_stats.argtypes = [c.c_void_p, P(StreamStats)]
+    L.lvr_brick_pin.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
     L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
     _lib = L
@@ -390,6 +420,13 @@ class Context:
     def sync(self):
         self._check(lib().lvr_ctx_sync(self._h), "sync")
 
+    def query(self):
+        """True once the queued work of the context has finished."""
+        rc = lib().lvr_ctx_query(self._h)
+        if rc < 0:
+            self._check(rc, "query")
+        return rc == 1
+
     def set_policy(self, policy):
         self._check(lib().lvr_pool_set_policy(self._h, policy), "set_policy")
 
@@ -422,6 +459,11 @@ class Context:
         self._check(lib().lvr_brick_download(self._h, int(node_id), out.ctypes.data,
                                              out.nbytes), "download")
         return out
+
+    def pin(self, ids, delta=1):
+        """Pin (+1) or unpin (-1) resident bricks (never evicted while pinned)."""
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_brick_pin(self._h, pids, len(ids), delta), "pin")
 
     def generate_noise(self, params, ids):
         ids, pids = _u64(ids)
@@ -698,4 +740,84 @@ class Server:
     def stats(self):
         st = ServerStats()
         self._check(lib().lvr_server_get_stats(self._h, ctypes.byref(st)), "stats")
+        return st
+
+
+def rendering_set(visible, resident):
+    """RenderingSetGenerator rule against an explicit resident set:
+    (ids, n_available)."""
+    visible, pv = _u64(visible)
+    resident, pr = _u64(resident)
+    out = np.zeros(max(1, len(visible)), np.uint64)
+    n, avail = ctypes.c_int(), ctypes.c_int()
+    rc = lib().lvr_rendering_set(pv, len(visible), pr, len(resident),
+                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(out),
+                                 ctypes.byref(n), ctypes.byref(avail))
+    if rc != LVR_OK:
+        raise LivreError("rendering set failed (%d)" % rc)
+    return out[:n.value].copy(), avail.value
+
+
+class Stream:
+    """Out-of-core streaming of a context's bricks from the host value-noise
+    DataSource (lvr_stream_*)."""
+
+    def __init__(self, ctx, noise, threads=4, staging_bricks=64):
+        self.ctx = ctx
+        self._data = NoiseSourceData(ctx.vi, noise)
+        self._h = ctypes.c_void_p()
+        fn = ctypes.cast(lib().lvr_noise_source, ctypes.c_void_p)
+        rc = lib().lvr_stream_create(ctx._h, fn, ctypes.cast(ctypes.pointer(self._data),
+                                                             ctypes.c_void_p),
+                                     threads, staging_bricks, ctypes.byref(self._h))
+        if rc != LVR_OK:
+            raise LivreError("lvr_stream_create failed (%d)" % rc)
+
+    def close(self):
+        if self._h:
+            lib().lvr_stream_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != LVR_OK:
+            msg = lib().lvr_stream_last_error(self._h).decode()
+            raise LivreError("%s failed (%d): %s" % (what, rc, msg))
+
+    def request(self, ids, ancestors=False):
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_stream_request_ex(self._h, pids, len(ids), 1 if ancestors else 0),
+                    "request")
+
+    def pump(self):
+        n = ctypes.c_int()
+        self._check(lib().lvr_stream_pump(self._h, ctypes.byref(n)), "pump")
+        return n.value
+
+    def wait(self):
+        self._check(lib().lvr_stream_wait(self._h), "wait")
+
+    def rendering_set(self, visible):
+        visible, pv = _u64(visible)
+        out = np.zeros(max(1, len(visible)), np.uint64)
+        n, avail = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().lvr_stream_rendering_set(
+            self._h, pv, len(visible), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+            len(out), ctypes.byref(n), ctypes.byref(avail)), "rendering_set")
+        return out[:n.value].copy(), avail.value
+
+    def stats(self):
+        st = StreamStats()
+        self._check(lib().lvr_stream_get_stats(self._h, ctypes.byref(st)), "stats")
         return st

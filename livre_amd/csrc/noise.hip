@@ -222,17 +222,125 @@ __global__ void noiseBricksKernel(NoiseShape s, const uint16_t* __restrict__ rem
     }
 }
 
+// Host generation of one brick, separable: per octave, the lattice values
+// the brick touches are hashed once, interpolated along x into rows, the rows
+// along y into planes, and each voxel takes the final lerp along z. Every
+// value goes through the same integer operations in the same order as
+// noiseVoxel (x, then y, then z), so the result equals the device brick.
 template <typename T>
-void noiseBrickHost(const NoiseShape& s, const uint16_t* remap,
-                    const BrickGeo& g, T* out)
+void noiseBrickHost(const NoiseShape& s, const uint16_t* remap, const BrickGeo& g, T* out)
+{
+    const uint32_t shift = s.depth - 1u - g.level;
+    const int64_t S = int64_t(1) << shift;
+    const int nx = int(g.dim[0]), ny = int(g.dim[1]), nz = int(g.dim[2]);
+    int octaves = 0;
+    for (int o = 0; o < s.octaves; ++o)
+        if (s.cellFinest[o] >= 2 * S || o == 0)
+            octaves = o + 1;
+    // per axis and octave: lattice cell and fade weight of every voxel
+    std::vector<int64_t> ci[3], fr[3];
+    const int n[3] = {nx, ny, nz};
+    for (int a = 0; a < 3; ++a)
+    {
+        ci[a].resize(size_t(octaves) * n[a]);
+        fr[a].resize(size_t(octaves) * n[a]);
+        for (int i = 0; i < n[a]; ++i)
+        {
+            int64_t c = (2 * (g.origin[a] + i) + 1) * S * 32768;
+            const int64_t lo = 32768;
+            const int64_t hi = (2 * int64_t(s.voxels[a]) - 1) * 32768;
+            c = c < lo ? lo : (c > hi ? hi : c);
+            for (int o = 0; o < octaves; ++o)
+            {
+                const int64_t cc = (c * s.cellMult[o]) >> 32;
+                ci[a][size_t(o) * n[a] + i] = cc >> 16;
+                fr[a][size_t(o) * n[a] + i] = fade(cc & 0xFFFF);
+            }
+        }
+    }
+    std::vector<int64_t> sum(size_t(nx) * ny * nz, 0);
+    std::vector<int64_t> lat, rows, planes;
+    for (int o = 0; o < octaves; ++o)
+    {
+        const int64_t* cx = &ci[0][size_t(o) * nx];
+        const int64_t* cy = &ci[1][size_t(o) * ny];
+        const int64_t* cz = &ci[2][size_t(o) * nz];
+        const int64_t* fx = &fr[0][size_t(o) * nx];
+        const int64_t* fy = &fr[1][size_t(o) * ny];
+        const int64_t* fz = &fr[2][size_t(o) * nz];
+        // lattice range touched (cells are monotone in the voxel index)
+        const int64_t x0 = cx[0], y0 = cy[0], z0 = cz[0];
+        const int lx = int(cx[nx - 1] - x0) + 2, ly = int(cy[ny - 1] - y0) + 2,
+                  lz = int(cz[nz - 1] - z0) + 2;
+        lat.resize(size_t(lx) * ly * lz);
+        for (int k = 0; k < lz; ++k)
+            for (int j = 0; j < ly; ++j)
+                for (int i = 0; i < lx; ++i)
+                    lat[(size_t(k) * ly + j) * lx + i] = lattice(s.seed, o, x0 + i, y0 + j, z0 + k);
+        // x-interpolated rows for every lattice (y, z) and voxel x
+        rows.resize(size_t(lz) * ly * nx);
+        for (int k = 0; k < lz; ++k)
+            for (int j = 0; j < ly; ++j)
+            {
+                const int64_t* L = &lat[(size_t(k) * ly + j) * lx];
+                int64_t* R = &rows[(size_t(k) * ly + j) * nx];
+                for (int x = 0; x < nx; ++x)
+                {
+                    const int i = int(cx[x] - x0);
+                    R[x] = lerp16(L[i], L[i + 1], fx[x]);
+                }
+            }
+        // y-interpolated planes for every lattice z and voxel (x, y)
+        planes.resize(size_t(lz) * ny * nx);
+        for (int k = 0; k < lz; ++k)
+            for (int y = 0; y < ny; ++y)
+            {
+                const int j = int(cy[y] - y0);
+                const int64_t* R0 = &rows[(size_t(k) * ly + j) * nx];
+                const int64_t* R1 = &rows[(size_t(k) * ly + j + 1) * nx];
+                int64_t* P = &planes[(size_t(k) * ny + y) * nx];
+                for (int x = 0; x < nx; ++x)
+                    P[x] = lerp16(R0[x], R1[x], fy[y]);
+            }
+        // z lerp per voxel, accumulated like octaveSum
+        for (int z = 0; z < nz; ++z)
+        {
+            const int k = int(cz[z] - z0);
+            for (int y = 0; y < ny; ++y)
+            {
+                const int64_t* P0 = &planes[(size_t(k) * ny + y) * nx];
+                const int64_t* P1 = &planes[(size_t(k + 1) * ny + y) * nx];
+                int64_t* acc = &sum[(size_t(z) * ny + y) * nx];
+                for (int x = 0; x < nx; ++x)
+                    acc[x] += lerp16(P0[x], P1[x], fz[z]) >> o;
+            }
+        }
+    }
+    for (size_t i = 0; i < sum.size(); ++i)
+    {
+        const int64_t t = (sum[i] * s.segMult) >> 16;
+        int64_t seg = t >> 16;
+        const int64_t f = t & 0xFFFF;
+        if (seg > 4095)
+            seg = 4095;
+        const int64_t a = remap[seg], b = remap[seg + 1];
+        int64_t v = a + (((b - a) * f) >> 16);
+        v = v < 0 ? 0 : (v > 65535 ? 65535 : v);
+        out[i] = convertNoise<T>(uint32_t(v));
+    }
+}
+
+// Per-voxel host evaluation (noiseVoxel at every voxel): the definition the
+// separable path is checked against.
+template <typename T>
+void noiseBrickHostDirect(const NoiseShape& s, const uint16_t* remap, const BrickGeo& g, T* out)
 {
     size_t i = 0;
     for (uint32_t z = 0; z < g.dim[2]; ++z)
         for (uint32_t y = 0; y < g.dim[1]; ++y)
             for (uint32_t x = 0; x < g.dim[0]; ++x)
                 out[i++] = convertNoise<T>(
-                    noiseVoxel(s, remap, g.level, g.origin[0] + x,
-                               g.origin[1] + y, g.origin[2] + z));
+                    noiseVoxel(s, remap, g.level, g.origin[0] + x, g.origin[1] + y, g.origin[2] + z));
 }
 } // namespace
 } // namespace lvr

@@ -447,6 +447,20 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     delete ctx;
 }
 
+int lvr_ctx_query(lvr_ctx* ctx)
+{
+    if (!ctx)
+        return LVR_EINVAL;
+    if (!ctx->stream)
+        return 1;
+    const hipError_t e = hipStreamQuery(ctx->stream);
+    if (e == hipSuccess)
+        return 1;
+    if (e == hipErrorNotReady)
+        return 0;
+    return lvr::hipFail(ctx, e, "hipStreamQuery");
+}
+
 const char* lvr_last_error(const lvr_ctx* ctx)
 {
     return ctx ? ctx->error.c_str() : "null context";

@@ -160,3 +160,24 @@ def poisson_arrivals(rate, duration_s, n_sessions, seed):
         if t > duration_s * 1000.0:
             return out
         out.append((t, int(rng.integers(n_sessions))))
+
+
+def flythrough(t, frames=300, seed=8):
+    """Camera of frame t on a seeded Catmull-Rom path that starts outside the
+    unit cube, crosses its inside and leaves again; it looks along the path."""
+    rng = np.random.default_rng(seed)
+    pts = [np.array([-2.2, 0.3, 1.6])]
+    for _ in range(4):
+        pts.append(rng.uniform(-0.35, 0.35, 3))
+    pts.append(np.array([2.0, -0.4, -1.8]))
+    pts = [2 * pts[0] - pts[1]] + pts + [2 * pts[-1] - pts[-2]]
+    u = (len(pts) - 3) * t / max(1, frames - 1)
+    i = min(int(u), len(pts) - 4)
+    f = u - i
+    p0, p1, p2, p3 = pts[i:i + 4]
+
+    pos = 0.5 * ((2 * p1) + (-p0 + p2) * f + (2 * p0 - 5 * p1 + 4 * p2 - p3) * f * f +
+                 (-p0 + 3 * p1 - 3 * p2 + p3) * f * f * f)
+    tangent = 0.5 * ((-p0 + p2) + 2 * (2 * p0 - 5 * p1 + 4 * p2 - p3) * f +
+                     3 * (-p0 + 3 * p1 - 3 * p2 + p3) * f * f)
+    return pos, pos + tangent / max(np.linalg.norm(tangent), 1e-9)

@@ -147,3 +147,75 @@ def test_invalid_arguments_are_rejected(lvr):
     vi = lvr.volume_info(64, 16, 1, lvr.DT_UINT8)
     with pytest.raises(lvr.LivreError):
         lvr.world_box(vi, scenes.node_id(7, 0, 0, 0))
+
+
+def _noise_voxel_py(p, vi, level, g):
+    """Per-voxel definition of the value-noise source in Python integers:
+    splitmix64 lattice hash, smoothstep fade and 16-bit lerps per octave
+    (x, then y, then z), band limit by level, remap table."""
+    M = (1 << 64) - 1
+
+    def splitmix(x):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        return x ^ (x >> 31)
+
+    def lattice(o, x, y, z):
+        key = (x & 0xFFFFF) | ((y & 0xFFFFF) << 20) | ((z & 0xFFFFF) << 40) | (o << 60)
+        return splitmix(p.seed ^ splitmix(key)) >> 48
+
+    def fade(f):
+        return (f * f * (3 * 65536 - 2 * f)) >> 32
+
+    def lerp(a, b, s):
+        return a + (((b - a) * s) >> 16)  # Python >> floors like the arithmetic shift
+
+    vmax = max(vi.voxels)
+    S = 1 << (vi.depth - 1 - level)
+    u = []
+    for a in range(3):
+        c = (2 * g[a] + 1) * S * 32768
+        u.append(min(max(c, 32768), (2 * vi.voxels[a] - 1) * 32768))
+    octaves = 0
+    m = 0
+    for o in range(p.octaves):
+        m += 65535 >> o
+        cells = p.base_cells << o
+        if vmax // cells >= 2 * S or o == 0:
+            octaves = o + 1
+    total = 0
+    for o in range(octaves):
+        mult = ((p.base_cells << o) << 32) // vmax
+        ci, fr = [], []
+        for a in range(3):
+            c = (u[a] * mult) >> 32
+            ci.append(c >> 16)
+            fr.append(fade(c & 0xFFFF))
+        v = [lattice(o, ci[0] + (k & 1), ci[1] + ((k >> 1) & 1), ci[2] + ((k >> 2) & 1))
+             for k in range(8)]
+        x00, x10 = lerp(v[0], v[1], fr[0]), lerp(v[2], v[3], fr[0])
+        x01, x11 = lerp(v[4], v[5], fr[0]), lerp(v[6], v[7], fr[0])
+        y0, y1 = lerp(x00, x10, fr[1]), lerp(x01, x11, fr[1])
+        total += lerp(y0, y1, fr[2]) >> o
+    seg_mult = (4096 << 32) // (m + 1)
+    t = (total * seg_mult) >> 16
+    seg, f = min(t >> 16, 4095), t & 0xFFFF
+    a, b = p.remap[seg], p.remap[seg + 1]
+    return min(max(a + (((b - a) * f) >> 16), 0), 65535)
+
+
+def test_host_noise_bricks_follow_the_per_voxel_definition(lvr):
+    rng = np.random.default_rng(5)
+    vi = lvr.volume_info((200, 160, 128), 32, 1, lvr.DT_UINT16)
+    p = lvr.noise_params(77, octaves=5, base_cells=3)
+    for level in range(vi.depth):
+        n = [max(1, int(np.ceil(vi.voxels[a] / (32 << (vi.depth - 1 - level)))))
+             for a in range(3)]
+        pos = [int(rng.integers(n[a])) for a in range(3)]
+        nid = scenes.node_id(level, *pos)
+        brick = lvr.noise_brick_host(vi, p, nid)
+        for _ in range(40):
+            x, y, z = (int(rng.integers(34)) for _ in range(3))
+            g = (pos[0] * 32 - 1 + x, pos[1] * 32 - 1 + y, pos[2] * 32 - 1 + z)
+            assert brick[z, y, x] == _noise_voxel_py(p, vi, level, g), (level, x, y, z)

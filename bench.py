@@ -37,6 +37,20 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def profiled_traffic():
+    """HBM bytes per ray-cast launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(HERE, "profiles", "raycast_2048u16_1080p_hbm.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def build_scene(livre_amd, scenes):
     vi = livre_amd.volume_info(2048, 64, 1, livre_amd.DT_UINT16)
     ids = scenes.all_nodes(vi)
@@ -556,7 +570,10 @@ def main():
         "tile_entries_mean": float(np.mean(entries)),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
-                     "traffic": None,
+                     "traffic": profiled_traffic(),
+                     "traffic_source": "profiles/raycast_2048u16_1080p_hbm.txt (rocprofv3 "
+                                       "FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a 1 GiB "
+                                       "copy)",
                      "kernel": "raycastKernel<unsigned short>",
                      "bytes_per_launch": "rendered bricks x 574992 B + 1920x1080x4 + "
                                          "4096x16"},

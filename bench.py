@@ -154,15 +154,29 @@ def batch_workload(args):
         t0 = time.perf_counter()
         tuned = tuner.tune_batch([(c, f, w, h, o, p) for c, f, w, h, o, p, _t, _r in items])
         t_tune = time.perf_counter() - t0
-        with livre_amd.Batch() as batch:
-            batch_pass(batch, items, tuned)  # warm-up and graph capture
-            gpu_ms, batch_samples = [], 0
+        with livre_amd.Batch() as b0, livre_amd.Batch() as b1:
+            pair = (b0, b1)
+            for batch in pair:  # warm-up and graph capture of both
+                batch_pass(batch, items, tuned)
+            # two batches in turn: the host assembles pass k+1 while pass k
+            # is on the GPU
+            gpu_ms, batch_samples, prev = [], 0, None
             t0 = time.perf_counter()
-            for _ in range(reps):
-                st = batch_pass(batch, items, tuned)
-                gpu_ms.append(st.gpu_ms)
-                batch_samples += st.samples
+            for r in range(reps):
+                batch = pair[r % 2]
+                for (ctx, fr, w, h, ordered, _p, tf, rng), tp in zip(items, tuned):
+                    batch.add(ctx, fr, w, h, ordered, tp, tf, rng)
+                batch.run()
+                if prev is not None:
+                    st = prev.wait()
+                    gpu_ms.append(st.gpu_ms)
+                    batch_samples += st.samples
+                prev = batch
+            st = prev.wait()
+            gpu_ms.append(st.gpu_ms)
+            batch_samples += st.samples
             t_batch = time.perf_counter() - t0
+            batch = prev
             # every batched frame equals the plain frame of the same request
             same = True
             for i in range(0, len(items), max(1, len(items) // 8)):
@@ -224,8 +238,9 @@ def batch_workload(args):
                                "the 2048^3 volume (4096-entry TF)",
                    "plain": "lvr_render + wait per request, default 8x8 tile, 2 samples "
                             "per thread",
-                   "batched": "one lvr_batch launch (graph replay), autotuned tiles and "
-                              "samples per thread"},
+                   "batched": "one lvr_batch launch per pass (graph replay), two batches "
+                              "in turn so the host assembles the next pass during the "
+                              "current one, autotuned tiles and samples per thread"},
         "uneven_requests": uneven,
         "sessions_1080p": sessions,
     }

@@ -467,7 +467,8 @@ int lvr_brick_pin(lvr_ctx* ctx, const uint64_t* ids, int n, int delta);
  * HistogramFilter.cpp:78-149): counts of interior voxels scaled by
  * 8^(depth - level - 1). 8-bit data: 256 bins, bin = value; 16-bit data:
  * 65536 bins, bin = value; float data: 256 bins over the TF range. The
- * result stays on the device until lvr_histogram_read. */
+ * kernels run on the context's histogram stream, beside the ray casting of
+ * the frame; the result stays on the device until lvr_histogram_read. */
 int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins);
 int lvr_histogram_read(lvr_ctx* ctx, uint64_t* out, int bins);
 

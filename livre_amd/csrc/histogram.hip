@@ -190,7 +190,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
 
     if (bins > ctx->histBins)
     {
-        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
         (void)hipFree(ctx->d_hist);
         ctx->d_hist = nullptr;
         ctx->histValid = false;
@@ -199,17 +199,17 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
     }
     ctx->histLast = bins;
     if (ctx->timing)
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->stream));
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->histStream));
     if (!delta)
         hipLaunchKernelGGL(clearKernel, dim3((bins + 255) / 256), dim3(256), 0,
-                           ctx->stream, ctx->d_hist, bins);
+                           ctx->histStream, ctx->d_hist, bins);
     if (!jobs.empty())
     {
         const size_t slotWords = slots.size();
         const size_t jobWords = jobs.size() * sizeof(HistJob) / sizeof(int32_t);
         if (slotWords > size_t(ctx->histCapacity) || jobWords > size_t(ctx->histJobCapacity))
         {
-            LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
             (void)hipFree(ctx->d_histSlots);
             (void)hipFree(ctx->d_histScale);
             ctx->d_histSlots = nullptr;
@@ -221,15 +221,15 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
             ctx->histCapacity = int(cs);
             ctx->histJobCapacity = int(cj);
         }
-        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
         LVR_HIP(ctx, hipMemcpyAsync(ctx->d_histSlots, slots.data(),
                                     sizeof(int32_t) * slots.size(), hipMemcpyHostToDevice,
-                                    ctx->stream));
+                                    ctx->histStream));
         LVR_HIP(ctx, hipMemcpyAsync(ctx->d_histScale, jobs.data(),
                                     sizeof(HistJob) * jobs.size(), hipMemcpyHostToDevice,
-                                    ctx->stream));
+                                    ctx->histStream));
         // the host vectors go out of scope at the end of this call
-        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
         const int M = int(ctx->vi.max_block[0]);
         const int ov = int(ctx->vi.overlap[0]);
         const size_t lds = sizeof(uint32_t) * binsPerJob;
@@ -240,7 +240,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&histKernel<uint8_t>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            hipLaunchKernelGGL(histKernel<uint8_t>, grid, block, lds, ctx->stream, dj,
+            hipLaunchKernelGGL(histKernel<uint8_t>, grid, block, lds, ctx->histStream, dj,
                                ctx->d_histSlots, ctx->d_pool, ctx->slotStride, M, ov,
                                binsPerJob, bins, r0, perBin, ctx->layout, ctx->d_hist);
         }
@@ -248,7 +248,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&histKernel<uint16_t>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            hipLaunchKernelGGL(histKernel<uint16_t>, grid, block, lds, ctx->stream, dj,
+            hipLaunchKernelGGL(histKernel<uint16_t>, grid, block, lds, ctx->histStream, dj,
                                ctx->d_histSlots, ctx->d_pool, ctx->slotStride, M, ov,
                                binsPerJob, bins, r0, perBin, ctx->layout, ctx->d_hist);
         }
@@ -256,17 +256,18 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         {
             (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&histKernel<float>),
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            hipLaunchKernelGGL(histKernel<float>, grid, block, lds, ctx->stream, dj,
+            hipLaunchKernelGGL(histKernel<float>, grid, block, lds, ctx->histStream, dj,
                                ctx->d_histSlots, ctx->d_pool, ctx->slotStride, M, ov,
                                binsPerJob, bins, r0, perBin, ctx->layout, ctx->d_hist);
         }
     }
     if (ctx->timing)
     {
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->stream));
+        LVR_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->histStream));
         ctx->histTimed = true;
     }
     LVR_HIP(ctx, hipGetLastError());
+    LVR_HIP(ctx, hipEventRecord(ctx->histDone, ctx->histStream));
     ctx->histSet.swap(next);
     ctx->histValid = true;
     ctx->histGeneration = ctx->poolGeneration;
@@ -284,8 +285,8 @@ int lvr_histogram_read(lvr_ctx* ctx, uint64_t* out, int bins)
     if (rc)
         return rc;
     LVR_HIP(ctx, hipMemcpyAsync(out, ctx->d_hist, sizeof(uint64_t) * bins,
-                                hipMemcpyDeviceToHost, ctx->stream));
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                                hipMemcpyDeviceToHost, ctx->histStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
     return LVR_OK;
 }
 }

@@ -147,6 +147,9 @@ struct lvr_ctx
     int numCUs = 256;
     hipStream_t stream = nullptr;
     hipStream_t uploadStream = nullptr;
+    // histogram kernels read the pool beside the ray casting of the frame
+    hipStream_t histStream = nullptr;
+    hipEvent_t histDone = nullptr; // end of the last histogram on histStream
     hipEvent_t renderDone = nullptr; // recorded on `stream` before slot reuse
     lvr_volume_info vi{};
     size_t voxelBytes = 0; // bytes per voxel

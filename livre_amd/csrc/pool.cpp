@@ -112,6 +112,14 @@ uint64_t evictableSlots(const lvr_ctx* ctx)
 
 int waitPoolReaders(lvr_ctx* ctx, hipStream_t st)
 {
+    // the last histogram (its own stream) may still read any slot
+    if (ctx->histDone)
+    {
+        if (st)
+            LVR_HIP(ctx, hipStreamWaitEvent(st, ctx->histDone, 0));
+        else
+            LVR_HIP(ctx, hipEventSynchronize(ctx->histDone));
+    }
     if (ctx->pinsGuardReaders || ctx->readers.empty())
         return LVR_OK;
     std::vector<lvr_batch*> still;
@@ -352,6 +360,8 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
             hipSuccess ||
         hipStreamCreateWithFlags(&ctx->uploadStream, hipStreamNonBlocking) !=
             hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->histStream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->histDone, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->renderDone, hipEventDisableTiming) !=
             hipSuccess)
         return bail(LVR_EHIP);
@@ -410,6 +420,8 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamSynchronize(ctx->stream);
     if (ctx->uploadStream)
         (void)hipStreamSynchronize(ctx->uploadStream);
+    if (ctx->histStream)
+        (void)hipStreamSynchronize(ctx->histStream);
     (void)hipFree(ctx->d_pool);
     (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->pt.d_slots);
@@ -444,6 +456,10 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamDestroy(ctx->stream);
     if (ctx->uploadStream)
         (void)hipStreamDestroy(ctx->uploadStream);
+    if (ctx->histStream)
+        (void)hipStreamDestroy(ctx->histStream);
+    if (ctx->histDone)
+        (void)hipEventDestroy(ctx->histDone);
     delete ctx;
 }
 
@@ -473,6 +489,7 @@ int lvr_ctx_sync(lvr_ctx* ctx)
     LVR_HIP(ctx, hipSetDevice(ctx->device));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
     return LVR_OK;
 }
 

@@ -819,7 +819,10 @@ int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out)
         LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, ctx->ev[0], ctx->ev[1]));
         LVR_HIP(ctx, hipEventElapsedTime(&out->raycast_ms, ctx->ev[1], ctx->ev[2]));
         if (ctx->histTimed)
+        {
+            LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
             LVR_HIP(ctx, hipEventElapsedTime(&out->histogram_ms, ctx->ev[3], ctx->ev[4]));
+        }
     }
     return LVR_OK;
 }

@@ -224,6 +224,11 @@ typedef struct
     int32_t tile_w, tile_h;      /* wavefront ray tile (w * h = 64); 0, 0 = 8 x 8 */
     int32_t samples_per_thread;  /* voxel fetches in flight per ray, 1..3 (NEAREST);
                                     0 = 2 */
+    int32_t classify;            /* 8/16-bit NEAREST: 1 = classification tabulated
+                                    per raw value (one table read per sample),
+                                    2 = computed per sample from the TF,
+                                    0 = automatic (table for 8-bit data);
+                                    identical images either way */
 } lvr_render_params;
 
 /* Render one frame of the ordered bricks (fragRayCast.glsl:113-207 per brick,
@@ -262,14 +267,15 @@ int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out);
 /* Autotuner of the launch shape (no reference counterpart: the GL path has
  * no tunable launch). Renders the given frame with every ray tile in
  * {8x8, 16x4, 4x16, 32x2, 2x32} and, for NEAREST, every samples_per_thread in
- * {1, 2, 3}; times the ray-cast kernel with HIP events (median of reps) and
- * writes the fastest tile_w, tile_h and samples_per_thread into *params.
+ * {1, 2, 3}, and for 8/16-bit data both classify modes; times the ray-cast
+ * kernel with HIP events (median of reps) and writes the fastest tile_w,
+ * tile_h, samples_per_thread and classify into *params.
  * All candidates produce the same image. The timings of the candidates are
  * copied to results (up to cap entries); *n_results = number of candidates.
  * Leaves the last candidate's frame in the framebuffer. */
 typedef struct
 {
-    int32_t tile_w, tile_h, samples_per_thread;
+    int32_t tile_w, tile_h, samples_per_thread, classify;
     float raycast_ms;
 } lvr_tune_entry;
 int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],

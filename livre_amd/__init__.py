@@ -121,12 +121,13 @@ class RenderParams(ctypes.Structure):
                 ("n_clip_planes", ctypes.c_int32),
                 ("clip_planes", ctypes.c_float * 24),
                 ("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
-                ("samples_per_thread", ctypes.c_int32)]
+                ("samples_per_thread", ctypes.c_int32), ("classify", ctypes.c_int32)]
 
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
-                ("samples_per_thread", ctypes.c_int32), ("raycast_ms", ctypes.c_float)]
+                ("samples_per_thread", ctypes.c_int32), ("classify", ctypes.c_int32),
+                ("raycast_ms", ctypes.c_float)]
 
 
 class NoiseParams(ctypes.Structure):
@@ -368,7 +369,7 @@ def field_brick_host(vi, params, node_id):
 
 def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
                   max_samples_per_ray=32, clip_planes=(), tile=(0, 0),
-                  samples_per_thread=0):
+                  samples_per_thread=0, classify=0):
     p = RenderParams()
     p.n_samples_per_ray = n_samples_per_ray
     p.max_samples_per_ray = max_samples_per_ray
@@ -380,6 +381,7 @@ def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
             p.clip_planes[4 * i + j] = pl[j]
     p.tile_w, p.tile_h = tile
     p.samples_per_thread = samples_per_thread
+    p.classify = classify
     return p
 
 
@@ -503,19 +505,19 @@ class Context:
         self.width, self.height = width, height
 
     def autotune(self, fr, w, h, ordered, params, reps=3):
-        """Fastest ray tile and samples per thread for this frame: returns a
-        copy of params with them set, and the list of (tile_w, tile_h,
-        samples_per_thread, raycast_ms) candidates."""
+        """Fastest ray tile, samples per thread and classification mode for
+        this frame: returns a copy of params with them set, and the list of
+        (tile_w, tile_h, samples_per_thread, classify, raycast_ms) candidates."""
         ordered, pids = _u64(ordered)
         vp = (ctypes.c_int * 4)(0, 0, w, h)
         out = RenderParams.from_buffer_copy(params)
-        res = (TuneEntry * 32)()
+        res = (TuneEntry * 64)()
         n = ctypes.c_int()
         self._check(lib().lvr_autotune(self._h, ctypes.byref(fr), vp, pids, len(ordered), reps,
-                                       ctypes.byref(out), res, 32, ctypes.byref(n)), "autotune")
+                                       ctypes.byref(out), res, 64, ctypes.byref(n)), "autotune")
         self.width, self.height = w, h
-        cands = [(r.tile_w, r.tile_h, r.samples_per_thread, r.raycast_ms)
-                 for r in res[:min(n.value, 32)]]
+        cands = [(r.tile_w, r.tile_h, r.samples_per_thread, r.classify, r.raycast_ms)
+                 for r in res[:min(n.value, 64)]]
         return out, cands
 
     def read_frame(self):

@@ -1,8 +1,9 @@
 """Launch-shape autotuner with a persistent cache.
 
-The ray tile (how a wavefront's 64 rays cover the screen) and the samples per
-thread (voxel fetches in flight per ray) change only the speed of a frame,
-never its pixels. `Tuner` picks them per request class -- voxel type,
+The ray tile (how a wavefront's 64 rays cover the screen), the samples per
+thread (voxel fetches in flight per ray) and, for 8/16-bit volumes, the
+classification mode (tabulated per raw value or computed per sample) change
+only the speed of a frame, never its pixels. `Tuner` picks them per request class -- voxel type,
 viewport size (log2 of the pixel count), sampling rate (samples per voxel of
 the finest level, log2) and filter -- by timing every candidate on the first
 frame of that class (`lvr_autotune`), and keeps the choices in a JSON file so
@@ -78,6 +79,7 @@ class Tuner:
             tuned, cands = ctx.autotune(fr, width, height, ordered, params, self.reps)
             c = {"tile": [tuned.tile_w, tuned.tile_h],
                  "samples_per_thread": tuned.samples_per_thread,
+                 "classify": tuned.classify,
                  "candidates": [list(x) for x in cands]}
             self.table[key] = c
             self.tuned += 1
@@ -85,6 +87,7 @@ class Tuner:
         out = RenderParams.from_buffer_copy(params)
         out.tile_w, out.tile_h = c["tile"]
         out.samples_per_thread = c["samples_per_thread"]
+        out.classify = c.get("classify", 0)
         return out
 
     def tune_batch(self, requests):

@@ -1,7 +1,8 @@
 // Autotuner of the ray-cast launch shape: the ray tile (how a wavefront's 64
-// rays cover the screen) and the samples per thread (voxel fetches in flight
-// per ray). Every candidate renders the caller's frame; the ray-cast kernel is
-// timed with HIP events on the context stream, median of `reps` launches, and
+// rays cover the screen), the samples per thread (voxel fetches in flight
+// per ray) and, for 8/16-bit volumes, tabulated or per-sample classification.
+// Every candidate renders the caller's frame; the ray-cast kernel is timed
+// with HIP events on the context stream, median of `reps` launches, and
 // the fastest candidate is written back into the parameters. Results are
 // bit-identical for every candidate, so the choice only affects speed.
 #include <algorithm>
@@ -28,14 +29,23 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
     if (rc)
         return rc;
     const int depths = params->linear_filter ? 1 : 3; // depth applies to NEAREST
+    // 8/16-bit NEAREST: tabulated or per-sample classification
+    const int dt = ctx->vi.data_type;
+    const bool tabulable =
+        !params->linear_filter && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16);
+    const int modes = tabulable ? 2 : 1;
     std::vector<lvr_tune_entry> all;
-    for (const auto& t : kTiles)
-        for (int d = 1; d <= depths; ++d)
+    for (int c = 0; c < int(sizeof kTiles / sizeof kTiles[0]) * depths * modes; ++c)
+    {
+        const auto& t = kTiles[c / (depths * modes)];
+        const int d = 1 + (c / modes) % depths;
+        const int m = 1 + c % modes;
         {
             lvr_render_params p = *params;
             p.tile_w = t[0];
             p.tile_h = t[1];
             p.samples_per_thread = params->linear_filter ? 0 : d;
+            p.classify = tabulable ? m : 0;
             std::vector<float> ms;
             for (int r = 0; r < reps && rc == LVR_OK; ++r)
             {
@@ -54,9 +64,11 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
             e.tile_w = p.tile_w;
             e.tile_h = p.tile_h;
             e.samples_per_thread = p.samples_per_thread;
+            e.classify = p.classify;
             e.raycast_ms = ms[ms.size() / 2];
             all.push_back(e);
         }
+    }
     if (!wasTiming)
         (void)lvr_set_timing(ctx, 0);
     if (rc)
@@ -68,6 +80,7 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
     params->tile_w = best->tile_w;
     params->tile_h = best->tile_h;
     params->samples_per_thread = best->samples_per_thread;
+    params->classify = best->classify;
     const int k = std::min(cap, int(all.size()));
     for (int i = 0; i < k; ++i)
         results[i] = all[size_t(i)];

@@ -124,6 +124,7 @@ struct FrameUniforms
     int32_t nBricks;
     int32_t linear;
     int32_t depth; // voxel fetches in flight per ray (NEAREST march)
+    int32_t classify; // 1 tabulated, 2 per sample (8/16-bit NEAREST)
     int32_t nClip;
     float clip[24];
     uint64_t slotStride;
@@ -131,12 +132,15 @@ struct FrameUniforms
     // position of this frame in a batch
     int32_t brickBase; // first brick record / tile range
     int32_t tileBase;  // first tile of the per-tile arrays
+    int32_t superBase; // first super-tile (binning) of the per-super-tile arrays
+    int32_t superX, superY; // super-tiles of kSuperTile x kSuperTile tiles
     int32_t itemBase;  // first work item
     int32_t nItems;    // work items (16 per 4x4 super-tile)
     int32_t dataType;
     int32_t request;   // index in the batch
     const uint8_t* pool;
     const float4* tf;  // nTf RGBA entries
+    const float4* cls; // classified table per raw value (integer NEAREST), or null
     uint32_t* frame;   // RGBA8 output, row 0 = bottom
 };
 } // namespace lvr
@@ -175,6 +179,11 @@ struct lvr_ctx
     float* d_tf = nullptr; // float4 entries
     int nTf = 0;
     int tfCapacity = 0;
+    uint64_t tfGeneration = 0; // bumped by every transfer-function upload
+    // classified table (r*a, g*a, b*a, a) per raw value of 8/16-bit volumes
+    // for the current TF, data range and sampling rate
+    float4* d_cls = nullptr;
+    uint64_t clsKey[4] = {~0ull, 0, 0, 0};
     float range0 = 0.f, range1 = 0.f;
     bool tfSet = false;
 
@@ -192,6 +201,11 @@ struct lvr_ctx
     size_t tileCapacity = 0;
     size_t tileOffsetCapacity = 0;
     size_t tileListCapacity = 0;
+    // first binning level: ordered brick lists per super-tile
+    int32_t* d_superCount = nullptr;
+    int32_t* d_superOffset = nullptr;
+    int32_t* d_superList = nullptr;
+    size_t superCapacity = 0, superOffsetCapacity = 0, superListCapacity = 0;
     lvr::FrameUniforms* d_req = nullptr; // the frame as a one-request table
     size_t reqCapacity = 0;
     lvr::FrameUniforms hostReq{};

@@ -153,6 +153,7 @@ void* genScratch(lvr_ctx* ctx, size_t bytes)
         return nullptr;
     }
     (void)hipFree(ctx->d_gen);
+    (void)hipFree(ctx->d_cls);
     ctx->d_gen = nullptr;
     ctx->genCapacity = 0;
     const size_t cap = std::max<size_t>(bytes, size_t(1) << 20);
@@ -433,6 +434,9 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->d_tileCount);
     (void)hipFree(ctx->d_tileOffset);
     (void)hipFree(ctx->d_tileList);
+    (void)hipFree(ctx->d_superCount);
+    (void)hipFree(ctx->d_superOffset);
+    (void)hipFree(ctx->d_superList);
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_hist);
     (void)hipFree(ctx->d_histSlots);
@@ -676,6 +680,7 @@ static int tfUpload(lvr_ctx* ctx, const std::vector<float>& rgba, int n,
     ctx->range0 = range0;
     ctx->range1 = range1;
     ctx->tfSet = true;
+    ++ctx->tfGeneration;
     return LVR_OK;
 }
 

@@ -130,18 +130,17 @@ __device__ __forceinline__ uint32_t nearestIndex(const FrameUniforms& U, const B
     return tiledIndexDev(U.layout, ix, iy, iz);
 }
 
-// Classification and compositing of one sample (fragRayCast.glsl:184-204):
-// texture(transferFnTex, density) with linear filtering and clamp to edge,
-// opacity correction, front-to-back "over". Returns true when the ray
-// reached the early-exit opacity.
-struct Composite
+// Classification of one sample value (fragRayCast.glsl:184-204 before the
+// blend): texture(transferFnTex, density) with linear filtering and clamp to
+// edge, then the opacity-corrected alpha. Returns (r*alpha, g*alpha, b*alpha,
+// alpha), the factors the front-to-back blend multiplies by (1 - dst.a).
+struct Classify
 {
-    float nTfF, nTfMax, alphaCorrection, maxA;
-    float acc0, acc1, acc2, acc3;
+    float mult, add, nTfF, nTfMax, alphaCorrection, maxA;
 
-    __device__ __forceinline__ bool add(const FrameUniforms& U, const float4* tf, float value)
+    __device__ __forceinline__ float4 operator()(const float4* tf, float value) const
     {
-        const float density = value * U.mult + U.add;
+        const float density = value * mult + add;
         // the two entries are floor(u) and floor(u) + 1 clamped to
         // [0, nTf - 1] (fmed3 + truncation, as for the voxel)
         const float u = density * nTfF - 0.5f;
@@ -160,12 +159,43 @@ struct Composite
         const float base = 1.0f - fminf(sa, maxA);
         const float alpha =
             1.0f - __builtin_amdgcn_exp2f(alphaCorrection * __builtin_amdgcn_logf(base));
+        return make_float4(sr * alpha, sg * alpha, sb * alpha, alpha);
+    }
+};
+
+__device__ __forceinline__ Classify makeClassify(const FrameUniforms& U)
+{
+    Classify c;
+    c.mult = U.mult;
+    c.add = U.add;
+    c.nTfF = float(U.nTf);
+    c.nTfMax = float(U.nTf - 1);
+    c.alphaCorrection = float(U.maxSamples) / float(U.nSamples);
+    c.maxA = 1.0f - 1.0f / 256.0f;
+    return c;
+}
+
+// Front-to-back "over" of classified samples (fragRayCast.glsl:103-111):
+// dst.rgb += (src.rgb * alpha) * (1 - dst.a); dst.a += alpha * (1 - dst.a).
+// Returns true when the ray reached the early-exit opacity.
+struct Composite
+{
+    Classify cls;
+    float acc0, acc1, acc2, acc3;
+
+    __device__ __forceinline__ bool blend(const FrameUniforms& U, float4 c)
+    {
         const float rem = 1.0f - acc3;
-        acc0 += sr * alpha * rem;
-        acc1 += sg * alpha * rem;
-        acc2 += sb * alpha * rem;
-        acc3 += alpha * rem;
+        acc0 += c.x * rem;
+        acc1 += c.y * rem;
+        acc2 += c.z * rem;
+        acc3 += c.w * rem;
         return acc3 > U.exitA;
+    }
+
+    __device__ __forceinline__ bool add(const FrameUniforms& U, const float4* tf, float value)
+    {
+        return blend(U, cls(tf, value));
     }
 };
 
@@ -193,7 +223,7 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
 // on a load. Fetches past the last composited sample (end of segment or early
 // exit) are discarded; their texel indices are clamped into the brick like
 // every other.
-template <typename T, int D>
+template <typename T, int D, bool CLS>
 __device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const FrameUniforms& U,
                                              const BrickEntry& b, const float texExt[3],
                                              float pos[3], const float step[3], float travel,
@@ -221,25 +251,32 @@ __device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const Fr
         {
             travel -= stepSize;
             --guard;
-            uint32_t next = 0;
+            float4 cls = make_float4(0.f, 0.f, 0.f, 0.f);
+            float value = 0.f;
+            if (CLS)
+            {
+                // classified table: one read per sample, issued before the
+                // next fetch's address arithmetic so the two overlap
+                cls = tf[uint32_t(raw[s])];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            else
+                value = float(raw[s]);
             if (D > 1)
             {
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                next = nearestIndex(U, b, texExt, pos);
-            }
-            const float value = float(raw[s]);
-            if (D > 1)
-            {
-                // the oldest voxel is converted before its register takes the
-                // next load; the load stays above this sample's compositing
+                const uint32_t next = nearestIndex(U, b, texExt, pos);
+                // the oldest voxel has been used before its register takes
+                // the next load; the load stays above this sample's blend
                 __builtin_amdgcn_sched_barrier(0);
                 raw[s] = vox[next];
                 __builtin_amdgcn_sched_barrier(0);
             }
             ++nsamp;
-            if (C.add(U, tf, value) || !(travel > 0.0f && guard > 0))
+            const bool opaque = CLS ? C.blend(U, cls) : C.add(U, tf, value);
+            if (opaque || !(travel > 0.0f && guard > 0))
             {
                 more = false;
                 break;
@@ -257,8 +294,9 @@ __device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const Fr
 
 // One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
 // ordered list, composited in registers, then the RGBA8 store. `tf` is the
-// transfer function (LDS or global, resolved at the call site).
-template <typename T, bool LINEAR, int D>
+// transfer function (LDS or global, resolved at the call site), or with CLS
+// the classified table indexed by the raw voxel value.
+template <typename T, bool LINEAR, int D, bool CLS = false>
 __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int lane,
                                          const BrickEntry* __restrict__ bricks,
                                          const int32_t* __restrict__ offsets,
@@ -314,10 +352,7 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
     const float texExt[3] = {U.texMax[0] - U.texMin[0], U.texMax[1] - U.texMin[1],
                              U.texMax[2] - U.texMin[2]};
     Composite C;
-    C.nTfF = float(U.nTf);
-    C.nTfMax = float(U.nTf - 1);
-    C.alphaCorrection = float(U.maxSamples) / float(U.nSamples);
-    C.maxA = 1.0f - 1.0f / 256.0f;
+    C.cls = makeClassify(U);
     C.acc0 = C.acc1 = C.acc2 = C.acc3 = 0.f;
     bool alive = inside;
 
@@ -383,7 +418,8 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         const T* vox = reinterpret_cast<const T*>(U.pool + size_t(b.slot) * U.slotStride);
 
         if (!LINEAR)
-            marchNearest<T, D>(vox, U, b, texExt, pos, step, segLen, stepSize, C, tf, nsamp);
+            marchNearest<T, D, CLS>(vox, U, b, texExt, pos, step, segLen, stepSize, C, tf,
+                                    nsamp);
         else
         {
             // A segment lies inside one brick, so it holds at most

@@ -32,10 +32,80 @@ namespace
 {
 // One wavefront per global tile: count the bricks of its frame whose tile
 // range covers the tile.
-__global__ __launch_bounds__(256) void binCountKernel(const FrameUniforms* __restrict__ reqs,
+// Binning in two levels, both keeping the front-to-back brick order that
+// the shader's draw order gives (RayCastRenderer.cpp:550-571): first every
+// super-tile of kSuperTile x kSuperTile ray tiles scans the frame's bricks
+// in order and keeps those whose screen rectangle meets it; then every tile
+// scans only its super-tile's list. A wavefront tests 64 bricks per step and
+// compacts the hits with a ballot, so the lists come out ordered.
+constexpr int kSuperTile = 8;
+
+__device__ inline int findBySuper(const FrameUniforms* __restrict__ reqs, int nReq, int gsuper)
+{
+    int lo = 0, hi = nReq - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].superBase <= gsuper)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// FILL = false: count into out[gsuper]; FILL = true: write the ordered list
+// at offsets[gsuper].
+template <bool FILL>
+__global__ __launch_bounds__(256) void superBinKernel(const FrameUniforms* __restrict__ reqs,
                                                       int nReq, const int4* __restrict__ ranges,
-                                                      int totalTiles,
-                                                      int32_t* __restrict__ counts)
+                                                      int totalSupers,
+                                                      const int32_t* __restrict__ offsets,
+                                                      int32_t* __restrict__ out)
+{
+    const int gsuper = __builtin_amdgcn_readfirstlane(
+        int(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
+    if (gsuper >= totalSupers)
+        return;
+    const int r = __builtin_amdgcn_readfirstlane(findBySuper(reqs, nReq, gsuper));
+    const FrameUniforms& U = reqs[r];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int st = gsuper - U.superBase;
+    const int x0 = (st % U.superX) * kSuperTile, y0 = (st / U.superX) * kSuperTile;
+    const int x1 = x0 + kSuperTile - 1, y1 = y0 + kSuperTile - 1;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int pos = FILL ? offsets[gsuper] : 0;
+    for (int base = 0; base < U.nBricks; base += kWave)
+    {
+        const int i = base + lane;
+        bool hit = false;
+        if (i < U.nBricks)
+        {
+            const int4 q = ranges[U.brickBase + i];
+            // empty ranges (x > z: brick off screen or not resident) never
+            // hit; the host sizes the lists by the same rule
+            hit = q.x <= q.z && q.y <= q.w && q.x <= x1 && q.z >= x0 && q.y <= y1 && q.w >= y0;
+        }
+        const uint64_t mask = __ballot(hit);
+        if (FILL && hit)
+            out[pos + __popcll(mask & below)] = U.brickBase + i;
+        pos += __popcll(mask);
+    }
+    if (!FILL && lane == 0)
+        out[gsuper] = pos;
+}
+
+// Second level: FILL = false counts into out[gtile], FILL = true writes the
+// tile's ordered list (global brick indices) at offsets[gtile].
+template <bool FILL>
+__global__ __launch_bounds__(256) void tileBinKernel(const FrameUniforms* __restrict__ reqs,
+                                                     int nReq, const int4* __restrict__ ranges,
+                                                     int totalTiles,
+                                                     const int32_t* __restrict__ superOffsets,
+                                                     const int32_t* __restrict__ superCounts,
+                                                     const int32_t* __restrict__ superLists,
+                                                     const int32_t* __restrict__ offsets,
+                                                     int32_t* __restrict__ out)
 {
     const int gtile = __builtin_amdgcn_readfirstlane(
         int(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
@@ -46,20 +116,28 @@ __global__ __launch_bounds__(256) void binCountKernel(const FrameUniforms* __res
     const int lane = threadIdx.x & (kWave - 1);
     const int tile = gtile - U.tileBase;
     const int tx = tile % U.tilesX, ty = tile / U.tilesX;
-    int count = 0;
-    for (int base = 0; base < U.nBricks; base += kWave)
+    const int gsuper = U.superBase + (ty / kSuperTile) * U.superX + tx / kSuperTile;
+    const int beg = superOffsets[gsuper], n = superCounts[gsuper];
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    int pos = FILL ? offsets[gtile] : 0;
+    for (int base = 0; base < n; base += kWave)
     {
-        const int i = base + lane;
+        const int k = base + lane;
         bool hit = false;
-        if (i < U.nBricks)
+        int b = 0;
+        if (k < n)
         {
-            const int4 q = ranges[U.brickBase + i];
+            b = superLists[beg + k];
+            const int4 q = ranges[b];
             hit = tx >= q.x && tx <= q.z && ty >= q.y && ty <= q.w;
         }
-        count += __popcll(__ballot(hit));
+        const uint64_t mask = __ballot(hit);
+        if (FILL && hit)
+            out[pos + __popcll(mask & below)] = b;
+        pos += __popcll(mask);
     }
-    if (lane == 0)
-        counts[gtile] = count;
+    if (!FILL && lane == 0)
+        out[gtile] = pos;
 }
 
 // Exclusive scan of the tile counts, one workgroup of 1024 threads. Also
@@ -96,38 +174,26 @@ __global__ __launch_bounds__(1024) void scanKernel(const int32_t* __restrict__ c
     }
 }
 
-// Same walk as binCountKernel, writing global brick indices in list order.
-__global__ __launch_bounds__(256) void binFillKernel(const FrameUniforms* __restrict__ reqs,
-                                                     int nReq, const int4* __restrict__ ranges,
-                                                     int totalTiles,
-                                                     const int32_t* __restrict__ offsets,
-                                                     int32_t* __restrict__ lists)
+// Both binning levels with their scans, on stream st. The second scan also
+// clears nCounters counters for the kernels that follow.
+static void enqueueBinning(hipStream_t st, const FrameUniforms* reqs, int nReq,
+                           const int4* ranges, int supers, int tiles, int32_t* sCount,
+                           int32_t* sOffset, int32_t* sList, int32_t* tCount, int32_t* tOffset,
+                           int32_t* tList, unsigned long long* counters, int nCounters)
 {
-    const int gtile = __builtin_amdgcn_readfirstlane(
-        int(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave));
-    if (gtile >= totalTiles)
-        return;
-    const int r = __builtin_amdgcn_readfirstlane(findByTile(reqs, nReq, gtile));
-    const FrameUniforms& U = reqs[r];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int tile = gtile - U.tileBase;
-    const int tx = tile % U.tilesX, ty = tile / U.tilesX;
-    int out = offsets[gtile];
-    const uint64_t below = (uint64_t(1) << lane) - 1;
-    for (int base = 0; base < U.nBricks; base += kWave)
-    {
-        const int i = base + lane;
-        bool hit = false;
-        if (i < U.nBricks)
-        {
-            const int4 q = ranges[U.brickBase + i];
-            hit = tx >= q.x && tx <= q.z && ty >= q.y && ty <= q.w;
-        }
-        const uint64_t mask = __ballot(hit);
-        if (hit)
-            lists[out + __popcll(mask & below)] = U.brickBase + i;
-        out += __popcll(mask);
-    }
+    const int sBlocks = (supers + 3) / 4, tBlocks = (tiles + 3) / 4;
+    hipLaunchKernelGGL(superBinKernel<false>, dim3(sBlocks), dim3(256), 0, st, reqs, nReq,
+                       ranges, supers, nullptr, sCount);
+    hipLaunchKernelGGL(scanKernel, dim3(1), dim3(1024), 0, st, sCount, supers, sOffset, counters,
+                       0);
+    hipLaunchKernelGGL(superBinKernel<true>, dim3(sBlocks), dim3(256), 0, st, reqs, nReq, ranges,
+                       supers, sOffset, sList);
+    hipLaunchKernelGGL(tileBinKernel<false>, dim3(tBlocks), dim3(256), 0, st, reqs, nReq, ranges,
+                       tiles, sOffset, sCount, sList, nullptr, tCount);
+    hipLaunchKernelGGL(scanKernel, dim3(1), dim3(1024), 0, st, tCount, tiles, tOffset, counters,
+                       nCounters);
+    hipLaunchKernelGGL(tileBinKernel<true>, dim3(tBlocks), dim3(256), 0, st, reqs, nReq, ranges,
+                       tiles, sOffset, sCount, sList, tOffset, tList);
 }
 
 __device__ inline int nextItem(unsigned long long* counter, int lane)
@@ -144,15 +210,28 @@ __device__ inline int nextItem(unsigned long long* counter, int lane)
 // Tiles differ by orders of magnitude in work (empty sky vs rays through the
 // whole volume), so pulling them dynamically keeps every CU busy to the end.
 // Two 1024-ray workgroups per CU need <= 64 VGPRs (8 wavefronts per SIMD).
-template <typename T, bool LINEAR, int D>
+template <typename T, bool LINEAR, int D, bool CLS>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void raycastKernel(
     const FrameUniforms U, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
     const int32_t* __restrict__ lists, unsigned long long* __restrict__ counters)
 {
+    // LDS: the transfer function, or with CLS the 8-bit classified table
+    // (16-bit tables stay in global memory)
     extern __shared__ float4 sTf[];
-    for (int i = threadIdx.x; i < U.nTf; i += blockDim.x)
-        sTf[i] = U.tf[i];
+    const float4* table = sTf;
+    if (!CLS)
+    {
+        for (int i = threadIdx.x; i < U.nTf; i += blockDim.x)
+            sTf[i] = U.tf[i];
+    }
+    else if (sizeof(T) == 1)
+    {
+        for (int i = threadIdx.x; i < 256; i += blockDim.x)
+            sTf[i] = U.cls[i];
+    }
+    else
+        table = U.cls;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t nsamp = 0;
@@ -163,7 +242,8 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
             break;
         const int tile = itemToTile(U, item);
         if (tile >= 0)
-            castTile<T, LINEAR, D>(U, tile, lane, bricks, offsets, counts, lists, sTf, nsamp);
+            castTile<T, LINEAR, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, table,
+                                        nsamp);
     }
     const unsigned long long s = waveSum(nsamp);
     if (lane == 0 && s)
@@ -213,15 +293,23 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                 nsamp = 0;
             }
             cur = r;
-            if (U.nTf <= kBatchTf)
+            if (U.dataType == LVR_DT_UINT8 && U.classify == 1)
+            {
+                // 8-bit request: its classified table, built from the
+                // request's TF into this wavefront's slice
+                const Classify c = makeClassify(U);
+                for (int i = lane; i < 256; i += kWave)
+                    myTf[i] = c(U.tf, float(i));
+            }
+            else if (U.nTf <= kBatchTf)
             {
                 // this wavefront's slice only: ordered by the wave barrier
                 for (int i = lane; i < U.nTf; i += kWave)
                     myTf[i] = U.tf[i];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
         const int tile = itemToTile(U, item - U.itemBase);
         if (tile < 0)
@@ -230,7 +318,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         switch (U.dataType)
         {
         case LVR_DT_UINT8:
-            if (lds)
+            if (U.classify == 1)
+                castTile<uint8_t, false, D, true>(U, tile, lane, bricks, offsets, counts, lists,
+                                                  myTf, nsamp);
+            else if (lds)
                 castTile<uint8_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
                 castTile<uint8_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
@@ -260,16 +351,42 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     }
 }
 
-template <typename T, bool LINEAR, int D>
+// Classified table of an 8- or 16-bit volume: entry v = the classification
+// of raw value v, computed by the same code as the in-loop path.
+__global__ __launch_bounds__(256) void classifyKernel(Classify c, const float4* __restrict__ tf,
+                                                      int nValues, float4* __restrict__ out)
+{
+    const int v = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (v < nValues)
+        out[v] = c(tf, float(v));
+}
+
+template <typename T, bool LINEAR, int D, bool CLS>
 void launchRaycastAs(int blocks, size_t lds, hipStream_t st, const FrameUniforms& U,
                      const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
                      const int32_t* lists, unsigned long long* counters)
 {
     const int threads = kSuper * kSuper * kWave;
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, LINEAR, D>),
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, LINEAR, D, CLS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-    hipLaunchKernelGGL((raycastKernel<T, LINEAR, D>), dim3(blocks), dim3(threads), lds, st, U,
-                       bricks, offsets, counts, lists, counters);
+    hipLaunchKernelGGL((raycastKernel<T, LINEAR, D, CLS>), dim3(blocks), dim3(threads), lds, st,
+                       U, bricks, offsets, counts, lists, counters);
+}
+
+template <typename T, bool CLS>
+void launchNearest(int blocks, size_t lds, hipStream_t st, const FrameUniforms& U,
+                   const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
+                   const int32_t* lists, unsigned long long* counters)
+{
+    if (U.depth == 1)
+        launchRaycastAs<T, false, 1, CLS>(blocks, lds, st, U, bricks, offsets, counts, lists,
+                                          counters);
+    else if (U.depth == 3)
+        launchRaycastAs<T, false, 3, CLS>(blocks, lds, st, U, bricks, offsets, counts, lists,
+                                          counters);
+    else
+        launchRaycastAs<T, false, 2, CLS>(blocks, lds, st, U, bricks, offsets, counts, lists,
+                                          counters);
 }
 
 template <typename T>
@@ -278,13 +395,10 @@ void launchRaycast(int blocks, size_t lds, hipStream_t st, const FrameUniforms& 
                    const int32_t* lists, unsigned long long* counters)
 {
     if (U.linear)
-        launchRaycastAs<T, true, 1>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
-    else if (U.depth == 1)
-        launchRaycastAs<T, false, 1>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
-    else if (U.depth == 3)
-        launchRaycastAs<T, false, 3>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
+        launchRaycastAs<T, true, 1, false>(blocks, lds, st, U, bricks, offsets, counts, lists,
+                                           counters);
     else
-        launchRaycastAs<T, false, 2>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
+        launchNearest<T, false>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
 }
 } // namespace
 
@@ -340,7 +454,7 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
                         const uint64_t* ids, int n, const lvr_render_params* params,
                         int nTf, float tf0, float tf1, FrameUniforms* U,
                         std::vector<BrickEntry>& bricks, std::vector<int4>& ranges,
-                        uint64_t* listTotal, uint64_t* onScreen)
+                        uint64_t* listTotal, uint64_t* superTotal, uint64_t* onScreen)
 {
     if (!f || !viewport || !params || n < 0 || (n > 0 && !ids))
         return fail(ctx, LVR_EINVAL, "invalid arguments");
@@ -361,6 +475,8 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     const int depth = params->samples_per_thread == 0 ? 2 : params->samples_per_thread;
     if (depth < 1 || depth > 3)
         return fail(ctx, LVR_EINVAL, "samples per thread must be 1, 2 or 3");
+    if (params->classify < 0 || params->classify > 2)
+        return fail(ctx, LVR_EINVAL, "classify must be 0, 1 or 2");
 
     std::memset(U, 0, sizeof *U);
     std::memcpy(U->invProj, f->inv_proj, sizeof U->invProj);
@@ -396,9 +512,15 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     U->tileH = th;
     U->tilesX = (W + tw - 1) / tw;
     U->tilesY = (H + th - 1) / th;
+    U->superX = (U->tilesX + kSuperTile - 1) / kSuperTile;
+    U->superY = (U->tilesY + kSuperTile - 1) / kSuperTile;
     U->nBricks = n;
     U->linear = params->linear_filter ? 1 : 0;
     U->depth = depth;
+    // automatic: a 256-entry table sits in LDS beside the march; a 65536-entry
+    // one is a dependent global gather per sample, slower than computing
+    U->classify = params->classify != 0 ? params->classify
+                                        : (ctx->vi.data_type == LVR_DT_UINT8 ? 1 : 2);
     U->nClip = params->n_clip_planes;
     std::memcpy(U->clip, params->clip_planes, sizeof U->clip);
     U->slotStride = ctx->slotStride;
@@ -474,6 +596,8 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
             {
                 r = make_int4(x0 / tw, y0 / th, x1 / tw, y1 / th);
                 *listTotal += uint64_t(r.z - r.x + 1) * uint64_t(r.w - r.y + 1);
+                *superTotal += uint64_t(r.z / kSuperTile - r.x / kSuperTile + 1) *
+                               uint64_t(r.w / kSuperTile - r.y / kSuperTile + 1);
                 ++*onScreen;
             }
         }
@@ -516,7 +640,7 @@ struct lvr_batch
     std::vector<int4> ranges;
     std::vector<float4> tf;
     std::vector<size_t> tfOffset, frameOffset;
-    uint64_t listTotal = 0, onScreen = 0, pixels = 0;
+    uint64_t listTotal = 0, superTotal = 0, onScreen = 0, pixels = 0;
 
     // device buffers
     lvr::FrameUniforms* d_reqs = nullptr;
@@ -534,6 +658,11 @@ struct lvr_batch
     size_t tileCap = 0;
     int32_t* d_lists = nullptr;
     size_t listCap = 0;
+    int32_t* d_scounts = nullptr;
+    int32_t* d_soffsets = nullptr;
+    size_t superCap = 0;
+    int32_t* d_slists = nullptr;
+    size_t slistCap = 0;
     unsigned long long* d_counters = nullptr;
     size_t counterCap = 0;
     std::vector<unsigned long long> h_counters;
@@ -624,6 +753,36 @@ static int bgrow(lvr_batch* b, T** ptr, size_t* cap, size_t need, bool* grew)
 
 extern "C" {
 
+// The classified table of the context for the frame's TF, data range and
+// sampling rate, recomputed on the render stream when any of them changed.
+static int ensureClassified(lvr_ctx* ctx, const lvr::FrameUniforms& U, int nValues)
+{
+    using namespace lvr;
+    uint32_t mult, add;
+    std::memcpy(&mult, &U.mult, 4);
+    std::memcpy(&add, &U.add, 4);
+    const uint64_t key[4] = {ctx->tfGeneration,
+                             (uint64_t(uint32_t(U.nSamples)) << 32) | uint32_t(U.maxSamples),
+                             (uint64_t(mult) << 32) | add,
+                             (uint64_t(uint32_t(U.nTf)) << 32) | uint32_t(nValues)};
+    if (ctx->d_cls && std::memcmp(key, ctx->clsKey, sizeof key) == 0)
+        return LVR_OK;
+    if (!ctx->d_cls)
+        LVR_HIP(ctx, hipMalloc((void**)&ctx->d_cls, sizeof(float4) * 65536));
+    Classify c;
+    c.mult = U.mult;
+    c.add = U.add;
+    c.nTfF = float(U.nTf);
+    c.nTfMax = float(U.nTf - 1);
+    c.alphaCorrection = float(U.maxSamples) / float(U.nSamples);
+    c.maxA = 1.0f - 1.0f / 256.0f;
+    hipLaunchKernelGGL(classifyKernel, dim3((nValues + 255) / 256), dim3(256), 0, ctx->stream, c,
+                       U.tf, nValues, ctx->d_cls);
+    LVR_HIP(ctx, hipGetLastError());
+    std::memcpy(ctx->clsKey, key, sizeof key);
+    return LVR_OK;
+}
+
 int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                const uint64_t* ordered_ids, int n, const lvr_render_params* params)
 {
@@ -641,9 +800,9 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     std::vector<int4>& ranges = ctx->hostRanges;
     bricks.clear();
     ranges.clear();
-    uint64_t listTotal = 0, onScreen = 0;
+    uint64_t listTotal = 0, superTotal = 0, onScreen = 0;
     rc = prepareFrame(ctx, f, viewport, ordered_ids, n, params, ctx->nTf, ctx->range0,
-                      ctx->range1, &U, bricks, ranges, &listTotal, &onScreen);
+                      ctx->range1, &U, bricks, ranges, &listTotal, &superTotal, &onScreen);
     if (rc)
         return rc;
     const int W = U.width, H = U.height;
@@ -679,6 +838,17 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                     size_t(std::max<uint64_t>(listTotal, 1)));
     if (rc)
         return rc;
+    const int nSupers = U.superX * U.superY;
+    rc = growBuffer(ctx, &ctx->d_superCount, &ctx->superCapacity, size_t(nSupers));
+    if (rc)
+        return rc;
+    rc = growBuffer(ctx, &ctx->d_superOffset, &ctx->superOffsetCapacity, size_t(nSupers));
+    if (rc)
+        return rc;
+    rc = growBuffer(ctx, &ctx->d_superList, &ctx->superListCapacity,
+                    size_t(std::max<uint64_t>(superTotal, 1)));
+    if (rc)
+        return rc;
     size_t reqCap = ctx->reqCapacity;
     rc = growBuffer(ctx, &ctx->d_req, &reqCap, 1);
     if (rc)
@@ -699,21 +869,42 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     ctx->lastEntries = listTotal;
     if (ctx->timing)
         LVR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
-    const int binBlocks = (nTiles + 3) / 4;
-    hipLaunchKernelGGL(binCountKernel, dim3(binBlocks), dim3(256), 0, ctx->stream, ctx->d_req, 1,
-                       ctx->d_ranges, nTiles, ctx->d_tileCount);
-    hipLaunchKernelGGL(scanKernel, dim3(1), dim3(1024), 0, ctx->stream, ctx->d_tileCount,
-                       nTiles, ctx->d_tileOffset, ctx->d_counters, 2);
-    hipLaunchKernelGGL(binFillKernel, dim3(binBlocks), dim3(256), 0, ctx->stream, ctx->d_req, 1,
-                       ctx->d_ranges, nTiles, ctx->d_tileOffset, ctx->d_tileList);
+    enqueueBinning(ctx->stream, ctx->d_req, 1, ctx->d_ranges, nSupers, nTiles, ctx->d_superCount,
+                   ctx->d_superOffset, ctx->d_superList, ctx->d_tileCount, ctx->d_tileOffset,
+                   ctx->d_tileList, ctx->d_counters, 2);
     if (ctx->timing)
         LVR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
     // persistent: two 1024-ray workgroups per CU are resident (LDS and VGPR
     // budget); more would only find the queue empty
     const int superTiles = U.nItems / (kSuper * kSuper);
     const int blocks = std::max(1, std::min(superTiles, 2 * ctx->numCUs));
-    const size_t lds = size_t(U.nTf) * sizeof(float4);
-    switch (ctx->vi.data_type)
+    size_t lds = size_t(U.nTf) * sizeof(float4);
+    const int dt = ctx->vi.data_type;
+    if (!U.linear && U.classify == 1 && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16))
+    {
+        // Integer volumes sampled NEAREST: classification depends on the raw
+        // value only, so it is tabulated once per (TF, range, rate) and the
+        // march reads one table entry per sample.
+        const int nValues = dt == LVR_DT_UINT8 ? 256 : 65536;
+        rc = ensureClassified(ctx, U, nValues);
+        if (rc)
+            return rc;
+        U.cls = ctx->d_cls;
+        lds = dt == LVR_DT_UINT8 ? 256 * sizeof(float4) : 0;
+        if (dt == LVR_DT_UINT8)
+            launchNearest<uint8_t, true>(blocks, lds, ctx->stream, U, ctx->d_bricks,
+                                         ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
+                                         ctx->d_counters);
+        else
+            launchNearest<uint16_t, true>(blocks, lds, ctx->stream, U, ctx->d_bricks,
+                                          ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
+                                          ctx->d_counters);
+        LVR_HIP(ctx, hipGetLastError());
+        if (ctx->timing)
+            LVR_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+        return LVR_OK;
+    }
+    switch (dt)
     {
     case LVR_DT_UINT8:
         launchRaycast<uint8_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
@@ -869,6 +1060,9 @@ void lvr_batch_destroy(lvr_batch* b)
     (void)hipFree(b->d_counts);
     (void)hipFree(b->d_offsets);
     (void)hipFree(b->d_lists);
+    (void)hipFree(b->d_scounts);
+    (void)hipFree(b->d_soffsets);
+    (void)hipFree(b->d_slists);
     (void)hipFree(b->d_counters);
     if (b->h_pinned)
         (void)hipHostFree(b->h_pinned);
@@ -906,7 +1100,7 @@ int lvr_batch_clear(lvr_batch* b)
     b->tfOffset.clear();
     b->frameOffset.clear();
     b->outputs.clear();
-    b->listTotal = b->onScreen = b->pixels = 0;
+    b->listTotal = b->superTotal = b->onScreen = b->pixels = 0;
     b->ran = false;
     return LVR_OK;
 }
@@ -932,10 +1126,10 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
             return rc;
     }
     FrameUniforms U;
-    uint64_t listTotal = 0, onScreen = 0;
+    uint64_t listTotal = 0, superTotal = 0, onScreen = 0;
     const size_t nb = b->bricks.size();
     const int rc = prepareFrame(ctx, f, viewport, ordered_ids, n, params, n_tf, range0, range1,
-                                &U, b->bricks, b->ranges, &listTotal, &onScreen);
+                                &U, b->bricks, b->ranges, &listTotal, &superTotal, &onScreen);
     if (rc)
     {
         b->error = ctx->error;
@@ -960,6 +1154,7 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
     b->outputs.push_back(nullptr);
     b->pixels += size_t(U.width) * U.height;
     b->listTotal += listTotal;
+    b->superTotal += superTotal;
     b->onScreen += onScreen;
     b->reqs.push_back(U);
     if (request)
@@ -983,6 +1178,8 @@ int lvr_batch_run(lvr_batch* b)
         (rc = bgrow(b, &b->d_tf, &b->tfCap, b->tf.size(), &grew)) ||
         (rc = bgrow(b, &b->d_frames, &b->frameCap, b->pixels, &grew)) ||
         (rc = bgrow(b, &b->d_lists, &b->listCap, std::max<uint64_t>(1, b->listTotal), &grew)) ||
+        (rc = bgrow(b, &b->d_slists, &b->slistCap, std::max<uint64_t>(1, b->superTotal),
+                    &grew)) ||
         (rc = bgrow(b, &b->d_counters, &b->counterCap, size_t(nReq) + 2, &grew)))
         return rc;
     if (b->pinnedCap < size_t(nReq) + 2)
@@ -997,20 +1194,31 @@ int lvr_batch_run(lvr_batch* b)
         b->pinnedCap = size_t(nReq) + 2;
     }
     // request table: positions in the batch-wide arrays
-    int tiles = 0, items = 0;
+    int tiles = 0, items = 0, supers = 0;
     for (int r = 0; r < nReq; ++r)
     {
         FrameUniforms& U = b->reqs[size_t(r)];
         U.tileBase = tiles;
         U.itemBase = items;
+        U.superBase = supers;
         tiles += U.tilesX * U.tilesY;
+        supers += U.superX * U.superY;
         items += U.nItems;
         U.tf = b->d_tf + b->tfOffset[size_t(r)];
         U.frame = b->outputs[size_t(r)] ? b->outputs[size_t(r)]
                                         : b->d_frames + b->frameOffset[size_t(r)];
     }
-    if ((rc = bgrow(b, &b->d_counts, &b->tileCap, size_t(tiles), &grew)))
+    bool superGrew = false;
+    if ((rc = bgrow(b, &b->d_counts, &b->tileCap, size_t(tiles), &grew)) ||
+        (rc = bgrow(b, &b->d_scounts, &b->superCap, size_t(supers), &superGrew)))
         return rc;
+    if (superGrew || !b->d_soffsets)
+    {
+        (void)hipFree(b->d_soffsets);
+        b->d_soffsets = nullptr;
+        LVR_BHIP(b, hipMalloc((void**)&b->d_soffsets, sizeof(int32_t) * b->superCap));
+        grew = true;
+    }
     if (b->tileCap > 0 && !b->d_offsets)
         grew = true;
     if (grew || !b->d_offsets)
@@ -1027,6 +1235,7 @@ int lvr_batch_run(lvr_batch* b)
                                 int64_t(b->bricks.size()),
                                 int64_t(b->tf.size()),
                                 tiles,
+                                supers,
                                 items,
                                 int64_t(reinterpret_cast<intptr_t>(b->d_reqs)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_bricks)),
@@ -1035,6 +1244,9 @@ int lvr_batch_run(lvr_batch* b)
                                 int64_t(reinterpret_cast<intptr_t>(b->d_counts)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_offsets)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_lists)),
+                                int64_t(reinterpret_cast<intptr_t>(b->d_scounts)),
+                                int64_t(reinterpret_cast<intptr_t>(b->d_soffsets)),
+                                int64_t(reinterpret_cast<intptr_t>(b->d_slists)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_counters))};
 
     // inputs: pageable host vectors are copied before the launch sequence
@@ -1052,13 +1264,9 @@ int lvr_batch_run(lvr_batch* b)
                                hipMemcpyHostToDevice, b->stream));
 
     auto enqueue = [&](hipStream_t st) -> hipError_t {
-        const int binBlocks = (tiles + 3) / 4;
-        hipLaunchKernelGGL(binCountKernel, dim3(binBlocks), dim3(256), 0, st, b->d_reqs, nReq,
-                           b->d_ranges, tiles, b->d_counts);
-        hipLaunchKernelGGL(scanKernel, dim3(1), dim3(1024), 0, st, b->d_counts, tiles,
-                           b->d_offsets, b->d_counters, nReq + 2);
-        hipLaunchKernelGGL(binFillKernel, dim3(binBlocks), dim3(256), 0, st, b->d_reqs, nReq,
-                           b->d_ranges, tiles, b->d_offsets, b->d_lists);
+        enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,
+                       b->d_soffsets, b->d_slists, b->d_counts, b->d_offsets, b->d_lists,
+                       b->d_counters, nReq + 2);
         // two persistent workgroups per CU the stream may use
         const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
         const int blocks = std::max(1, std::min(items / (kSuper * kSuper), slots));

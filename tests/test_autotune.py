@@ -23,7 +23,7 @@ class FakeCtx:
         self.calls += 1
         out = livre_amd.RenderParams.from_buffer_copy(params)
         out.tile_w, out.tile_h, out.samples_per_thread = self.choice
-        return out, [(8, 8, 2, 1.0), (self.choice[0], self.choice[1], self.choice[2], 0.5)]
+        return out, [(8, 8, 2, 2, 1.0), (self.choice[0], self.choice[1], self.choice[2], 2, 0.5)]
 
 
 def test_request_class_buckets(lvr):
@@ -77,14 +77,15 @@ def test_autotune_candidates_render_the_same_frame(lvr):
         ctx.render(fr, 96, 80, ordered, base)
         ref, ref_smp = ctx.read_frame()
         tuned, cands = ctx.autotune(fr, 96, 80, ordered, base, reps=2)
-        assert len(cands) == 15
+        assert len(cands) == 30
         assert {(c[0], c[1]) for c in cands} == {(8, 8), (16, 4), (4, 16), (32, 2), (2, 32)}
-        assert {c[2] for c in cands} == {1, 2, 3}
-        best = min(cands, key=lambda c: c[3])
-        assert (tuned.tile_w, tuned.tile_h, tuned.samples_per_thread) == best[:3]
-        for tw, th, d, _ms in cands:
+        assert {c[2] for c in cands} == {1, 2, 3} and {c[3] for c in cands} == {1, 2}
+        best = min(cands, key=lambda c: c[4])
+        assert (tuned.tile_w, tuned.tile_h, tuned.samples_per_thread, tuned.classify) == \
+            best[:4]
+        for tw, th, d, m, _ms in cands:
             ctx.render(fr, 96, 80, ordered,
-                       lvr.render_params(128, tile=(tw, th), samples_per_thread=d))
+                       lvr.render_params(128, tile=(tw, th), samples_per_thread=d, classify=m))
             img, smp = ctx.read_frame()
             assert smp == ref_smp and np.array_equal(img, ref), (tw, th, d)
     finally:

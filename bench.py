@@ -496,7 +496,7 @@ def main():
                          "ooc: out-of-core 8192^3 fly-through")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
-    ap.add_argument("--max-batch", type=int, default=16)
+    ap.add_argument("--max-batch", type=int, default=2)
     ap.add_argument("--pool-frac", type=float, default=1.0)
     ap.add_argument("--pool-gb", type=float, default=200.0)
     ap.add_argument("--frames", type=int, default=300)

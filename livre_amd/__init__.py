@@ -488,8 +488,18 @@ class Context:
             self._h, lut.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), lut.size // 4,
             data_range[0], data_range[1]), "set_tf_rgba8")
 
-    def select_visibles(self, fr, window_height, sse, min_lod, max_lod, cap=1 << 22):
-        out = np.zeros(cap, np.uint64)
+    def node_count(self):
+        """Nodes of the octree: the largest possible visible set."""
+        if getattr(self, "_nodes", None) is None:
+            # every level is the root block grid refined 2x per axis
+            vi = self.vi
+            self._nodes = sum((vi.root_blocks[0] << lv) * (vi.root_blocks[1] << lv) *
+                              (vi.root_blocks[2] << lv) for lv in range(vi.depth))
+        return self._nodes
+
+    def select_visibles(self, fr, window_height, sse, min_lod, max_lod, cap=None):
+        cap = cap or self.node_count()
+        out = np.empty(cap, np.uint64)
         n = ctypes.c_int(0)
         self._check(lib().lvr_select_visibles(
             self._h, ctypes.byref(fr), window_height, sse, min_lod, max_lod,

@@ -376,7 +376,16 @@ public:
                 const Params& p, std::vector<uint64_t>* histogram = nullptr)
     {
         const lvr_volume_info& vi = _pool.volumeInfo().info;
-        std::vector<uint64_t> vis(1u << 20);
+        if (_vis.empty())
+        {
+            // every level is the root block grid refined 2x per axis
+            size_t nodes = 0;
+            for (uint32_t l = 0; l < vi.depth; ++l)
+                nodes += size_t(vi.root_blocks[0] << l) * (vi.root_blocks[1] << l) *
+                         (vi.root_blocks[2] << l);
+            _vis.resize(nodes);
+        }
+        std::vector<uint64_t>& vis = _vis;
         int n = 0;
         check(lvr_select_visibles(_pool.ctx(), &frustum.raw(), uint32_t(view.h), p.sse, p.minLod,
                                   std::min(p.maxLod, vi.depth - 1), vis.data(), int(vis.size()),
@@ -407,5 +416,6 @@ public:
 private:
     DataSourcePlugin& _source;
     BrickPool& _pool;
+    std::vector<uint64_t> _vis; // visible-set buffer, one entry per octree node
 };
 } // namespace livre

@@ -414,6 +414,7 @@ def ooc_workload(args):
         log("preloaded %d coarse bricks in %.2f s (%d source threads)"
             % (len(lv), time.perf_counter() - t0, threads))
         start = st.stats()
+        host_ms = []
         t_start = time.perf_counter()
         for t in range(args.frames):
             f0 = time.perf_counter()
@@ -422,11 +423,18 @@ def ooc_workload(args):
             mv = scenes.look_at(tuple(pos), tuple(ahead), up)
             proj = scenes.perspective(60.0, W4 / H4, 0.01, 6.0)
             fr = livre_amd.frustum(mv, proj)
+            h0 = time.perf_counter()
             vis = ctx.select_visibles(fr, H4, 1.0, 0, vi.depth - 1)
-            st.request(livre_amd.order(vi, fr, vis), ancestors=True)
+            h1 = time.perf_counter()
+            # wishes in selection (depth-first, spatially coherent) order, with
+            # their ancestors; the stream loads coarse levels first
+            st.request(vis, ancestors=True)
+            h2 = time.perf_counter()
             st.pump()
             rset, avail = st.rendering_set(vis)
+            h3 = time.perf_counter()
             ordered = livre_amd.order(vi, fr, rset)
+            host_ms.append(((h1 - h0) * 1e3, (h2 - h1) * 1e3, (h3 - h2) * 1e3))
             # samples per ray from the finest rendered level, as
             # RayCastRenderer::onFrameStart does when none is given
             # (RayCastRenderer.cpp:216-234)
@@ -474,6 +482,8 @@ def ooc_workload(args):
         "full_lod_fraction_mean": float(np.mean(a[:, 4] / np.maximum(a[:, 3], 1))),
         "full_lod_fraction_last": float(a[-1, 4] / max(a[-1, 3], 1)),
         "finest_rendered_level_mean": float(a[:, 7].mean()),
+        "host_ms_mean": dict(zip(("select", "order_and_request", "pump_and_rendering_set"),
+                                 np.mean(np.array(host_ms), axis=0).round(2).tolist())),
         "bricks_streamed": int(loaded),
         "streamed_gb_per_s": loaded * vi.max_block[0] ** 3 / total / 1e9,
         "pool_bricks_resident": int(pool_used),

@@ -263,6 +263,8 @@ int fail(lvr_ctx* ctx, int code, const std::string& msg);
 int hipFail(lvr_ctx* ctx, hipError_t e, const char* what);
 int ensureDevice(lvr_ctx* ctx);
 int32_t slotOf(const lvr_ctx* ctx, uint64_t id);
+// Dense index of a node in the page table (false outside the octree).
+bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx);
 void touch(lvr_ctx* ctx, uint64_t id);
 int flushPageTable(lvr_ctx* ctx);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);

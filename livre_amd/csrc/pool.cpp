@@ -32,7 +32,7 @@ int ensureDevice(lvr_ctx* ctx)
     return LVR_OK;
 }
 
-static bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx)
+bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx)
 {
     const NodeKey k = decodeNode(id);
     if (k.level >= ctx->vi.depth || k.t != 0)

@@ -116,7 +116,6 @@ struct lvr_stream
     std::mutex m;
     std::condition_variable cv;
     std::deque<uint64_t> queue;                   // wanted, most wanted first
-    std::unordered_set<uint64_t> queued;          // ids in `queue`
     std::unordered_set<uint64_t> generating;      // taken by a thread
     std::vector<std::pair<uint64_t, int>> ready;  // generated: (id, buffer)
     std::unordered_set<uint64_t> readyIds;        // ids in `ready`
@@ -126,6 +125,11 @@ struct lvr_stream
 
     // device side (caller's thread only)
     std::unordered_set<uint64_t> loading; // slots taken, copies in flight
+    // per octree node (page-table index): stamp of the last request that
+    // saw it, and 1 while its copy is in flight
+    std::vector<uint32_t> seen;
+    std::vector<uint8_t> loadingFlag;
+    uint32_t stamp = 0;
     std::deque<Upload> uploads;
     std::vector<hipEvent_t> spareEvents;
     int32_t* h_slots = nullptr; // pinned slot lists of the conversions
@@ -158,7 +162,6 @@ void worker(lvr_stream* s)
                 return;
             id = s->queue.front();
             s->queue.pop_front();
-            s->queued.erase(id);
             buf = s->freeBuffers.back();
             s->freeBuffers.pop_back();
             s->generating.insert(id);
@@ -197,6 +200,9 @@ int retireUploads(lvr_stream* s, bool block, int* completed)
         {
             (void)lvr::pinBrick(s->ctx, id, -1);
             s->loading.erase(id);
+            uint64_t idx;
+            if (lvr::pageIndex(s->ctx, id, &idx))
+                s->loadingFlag[idx] = 0;
         }
         s->stats.bricks_loaded += u.ids.size();
         s->stats.bytes_loaded += u.ids.size() * s->ctx->brickBytes;
@@ -269,6 +275,11 @@ int issueUploads(lvr_stream* s)
                 return stfail(s, rc, ctx->error);
             (void)lvr::pinBrick(ctx, id, 1);
             s->loading.insert(id);
+            {
+                uint64_t idx;
+                if (lvr::pageIndex(ctx, id, &idx))
+                    s->loadingFlag[idx] = 1;
+            }
             slots[k] = slot;
             LVR_HIP(ctx, hipMemcpyAsync(ctx->d_staging + k * ctx->brickBytes,
                                         s->buffers[size_t(buf)], ctx->brickBytes,
@@ -351,6 +362,8 @@ int lvr_stream_create(lvr_ctx* ctx, lvr_brick_source_fn source, void* user, int 
         }
         s->freeBuffers.push_back(i);
     }
+    s->seen.assign(size_t(ctx->pt.levelOffset.back()), 0u);
+    s->loadingFlag.assign(size_t(ctx->pt.levelOffset.back()), uint8_t(0));
     s->slotCap = size_t(staging_bricks) + 64;
     if (hipHostMalloc((void**)&s->h_slots, sizeof(int32_t) * s->slotCap,
                       hipHostMallocDefault) != hipSuccess)
@@ -400,47 +413,47 @@ int lvr_stream_request_ex(lvr_stream* s, const uint64_t* ids, int n, int ancesto
 {
     if (!s || n < 0 || (n > 0 && !ids))
         return LVR_EINVAL;
-    std::vector<uint64_t> want;
-    want.reserve(size_t(n));
-    std::unordered_set<uint64_t> seen;
-    auto consider = [&](uint64_t id) {
-        if (s->ctx->resident.count(id))
-            lvr::touch(s->ctx, id); // wanted: last in line for eviction
-        else
-            want.push_back(id);
-    };
+    lvr_ctx* ctx = s->ctx;
+    if (++s->stamp == 0) // wrapped: forget every old stamp
+    {
+        std::fill(s->seen.begin(), s->seen.end(), 0u);
+        s->stamp = 1;
+    }
+    const uint32_t stamp = s->stamp;
+    // wanted bricks bucketed by level (coarse first), each once
+    std::vector<std::vector<uint64_t>> byLevel(ctx->vi.depth);
     for (int i = 0; i < n; ++i)
     {
-        if (lvr::decodeNode(ids[i]).level >= s->ctx->vi.depth)
-            return stfail(s, LVR_EINVAL, "node id outside the octree");
-        if (ancestors)
+        uint64_t a = ids[i];
+        for (;;)
         {
-            // the chain up to the root, each once: intermediate fallbacks
-            for (uint64_t a = ids[i]; seen.insert(a).second;)
+            uint64_t idx;
+            if (!lvr::pageIndex(ctx, a, &idx))
+                return stfail(s, LVR_EINVAL, "node id outside the octree");
+            if (s->seen[idx] == stamp)
+                break; // this node and its ancestors are already handled
+            s->seen[idx] = stamp;
+            if (ctx->pt.h_slots[idx] >= 0)
             {
-                consider(a);
-                if (lvr::decodeNode(a).level == 0)
-                    break;
-                a = parentOf(a);
+                if (!s->loadingFlag[idx])
+                    lvr::touch(ctx, a); // wanted: last in line for eviction
             }
+            else
+                byLevel[lvr::decodeNode(a).level].push_back(a);
+            if (!ancestors || lvr::decodeNode(a).level == 0)
+                break;
+            a = parentOf(a);
         }
-        else
-            consider(ids[i]);
     }
-    // coarse levels first (they cover the most screen as fallbacks), then in
-    // the caller's order
-    std::stable_sort(want.begin(), want.end(), [](uint64_t a, uint64_t b) {
-        return lvr::decodeNode(a).level < lvr::decodeNode(b).level;
-    });
     {
         std::lock_guard<std::mutex> lk(s->m);
-        // the newest wishes replace the queue; bricks being generated finish
+        // the newest wishes replace the queue; bricks being generated or
+        // waiting for a slot finish
         s->queue.clear();
-        s->queued.clear();
-        for (uint64_t id : want)
-            if (!s->generating.count(id) && !s->readyIds.count(id) &&
-                s->queued.insert(id).second)
-                s->queue.push_back(id);
+        for (const auto& level : byLevel)
+            for (uint64_t id : level)
+                if (!s->generating.count(id) && !s->readyIds.count(id))
+                    s->queue.push_back(id);
         s->stats.requested += s->queue.size();
     }
     s->cv.notify_all();
@@ -506,8 +519,12 @@ int lvr_stream_rendering_set(lvr_stream* s, const uint64_t* visible, int n, uint
     const lvr_ctx* ctx = s->ctx;
     const int rc = renderingSet(
         visible, n,
-        [&](uint64_t id) { return ctx->resident.count(id) && !s->loading.count(id); }, out, cap,
-        n_out, n_available);
+        [&](uint64_t id) {
+            uint64_t idx;
+            return lvr::pageIndex(ctx, id, &idx) && ctx->pt.h_slots[idx] >= 0 &&
+                   !s->loadingFlag[idx];
+        },
+        out, cap, n_out, n_available);
     if (rc)
         return stfail(s, rc, "rendering set larger than the output buffer");
     return LVR_OK;

@@ -116,6 +116,11 @@ struct FrameUniforms
     float gmin[3], gmax[3];
     float texMin[3], texMax[3];
     int32_t texDim[3];
+    // host-computed per-frame constants (scalar operands in the kernels):
+    // texMax - texMin, float(texDim), float(texDim - 1), 1 / nSamples,
+    // float(nTf), float(nTf - 1), maxSamples / nSamples
+    float texExt[3], texDimF[3], texLastF[3];
+    float stepSize, nTfF, nTfLastF, alphaCorrection;
     int32_t nSamples, maxSamples;
     float exitA;
     float mult, add;

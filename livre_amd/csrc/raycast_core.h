@@ -124,9 +124,9 @@ __device__ __forceinline__ uint32_t nearestIndex(const FrameUniforms& U, const B
 {
     float tp[3];
     texturePosition(U, b, texExt, pos, tp);
-    const uint32_t ix = nearestTexel(tp[0] * float(U.texDim[0]), float(U.texDim[0] - 1));
-    const uint32_t iy = nearestTexel(tp[1] * float(U.texDim[1]), float(U.texDim[1] - 1));
-    const uint32_t iz = nearestTexel(tp[2] * float(U.texDim[2]), float(U.texDim[2] - 1));
+    const uint32_t ix = nearestTexel(tp[0] * U.texDimF[0], U.texLastF[0]);
+    const uint32_t iy = nearestTexel(tp[1] * U.texDimF[1], U.texLastF[1]);
+    const uint32_t iz = nearestTexel(tp[2] * U.texDimF[2], U.texLastF[2]);
     return tiledIndexDev(U.layout, ix, iy, iz);
 }
 
@@ -168,9 +168,9 @@ __device__ __forceinline__ Classify makeClassify(const FrameUniforms& U)
     Classify c;
     c.mult = U.mult;
     c.add = U.add;
-    c.nTfF = float(U.nTf);
-    c.nTfMax = float(U.nTf - 1);
-    c.alphaCorrection = float(U.maxSamples) / float(U.nSamples);
+    c.nTfF = U.nTfF;
+    c.nTfMax = U.nTfLastF;
+    c.alphaCorrection = U.alphaCorrection;
     c.maxA = 1.0f - 1.0f / 256.0f;
     return c;
 }
@@ -348,9 +348,8 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
     float tnearGlobal, tfarGlobal;
     intersectBoxInv(U.eye, inv, U.gmin, U.gmax, tnearGlobal, tfarGlobal);
 
-    const float stepSize = 1.0f / float(U.nSamples);
-    const float texExt[3] = {U.texMax[0] - U.texMin[0], U.texMax[1] - U.texMin[1],
-                             U.texMax[2] - U.texMin[2]};
+    const float stepSize = U.stepSize;
+    const float* texExt = U.texExt;
     Composite C;
     C.cls = makeClassify(U);
     C.acc0 = C.acc1 = C.acc2 = C.acc3 = 0.f;

@@ -502,12 +502,23 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     }
     U->nSamples = params->n_samples_per_ray;
     U->maxSamples = params->max_samples_per_ray;
+    for (int i = 0; i < 3; ++i)
+    {
+        // the same FP32 operations the shader performs per fragment
+        U->texExt[i] = U->texMax[i] - U->texMin[i];
+        U->texDimF[i] = float(U->texDim[i]);
+        U->texLastF[i] = float(U->texDim[i] - 1);
+    }
+    U->stepSize = 1.0f / float(U->nSamples);
+    U->alphaCorrection = float(U->maxSamples) / float(U->nSamples);
     U->exitA = params->early_exit;
     float r0, r1;
     dataRange(vi.data_type, tf0, tf1, &r0, &r1);
     U->mult = 1.0f / (r1 - r0);
     U->add = -r0 / (r1 - r0);
     U->nTf = nTf;
+    U->nTfF = float(nTf);
+    U->nTfLastF = float(nTf - 1);
     U->tileW = tw;
     U->tileH = th;
     U->tilesX = (W + tw - 1) / tw;
@@ -772,9 +783,9 @@ static int ensureClassified(lvr_ctx* ctx, const lvr::FrameUniforms& U, int nValu
     Classify c;
     c.mult = U.mult;
     c.add = U.add;
-    c.nTfF = float(U.nTf);
-    c.nTfMax = float(U.nTf - 1);
-    c.alphaCorrection = float(U.maxSamples) / float(U.nSamples);
+    c.nTfF = U.nTfF;
+    c.nTfMax = U.nTfLastF;
+    c.alphaCorrection = U.alphaCorrection;
     c.maxA = 1.0f - 1.0f / 256.0f;
     hipLaunchKernelGGL(classifyKernel, dim3((nValues + 255) / 256), dim3(256), 0, ctx->stream, c,
                        U.tf, nValues, ctx->d_cls);

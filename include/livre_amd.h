@@ -229,6 +229,9 @@ typedef struct
                                     2 = computed per sample from the TF,
                                     0 = automatic (table for 8-bit data);
                                     identical images either way */
+    int32_t waves_per_simd;      /* wavefronts per SIMD the ray cast keeps
+                                    resident: 2, 4, 6 or 8; 0 = 4. More hide
+                                    more latency, fewer thrash the L1 less */
 } lvr_render_params;
 
 /* Render one frame of the ordered bricks (fragRayCast.glsl:113-207 per brick,
@@ -266,16 +269,17 @@ int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out);
 
 /* Autotuner of the launch shape (no reference counterpart: the GL path has
  * no tunable launch). Renders the given frame with every ray tile in
- * {8x8, 16x4, 4x16, 32x2, 2x32} and, for NEAREST, every samples_per_thread in
- * {1, 2, 3}, and for 8/16-bit data both classify modes; times the ray-cast
- * kernel with HIP events (median of reps) and writes the fastest tile_w,
- * tile_h, samples_per_thread and classify into *params.
+ * {8x8, 16x4, 4x16, 32x2, 2x32}, every waves_per_simd in {4, 6, 8}, for
+ * NEAREST every samples_per_thread in {1, 2, 3}, and for 8/16-bit data both
+ * classify modes; times the ray-cast kernel with HIP events (median of reps)
+ * and writes the fastest tile_w, tile_h, samples_per_thread, classify and
+ * waves_per_simd into *params.
  * All candidates produce the same image. The timings of the candidates are
  * copied to results (up to cap entries); *n_results = number of candidates.
  * Leaves the last candidate's frame in the framebuffer. */
 typedef struct
 {
-    int32_t tile_w, tile_h, samples_per_thread, classify;
+    int32_t tile_w, tile_h, samples_per_thread, classify, waves_per_simd;
     float raycast_ms;
 } lvr_tune_entry;
 int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
@@ -310,7 +314,8 @@ const char* lvr_batch_last_error(const lvr_batch* batch);
 int lvr_batch_set_graph(lvr_batch* batch, int enable);
 /* Queue one frame of ctx's volume; request receives its index. The TF is
  * n_tf FP32 RGBA entries over [range0, range1] (as lvr_tf_set). Adding after
- * a run starts a new batch. */
+ * a run starts a new batch. One launch renders all requests, so their
+ * samples_per_thread and waves_per_simd must agree (LVR_EINVAL otherwise). */
 int lvr_batch_add(lvr_batch* batch, lvr_ctx* ctx, const lvr_frustum* f,
                   const int viewport[4], const uint64_t* ordered_ids, int n,
                   const lvr_render_params* params, const float* tf_rgba,

@@ -121,13 +121,14 @@ class RenderParams(ctypes.Structure):
                 ("n_clip_planes", ctypes.c_int32),
                 ("clip_planes", ctypes.c_float * 24),
                 ("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
-                ("samples_per_thread", ctypes.c_int32), ("classify", ctypes.c_int32)]
+                ("samples_per_thread", ctypes.c_int32), ("classify", ctypes.c_int32),
+                ("waves_per_simd", ctypes.c_int32)]
 
 
 class TuneEntry(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
                 ("samples_per_thread", ctypes.c_int32), ("classify", ctypes.c_int32),
-                ("raycast_ms", ctypes.c_float)]
+                ("waves_per_simd", ctypes.c_int32), ("raycast_ms", ctypes.c_float)]
 
 
 class NoiseParams(ctypes.Structure):
@@ -369,7 +370,7 @@ def field_brick_host(vi, params, node_id):
 
 def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
                   max_samples_per_ray=32, clip_planes=(), tile=(0, 0),
-                  samples_per_thread=0, classify=0):
+                  samples_per_thread=0, classify=0, waves_per_simd=0):
     p = RenderParams()
     p.n_samples_per_ray = n_samples_per_ray
     p.max_samples_per_ray = max_samples_per_ray
@@ -382,6 +383,7 @@ def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
     p.tile_w, p.tile_h = tile
     p.samples_per_thread = samples_per_thread
     p.classify = classify
+    p.waves_per_simd = waves_per_simd
     return p
 
 
@@ -515,19 +517,21 @@ class Context:
         self.width, self.height = width, height
 
     def autotune(self, fr, w, h, ordered, params, reps=3):
-        """Fastest ray tile, samples per thread and classification mode for
-        this frame: returns a copy of params with them set, and the list of
-        (tile_w, tile_h, samples_per_thread, classify, raycast_ms) candidates."""
+        """Fastest ray tile, samples per thread, classification mode and
+        wavefronts per SIMD for this frame: returns a copy of params with them
+        set, and the list of (tile_w, tile_h, samples_per_thread, classify,
+        waves_per_simd, raycast_ms) candidates."""
         ordered, pids = _u64(ordered)
         vp = (ctypes.c_int * 4)(0, 0, w, h)
         out = RenderParams.from_buffer_copy(params)
-        res = (TuneEntry * 64)()
+        cap = 256
+        res = (TuneEntry * cap)()
         n = ctypes.c_int()
         self._check(lib().lvr_autotune(self._h, ctypes.byref(fr), vp, pids, len(ordered), reps,
-                                       ctypes.byref(out), res, 64, ctypes.byref(n)), "autotune")
+                                       ctypes.byref(out), res, cap, ctypes.byref(n)), "autotune")
         self.width, self.height = w, h
-        cands = [(r.tile_w, r.tile_h, r.samples_per_thread, r.classify, r.raycast_ms)
-                 for r in res[:min(n.value, 64)]]
+        cands = [(r.tile_w, r.tile_h, r.samples_per_thread, r.classify, r.waves_per_simd,
+                  r.raycast_ms) for r in res[:min(n.value, cap)]]
         return out, cands
 
     def read_frame(self):

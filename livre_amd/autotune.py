@@ -1,21 +1,24 @@
 """Launch-shape autotuner with a persistent cache.
 
 The ray tile (how a wavefront's 64 rays cover the screen), the samples per
-thread (voxel fetches in flight per ray) and, for 8/16-bit volumes, the
-classification mode (tabulated per raw value or computed per sample) change
-only the speed of a frame, never its pixels. `Tuner` picks them per request class -- voxel type,
-viewport size (log2 of the pixel count), sampling rate (samples per voxel of
-the finest level, log2) and filter -- by timing every candidate on the first
-frame of that class (`lvr_autotune`), and keeps the choices in a JSON file so
-that later runs start tuned. The package ships the table measured on MI355X
-(tune_gfx950.json beside this module); the user cache (LIVRE_AMD_TUNE_CACHE,
-default ~/.cache/livre_amd/tune_gfx950.json) takes precedence over it and
-receives every class tuned later.
+thread (voxel fetches in flight per ray), the resident wavefronts per SIMD
+and, for 8/16-bit volumes, the classification mode (tabulated per raw value
+or computed per sample) change only the speed of a frame, never its pixels.
+`Tuner` picks them per request class -- voxel type, viewport size (log2 of
+the pixel count), sampling rate (samples per voxel of the finest level, log2)
+and filter -- by timing every candidate on the first frame of that class
+(`lvr_autotune`), and keeps the choices in a JSON file so that later runs
+start tuned. The package ships the table measured on MI355X (tune_gfx950.json
+beside this module); the user cache (LIVRE_AMD_TUNE_CACHE, default
+~/.cache/livre_amd/tune_gfx950.json) takes precedence over it and receives
+every class tuned later. Entries without a wavefronts-per-SIMD choice (older
+caches) are tuned again.
 
-A batch renders with one samples-per-thread value for all of its requests
-(it selects the kernel), so `tune_batch` gives every request its class's
-ray tile and the batch the samples-per-thread value that the classes chose
-most often, weighted by pixels.
+A batch renders with one samples-per-thread value (it selects the kernel)
+and one workgroup size for all of its requests, so `tune_batch` gives every
+request its class's ray tile and the batch the samples-per-thread and
+wavefronts-per-SIMD values that the classes chose most often, weighted by
+pixels.
 """
 import json
 import math
@@ -75,11 +78,12 @@ class Tuner:
         lib()
         key = request_class(ctx.vi, width, height, params)
         c = self.table.get(key)
-        if c is None:
+        if c is None or "waves_per_simd" not in c:
             tuned, cands = ctx.autotune(fr, width, height, ordered, params, self.reps)
             c = {"tile": [tuned.tile_w, tuned.tile_h],
                  "samples_per_thread": tuned.samples_per_thread,
                  "classify": tuned.classify,
+                 "waves_per_simd": tuned.waves_per_simd,
                  "candidates": [list(x) for x in cands]}
             self.table[key] = c
             self.tuned += 1
@@ -88,19 +92,22 @@ class Tuner:
         out.tile_w, out.tile_h = c["tile"]
         out.samples_per_thread = c["samples_per_thread"]
         out.classify = c.get("classify", 0)
+        out.waves_per_simd = c.get("waves_per_simd", 0)
         return out
 
     def tune_batch(self, requests):
         """requests: list of (ctx, fr, width, height, ordered, params).
         Returns one RenderParams per request: the class's tile, and the
-        batch-wide samples per thread."""
+        batch-wide samples per thread and wavefronts per SIMD."""
         tuned = [self.params_for(*r) for r in requests]
-        votes = {}
-        for (_ctx, _fr, w, h, _o, _p), t in zip(requests, tuned):
-            if not t.linear_filter:
-                votes[t.samples_per_thread] = votes.get(t.samples_per_thread, 0) + w * h
-        if votes:
-            depth = max(votes.items(), key=lambda kv: kv[1])[0]
-            for t in tuned:
-                t.samples_per_thread = depth
+        for field, nearest_only in (("samples_per_thread", True), ("waves_per_simd", False)):
+            votes = {}
+            for (_ctx, _fr, w, h, _o, _p), t in zip(requests, tuned):
+                if not (nearest_only and t.linear_filter):
+                    v = getattr(t, field)
+                    votes[v] = votes.get(v, 0) + w * h
+            if votes:
+                best = max(votes.items(), key=lambda kv: kv[1])[0]
+                for t in tuned:
+                    setattr(t, field, best)
         return tuned

@@ -1,6 +1,7 @@
 // Autotuner of the ray-cast launch shape: the ray tile (how a wavefront's 64
-// rays cover the screen), the samples per thread (voxel fetches in flight
-// per ray) and, for 8/16-bit volumes, tabulated or per-sample classification.
+// rays cover the screen), the resident wavefronts per SIMD, the samples per
+// thread (voxel fetches in flight per ray) and, for 8/16-bit volumes,
+// tabulated or per-sample classification.
 // Every candidate renders the caller's frame; the ray-cast kernel is timed
 // with HIP events on the context stream, median of `reps` launches, and
 // the fastest candidate is written back into the parameters. Results are
@@ -14,6 +15,7 @@
 namespace
 {
 const int kTiles[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}};
+const int kWavesPerSimd[] = {4, 6, 8};
 } // namespace
 
 extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
@@ -35,39 +37,42 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
         !params->linear_filter && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16);
     const int modes = tabulable ? 2 : 1;
     std::vector<lvr_tune_entry> all;
-    for (int c = 0; c < int(sizeof kTiles / sizeof kTiles[0]) * depths * modes; ++c)
+    const int nTiles = int(sizeof kTiles / sizeof kTiles[0]);
+    const int nOcc = int(sizeof kWavesPerSimd / sizeof kWavesPerSimd[0]);
+    for (int c = 0; c < nTiles * nOcc * depths * modes; ++c)
     {
-        const auto& t = kTiles[c / (depths * modes)];
+        const auto& t = kTiles[c / (nOcc * depths * modes)];
+        const int w = kWavesPerSimd[(c / (depths * modes)) % nOcc];
         const int d = 1 + (c / modes) % depths;
         const int m = 1 + c % modes;
+        lvr_render_params p = *params;
+        p.tile_w = t[0];
+        p.tile_h = t[1];
+        p.samples_per_thread = params->linear_filter ? 0 : d;
+        p.classify = tabulable ? m : 0;
+        p.waves_per_simd = w;
+        std::vector<float> ms;
+        for (int r = 0; r < reps && rc == LVR_OK; ++r)
         {
-            lvr_render_params p = *params;
-            p.tile_w = t[0];
-            p.tile_h = t[1];
-            p.samples_per_thread = params->linear_filter ? 0 : d;
-            p.classify = tabulable ? m : 0;
-            std::vector<float> ms;
-            for (int r = 0; r < reps && rc == LVR_OK; ++r)
-            {
-                rc = lvr_render(ctx, f, viewport, ordered_ids, n, &p);
-                lvr_frame_stats st;
-                if (rc == LVR_OK)
-                    rc = lvr_frame_stats_get(ctx, &st);
-                if (rc == LVR_OK)
-                    ms.push_back(st.raycast_ms);
-            }
-            if (rc)
-                break;
-            std::nth_element(ms.begin(), ms.begin() + ms.size() / 2, ms.end());
-            lvr_tune_entry e;
-            std::memset(&e, 0, sizeof e);
-            e.tile_w = p.tile_w;
-            e.tile_h = p.tile_h;
-            e.samples_per_thread = p.samples_per_thread;
-            e.classify = p.classify;
-            e.raycast_ms = ms[ms.size() / 2];
-            all.push_back(e);
+            rc = lvr_render(ctx, f, viewport, ordered_ids, n, &p);
+            lvr_frame_stats st;
+            if (rc == LVR_OK)
+                rc = lvr_frame_stats_get(ctx, &st);
+            if (rc == LVR_OK)
+                ms.push_back(st.raycast_ms);
         }
+        if (rc)
+            break;
+        std::nth_element(ms.begin(), ms.begin() + ms.size() / 2, ms.end());
+        lvr_tune_entry e;
+        std::memset(&e, 0, sizeof e);
+        e.tile_w = p.tile_w;
+        e.tile_h = p.tile_h;
+        e.samples_per_thread = p.samples_per_thread;
+        e.classify = p.classify;
+        e.waves_per_simd = p.waves_per_simd;
+        e.raycast_ms = ms[ms.size() / 2];
+        all.push_back(e);
     }
     if (!wasTiming)
         (void)lvr_set_timing(ctx, 0);
@@ -81,6 +86,7 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
     params->tile_h = best->tile_h;
     params->samples_per_thread = best->samples_per_thread;
     params->classify = best->classify;
+    params->waves_per_simd = best->waves_per_simd;
     const int k = std::min(cap, int(all.size()));
     for (int i = 0; i < k; ++i)
         results[i] = all[size_t(i)];

@@ -130,6 +130,7 @@ struct FrameUniforms
     int32_t linear;
     int32_t depth; // voxel fetches in flight per ray (NEAREST march)
     int32_t classify; // 1 tabulated, 2 per sample (8/16-bit NEAREST)
+    int32_t wavesPerSimd; // resident wavefronts per SIMD (single-frame launch)
     int32_t nClip;
     float clip[24];
     uint64_t slotStride;
@@ -223,10 +224,12 @@ struct lvr_ctx
     unsigned long long* d_hist = nullptr;
     int histBins = 0; // allocated bins
     int histLast = 0; // bins of the last histogram
-    int32_t* d_histSlots = nullptr;
-    uint32_t* d_histScale = nullptr; // job records
-    int histCapacity = 0;            // slot entries
-    int histJobCapacity = 0;         // 32-bit words of job records
+    // job lists of the histogram kernels: two pinned, device-mapped buffers
+    // used in turn, each with the event of the last kernel that read it
+    uint8_t* h_histRing[2] = {nullptr, nullptr};
+    size_t histRingBytes[2] = {0, 0};
+    hipEvent_t histRingDone[2] = {nullptr, nullptr};
+    int histRingNext = 0;
     // running histogram of the last set (d_hist holds its sum)
     std::vector<uint64_t> histSet;   // sorted resident ids
     bool histValid = false;

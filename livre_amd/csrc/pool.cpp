@@ -439,8 +439,13 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->d_superList);
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_hist);
-    (void)hipFree(ctx->d_histSlots);
-    (void)hipFree(ctx->d_histScale);
+    for (int r = 0; r < 2; ++r)
+    {
+        if (ctx->h_histRing[r])
+            (void)hipHostFree(ctx->h_histRing[r]);
+        if (ctx->histRingDone[r])
+            (void)hipEventDestroy(ctx->histRingDone[r]);
+    }
     (void)hipFree(ctx->d_selA);
     (void)hipFree(ctx->d_selB);
     (void)hipFree(ctx->d_selOut);

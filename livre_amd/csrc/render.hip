@@ -204,12 +204,13 @@ __device__ inline int nextItem(unsigned long long* counter, int lane)
     return __builtin_amdgcn_readfirstlane(int(__shfl(got, 0, kWave)));
 }
 
-// Persistent single-frame kernel: each workgroup (16 wavefronts) stages the
-// whole transfer function in LDS once (up to 8192 FP32 entries), then every
-// wavefront pulls tiles from the frame's atomic counter until none are left.
-// Tiles differ by orders of magnitude in work (empty sky vs rays through the
-// whole volume), so pulling them dynamically keeps every CU busy to the end.
-// Two 1024-ray workgroups per CU need <= 64 VGPRs (8 wavefronts per SIMD).
+// Persistent single-frame kernel: each workgroup (4 to 16 wavefronts) stages
+// the whole transfer function in LDS once (up to 8192 FP32 entries), then
+// every wavefront pulls tiles from the frame's atomic counter until none are
+// left. Tiles differ by orders of magnitude in work (empty sky vs rays
+// through the whole volume), so pulling them dynamically keeps every CU busy
+// to the end. Up to 8 wavefronts per SIMD fit in <= 64 VGPRs; fewer keep
+// fewer voxel lines in flight per CU, which the 32 KB L1 holds better.
 template <typename T, bool LINEAR, int D, bool CLS>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void raycastKernel(
     const FrameUniforms U, const BrickEntry* __restrict__ bricks,
@@ -259,7 +260,7 @@ constexpr int kBatchTf = 256;
 // restages that frame's transfer function in its own LDS slice when the
 // frame changes (tables above kBatchTf entries are read from global memory),
 // and dispatches on the frame's voxel type. Composite steps are counted per
-// frame.
+// frame. A workgroup holds 2 * wavesPerSimd (4 to 16) wavefronts.
 template <int D>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void batchKernel(
     const FrameUniforms* __restrict__ reqs, int nReq, int totalItems,
@@ -366,7 +367,9 @@ void launchRaycastAs(int blocks, size_t lds, hipStream_t st, const FrameUniforms
                      const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
                      const int32_t* lists, unsigned long long* counters)
 {
-    const int threads = kSuper * kSuper * kWave;
+    // two workgroups per CU, each with wavesPerSimd * 2 wavefronts (spread
+    // over the CU's 4 SIMDs)
+    const int threads = U.wavesPerSimd * 2 * kWave;
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&raycastKernel<T, LINEAR, D, CLS>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
     hipLaunchKernelGGL((raycastKernel<T, LINEAR, D, CLS>), dim3(blocks), dim3(threads), lds, st,
@@ -477,6 +480,9 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
         return fail(ctx, LVR_EINVAL, "samples per thread must be 1, 2 or 3");
     if (params->classify < 0 || params->classify > 2)
         return fail(ctx, LVR_EINVAL, "classify must be 0, 1 or 2");
+    const int wps = params->waves_per_simd == 0 ? 4 : params->waves_per_simd;
+    if (wps < 2 || wps > 8 || (wps & 1))
+        return fail(ctx, LVR_EINVAL, "waves per SIMD must be 2, 4, 6 or 8");
 
     std::memset(U, 0, sizeof *U);
     std::memcpy(U->invProj, f->inv_proj, sizeof U->invProj);
@@ -528,6 +534,7 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     U->nBricks = n;
     U->linear = params->linear_filter ? 1 : 0;
     U->depth = depth;
+    U->wavesPerSimd = wps;
     // automatic: a 256-entry table sits in LDS beside the march; a 65536-entry
     // one is a dependent global gather per sample, slower than computing
     U->classify = params->classify != 0 ? params->classify
@@ -885,10 +892,11 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                    ctx->d_tileList, ctx->d_counters, 2);
     if (ctx->timing)
         LVR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
-    // persistent: two 1024-ray workgroups per CU are resident (LDS and VGPR
-    // budget); more would only find the queue empty
-    const int superTiles = U.nItems / (kSuper * kSuper);
-    const int blocks = std::max(1, std::min(superTiles, 2 * ctx->numCUs));
+    // persistent: two workgroups per CU are resident (the LDS budget of a
+    // 4096-entry transfer function); more would only find the queue empty
+    const int perBlock = U.wavesPerSimd * 2;
+    const int blocks =
+        std::max(1, std::min((U.nItems + perBlock - 1) / perBlock, 2 * ctx->numCUs));
     size_t lds = size_t(U.nTf) * sizeof(float4);
     const int dt = ctx->vi.data_type;
     if (!U.linear && U.classify == 1 && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16))
@@ -1154,6 +1162,12 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
         b->ranges.resize(nb);
         return bfail(b, LVR_EINVAL, "samples per thread differs from the batch's");
     }
+    if (!b->reqs.empty() && U.wavesPerSimd != b->reqs.front().wavesPerSimd)
+    {
+        b->bricks.resize(nb);
+        b->ranges.resize(nb);
+        return bfail(b, LVR_EINVAL, "waves per SIMD differs from the batch's");
+    }
     U.request = int32_t(b->reqs.size());
     if (std::find(b->ctxs.begin(), b->ctxs.end(), ctx) == b->ctxs.end())
         b->ctxs.push_back(ctx);
@@ -1241,8 +1255,10 @@ int lvr_batch_run(lvr_batch* b)
 
     // the launch sequence depends on these sizes and buffers only
     const int depth = b->reqs.front().depth;
+    const int wps = b->reqs.front().wavesPerSimd;
     std::vector<int64_t> key = {nReq,
                                 depth,
+                                wps,
                                 int64_t(b->bricks.size()),
                                 int64_t(b->tf.size()),
                                 tiles,
@@ -1278,10 +1294,11 @@ int lvr_batch_run(lvr_batch* b)
         enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,
                        b->d_soffsets, b->d_slists, b->d_counts, b->d_offsets, b->d_lists,
                        b->d_counters, nReq + 2);
-        // two persistent workgroups per CU the stream may use
+        // two persistent workgroups of wps * 2 wavefronts per CU the stream
+        // may use
         const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
-        const int blocks = std::max(1, std::min(items / (kSuper * kSuper), slots));
-        const dim3 grid(blocks), block(kSuper * kSuper * kWave);
+        const int blocks = std::max(1, std::min((items + 2 * wps - 1) / (2 * wps), slots));
+        const dim3 grid(blocks), block(2 * wps * kWave);
         if (depth == 1)
             hipLaunchKernelGGL(batchKernel<1>, grid, block, 0, st, b->d_reqs, nReq, items,
                                b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);

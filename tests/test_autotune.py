@@ -22,8 +22,8 @@ class FakeCtx:
     def autotune(self, fr, w, h, ordered, params, reps):
         self.calls += 1
         out = livre_amd.RenderParams.from_buffer_copy(params)
-        out.tile_w, out.tile_h, out.samples_per_thread = self.choice
-        return out, [(8, 8, 2, 2, 1.0), (self.choice[0], self.choice[1], self.choice[2], 2, 0.5)]
+        out.tile_w, out.tile_h, out.samples_per_thread, out.waves_per_simd = self.choice
+        return out, [(8, 8, 2, 2, 4, 1.0), tuple(self.choice[:3]) + (2, self.choice[3], 0.5)]
 
 
 def test_request_class_buckets(lvr):
@@ -40,27 +40,35 @@ def test_cache_hits_persist_and_vote(lvr, tmp_path):
     path = str(tmp_path / "tune.json")
     t = autotune.Tuner(path=path)
     t.table = {}  # ignore the shipped table for this test
-    u8 = FakeCtx(lvr.volume_info(256, 32, 1, lvr.DT_UINT8), (16, 4, 3))
-    f32 = FakeCtx(lvr.volume_info(256, 32, 1, lvr.DT_FLOAT), (4, 16, 1))
+    u8 = FakeCtx(lvr.volume_info(256, 32, 1, lvr.DT_UINT8), (16, 4, 3, 6))
+    f32 = FakeCtx(lvr.volume_info(256, 32, 1, lvr.DT_FLOAT), (4, 16, 1, 8))
     p = lvr.render_params(256)
     a = t.params_for(u8, None, 512, 512, [], p)
-    assert (a.tile_w, a.tile_h, a.samples_per_thread) == (16, 4, 3)
+    assert (a.tile_w, a.tile_h, a.samples_per_thread, a.waves_per_simd) == (16, 4, 3, 6)
     t.params_for(u8, None, 500, 520, [], p)  # same class: cached
     assert u8.calls == 1 and t.tuned == 1
     with open(path) as f:
         saved = json.load(f)
     assert saved[autotune.request_class(u8.vi, 512, 512, p)]["tile"] == [16, 4]
+    assert saved[autotune.request_class(u8.vi, 512, 512, p)]["waves_per_simd"] == 6
     # a new tuner reads the user cache back
     t2 = autotune.Tuner(path=path)
     b = t2.params_for(u8, None, 512, 512, [], p)
     assert u8.calls == 1 and (b.tile_w, b.tile_h) == (16, 4)
-    # batch: tiles per class, samples per thread by pixel-weighted vote
+    # an entry without a waves-per-SIMD choice (older cache) is tuned again
+    key = autotune.request_class(u8.vi, 512, 512, p)
+    del t2.table[key]["waves_per_simd"]
+    t2.params_for(u8, None, 512, 512, [], p)
+    assert u8.calls == 2 and t2.table[key]["waves_per_simd"] == 6
+    # batch: tiles per class, samples per thread and waves per SIMD by
+    # pixel-weighted vote
     reqs = [(u8, None, 512, 512, [], p), (f32, None, 2048, 2048, [], p),
             (u8, None, 512, 512, [], p)]
     tuned = t2.tune_batch(reqs)
     assert [(x.tile_w, x.tile_h) for x in tuned] == [(16, 4), (4, 16), (16, 4)]
     assert {x.samples_per_thread for x in tuned} == {1}
-    assert p.samples_per_thread == 0  # inputs untouched
+    assert {x.waves_per_simd for x in tuned} == {8}
+    assert p.samples_per_thread == 0 and p.waves_per_simd == 0  # inputs untouched
 
 
 @pytest.mark.gpu
@@ -77,16 +85,18 @@ def test_autotune_candidates_render_the_same_frame(lvr):
         ctx.render(fr, 96, 80, ordered, base)
         ref, ref_smp = ctx.read_frame()
         tuned, cands = ctx.autotune(fr, 96, 80, ordered, base, reps=2)
-        assert len(cands) == 30
+        assert len(cands) == 90
         assert {(c[0], c[1]) for c in cands} == {(8, 8), (16, 4), (4, 16), (32, 2), (2, 32)}
         assert {c[2] for c in cands} == {1, 2, 3} and {c[3] for c in cands} == {1, 2}
-        best = min(cands, key=lambda c: c[4])
-        assert (tuned.tile_w, tuned.tile_h, tuned.samples_per_thread, tuned.classify) == \
-            best[:4]
-        for tw, th, d, m, _ms in cands:
+        assert {c[4] for c in cands} == {4, 6, 8}
+        best = min(cands, key=lambda c: c[5])
+        assert (tuned.tile_w, tuned.tile_h, tuned.samples_per_thread, tuned.classify,
+                tuned.waves_per_simd) == best[:5]
+        for tw, th, d, m, w, _ms in cands:
             ctx.render(fr, 96, 80, ordered,
-                       lvr.render_params(128, tile=(tw, th), samples_per_thread=d, classify=m))
+                       lvr.render_params(128, tile=(tw, th), samples_per_thread=d, classify=m,
+                                         waves_per_simd=w))
             img, smp = ctx.read_frame()
-            assert smp == ref_smp and np.array_equal(img, ref), (tw, th, d)
+            assert smp == ref_smp and np.array_equal(img, ref), (tw, th, d, m, w)
     finally:
         ctx.close()

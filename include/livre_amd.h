@@ -230,7 +230,7 @@ typedef struct
                                     0 = automatic (table for 8-bit data);
                                     identical images either way */
     int32_t waves_per_simd;      /* wavefronts per SIMD the ray cast keeps
-                                    resident: 2, 4, 6 or 8; 0 = 4. More hide
+                                    resident: 2 to 8; 0 = 4. More hide
                                     more latency, fewer thrash the L1 less */
 } lvr_render_params;
 

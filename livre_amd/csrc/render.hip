@@ -481,8 +481,8 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     if (params->classify < 0 || params->classify > 2)
         return fail(ctx, LVR_EINVAL, "classify must be 0, 1 or 2");
     const int wps = params->waves_per_simd == 0 ? 4 : params->waves_per_simd;
-    if (wps < 2 || wps > 8 || (wps & 1))
-        return fail(ctx, LVR_EINVAL, "waves per SIMD must be 2, 4, 6 or 8");
+    if (wps < 2 || wps > 8)
+        return fail(ctx, LVR_EINVAL, "waves per SIMD must be 2 to 8");
 
     std::memset(U, 0, sizeof *U);
     std::memcpy(U->invProj, f->inv_proj, sizeof U->invProj);

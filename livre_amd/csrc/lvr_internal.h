@@ -102,7 +102,57 @@ struct PoolEntry
 {
     int32_t slot;
     int pinned;
-    std::list<uint64_t>::iterator lru;
+};
+
+// Eviction order of the resident bricks (front = next to evict) as a doubly
+// linked list over page indices, in arrays sized by the octree's node count:
+// moving a brick to the back rewrites three array entries, where a heap list
+// would chase nodes scattered over memory for every rendered brick.
+struct DenseLru
+{
+    std::vector<int32_t> prev, next; // -1 = none
+    std::vector<uint64_t> ids;       // node id of each listed page index
+    int32_t head = -1, tail = -1;
+
+    void init(size_t pages)
+    {
+        prev.assign(pages, -1);
+        next.assign(pages, -1);
+        ids.assign(pages, 0);
+        head = tail = -1;
+    }
+    void pushBack(int32_t i, uint64_t id)
+    {
+        ids[size_t(i)] = id;
+        prev[size_t(i)] = tail;
+        next[size_t(i)] = -1;
+        if (tail >= 0)
+            next[size_t(tail)] = i;
+        else
+            head = i;
+        tail = i;
+    }
+    void erase(int32_t i)
+    {
+        const int32_t p = prev[size_t(i)], n = next[size_t(i)];
+        if (p >= 0)
+            next[size_t(p)] = n;
+        else
+            head = n;
+        if (n >= 0)
+            prev[size_t(n)] = p;
+        else
+            tail = p;
+        prev[size_t(i)] = next[size_t(i)] = -1;
+    }
+    void moveToBack(int32_t i)
+    {
+        if (i == tail)
+            return;
+        const uint64_t id = ids[size_t(i)];
+        erase(i);
+        pushBack(i, id);
+    }
 };
 
 // Everything one frame (one request of a batch) needs on the device.
@@ -149,6 +199,27 @@ struct FrameUniforms
     const float4* cls; // classified table per raw value (integer NEAREST), or null
     uint32_t* frame;   // RGBA8 output, row 0 = bottom
 };
+
+// Pinned staging of the per-frame host records (brick entries, screen
+// ranges, request tables): hipMemcpyAsync from pinned memory is a plain DMA
+// that returns at once, where a pageable source is first staged by the host
+// in chunks. Two buffers are used in turn, and one is rewritten only after
+// the copies that read it have completed.
+struct PinnedRing
+{
+    uint8_t* buf[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    bool armed[2] = {false, false};
+    int next = 0;
+
+    // a buffer of at least `bytes` whose previous copies have completed;
+    // *slot names it for release()
+    hipError_t acquire(size_t bytes, uint8_t** out, int* slot);
+    // the copies reading buffer `slot` are queued on st
+    hipError_t release(int slot, hipStream_t st);
+    void destroy();
+};
 } // namespace lvr
 
 struct lvr_ctx
@@ -171,7 +242,7 @@ struct lvr_ctx
     lvr::PageTable pt;
     std::vector<int32_t> freeSlots;
     std::unordered_map<uint64_t, lvr::PoolEntry> resident;
-    std::list<uint64_t> lruList; // front = next to evict
+    lvr::DenseLru lru; // over page indices
     int policy = 0;
 
     // pinned staging for uploads, device staging for layout conversion
@@ -214,9 +285,10 @@ struct lvr_ctx
     size_t superCapacity = 0, superOffsetCapacity = 0, superListCapacity = 0;
     lvr::FrameUniforms* d_req = nullptr; // the frame as a one-request table
     size_t reqCapacity = 0;
-    lvr::FrameUniforms hostReq{};
     std::vector<lvr::BrickEntry> hostBricks;
     std::vector<int4> hostRanges;
+    lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
+    lvr::PinnedRing selStaging;   // selection results on their way to the host
     unsigned long long* d_counters = nullptr; // [0] samples, [1] work queue
     unsigned long long* h_counters = nullptr; // pinned
 
@@ -274,6 +346,8 @@ int32_t slotOf(const lvr_ctx* ctx, uint64_t id);
 // Dense index of a node in the page table (false outside the octree).
 bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx);
 void touch(lvr_ctx* ctx, uint64_t id);
+// slotOf + touch with one lookup
+int32_t slotAndTouch(lvr_ctx* ctx, uint64_t id);
 int flushPageTable(lvr_ctx* ctx);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
 // Undo acquireSlotFor for a brick whose contents were never written.

@@ -5,6 +5,7 @@
 #include <cstring>
 
 #include "lvr_internal.h"
+#include "radix.h"
 
 namespace lvr
 {
@@ -194,6 +195,8 @@ int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
     if (!info || !f || (n > 0 && (!ids || !out)) || n < 0)
         return LVR_EINVAL;
     std::vector<std::pair<float, uint64_t>> v(static_cast<size_t>(n));
+    std::vector<std::pair<uint32_t, uint64_t>> bits(static_cast<size_t>(n)), tmp;
+    bool plain = true; // every distance a non-negative number (not NaN)
     for (int i = 0; i < n; ++i)
     {
         float mn[3], mx[3];
@@ -206,7 +209,25 @@ int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
             r[a] = lvr::at(f->mv, a, 0) * c[0] + lvr::at(f->mv, a, 1) * c[1] +
                    lvr::at(f->mv, a, 2) * c[2] + lvr::at(f->mv, a, 3);
         const float x = r[0] / r[3], y = r[1] / r[3], z = r[2] / r[3];
-        v[size_t(i)] = {std::sqrt(x * x + y * y + z * z), ids[i]};
+        const float d = std::sqrt(x * x + y * y + z * z);
+        v[size_t(i)] = {d, ids[i]};
+        uint32_t u;
+        std::memcpy(&u, &d, sizeof u);
+        plain = plain && !(d < 0.0f) && u <= 0x7f800000u;
+        bits[size_t(i)] = {u, ids[i]};
+    }
+    // Non-negative floats order as their bit patterns. With all distances
+    // distinct the sorted order is unique, so the radix sort gives exactly
+    // what std::sort gives; equal distances keep std::sort's tie order.
+    if (plain)
+    {
+        lvr::radixSortByKey(bits, tmp);
+        if (!lvr::hasEqualKeys(bits))
+        {
+            for (int i = 0; i < n; ++i)
+                out[i] = bits[size_t(i)].second;
+            return LVR_OK;
+        }
     }
     std::sort(v.begin(), v.end(),
               [](const std::pair<float, uint64_t>& a,

@@ -47,18 +47,74 @@ bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx)
 
 int32_t slotOf(const lvr_ctx* ctx, uint64_t id)
 {
-    auto it = ctx->resident.find(id);
-    return it == ctx->resident.end() ? -1 : it->second.slot;
+    uint64_t idx;
+    return pageIndex(ctx, id, &idx) ? ctx->pt.h_slots[idx] : -1;
+}
+
+hipError_t PinnedRing::acquire(size_t bytes, uint8_t** out, int* slot)
+{
+    const int r = next;
+    next ^= 1;
+    hipError_t e = hipSuccess;
+    if (!done[r] && (e = hipEventCreateWithFlags(&done[r], hipEventDisableTiming)) != hipSuccess)
+        return e;
+    if (armed[r] && (e = hipEventSynchronize(done[r])) != hipSuccess)
+        return e;
+    armed[r] = false;
+    if (bytes > cap[r])
+    {
+        if (buf[r])
+            (void)hipHostFree(buf[r]);
+        buf[r] = nullptr;
+        cap[r] = 0;
+        const size_t n = bytes + bytes / 4 + 4096;
+        if ((e = hipHostMalloc((void**)&buf[r], n, hipHostMallocDefault)) != hipSuccess)
+            return e;
+        cap[r] = n;
+    }
+    *out = buf[r];
+    *slot = r;
+    return hipSuccess;
+}
+
+hipError_t PinnedRing::release(int slot, hipStream_t st)
+{
+    const hipError_t e = hipEventRecord(done[slot], st);
+    armed[slot] = e == hipSuccess;
+    return e;
+}
+
+void PinnedRing::destroy()
+{
+    for (int r = 0; r < 2; ++r)
+    {
+        if (armed[r])
+            (void)hipEventSynchronize(done[r]);
+        if (buf[r])
+            (void)hipHostFree(buf[r]);
+        if (done[r])
+            (void)hipEventDestroy(done[r]);
+        buf[r] = nullptr;
+        done[r] = nullptr;
+        cap[r] = 0;
+        armed[r] = false;
+    }
+}
+
+int32_t slotAndTouch(lvr_ctx* ctx, uint64_t id)
+{
+    uint64_t idx;
+    if (!pageIndex(ctx, id, &idx))
+        return -1;
+    const int32_t slot = ctx->pt.h_slots[idx];
+    if (slot >= 0 && ctx->policy == 0)
+        ctx->lru.moveToBack(int32_t(idx));
+    return slot;
 }
 
 void touch(lvr_ctx* ctx, uint64_t id)
 {
-    if (ctx->policy != 0)
-        return;
-    auto it = ctx->resident.find(id);
-    if (it == ctx->resident.end())
-        return;
-    ctx->lruList.splice(ctx->lruList.end(), ctx->lruList, it->second.lru);
+    (void)slotAndTouch(ctx, id);
 }
 
 static int setPage(lvr_ctx* ctx, uint64_t id, int32_t slot)
@@ -170,14 +226,14 @@ void* genScratch(lvr_ctx* ctx, size_t bytes)
 // the queue in insertion order and skips referenced objects).
 static int evictOne(lvr_ctx* ctx)
 {
-    for (auto it = ctx->lruList.begin(); it != ctx->lruList.end(); ++it)
+    for (int32_t i = ctx->lru.head; i >= 0; i = ctx->lru.next[size_t(i)])
     {
-        auto r = ctx->resident.find(*it);
+        const uint64_t id = ctx->lru.ids[size_t(i)];
+        auto r = ctx->resident.find(id);
         if (r->second.pinned)
             continue;
-        const uint64_t id = *it;
         const int32_t slot = r->second.slot;
-        ctx->lruList.erase(it);
+        ctx->lru.erase(i);
         ctx->resident.erase(r);
         ctx->freeSlots.push_back(slot);
         return setPage(ctx, id, -1);
@@ -207,11 +263,10 @@ static int acquireSlot(lvr_ctx* ctx, uint64_t id, int32_t* slot)
     }
     *slot = ctx->freeSlots.back();
     ctx->freeSlots.pop_back();
-    ctx->lruList.push_back(id);
+    ctx->lru.pushBack(int32_t(idx), id);
     PoolEntry e;
     e.slot = *slot;
     e.pinned = 0;
-    e.lru = std::prev(ctx->lruList.end());
     ctx->resident.emplace(id, e);
     return setPage(ctx, id, *slot);
 }
@@ -229,7 +284,9 @@ int releaseSlot(lvr_ctx* ctx, uint64_t id)
     if (it->second.pinned)
         --ctx->pinnedBricks;
     ctx->freeSlots.push_back(it->second.slot);
-    ctx->lruList.erase(it->second.lru);
+    uint64_t idx;
+    if (pageIndex(ctx, id, &idx))
+        ctx->lru.erase(int32_t(idx));
     ctx->resident.erase(it);
     ++ctx->poolGeneration;
     return setPage(ctx, id, -1);
@@ -387,7 +444,10 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
     }
     ctx->pt.levelOffset[vi.depth] = total;
     ctx->pt.size = total;
+    if (total > uint64_t(INT32_MAX))
+        return bail(LVR_EINVAL);
     ctx->pt.h_slots.assign(total, -1);
+    ctx->lru.init(size_t(total));
     if (hipMalloc((void**)&ctx->pt.d_slots, total * sizeof(int32_t)) !=
         hipSuccess)
         return bail(LVR_ENOMEM);
@@ -456,6 +516,8 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipHostFree(ctx->h_staging);
     if (ctx->h_counters)
         (void)hipHostFree(ctx->h_counters);
+    ctx->frameStaging.destroy();
+    ctx->selStaging.destroy();
     if (ctx->renderDone)
         (void)hipEventDestroy(ctx->renderDone);
     for (auto& e : ctx->ev)
@@ -618,7 +680,9 @@ int lvr_brick_evict(lvr_ctx* ctx, uint64_t node_id)
         return lvr::fail(ctx, LVR_EINVAL, "brick is pinned");
     ++ctx->poolGeneration;
     ctx->freeSlots.push_back(it->second.slot);
-    ctx->lruList.erase(it->second.lru);
+    uint64_t idx;
+    if (lvr::pageIndex(ctx, node_id, &idx))
+        ctx->lru.erase(int32_t(idx));
     ctx->resident.erase(it);
     rc = lvr::setPage(ctx, node_id, -1);
     if (rc)

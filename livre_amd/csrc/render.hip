@@ -573,20 +573,27 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
             e.rcp[a] = 1.0f / e.ext[a];
         }
         e.level = key.level;
-        e.slot = slotOf(ctx, ids[b]);
+        e.slot = slotAndTouch(ctx, ids[b]);
         if (e.slot >= 0)
         {
-            touch(ctx, ids[b]);
+            // clip coordinates of the 8 corners: the min corner plus the
+            // projected edge vectors
+            double base[4], edge[3][4];
+            for (int i = 0; i < 4; ++i)
+            {
+                base[i] = mvp[i] * e.bmin[0] + mvp[4 + i] * e.bmin[1] + mvp[8 + i] * e.bmin[2] +
+                          mvp[12 + i];
+                for (int a = 0; a < 3; ++a)
+                    edge[a][i] = mvp[4 * a + i] * (double(e.bmax[a]) - double(e.bmin[a]));
+            }
             double mnx = 1e300, mny = 1e300, mxx = -1e300, mxy = -1e300;
             bool full = false;
             for (int c = 0; c < 8 && !full; ++c)
             {
-                const double v[3] = {(c & 1) ? e.bmax[0] : e.bmin[0],
-                                     (c & 2) ? e.bmax[1] : e.bmin[1],
-                                     (c & 4) ? e.bmax[2] : e.bmin[2]};
                 double clip[4];
                 for (int i = 0; i < 4; ++i)
-                    clip[i] = mvp[i] * v[0] + mvp[4 + i] * v[1] + mvp[8 + i] * v[2] + mvp[12 + i];
+                    clip[i] = base[i] + ((c & 1) ? edge[0][i] : 0.0) +
+                              ((c & 2) ? edge[1][i] : 0.0) + ((c & 4) ? edge[2][i] : 0.0);
                 if (clip[3] <= 1e-6)
                 {
                     full = true;
@@ -684,6 +691,7 @@ struct lvr_batch
     unsigned long long* d_counters = nullptr;
     size_t counterCap = 0;
     std::vector<unsigned long long> h_counters;
+    lvr::PinnedRing staging; // host tables on their way to d_*
     bool ran = false;
 
     // graph of the last launch sequence and the shape it was captured for
@@ -812,7 +820,6 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     int rc = ensureDevice(ctx);
     if (rc)
         return rc;
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream)); // host staging is reused
     FrameUniforms U;
     std::vector<BrickEntry>& bricks = ctx->hostBricks;
     std::vector<int4>& ranges = ctx->hostRanges;
@@ -872,17 +879,28 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     if (rc)
         return rc;
     ctx->reqCapacity = reqCap;
-    ctx->hostReq = U;
 
-    if (n > 0)
+    // uniforms, brick entries and ranges through one pinned staging buffer
     {
-        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_bricks, bricks.data(), sizeof(BrickEntry) * n,
+        const size_t bBytes = sizeof(BrickEntry) * size_t(n), rBytes = sizeof(int4) * size_t(n);
+        uint8_t* h = nullptr;
+        int slot = 0;
+        LVR_HIP(ctx, ctx->frameStaging.acquire(sizeof(FrameUniforms) + bBytes + rBytes, &h,
+                                               &slot));
+        std::memcpy(h, &U, sizeof(FrameUniforms));
+        std::memcpy(h + sizeof(FrameUniforms), bricks.data(), bBytes);
+        std::memcpy(h + sizeof(FrameUniforms) + bBytes, ranges.data(), rBytes);
+        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_req, h, sizeof(FrameUniforms),
                                     hipMemcpyHostToDevice, ctx->stream));
-        LVR_HIP(ctx, hipMemcpyAsync(ctx->d_ranges, ranges.data(), sizeof(int4) * n,
-                                    hipMemcpyHostToDevice, ctx->stream));
+        if (n > 0)
+        {
+            LVR_HIP(ctx, hipMemcpyAsync(ctx->d_bricks, h + sizeof(FrameUniforms), bBytes,
+                                        hipMemcpyHostToDevice, ctx->stream));
+            LVR_HIP(ctx, hipMemcpyAsync(ctx->d_ranges, h + sizeof(FrameUniforms) + bBytes,
+                                        rBytes, hipMemcpyHostToDevice, ctx->stream));
+        }
+        LVR_HIP(ctx, ctx->frameStaging.release(slot, ctx->stream));
     }
-    LVR_HIP(ctx, hipMemcpyAsync(ctx->d_req, &ctx->hostReq, sizeof(FrameUniforms),
-                                hipMemcpyHostToDevice, ctx->stream));
     ctx->lastBricks = onScreen;
     ctx->lastEntries = listTotal;
     if (ctx->timing)
@@ -1085,6 +1103,7 @@ void lvr_batch_destroy(lvr_batch* b)
     (void)hipFree(b->d_counters);
     if (b->h_pinned)
         (void)hipHostFree(b->h_pinned);
+    b->staging.destroy();
     for (auto& e : b->ev)
         if (e)
             (void)hipEventDestroy(e);
@@ -1276,19 +1295,33 @@ int lvr_batch_run(lvr_batch* b)
                                 int64_t(reinterpret_cast<intptr_t>(b->d_slists)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_counters))};
 
-    // inputs: pageable host vectors are copied before the launch sequence
-    LVR_BHIP(b, hipMemcpyAsync(b->d_reqs, b->reqs.data(), sizeof(FrameUniforms) * nReq,
-                               hipMemcpyHostToDevice, b->stream));
-    if (!b->bricks.empty())
+    // inputs: the host tables go through one pinned staging buffer and are
+    // copied before the launch sequence
     {
-        LVR_BHIP(b, hipMemcpyAsync(b->d_bricks, b->bricks.data(),
-                                   sizeof(BrickEntry) * b->bricks.size(),
-                                   hipMemcpyHostToDevice, b->stream));
-        LVR_BHIP(b, hipMemcpyAsync(b->d_ranges, b->ranges.data(), sizeof(int4) * b->ranges.size(),
-                                   hipMemcpyHostToDevice, b->stream));
+        const size_t qB = sizeof(FrameUniforms) * size_t(nReq);
+        const size_t bB = sizeof(BrickEntry) * b->bricks.size();
+        const size_t rB = sizeof(int4) * b->ranges.size();
+        const size_t tB = sizeof(float4) * b->tf.size();
+        uint8_t* h = nullptr;
+        int slot = 0;
+        LVR_BHIP(b, b->staging.acquire(qB + bB + rB + tB, &h, &slot));
+        std::memcpy(h, b->reqs.data(), qB);
+        std::memcpy(h + qB, b->bricks.data(), bB);
+        std::memcpy(h + qB + bB, b->ranges.data(), rB);
+        std::memcpy(h + qB + bB + rB, b->tf.data(), tB);
+        LVR_BHIP(b, hipMemcpyAsync(b->d_reqs, h, qB, hipMemcpyHostToDevice, b->stream));
+        if (bB)
+        {
+            LVR_BHIP(b, hipMemcpyAsync(b->d_bricks, h + qB, bB, hipMemcpyHostToDevice,
+                                       b->stream));
+            LVR_BHIP(b, hipMemcpyAsync(b->d_ranges, h + qB + bB, rB, hipMemcpyHostToDevice,
+                                       b->stream));
+        }
+        if (tB)
+            LVR_BHIP(b, hipMemcpyAsync(b->d_tf, h + qB + bB + rB, tB, hipMemcpyHostToDevice,
+                                       b->stream));
+        LVR_BHIP(b, b->staging.release(slot, b->stream));
     }
-    LVR_BHIP(b, hipMemcpyAsync(b->d_tf, b->tf.data(), sizeof(float4) * b->tf.size(),
-                               hipMemcpyHostToDevice, b->stream));
 
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "lvr_internal.h"
+#include "radix.h"
 
 namespace lvr
 {
@@ -249,8 +250,14 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     hipLaunchKernelGGL(selectKernel, dim3(1), dim3(1024), 0, ctx->stream, U,
                        ctx->d_selA, ctx->d_selB, ctx->d_selOut, ctx->d_selCount);
     LVR_HIP(ctx, hipGetLastError());
-    int32_t counts[2];
-    LVR_HIP(ctx, hipMemcpyAsync(counts, ctx->d_selCount, sizeof counts,
+    // read back through pinned memory (a DMA straight into it), on the
+    // context's own stream: a null-stream copy would also wait for every
+    // blocking stream of the process
+    uint8_t* h = nullptr;
+    int slot = 0;
+    LVR_HIP(ctx, ctx->selStaging.acquire(sizeof(uint64_t) * (size_t(U.capOut) + 1), &h, &slot));
+    int32_t* counts = reinterpret_cast<int32_t*>(h);
+    LVR_HIP(ctx, hipMemcpyAsync(counts, ctx->d_selCount, 2 * sizeof(int32_t),
                                 hipMemcpyDeviceToHost, ctx->stream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (counts[1])
@@ -259,19 +266,21 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     *n_out = n;
     if (n > cap)
         return fail(ctx, LVR_ERANGE, "visible set larger than the output buffer");
-    std::vector<uint64_t> ids(static_cast<size_t>(n));
+    const uint64_t* ids = reinterpret_cast<const uint64_t*>(h + sizeof(uint64_t));
     if (n > 0)
     {
-        // on the context's own stream: a null-stream copy would also wait
-        // for every blocking stream of the process
-        LVR_HIP(ctx, hipMemcpyAsync(ids.data(), ctx->d_selOut, sizeof(uint64_t) * n,
+        LVR_HIP(ctx, hipMemcpyAsync(h + sizeof(uint64_t), ctx->d_selOut, sizeof(uint64_t) * n,
                                     hipMemcpyDeviceToHost, ctx->stream));
         LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     }
-    std::vector<std::pair<uint64_t, uint64_t>> keyed(static_cast<size_t>(n));
+    std::vector<std::pair<uint64_t, uint64_t>> keyed(static_cast<size_t>(n)), tmp;
     for (int i = 0; i < n; ++i)
-        keyed[size_t(i)] = {dfsKey(ctx->vi, ids[size_t(i)]), ids[size_t(i)]};
-    std::sort(keyed.begin(), keyed.end());
+        keyed[size_t(i)] = {dfsKey(ctx->vi, ids[i]), ids[i]};
+    // (key, id) order: the radix pass orders by key; equal keys (an ancestor
+    // beside its first-child descendants) are ordered by id as well
+    radixSortByKey(keyed, tmp);
+    if (hasEqualKeys(keyed))
+        std::sort(keyed.begin(), keyed.end());
     for (int i = 0; i < n; ++i)
         out[i] = keyed[size_t(i)].second;
     return LVR_OK;

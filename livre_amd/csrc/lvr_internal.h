@@ -185,6 +185,12 @@ struct FrameUniforms
     float clip[24];
     uint64_t slotStride;
     BrickLayout layout; // tiled voxel order inside a pool slot
+    // byte offset of voxel (x, y, z) in a slot, as
+    //   bpv * (x + 4 y + 16 z) + tileK[0] * (x >> 2) + tileK[1] * (y >> 2)
+    //   + tileK[2] * (z >> 2)
+    // with tileK = bpv * {60, 64 T0 - 16, 64 T0 T1 - 64} (T = tiles per axis):
+    // the 4x4x4 micro-tile index times 64 plus the voxel inside the tile
+    uint32_t tileK[3];
     // position of this frame in a batch
     int32_t brickBase; // first brick record / tile range
     int32_t tileBase;  // first tile of the per-tile arrays

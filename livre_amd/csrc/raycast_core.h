@@ -59,6 +59,72 @@ __device__ inline uint32_t tiledIndexDev(const BrickLayout& L, uint32_t x, uint3
     return (tile << 6) | ((z & 3u) << 4) | ((y & 3u) << 2) | (x & 3u);
 }
 
+// Byte offset of voxel (x, y, z) in a pool slot: tiledIndexDev() times the
+// voxel size, in 9 integer operations. x + 4 y + 16 z places the voxel inside
+// its micro-tile but advances only 4, 16 and 64 per tile step in x, y and z;
+// the per-axis constants U.tileK add the rest of 64, 64 T0 and 64 T0 T1.
+template <typename T>
+__device__ inline uint32_t tiledOffsetDev(const FrameUniforms& U, uint32_t x, uint32_t y,
+                                          uint32_t z)
+{
+    constexpr uint32_t lb = sizeof(T) == 1 ? 0u : (sizeof(T) == 2 ? 1u : 2u);
+    uint32_t a = __umul24(x >> 2, U.tileK[0]);
+    a = __umul24(y >> 2, U.tileK[1]) + a;
+    a = __umul24(z >> 2, U.tileK[2]) + a;
+    a = (x << lb) + a;
+    a = (y << (lb + 2)) + a;
+    a = (z << (lb + 4)) + a;
+    return a;
+}
+
+// Voxel loads through a buffer resource over one pool slot: a 32-bit byte
+// offset per lane instead of a 64-bit address, and the hardware returns 0
+// for any offset past the slot.
+__device__ inline __amdgpu_buffer_rsrc_t slotResource(const void* slot, uint32_t bytes)
+{
+    // word 3 of the buffer descriptor for raw (untyped) loads on gfx9 parts
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(slot), short(0), int(bytes),
+                                             0x00020000);
+}
+
+template <typename T>
+__device__ inline T loadSlot(__amdgpu_buffer_rsrc_t r, uint32_t off);
+template <>
+__device__ inline uint8_t loadSlot<uint8_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0);
+}
+template <>
+__device__ inline int8_t loadSlot<int8_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return int8_t(__builtin_amdgcn_raw_buffer_load_b8(r, off, 0, 0));
+}
+template <>
+__device__ inline uint16_t loadSlot<uint16_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0);
+}
+template <>
+__device__ inline int16_t loadSlot<int16_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return int16_t(__builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+}
+template <>
+__device__ inline uint32_t loadSlot<uint32_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
+template <>
+__device__ inline int32_t loadSlot<int32_t>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return int32_t(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ inline float loadSlot<float>(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 // texture() on the brick: GL_NEAREST picks floor(u * N) clamped; GL_LINEAR
 // blends the 8 texels around u * N - 0.5 with clamp-to-edge.
 template <typename T, bool LINEAR>
@@ -118,16 +184,17 @@ __device__ __forceinline__ void texturePosition(const FrameUniforms& U, const Br
     }
 }
 
-// Pool index of the GL_NEAREST texel of a sample position.
-__device__ __forceinline__ uint32_t nearestIndex(const FrameUniforms& U, const BrickEntry& b,
-                                                 const float texExt[3], const float pos[3])
+// Byte offset in the slot of the GL_NEAREST texel of a sample position.
+template <typename T>
+__device__ __forceinline__ uint32_t nearestOffset(const FrameUniforms& U, const BrickEntry& b,
+                                                  const float texExt[3], const float pos[3])
 {
     float tp[3];
     texturePosition(U, b, texExt, pos, tp);
     const uint32_t ix = nearestTexel(tp[0] * U.texDimF[0], U.texLastF[0]);
     const uint32_t iy = nearestTexel(tp[1] * U.texDimF[1], U.texLastF[1]);
     const uint32_t iz = nearestTexel(tp[2] * U.texDimF[2], U.texLastF[2]);
-    return tiledIndexDev(U.layout, ix, iy, iz);
+    return tiledOffsetDev<T>(U, ix, iy, iz);
 }
 
 // Classification of one sample value (fragRayCast.glsl:184-204 before the
@@ -224,7 +291,7 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
 // exit) are discarded; their texel indices are clamped into the brick like
 // every other.
 template <typename T, int D, bool CLS>
-__device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const FrameUniforms& U,
+__device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
                                              const BrickEntry& b, const float texExt[3],
                                              float pos[3], const float step[3], float travel,
                                              float stepSize, Composite& C, const float4* tf,
@@ -235,14 +302,14 @@ __device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const Fr
         return;
     int guard = 1 << 22;
     T raw[S];
-    raw[0] = vox[nearestIndex(U, b, texExt, pos)];
+    raw[0] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
 #pragma unroll
     for (int i = 1; i < S; ++i)
     {
         pos[0] = pos[0] + step[0];
         pos[1] = pos[1] + step[1];
         pos[2] = pos[2] + step[2];
-        raw[i] = vox[nearestIndex(U, b, texExt, pos)];
+        raw[i] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
     }
     for (bool more = true; more;)
     {
@@ -267,11 +334,11 @@ __device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const Fr
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                const uint32_t next = nearestIndex(U, b, texExt, pos);
+                const uint32_t next = nearestOffset<T>(U, b, texExt, pos);
                 // the oldest voxel has been used before its register takes
                 // the next load; the load stays above this sample's blend
                 __builtin_amdgcn_sched_barrier(0);
-                raw[s] = vox[next];
+                raw[s] = loadSlot<T>(vox, next);
                 __builtin_amdgcn_sched_barrier(0);
             }
             ++nsamp;
@@ -286,9 +353,83 @@ __device__ __forceinline__ void marchNearest(const T* __restrict__ vox, const Fr
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                raw[0] = vox[nearestIndex(U, b, texExt, pos)];
+                raw[0] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
             }
         }
+    }
+}
+
+// The march with a classified table (integer voxels sampled NEAREST: the
+// classification is a function of the raw value, tabulated per value). The
+// table read is a dependent load, so it is pipelined too. Step k blends
+// sample k with the table entry read during step k - 1, then reads the
+// entry of sample k + 1 (whose voxel was fetched S - 1 steps earlier) into
+// the same register, then fetches the voxel of sample k + S (S = D + 1).
+// Loads complete in issue order, so the blend waits only for the entry and
+// the entry read only for a voxel two steps old. Voxels sit in a ring of S
+// registers, so no register copy waits on a load.
+//
+// The S steps of a pass have no branches, so the compiler keeps every load
+// where it is issued: a ray that ends its segment (or turns opaque) inside a
+// pass blends with weight 0 for the pass's remaining steps (its colour stays
+// exactly as it was; table entries are finite) and leaves the loop at the end
+// of the pass. Its extra fetches are clamped into the brick like every other.
+// Per sample this leaves the position step, the texel address and the blend;
+// the table holds exactly what the per-sample classification computes.
+template <typename T, int D>
+__device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
+                                           const BrickEntry& b, const float texExt[3],
+                                           float pos[3], const float step[3], float travel,
+                                           float stepSize, Composite& C,
+                                           const float4* __restrict__ tab, uint32_t& nsamp)
+{
+    constexpr int S = D + 1;
+    if (!(travel > 0.0f))
+        return;
+    int guard = 1 << 22;
+    const uint32_t first = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+    // the voxel of sample m (m >= 1) of the segment is raw[(m - 1) % S]; one
+    // full register each (two 16-bit voxels packed in one register would
+    // have to wait for both loads to merge them)
+    uint32_t raw[S];
+#pragma unroll
+    for (int i = 0; i < S - 1; ++i)
+    {
+        pos[0] = pos[0] + step[0];
+        pos[1] = pos[1] + step[1];
+        pos[2] = pos[2] + step[2];
+        raw[i] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+    }
+    float4 cls = tab[first];
+    bool alive = true;
+    for (;;)
+    {
+#pragma unroll
+        for (int u = 0; u < S; ++u)
+        {
+            // blend (fragRayCast.glsl:103-111), weight 0 once the ray is done
+            const float rem = alive ? 1.0f - C.acc3 : 0.0f;
+            C.acc0 += cls.x * rem;
+            C.acc1 += cls.y * rem;
+            C.acc2 += cls.z * rem;
+            C.acc3 += cls.w * rem;
+            nsamp += alive ? 1u : 0u;
+            travel -= stepSize;
+            --guard;
+            // bitwise: a short-circuit would put branches in the pass
+            alive = alive & !(C.acc3 > U.exitA) & (travel > 0.0f) & (guard > 0);
+            cls = tab[raw[u]];
+            __builtin_amdgcn_sched_barrier(0);
+            pos[0] = pos[0] + step[0];
+            pos[1] = pos[1] + step[1];
+            pos[2] = pos[2] + step[2];
+            const uint32_t next = nearestOffset<T>(U, b, texExt, pos);
+            __builtin_amdgcn_sched_barrier(0);
+            raw[(u + S - 1) % S] = loadSlot<T>(vox, next);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (!alive)
+            break;
     }
 }
 
@@ -416,9 +557,13 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         float pos[3] = {rs[0], rs[1], rs[2]};
         const T* vox = reinterpret_cast<const T*>(U.pool + size_t(b.slot) * U.slotStride);
 
-        if (!LINEAR)
-            marchNearest<T, D, CLS>(vox, U, b, texExt, pos, step, segLen, stepSize, C, tf,
-                                    nsamp);
+        if (!LINEAR && CLS)
+            marchTable<T, D>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt, pos,
+                             step, segLen, stepSize, C, tf, nsamp);
+        else if (!LINEAR)
+            marchNearest<T, D, false>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt,
+                                      pos, step, segLen, stepSize, C, tf,
+                                      nsamp);
         else
         {
             // A segment lies inside one brick, so it holds at most

@@ -543,6 +543,13 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     std::memcpy(U->clip, params->clip_planes, sizeof U->clip);
     U->slotStride = ctx->slotStride;
     U->layout = ctx->layout;
+    {
+        const uint32_t bpv = uint32_t(ctx->voxelBytes);
+        const uint32_t T0 = ctx->layout.tiles[0], T1 = ctx->layout.tiles[1];
+        U->tileK[0] = bpv * 60u;
+        U->tileK[1] = bpv * (64u * T0 - 16u);
+        U->tileK[2] = bpv * (64u * T0 * T1 - 64u);
+    }
     U->dataType = vi.data_type;
     U->pool = ctx->d_pool;
     U->nItems = ((U->tilesX + kSuper - 1) / kSuper) * ((U->tilesY + kSuper - 1) / kSuper) *

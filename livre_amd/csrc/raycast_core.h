@@ -217,10 +217,13 @@ struct Classify
         const int ib = int(__builtin_amdgcn_fmed3f(fl + 1.0f, 0.0f, nTfMax));
         const float4 ca = tf[ia];
         const float4 cb = tf[ib];
-        const float sr = ca.x * (1.0f - w) + cb.x * w;
-        const float sg = ca.y * (1.0f - w) + cb.y * w;
-        const float sb = ca.z * (1.0f - w) + cb.z * w;
-        const float sa = ca.w * (1.0f - w) + cb.w * w;
+        // the linear filter as a + w (b - a), one FMA per channel (a GLSL
+        // compiler may contract the same way; the texel and TF indices above
+        // stay exact, only colour roundings move by an ulp)
+        const float sr = __builtin_fmaf(w, cb.x - ca.x, ca.x);
+        const float sg = __builtin_fmaf(w, cb.y - ca.y, ca.y);
+        const float sb = __builtin_fmaf(w, cb.z - ca.z, ca.z);
+        const float sa = __builtin_fmaf(w, cb.w - ca.w, ca.w);
         // pow(x, c) for x in [1/256, 1] as exp2(c * log2 x) on the hardware
         // transcendental units (1 ulp each)
         const float base = 1.0f - fminf(sa, maxA);
@@ -252,11 +255,12 @@ struct Composite
 
     __device__ __forceinline__ bool blend(const FrameUniforms& U, float4 c)
     {
+        // dst += src * (1 - dst.a) contracted to one FMA per channel
         const float rem = 1.0f - acc3;
-        acc0 += c.x * rem;
-        acc1 += c.y * rem;
-        acc2 += c.z * rem;
-        acc3 += c.w * rem;
+        acc0 = __builtin_fmaf(c.x, rem, acc0);
+        acc1 = __builtin_fmaf(c.y, rem, acc1);
+        acc2 = __builtin_fmaf(c.z, rem, acc2);
+        acc3 = __builtin_fmaf(c.w, rem, acc3);
         return acc3 > U.exitA;
     }
 
@@ -409,10 +413,10 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
         {
             // blend (fragRayCast.glsl:103-111), weight 0 once the ray is done
             const float rem = alive ? 1.0f - C.acc3 : 0.0f;
-            C.acc0 += cls.x * rem;
-            C.acc1 += cls.y * rem;
-            C.acc2 += cls.z * rem;
-            C.acc3 += cls.w * rem;
+            C.acc0 = __builtin_fmaf(cls.x, rem, C.acc0);
+            C.acc1 = __builtin_fmaf(cls.y, rem, C.acc1);
+            C.acc2 = __builtin_fmaf(cls.z, rem, C.acc2);
+            C.acc3 = __builtin_fmaf(cls.w, rem, C.acc3);
             nsamp += alive ? 1u : 0u;
             travel -= stepSize;
             --guard;

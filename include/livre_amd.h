@@ -202,7 +202,9 @@ int lvr_tf_set_rgba8(lvr_ctx* ctx, const uint8_t* lut, int n, float range0,
 /* SelectVisibles + DFSTraversal (livre/data/SelectVisibles.cpp:45-131,
  * livre/data/DFSTraversal.cpp:29-76) on the device, level by level, with
  * the reference's FP32 expressions. Output in the reference's DFS order.
- * n_out receives the count; LVR_ERANGE if cap is too small. */
+ * n_out receives the count; LVR_ERANGE if cap is too small. Runs on a
+ * stream of its own: it does not wait for frames still rendering, so the
+ * next frame's selection can overlap the current frame's ray casting. */
 int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
                         uint32_t window_height, float sse, uint32_t min_lod,
                         uint32_t max_lod, uint64_t* out, int cap, int* n_out);

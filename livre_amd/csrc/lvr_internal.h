@@ -236,6 +236,9 @@ struct lvr_ctx
     hipStream_t uploadStream = nullptr;
     // histogram kernels read the pool beside the ray casting of the frame
     hipStream_t histStream = nullptr;
+    // selection reads only its own buffers, so it runs on a stream of its own
+    // and does not wait for frames still rendering on `stream`
+    hipStream_t selStream = nullptr;
     hipEvent_t histDone = nullptr; // end of the last histogram on histStream
     hipEvent_t renderDone = nullptr; // recorded on `stream` before slot reuse
     lvr_volume_info vi{};

@@ -419,6 +419,7 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
         hipStreamCreateWithFlags(&ctx->uploadStream, hipStreamNonBlocking) !=
             hipSuccess ||
         hipStreamCreateWithFlags(&ctx->histStream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->selStream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->histDone, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->renderDone, hipEventDisableTiming) !=
             hipSuccess)
@@ -483,6 +484,8 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamSynchronize(ctx->uploadStream);
     if (ctx->histStream)
         (void)hipStreamSynchronize(ctx->histStream);
+    if (ctx->selStream)
+        (void)hipStreamSynchronize(ctx->selStream);
     (void)hipFree(ctx->d_pool);
     (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->pt.d_slots);
@@ -529,6 +532,8 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamDestroy(ctx->uploadStream);
     if (ctx->histStream)
         (void)hipStreamDestroy(ctx->histStream);
+    if (ctx->selStream)
+        (void)hipStreamDestroy(ctx->selStream);
     if (ctx->histDone)
         (void)hipEventDestroy(ctx->histDone);
     delete ctx;
@@ -561,6 +566,7 @@ int lvr_ctx_sync(lvr_ctx* ctx)
     LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
     return LVR_OK;
 }
 

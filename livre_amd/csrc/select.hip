@@ -216,7 +216,7 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     const size_t capNeed = maxLevel;
     if (capNeed > ctx->selCapacity)
     {
-        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
         (void)hipFree(ctx->d_selA);
         (void)hipFree(ctx->d_selB);
         (void)hipFree(ctx->d_selOut);
@@ -247,19 +247,19 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     U.maxLod = max_lod;
     U.capFrontier = int(std::min<size_t>(capNeed, 1u << 30));
     U.capOut = U.capFrontier;
-    hipLaunchKernelGGL(selectKernel, dim3(1), dim3(1024), 0, ctx->stream, U,
+    hipLaunchKernelGGL(selectKernel, dim3(1), dim3(1024), 0, ctx->selStream, U,
                        ctx->d_selA, ctx->d_selB, ctx->d_selOut, ctx->d_selCount);
     LVR_HIP(ctx, hipGetLastError());
     // read back through pinned memory (a DMA straight into it), on the
-    // context's own stream: a null-stream copy would also wait for every
-    // blocking stream of the process
+    // selection stream: a null-stream copy would also wait for every blocking
+    // stream of the process, and the render stream for the frame in flight
     uint8_t* h = nullptr;
     int slot = 0;
     LVR_HIP(ctx, ctx->selStaging.acquire(sizeof(uint64_t) * (size_t(U.capOut) + 1), &h, &slot));
     int32_t* counts = reinterpret_cast<int32_t*>(h);
     LVR_HIP(ctx, hipMemcpyAsync(counts, ctx->d_selCount, 2 * sizeof(int32_t),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                                hipMemcpyDeviceToHost, ctx->selStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
     if (counts[1])
         return fail(ctx, LVR_ENOMEM, "selection frontier overflow");
     const int n = counts[0];
@@ -270,8 +270,8 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     if (n > 0)
     {
         LVR_HIP(ctx, hipMemcpyAsync(h + sizeof(uint64_t), ctx->d_selOut, sizeof(uint64_t) * n,
-                                    hipMemcpyDeviceToHost, ctx->stream));
-        LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                                    hipMemcpyDeviceToHost, ctx->selStream));
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
     }
     std::vector<std::pair<uint64_t, uint64_t>> keyed(static_cast<size_t>(n)), tmp;
     for (int i = 0; i < n; ++i)

@@ -540,33 +540,33 @@ def main():
         frame(livre_amd, vi, ctx, camera(livre_amd, scenes, t)[2], params)
     ctx.sync()
 
-    # Frames are pipelined as RenderPipeline::renderAsync does: the next
-    # frame's selection (its own stream) and draw order run while the current
-    # frame ray-casts. A frame's latency runs from the start of its selection
-    # to the end of its ray cast and histogram; its statistics are read before
-    # the next frame is submitted.
+    # Frames are pipelined as RenderPipeline::renderAsync does: while frame t
+    # ray-casts, frame t+1 is selected (on the selection stream), ordered,
+    # prepared and queued behind it; then frame t's statistics are read (the
+    # context keeps the last four frames'). A frame's latency runs from the
+    # start of its selection to the moment its completion is observed.
     lat, samples, bricks, entries, rc_ms, bin_ms, hist_ms = [], 0, [], [], [], [], []
     alg_bytes = []
 
-    def select_order(t):
+    def submit(t):
         f0 = time.perf_counter()
         fr = camera(livre_amd, scenes, t)[2]
         vis = ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1)
-        return f0, fr, livre_amd.order(vi, fr, vis)
+        ordered = livre_amd.order(vi, fr, vis)
+        ctx.render(fr, W, H, ordered, params)
+        ctx.histogram(ordered, BINS, read=False)
+        return f0
 
     t_start = time.perf_counter()
-    f0, fr, ordered = select_order(args.warmup)
-    ctx.render(fr, W, H, ordered, params)
-    ctx.histogram(ordered, BINS, read=False)
+    starts = [submit(args.warmup)]
     for i in range(args.steps):
         t = args.warmup + i
-        nxt = select_order(t + 1) if i + 1 < args.steps else None
-        st = ctx.frame_stats()  # waits for frame t
-        lat.append((time.perf_counter() - f0) * 1e3)
-        if nxt is not None:
-            f0, fr, ordered = nxt
-            ctx.render(fr, W, H, ordered, params)
-            ctx.histogram(ordered, BINS, read=False)
+        if i + 1 < args.steps:
+            starts.append(submit(t + 1))
+            st = ctx.frame_stats(back=1)  # waits for frame t only
+        else:
+            st = ctx.frame_stats()
+        lat.append((time.perf_counter() - starts[i]) * 1e3)
         samples += st.samples
         bricks.append(st.bricks)
         entries.append(st.tile_entries)
@@ -609,7 +609,8 @@ def main():
                                "samples/unit, FP32 4096-entry TF, sse 1, 65536-bin "
                                "histogram",
                    "frame": "select (device) + order + bin + raycast + histogram; the "
-                            "next frame's select + order overlap the current ray cast"},
+                            "next frame is selected, ordered and queued while the "
+                            "current one ray-casts"},
         "fps": args.steps / total,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),

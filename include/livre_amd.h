@@ -267,7 +267,14 @@ typedef struct
     float histogram_ms;    /* histogram kernels of the last lvr_histogram */
 } lvr_frame_stats;
 int lvr_set_timing(lvr_ctx* ctx, int enable);
+/* Statistics of the last rendered frame (waits for it). */
 int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out);
+/* Statistics of the frame rendered `back` frames before the last one
+ * (0 = the last, up to 3), for callers that keep frames in flight
+ * (RenderPipeline::renderAsync). Waits for that frame only; LVR_ERANGE once
+ * it is no longer kept. No reference counterpart (the GL path reads
+ * nothing back per frame). */
+int lvr_frame_stats_back(lvr_ctx* ctx, int back, lvr_frame_stats* out);
 
 /* Autotuner of the launch shape (no reference counterpart: the GL path has
  * no tunable launch). Renders the given frame with every ray tile in

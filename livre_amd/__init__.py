@@ -31,7 +31,8 @@ EXPORTS = (
     "lvr_field_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
-    "lvr_set_timing", "lvr_frame_stats_get", "lvr_slot_bytes", "lvr_autotune",
+    "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
+    "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
     "lvr_batch_set_graph", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
     "lvr_batch_wait", "lvr_batch_read", "lvr_batch_query", "lvr_batch_set_output",
@@ -263,6 +264,7 @@ def lib():
     L.lvr_brick_pin.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
     L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
+    L.lvr_frame_stats_back.argtypes = [c.c_void_p, c.c_int, P(FrameStats)]
     _lib = L
     return L
 
@@ -550,9 +552,11 @@ class Context:
     def set_timing(self, enable=True):
         self._check(lib().lvr_set_timing(self._h, 1 if enable else 0), "set_timing")
 
-    def frame_stats(self):
+    def frame_stats(self, back=0):
+        """Statistics of the last rendered frame, or of the one `back` frames
+        before it (up to 3) while later frames are in flight."""
         s = FrameStats()
-        self._check(lib().lvr_frame_stats_get(self._h, ctypes.byref(s)), "frame_stats")
+        self._check(lib().lvr_frame_stats_back(self._h, back, ctypes.byref(s)), "frame_stats")
         return s
 
     def histogram(self, ids, bins, read=True):

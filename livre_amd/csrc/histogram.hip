@@ -273,8 +273,15 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         ctx->histBins = bins;
     }
     ctx->histLast = bins;
-    if (ctx->timing)
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[3], ctx->histStream));
+    // timed in the statistics of the last rendered frame
+    lvr_ctx::FrameRecord* rec =
+        ctx->timing && ctx->frameCount > 0
+            ? &ctx->frames[(ctx->frameCount - 1) % lvr_ctx::kFrameSlots]
+            : nullptr;
+    if (rec && !rec->timed)
+        rec = nullptr;
+    if (rec)
+        LVR_HIP(ctx, hipEventRecord(rec->ev[3], ctx->histStream));
     if (!delta)
         hipLaunchKernelGGL(clearKernel, dim3((bins + 255) / 256), dim3(256), 0,
                            ctx->histStream, ctx->d_hist, bins);
@@ -333,10 +340,10 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         LVR_HIP(ctx, hipGetLastError());
         LVR_HIP(ctx, hipEventRecord(ctx->histRingDone[r], ctx->histStream));
     }
-    if (ctx->timing)
+    if (rec)
     {
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[4], ctx->histStream));
-        ctx->histTimed = true;
+        LVR_HIP(ctx, hipEventRecord(rec->ev[4], ctx->histStream));
+        rec->histTimed = true;
     }
     LVR_HIP(ctx, hipGetLastError());
     LVR_HIP(ctx, hipEventRecord(ctx->histDone, ctx->histStream));

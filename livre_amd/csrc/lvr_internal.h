@@ -327,11 +327,22 @@ struct lvr_ctx
     int32_t* d_selCount = nullptr; // [0] next frontier, [1] visible
     size_t selCapacity = 0;
 
-    // frame statistics and stage timing (events on `stream`)
+    // statistics of the last kFrameSlots frames, so a caller that keeps
+    // frames in flight reads each one's after the next has been submitted
+    static constexpr int kFrameSlots = 4;
+    struct FrameRecord
+    {
+        uint64_t bricks = 0, entries = 0;
+        bool timed = false, histTimed = false, armed = false;
+        // binning start, ray-cast start, ray-cast end (on `stream`),
+        // histogram start and end (on `histStream`)
+        hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        hipEvent_t done = nullptr; // the frame's composite steps are in h_frameSamples
+    };
+    FrameRecord frames[kFrameSlots];
+    uint64_t* h_frameSamples = nullptr; // pinned, one per slot
+    uint64_t frameCount = 0;            // frames rendered
     bool timing = false;
-    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    bool histTimed = false;
-    uint64_t lastBricks = 0, lastEntries = 0;
 
     // batches launched on their own streams that read this pool; slot reuse
     // waits for them unless the owner pins every brick a launch reads

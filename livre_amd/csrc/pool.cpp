@@ -459,6 +459,13 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
     ctx->freeSlots.reserve(ctx->nSlots);
     for (int64_t s = int64_t(ctx->nSlots) - 1; s >= 0; --s)
         ctx->freeSlots.push_back(int32_t(s));
+    for (auto& f : ctx->frames)
+        if (hipEventCreateWithFlags(&f.done, hipEventDisableTiming) != hipSuccess)
+            return bail(LVR_EHIP);
+    if (hipHostMalloc((void**)&ctx->h_frameSamples,
+                      lvr_ctx::kFrameSlots * sizeof(uint64_t), hipHostMallocDefault) !=
+        hipSuccess)
+        return bail(LVR_ENOMEM);
     if (hipHostMalloc((void**)&ctx->h_counters, 4 * sizeof(unsigned long long),
                       hipHostMallocDefault) != hipSuccess ||
         hipMalloc((void**)&ctx->d_counters, 4 * sizeof(unsigned long long)) !=
@@ -523,9 +530,16 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     ctx->selStaging.destroy();
     if (ctx->renderDone)
         (void)hipEventDestroy(ctx->renderDone);
-    for (auto& e : ctx->ev)
-        if (e)
-            (void)hipEventDestroy(e);
+    for (auto& f : ctx->frames)
+    {
+        for (auto& e : f.ev)
+            if (e)
+                (void)hipEventDestroy(e);
+        if (f.done)
+            (void)hipEventDestroy(f.done);
+    }
+    if (ctx->h_frameSamples)
+        (void)hipHostFree(ctx->h_frameSamples);
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     if (ctx->uploadStream)

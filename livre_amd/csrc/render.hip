@@ -816,6 +816,22 @@ static int ensureClassified(lvr_ctx* ctx, const lvr::FrameUniforms& U, int nValu
     return LVR_OK;
 }
 
+// End of a frame on the render stream: ray-cast end event, the composite
+// step count into the frame's pinned slot (copied before the next frame's
+// binning clears the counter), and the frame's completion event.
+static int finishFrame(lvr_ctx* ctx, lvr_ctx::FrameRecord& rec)
+{
+    const int slot = int(ctx->frameCount % lvr_ctx::kFrameSlots);
+    if (rec.timed)
+        LVR_HIP(ctx, hipEventRecord(rec.ev[2], ctx->stream));
+    LVR_HIP(ctx, hipMemcpyAsync(&ctx->h_frameSamples[slot], ctx->d_counters, sizeof(uint64_t),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    LVR_HIP(ctx, hipEventRecord(rec.done, ctx->stream));
+    rec.armed = true;
+    ++ctx->frameCount;
+    return LVR_OK;
+}
+
 int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                const uint64_t* ordered_ids, int n, const lvr_render_params* params)
 {
@@ -908,15 +924,23 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         }
         LVR_HIP(ctx, ctx->frameStaging.release(slot, ctx->stream));
     }
-    ctx->lastBricks = onScreen;
-    ctx->lastEntries = listTotal;
-    if (ctx->timing)
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+    // this frame's statistics slot; its previous frame (kFrameSlots ago) has
+    // long completed, but its events are only reused once it has
+    lvr_ctx::FrameRecord& rec = ctx->frames[ctx->frameCount % lvr_ctx::kFrameSlots];
+    if (rec.armed)
+        LVR_HIP(ctx, hipEventSynchronize(rec.done));
+    rec.armed = false;
+    rec.bricks = onScreen;
+    rec.entries = listTotal;
+    rec.timed = ctx->timing;
+    rec.histTimed = false;
+    if (rec.timed)
+        LVR_HIP(ctx, hipEventRecord(rec.ev[0], ctx->stream));
     enqueueBinning(ctx->stream, ctx->d_req, 1, ctx->d_ranges, nSupers, nTiles, ctx->d_superCount,
                    ctx->d_superOffset, ctx->d_superList, ctx->d_tileCount, ctx->d_tileOffset,
                    ctx->d_tileList, ctx->d_counters, 2);
-    if (ctx->timing)
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+    if (rec.timed)
+        LVR_HIP(ctx, hipEventRecord(rec.ev[1], ctx->stream));
     // persistent: two workgroups per CU are resident (the LDS budget of a
     // 4096-entry transfer function); more would only find the queue empty
     const int perBlock = U.wavesPerSimd * 2;
@@ -944,9 +968,7 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                                           ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
                                           ctx->d_counters);
         LVR_HIP(ctx, hipGetLastError());
-        if (ctx->timing)
-            LVR_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-        return LVR_OK;
+        return finishFrame(ctx, rec);
     }
     switch (dt)
     {
@@ -980,9 +1002,7 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         break;
     }
     LVR_HIP(ctx, hipGetLastError());
-    if (ctx->timing)
-        LVR_HIP(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
-    return LVR_OK;
+    return finishFrame(ctx, rec);
 }
 
 int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes, uint64_t* samples)
@@ -1027,39 +1047,50 @@ int lvr_set_timing(lvr_ctx* ctx, int enable)
     int rc = lvr::ensureDevice(ctx);
     if (rc)
         return rc;
-    if (enable && !ctx->ev[0])
-        for (auto& e : ctx->ev)
-            LVR_HIP(ctx, hipEventCreate(&e));
+    if (enable && !ctx->frames[0].ev[0])
+        for (auto& f : ctx->frames)
+            for (auto& e : f.ev)
+                LVR_HIP(ctx, hipEventCreate(&e));
     ctx->timing = enable != 0;
     return LVR_OK;
 }
 
-int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out)
+int lvr_frame_stats_back(lvr_ctx* ctx, int back, lvr_frame_stats* out)
 {
-    if (!ctx || !out)
+    if (!ctx || !out || back < 0)
         return LVR_EINVAL;
     int rc = lvr::ensureDevice(ctx);
     if (rc)
         return rc;
     std::memset(out, 0, sizeof *out);
-    LVR_HIP(ctx, hipMemcpyAsync(ctx->h_counters, ctx->d_counters, sizeof(unsigned long long),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    out->samples = ctx->h_counters[0];
-    out->bricks = ctx->lastBricks;
-    out->tile_entries = ctx->lastEntries;
     out->brick_bytes = ctx->brickBytes;
-    if (ctx->timing)
+    if (ctx->frameCount == 0 && back == 0)
+        return LVR_OK; // nothing rendered yet
+    if (uint64_t(back) >= std::min<uint64_t>(ctx->frameCount, lvr_ctx::kFrameSlots))
+        return lvr::fail(ctx, LVR_ERANGE, "frame statistics no longer kept");
+    const uint64_t frame = ctx->frameCount - 1 - uint64_t(back);
+    const int slot = int(frame % lvr_ctx::kFrameSlots);
+    const lvr_ctx::FrameRecord& rec = ctx->frames[slot];
+    LVR_HIP(ctx, hipEventSynchronize(rec.done));
+    out->samples = ctx->h_frameSamples[slot];
+    out->bricks = rec.bricks;
+    out->tile_entries = rec.entries;
+    if (rec.timed)
     {
-        LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, ctx->ev[0], ctx->ev[1]));
-        LVR_HIP(ctx, hipEventElapsedTime(&out->raycast_ms, ctx->ev[1], ctx->ev[2]));
-        if (ctx->histTimed)
+        LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, rec.ev[0], rec.ev[1]));
+        LVR_HIP(ctx, hipEventElapsedTime(&out->raycast_ms, rec.ev[1], rec.ev[2]));
+        if (rec.histTimed)
         {
-            LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
-            LVR_HIP(ctx, hipEventElapsedTime(&out->histogram_ms, ctx->ev[3], ctx->ev[4]));
+            LVR_HIP(ctx, hipEventSynchronize(rec.ev[4]));
+            LVR_HIP(ctx, hipEventElapsedTime(&out->histogram_ms, rec.ev[3], rec.ev[4]));
         }
     }
     return LVR_OK;
+}
+
+int lvr_frame_stats_get(lvr_ctx* ctx, lvr_frame_stats* out)
+{
+    return lvr_frame_stats_back(ctx, 0, out);
 }
 
 // ---------------------------------------------------------------- batch

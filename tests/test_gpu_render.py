@@ -84,6 +84,32 @@ def test_tile_shapes_agree(blobs_u8):
         assert np.array_equal(img, imgs[0][0]) and smp == imgs[0][1]
 
 
+def test_frame_stats_of_frames_in_flight(blobs_u8):
+    """Frames queued back to back keep their own statistics: the stats read
+    `back` frames later equal those of the same frame rendered alone."""
+    s, ctx = blobs_u8
+    ctx.set_tf(scenes.piecewise_tf(256, seed=6), s.range)
+    ctx.set_timing(True)
+    eyes = [(1.2, 0.9, 1.4), (-1.3, 0.5, 1.1), (0.8, -1.2, 1.3)]
+    frames = []
+    for eye in eyes:
+        mv, proj, fr = s.camera(eye, 80, 64)
+        frames.append((fr, s.visible_ordered(mv, proj, 64)))
+    alone = []
+    for fr, ordered in frames:
+        ctx.render(fr, 80, 64, ordered, livre_amd.render_params(64))
+        st = ctx.frame_stats()
+        alone.append((st.samples, st.bricks, st.tile_entries))
+    for fr, ordered in frames:
+        ctx.render(fr, 80, 64, ordered, livre_amd.render_params(64))
+    for back, want in zip((2, 1, 0), alone):
+        st = ctx.frame_stats(back=back)
+        assert (st.samples, st.bricks, st.tile_entries) == want and st.raycast_ms > 0
+    with pytest.raises(livre_amd.LivreError):
+        ctx.frame_stats(back=4)
+    ctx.set_timing(False)
+
+
 def test_camera_inside_volume(blobs_u8):
     s, ctx = blobs_u8
     tf = scenes.piecewise_tf(256, seed=4, max_alpha=0.3)

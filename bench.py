@@ -541,28 +541,37 @@ def main():
     ctx.sync()
 
     # Frames are pipelined as RenderPipeline::renderAsync does: while frame t
-    # ray-casts, frame t+1 is selected (on the selection stream), ordered,
-    # prepared and queued behind it; then frame t's statistics are read (the
-    # context keeps the last four frames'). A frame's latency runs from the
-    # start of its selection to the moment its completion is observed.
+    # ray-casts, frame t+1 (whose selection was queued one frame earlier, so
+    # it ran between two ray casts) is ordered, prepared and queued behind it,
+    # and frame t+2's selection is queued; then frame t's statistics are read
+    # (the context keeps the last four frames'). A frame's latency runs from
+    # the queueing of its selection to the moment its completion is observed.
     lat, samples, bricks, entries, rc_ms, bin_ms, hist_ms = [], 0, [], [], [], [], []
     alg_bytes = []
+    last = args.warmup + args.steps - 1
 
-    def submit(t):
-        f0 = time.perf_counter()
+    def select(t):
         fr = camera(livre_amd, scenes, t)[2]
-        vis = ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1)
+        ctx.select_visibles_async(fr, H, 1.0, 0, vi.depth - 1)
+        return time.perf_counter(), fr
+
+    def submit(pending, t):
+        f0, fr = pending
+        vis = ctx.select_visibles_wait()
+        nxt = select(t + 1) if t < last else None
         ordered = livre_amd.order(vi, fr, vis)
         ctx.render(fr, W, H, ordered, params)
         ctx.histogram(ordered, BINS, read=False)
-        return f0
+        return f0, nxt
 
     t_start = time.perf_counter()
-    starts = [submit(args.warmup)]
+    f0, pending = submit(select(args.warmup), args.warmup)
+    starts = [f0]
     for i in range(args.steps):
         t = args.warmup + i
-        if i + 1 < args.steps:
-            starts.append(submit(t + 1))
+        if t < last:
+            f0, pending = submit(pending, t + 1)
+            starts.append(f0)
             st = ctx.frame_stats(back=1)  # waits for frame t only
         else:
             st = ctx.frame_stats()
@@ -609,12 +618,12 @@ def main():
                                "samples/unit, FP32 4096-entry TF, sse 1, 65536-bin "
                                "histogram",
                    "frame": "select (device) + order + bin + raycast + histogram; the "
-                            "next frame is selected, ordered and queued while the "
-                            "current one ray-casts"},
+                            "next frame is ordered and queued while the current one "
+                            "ray-casts, its selection queued one frame ahead"},
         "fps": args.steps / total,
         "p50_ms": float(np.percentile(lat, 50)),
         "p99_ms": float(np.percentile(lat, 99)),
-        "latency": "p50/p99_ms: pipelined, selection start to frame end; "
+        "latency": "p50/p99_ms: pipelined, selection queued to frame end; "
                    "sync_p50/p99_ms: the frame alone",
         "sync_p50_ms": float(np.percentile(sync_lat, 50)),
         "sync_p99_ms": float(np.percentile(sync_lat, 99)),

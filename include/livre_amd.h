@@ -203,11 +203,21 @@ int lvr_tf_set_rgba8(lvr_ctx* ctx, const uint8_t* lut, int n, float range0,
  * livre/data/DFSTraversal.cpp:29-76) on the device, level by level, with
  * the reference's FP32 expressions. Output in the reference's DFS order.
  * n_out receives the count; LVR_ERANGE if cap is too small. Runs on a
- * stream of its own: it does not wait for frames still rendering, so the
- * next frame's selection can overlap the current frame's ray casting. */
+ * stream of its own: it does not wait for frames still rendering. */
 int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
                         uint32_t window_height, float sse, uint32_t min_lod,
                         uint32_t max_lod, uint64_t* out, int cap, int* n_out);
+/* lvr_select_visibles in two steps, for pipelined frames: _async queues the
+ * selection (the kernel writes its result into pinned host memory) and
+ * returns at once; _wait returns that result, as lvr_select_visibles does.
+ * One selection may be pending per context (LVR_EINVAL otherwise). A
+ * selection queued one frame ahead runs whenever the GPU has room, typically
+ * between two ray casts. No reference counterpart (the reference selects on
+ * the render thread). */
+int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f,
+                              uint32_t window_height, float sse, uint32_t min_lod,
+                              uint32_t max_lod);
+int lvr_select_visibles_wait(lvr_ctx* ctx, uint64_t* out, int cap, int* n_out);
 
 /* RayCastRenderer::order (RayCastRenderer.cpp:57-81, 153-159): front to
  * back by |MV * brick centre|, with libstdc++ std::sort on the given order. */

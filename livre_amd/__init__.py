@@ -31,6 +31,7 @@ EXPORTS = (
     "lvr_field_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
+    "lvr_select_visibles_async", "lvr_select_visibles_wait",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
     "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
@@ -200,6 +201,9 @@ def lib():
     L.lvr_select_visibles.argtypes = [c.c_void_p, P(Frustum), c.c_uint32, c.c_float,
                                       c.c_uint32, c.c_uint32, u64p, c.c_int,
                                       P(c.c_int)]
+    L.lvr_select_visibles_async.argtypes = [c.c_void_p, P(Frustum), c.c_uint32, c.c_float,
+                                            c.c_uint32, c.c_uint32]
+    L.lvr_select_visibles_wait.argtypes = [c.c_void_p, u64p, c.c_int, P(c.c_int)]
     L.lvr_order.argtypes = [P(VolumeInfo), P(Frustum), u64p, c.c_int, u64p]
     L.lvr_render.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
                              P(RenderParams)]
@@ -509,6 +513,21 @@ class Context:
             self._h, ctypes.byref(fr), window_height, sse, min_lod, max_lod,
             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap,
             ctypes.byref(n)), "select_visibles")
+        return out[:n.value].copy()
+
+    def select_visibles_async(self, fr, window_height, sse, min_lod, max_lod):
+        """Queue a selection; select_visibles_wait() returns its result."""
+        self._check(lib().lvr_select_visibles_async(
+            self._h, ctypes.byref(fr), window_height, sse, min_lod, max_lod),
+            "select_visibles_async")
+
+    def select_visibles_wait(self, cap=None):
+        cap = cap or self.node_count()
+        out = np.empty(cap, np.uint64)
+        n = ctypes.c_int(0)
+        self._check(lib().lvr_select_visibles_wait(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap,
+            ctypes.byref(n)), "select_visibles_wait")
         return out[:n.value].copy()
 
     def render(self, fr, width, height, ordered, params):

@@ -297,7 +297,11 @@ struct lvr_ctx
     std::vector<lvr::BrickEntry> hostBricks;
     std::vector<int4> hostRanges;
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
-    lvr::PinnedRing selStaging;   // selection results on their way to the host
+    // selection result (counters, then ids), written by the kernel into
+    // pinned host memory; one selection in flight
+    uint8_t* h_sel = nullptr;
+    hipEvent_t selDone = nullptr;
+    bool selPending = false;
     unsigned long long* d_counters = nullptr; // [0] samples, [1] work queue
     unsigned long long* h_counters = nullptr; // pinned
 

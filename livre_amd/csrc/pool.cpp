@@ -527,7 +527,10 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     if (ctx->h_counters)
         (void)hipHostFree(ctx->h_counters);
     ctx->frameStaging.destroy();
-    ctx->selStaging.destroy();
+    if (ctx->h_sel)
+        (void)hipHostFree(ctx->h_sel);
+    if (ctx->selDone)
+        (void)hipEventDestroy(ctx->selDone);
     if (ctx->renderDone)
         (void)hipEventDestroy(ctx->renderDone);
     for (auto& f : ctx->frames)

@@ -94,7 +94,8 @@ __device__ int visitNode(const SelectUniforms& U, uint64_t id)
 
 __global__ __launch_bounds__(1024) void selectKernel(
     const SelectUniforms U, uint64_t* __restrict__ bufA, uint64_t* __restrict__ bufB,
-    uint64_t* __restrict__ out, int32_t* __restrict__ counters)
+    uint64_t* __restrict__ out, int32_t* __restrict__ counters, uint64_t* __restrict__ hostOut,
+    int32_t* __restrict__ hostCounters)
 {
     __shared__ int nCur, nNext, nOut, overflow;
     const uint32_t rx = U.vi.root_blocks[0], ry = U.vi.root_blocks[1],
@@ -112,8 +113,8 @@ __global__ __launch_bounds__(1024) void selectKernel(
     {
         if (threadIdx.x == 0)
         {
-            counters[0] = 0;
-            counters[1] = 1;
+            counters[0] = hostCounters[0] = 0;
+            counters[1] = hostCounters[1] = 1;
         }
         return;
     }
@@ -168,10 +169,14 @@ __global__ __launch_bounds__(1024) void selectKernel(
         next = t;
     }
     __syncthreads();
+    // the result straight into the caller's pinned host buffer
+    const int n = min(nOut, U.capOut);
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        hostOut[i] = out[i];
     if (threadIdx.x == 0)
     {
-        counters[0] = min(nOut, U.capOut);
-        counters[1] = overflow;
+        counters[0] = hostCounters[0] = n;
+        counters[1] = hostCounters[1] = overflow;
     }
 }
 
@@ -198,16 +203,17 @@ uint64_t dfsKey(const lvr_volume_info& vi, uint64_t id)
 
 extern "C" {
 
-int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
-                        uint32_t window_height, float sse, uint32_t min_lod,
-                        uint32_t max_lod, uint64_t* out, int cap, int* n_out)
+int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f, uint32_t window_height,
+                              float sse, uint32_t min_lod, uint32_t max_lod)
 {
     using namespace lvr;
-    if (!ctx || !f || !n_out || cap < 0 || (cap > 0 && !out) || window_height == 0)
+    if (!ctx || !f || window_height == 0)
         return LVR_EINVAL;
     int rc = ensureDevice(ctx);
     if (rc)
         return rc;
+    if (ctx->selPending)
+        return fail(ctx, LVR_EINVAL, "a selection is already pending");
     // frontier capacity: the largest level of the octree
     size_t maxLevel = 1;
     for (uint32_t l = 0; l < ctx->vi.depth; ++l)
@@ -221,13 +227,22 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
         (void)hipFree(ctx->d_selB);
         (void)hipFree(ctx->d_selOut);
         ctx->d_selA = ctx->d_selB = ctx->d_selOut = nullptr;
+        if (ctx->h_sel)
+            (void)hipHostFree(ctx->h_sel);
+        ctx->h_sel = nullptr;
+        ctx->selCapacity = 0;
         LVR_HIP(ctx, hipMalloc((void**)&ctx->d_selA, sizeof(uint64_t) * capNeed));
         LVR_HIP(ctx, hipMalloc((void**)&ctx->d_selB, sizeof(uint64_t) * capNeed));
         LVR_HIP(ctx, hipMalloc((void**)&ctx->d_selOut, sizeof(uint64_t) * capNeed));
+        // counters, then the ids; written by the kernel over the bus
+        LVR_HIP(ctx, hipHostMalloc((void**)&ctx->h_sel, sizeof(uint64_t) * (capNeed + 1),
+                                   hipHostMallocMapped | hipHostMallocCoherent));
         ctx->selCapacity = capNeed;
     }
     if (!ctx->d_selCount)
         LVR_HIP(ctx, hipMalloc((void**)&ctx->d_selCount, sizeof(int32_t) * 2));
+    if (!ctx->selDone)
+        LVR_HIP(ctx, hipEventCreateWithFlags(&ctx->selDone, hipEventDisableTiming));
     SelectUniforms U;
     U.vi = ctx->vi;
     for (int p = 0; p < 6; ++p)
@@ -247,32 +262,35 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     U.maxLod = max_lod;
     U.capFrontier = int(std::min<size_t>(capNeed, 1u << 30));
     U.capOut = U.capFrontier;
+    uint8_t* dHost = nullptr;
+    LVR_HIP(ctx, hipHostGetDevicePointer((void**)&dHost, ctx->h_sel, 0));
     hipLaunchKernelGGL(selectKernel, dim3(1), dim3(1024), 0, ctx->selStream, U,
-                       ctx->d_selA, ctx->d_selB, ctx->d_selOut, ctx->d_selCount);
+                       ctx->d_selA, ctx->d_selB, ctx->d_selOut, ctx->d_selCount,
+                       reinterpret_cast<uint64_t*>(dHost + sizeof(uint64_t)),
+                       reinterpret_cast<int32_t*>(dHost));
     LVR_HIP(ctx, hipGetLastError());
-    // read back through pinned memory (a DMA straight into it), on the
-    // selection stream: a null-stream copy would also wait for every blocking
-    // stream of the process, and the render stream for the frame in flight
-    uint8_t* h = nullptr;
-    int slot = 0;
-    LVR_HIP(ctx, ctx->selStaging.acquire(sizeof(uint64_t) * (size_t(U.capOut) + 1), &h, &slot));
-    int32_t* counts = reinterpret_cast<int32_t*>(h);
-    LVR_HIP(ctx, hipMemcpyAsync(counts, ctx->d_selCount, 2 * sizeof(int32_t),
-                                hipMemcpyDeviceToHost, ctx->selStream));
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
+    LVR_HIP(ctx, hipEventRecord(ctx->selDone, ctx->selStream));
+    ctx->selPending = true;
+    return LVR_OK;
+}
+
+int lvr_select_visibles_wait(lvr_ctx* ctx, uint64_t* out, int cap, int* n_out)
+{
+    using namespace lvr;
+    if (!ctx || !n_out || cap < 0 || (cap > 0 && !out))
+        return LVR_EINVAL;
+    if (!ctx->selPending)
+        return fail(ctx, LVR_EINVAL, "no selection pending");
+    ctx->selPending = false;
+    LVR_HIP(ctx, hipEventSynchronize(ctx->selDone));
+    const int32_t* counts = reinterpret_cast<const int32_t*>(ctx->h_sel);
     if (counts[1])
         return fail(ctx, LVR_ENOMEM, "selection frontier overflow");
     const int n = counts[0];
     *n_out = n;
     if (n > cap)
         return fail(ctx, LVR_ERANGE, "visible set larger than the output buffer");
-    const uint64_t* ids = reinterpret_cast<const uint64_t*>(h + sizeof(uint64_t));
-    if (n > 0)
-    {
-        LVR_HIP(ctx, hipMemcpyAsync(h + sizeof(uint64_t), ctx->d_selOut, sizeof(uint64_t) * n,
-                                    hipMemcpyDeviceToHost, ctx->selStream));
-        LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
-    }
+    const uint64_t* ids = reinterpret_cast<const uint64_t*>(ctx->h_sel + sizeof(uint64_t));
     std::vector<std::pair<uint64_t, uint64_t>> keyed(static_cast<size_t>(n)), tmp;
     for (int i = 0; i < n; ++i)
         keyed[size_t(i)] = {dfsKey(ctx->vi, ids[i]), ids[i]};
@@ -284,5 +302,16 @@ int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
     for (int i = 0; i < n; ++i)
         out[i] = keyed[size_t(i)].second;
     return LVR_OK;
+}
+
+int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,
+                        uint32_t window_height, float sse, uint32_t min_lod,
+                        uint32_t max_lod, uint64_t* out, int cap, int* n_out)
+{
+    if (!ctx || !f || !n_out || cap < 0 || (cap > 0 && !out) || window_height == 0)
+        return LVR_EINVAL;
+    const int rc =
+        lvr_select_visibles_async(ctx, f, window_height, sse, min_lod, max_lod);
+    return rc ? rc : lvr_select_visibles_wait(ctx, out, cap, n_out);
 }
 }

@@ -273,6 +273,26 @@ def test_selection_dfs_order_matches_oracle(lvr, orc):
             assert np.array_equal(got, ref), k
 
 
+def test_async_selection_matches_blocking_selection(lvr, orc):
+    """A selection queued while a frame renders returns what the blocking
+    call returns; one selection may be pending at a time."""
+    vi = livre_amd.volume_info(2048, 64, 1, livre_amd.DT_UINT16)
+    ovi = orc.volume_info([2048] * 3, [64] * 3, [1] * 3, orc.DT_UINT16)
+    with livre_amd.Context(vi, 1 << 26) as ctx:
+        for k, eye in enumerate(((1.5, 0.4, 1.1), (-0.9, 1.6, 0.7), (0.3, -0.2, 2.4))):
+            mv, proj = scenes.camera(eye, 640, 360)
+            fr = livre_amd.frustum(mv, proj)
+            ctx.select_visibles_async(fr, 360, 1.0, 0, vi.depth - 1)
+            with pytest.raises(livre_amd.LivreError):
+                ctx.select_visibles_async(fr, 360, 1.0, 0, vi.depth - 1)
+            got = ctx.select_visibles_wait()
+            assert np.array_equal(got, ctx.select_visibles(fr, 360, 1.0, 0, vi.depth - 1)), k
+            assert np.array_equal(got, orc.select_visibles(ovi, mv, proj, 360, 1.0, 0,
+                                                           ovi.depth - 1)), k
+        with pytest.raises(livre_amd.LivreError):
+            ctx.select_visibles_wait()
+
+
 def test_pool_policy_and_eviction(lvr):
     """A pool of 2 slots evicts in policy order (Cache.cpp:111-123) and the
     load-order policy reproduces the reference cache test

@@ -415,12 +415,14 @@ def ooc_workload(args):
             % (len(lv), time.perf_counter() - t0, threads))
         start = st.stats()
         host_ms = []
-        t_start = time.perf_counter()
-        for t in range(args.frames):
+
+        def prepare(t):
+            """Frame t's camera, visible set, stream requests, rendering set
+            and order; its bricks are pinned at once, so uploads for later
+            frames cannot evict them before it renders."""
             f0 = time.perf_counter()
             pos, ahead = workloads.flythrough(t, args.frames)
-            up = (0.0, 1.0, 0.0)
-            mv = scenes.look_at(tuple(pos), tuple(ahead), up)
+            mv = scenes.look_at(tuple(pos), tuple(ahead), (0.0, 1.0, 0.0))
             proj = scenes.perspective(60.0, W4 / H4, 0.01, 6.0)
             fr = livre_amd.frustum(mv, proj)
             h0 = time.perf_counter()
@@ -435,27 +437,38 @@ def ooc_workload(args):
             h3 = time.perf_counter()
             ordered = livre_amd.order(vi, fr, rset)
             host_ms.append(((h1 - h0) * 1e3, (h2 - h1) * 1e3, (h3 - h2) * 1e3))
+            ctx.pin(ordered, 1)
             # samples per ray from the finest rendered level, as
             # RayCastRenderer::onFrameStart does when none is given
             # (RayCastRenderer.cpp:216-234)
             max_lod = max(int(i) & 0xF for i in ordered) if len(ordered) else 0
             n_samples = max(8192 >> (vi.depth - max_lod - 1), 512)
-            ctx.pin(ordered, 1)
-            ctx.render(fr, W4, H4, ordered, livre_amd.render_params(n_samples))
-            # transfers overlap the ray casting of this frame
-            while not ctx.query():
+            return f0, fr, ordered, len(vis), avail, len(rset), max_lod, n_samples
+
+        # pipelined as RenderPipeline::renderAsync: frame t+1 is prepared on
+        # the host while frame t ray-casts; transfers are started throughout
+        t_start = time.perf_counter()
+        cur = prepare(0)
+        ctx.render(cur[1], W4, H4, cur[2], livre_amd.render_params(cur[7]))
+        for t in range(args.frames):
+            nxt = prepare(t + 1) if t + 1 < args.frames else None
+            while not ctx.query():  # transfers overlap the ray casting
                 st.pump()
                 time.sleep(0.0002)
             fs = ctx.frame_stats()
+            f0, fr, ordered, n_vis, avail, n_rset, max_lod, _ = cur
             ctx.pin(ordered, -1)
-            rows.append((time.perf_counter() - f0, fs.raycast_ms, fs.samples, len(vis), avail,
-                         len(rset), fs.bricks, max_lod))
+            if nxt is not None:
+                ctx.render(nxt[1], W4, H4, nxt[2], livre_amd.render_params(nxt[7]))
+            rows.append((time.perf_counter() - f0, fs.raycast_ms, fs.samples, n_vis, avail,
+                         n_rset, fs.bricks, max_lod))
             if t % 30 == 0:
                 s = st.stats()
                 log("frame %d: visible %d, at full LOD %d, rendered %d (finest level %d), "
                     "raycast %.1f ms, frame %.1f ms, loaded %d"
-                    % (t, len(vis), avail, len(rset), max_lod, fs.raycast_ms,
+                    % (t, n_vis, avail, n_rset, max_lod, fs.raycast_ms,
                        rows[-1][0] * 1e3, s.bricks_loaded))
+            cur = nxt
         total = time.perf_counter() - t_start
         end = st.stats()
     pool_used = ctx.stats()[1]

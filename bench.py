@@ -416,17 +416,27 @@ def ooc_workload(args):
         start = st.stats()
         host_ms = []
 
-        def prepare(t):
-            """Frame t's camera, visible set, stream requests, rendering set
-            and order; its bricks are pinned at once, so uploads for later
-            frames cannot evict them before it renders."""
-            f0 = time.perf_counter()
+        def camera4k(t):
             pos, ahead = workloads.flythrough(t, args.frames)
             mv = scenes.look_at(tuple(pos), tuple(ahead), (0.0, 1.0, 0.0))
             proj = scenes.perspective(60.0, W4 / H4, 0.01, 6.0)
-            fr = livre_amd.frustum(mv, proj)
+            return livre_amd.frustum(mv, proj)
+
+        def select(t):
+            """Queue frame t's selection; it runs between two ray casts."""
+            fr = camera4k(t)
+            ctx.select_visibles_async(fr, H4, 1.0, 0, vi.depth - 1)
+            return time.perf_counter(), fr
+
+        def prepare(t, pending):
+            """Frame t's visible set (queued one frame earlier), stream
+            requests, rendering set and order; its bricks are pinned at once,
+            so uploads for later frames cannot evict them before it renders.
+            Frame t+1's selection is queued."""
+            f0, fr = pending
             h0 = time.perf_counter()
-            vis = ctx.select_visibles(fr, H4, 1.0, 0, vi.depth - 1)
+            vis = ctx.select_visibles_wait()
+            nxt = select(t + 1) if t + 1 < args.frames else None
             h1 = time.perf_counter()
             # wishes in selection (depth-first, spatially coherent) order, with
             # their ancestors; the stream loads coarse levels first
@@ -443,15 +453,17 @@ def ooc_workload(args):
             # (RayCastRenderer.cpp:216-234)
             max_lod = max(int(i) & 0xF for i in ordered) if len(ordered) else 0
             n_samples = max(8192 >> (vi.depth - max_lod - 1), 512)
-            return f0, fr, ordered, len(vis), avail, len(rset), max_lod, n_samples
+            return (f0, fr, ordered, len(vis), avail, len(rset), max_lod, n_samples), nxt
 
         # pipelined as RenderPipeline::renderAsync: frame t+1 is prepared on
         # the host while frame t ray-casts; transfers are started throughout
         t_start = time.perf_counter()
-        cur = prepare(0)
+        cur, pending = prepare(0, select(0))
         ctx.render(cur[1], W4, H4, cur[2], livre_amd.render_params(cur[7]))
         for t in range(args.frames):
-            nxt = prepare(t + 1) if t + 1 < args.frames else None
+            nxt = None
+            if pending is not None:
+                nxt, pending = prepare(t + 1, pending)
             while not ctx.query():  # transfers overlap the ray casting
                 st.pump()
                 time.sleep(0.0002)

@@ -187,6 +187,51 @@ int lvr_field_brick_host(const lvr_volume_info* info,
 int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p,
                             const uint64_t* node_ids, int n);
 
+/* ------------------------------------------------------- raw / NRRD data */
+/* RawDataSource (livre/data/RawDataSource.cpp:62-219): a memory-mapped raw
+ * (x fastest) or NRRD volume. uri:
+ *   [raw://]/path.raw|.img[?output=type]#nx,ny,nz[,type]   (type default uint8)
+ *   [raw://]/path.nrrd[?output=type]     (type, dimension 3, sizes, endian,
+ *                                         encoding raw, optional datafile)
+ * with the reference's type names (char, int8, unsigned char, uint8, short,
+ * int16, unsigned short, uint16, int, int32, unsigned int, uint32, float, and
+ * NRRD's aliases such as uchar). An output type narrower than the input keeps
+ * the high bits (uint16 -> uint8, uint32 -> uint8 or uint16, as _scale does).
+ * The volume information is the reference's: the whole volume is one brick
+ * (depth 1, no overlap). On failure *out is null and error receives the
+ * reason. */
+typedef struct lvr_raw_source lvr_raw_source;
+typedef struct
+{
+    lvr_volume_info info;  /* one brick, output type */
+    int32_t input_type;    /* LVR_DT_* of the file */
+    int32_t big_endian;    /* NRRD "endian: big"; bytes are passed on as stored */
+    uint64_t data_bytes;   /* voxels * input bytes per voxel */
+} lvr_raw_info;
+int lvr_raw_open(const char* uri, lvr_raw_source** out, char* error, size_t error_len);
+void lvr_raw_close(lvr_raw_source* src);
+int lvr_raw_get_info(const lvr_raw_source* src, lvr_raw_info* info);
+/* DataSourcePlugin::getData of the one brick (node id of level 0 at the
+ * origin), an lvr_brick_source_fn with user = the source. */
+int lvr_raw_brick(void* user, uint64_t node_id, void* out, size_t bytes);
+
+/* LOD pyramid of a raw source (no reference counterpart: the reference
+ * serves a raw volume as one brick). The file becomes the finest level of the
+ * regular octree lvr_pyramid_volume_info describes (block and overlap chosen
+ * by the caller; zero padding up to the octree's extent); each coarser level
+ * is the 2x2x2 mean of the next (integers rounded half up, floats
+ * ((a+b)+(c+d))+((e+f)+(g+h)) times 1/8); a brick holds its level's voxels
+ * [b*block - overlap, (b+1)*block + overlap), clamped to the level. The
+ * levels are built in HBM on the first load (about 8/7 of the padded finest
+ * level) and kept until lvr_pyramid_release or another source is loaded;
+ * bricks are cut straight into pool slots. The context must have been
+ * created with the pyramid's volume information; cubic brick grids only. */
+int lvr_pyramid_volume_info(const lvr_raw_source* src, uint32_t block, uint32_t overlap,
+                            lvr_volume_info* info);
+int lvr_pyramid_load(lvr_ctx* ctx, const lvr_raw_source* src, const uint64_t* node_ids,
+                     int n);
+int lvr_pyramid_release(lvr_ctx* ctx);
+
 /* ----------------------------------------------------- transfer function */
 /* FP32 RGBA transfer function of n entries over the data range
  * [range0, range1] (RayCastRenderer.cpp:161-193, 280-285). Sampled with

@@ -32,6 +32,8 @@ EXPORTS = (
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
     "lvr_select_visibles_async", "lvr_select_visibles_wait",
+    "lvr_raw_open", "lvr_raw_close", "lvr_raw_get_info", "lvr_raw_brick",
+    "lvr_pyramid_volume_info", "lvr_pyramid_load", "lvr_pyramid_release",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
     "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
@@ -154,6 +156,11 @@ class NoiseSourceData(ctypes.Structure):
     _fields_ = [("info", VolumeInfo), ("params", NoiseParams)]
 
 
+class RawInfo(ctypes.Structure):
+    _fields_ = [("info", VolumeInfo), ("input_type", ctypes.c_int32),
+                ("big_endian", ctypes.c_int32), ("data_bytes", ctypes.c_uint64)]
+
+
 _lib = None
 
 
@@ -196,6 +203,14 @@ def lib():
     L.lvr_field_brick_host.argtypes = [P(VolumeInfo), P(FieldParams), c.c_uint64,
                                        c.c_void_p, c.c_size_t]
     L.lvr_field_bricks_device.argtypes = [c.c_void_p, P(FieldParams), u64p, c.c_int]
+    L.lvr_raw_open.argtypes = [c.c_char_p, P(c.c_void_p), c.c_char_p, c.c_size_t]
+    L.lvr_raw_close.argtypes = [c.c_void_p]
+    L.lvr_raw_close.restype = None
+    L.lvr_raw_get_info.argtypes = [c.c_void_p, P(RawInfo)]
+    L.lvr_raw_brick.argtypes = [c.c_void_p, c.c_uint64, c.c_void_p, c.c_size_t]
+    L.lvr_pyramid_volume_info.argtypes = [c.c_void_p, c.c_uint32, c.c_uint32, P(VolumeInfo)]
+    L.lvr_pyramid_load.argtypes = [c.c_void_p, c.c_void_p, u64p, c.c_int]
+    L.lvr_pyramid_release.argtypes = [c.c_void_p]
     L.lvr_tf_set.argtypes = [c.c_void_p, f32p, c.c_int, c.c_float, c.c_float]
     L.lvr_tf_set_rgba8.argtypes = [c.c_void_p, u8p, c.c_int, c.c_float, c.c_float]
     L.lvr_select_visibles.argtypes = [c.c_void_p, P(Frustum), c.c_uint32, c.c_float,
@@ -393,6 +408,55 @@ def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
     return p
 
 
+class RawSource:
+    """A raw or NRRD volume (RawDataSource): `uri` as lvr_raw_open takes it."""
+
+    def __init__(self, uri):
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().lvr_raw_open(uri.encode(), ctypes.byref(h), err, len(err))
+        if rc != LVR_OK:
+            raise LivreError(err.value.decode() or "raw source failed (%d)" % rc)
+        self._h = h
+        ri = RawInfo()
+        lib().lvr_raw_get_info(self._h, ctypes.byref(ri))
+        self.raw_info = ri
+        self.vi = ri.info
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvr_raw_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def brick(self, node_id=0):
+        """The one brick of the reference's depth-1 volume (getData)."""
+        vi = self.vi
+        out = np.empty(vi.brick_shape, vi.dtype)
+        rc = lib().lvr_raw_brick(self._h, int(node_id), out.ctypes.data, out.nbytes)
+        if rc != LVR_OK:
+            raise LivreError("raw brick failed (%d)" % rc)
+        return out
+
+    def pyramid_info(self, block, overlap):
+        vi = VolumeInfo()
+        rc = lib().lvr_pyramid_volume_info(self._h, block, overlap, ctypes.byref(vi))
+        if rc != LVR_OK:
+            raise LivreError("invalid pyramid shape (%d)" % rc)
+        return vi
+
+
 class Context:
     """One volume's HBM brick pool and render state on one device."""
 
@@ -474,6 +538,16 @@ class Context:
         """Pin (+1) or unpin (-1) resident bricks (never evicted while pinned)."""
         ids, pids = _u64(ids)
         self._check(lib().lvr_brick_pin(self._h, pids, len(ids), delta), "pin")
+
+    def load_pyramid(self, source, ids):
+        """Bricks of a RawSource's LOD pyramid into the pool (the context's
+        volume information must be source.pyramid_info(block, overlap))."""
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_pyramid_load(self._h, source._h, pids, len(ids)),
+                    "load_pyramid")
+
+    def release_pyramid(self):
+        self._check(lib().lvr_pyramid_release(self._h), "release_pyramid")
 
     def generate_noise(self, params, ids):
         ids, pids = _u64(ids)

@@ -211,6 +211,8 @@ struct FrameUniforms
 // that returns at once, where a pageable source is first staged by the host
 // in chunks. Two buffers are used in turn, and one is rewritten only after
 // the copies that read it have completed.
+struct RawPyramid;
+
 struct PinnedRing
 {
     uint8_t* buf[2] = {nullptr, nullptr};
@@ -297,6 +299,7 @@ struct lvr_ctx
     std::vector<lvr::BrickEntry> hostBricks;
     std::vector<int4> hostRanges;
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
+    lvr::RawPyramid* pyramid = nullptr; // levels of the last raw source loaded
     // selection result (counters, then ids), written by the kernel into
     // pinned host memory; one selection in flight
     uint8_t* h_sel = nullptr;
@@ -372,6 +375,9 @@ bool pageIndex(const lvr_ctx* ctx, uint64_t id, uint64_t* idx);
 void touch(lvr_ctx* ctx, uint64_t id);
 // slotOf + touch with one lookup
 int32_t slotAndTouch(lvr_ctx* ctx, uint64_t id);
+// Device levels of a raw source's LOD pyramid (raw.hip), freed with the
+// context or by lvr_pyramid_release.
+void destroyPyramid(lvr_ctx* ctx);
 int flushPageTable(lvr_ctx* ctx);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
 // Undo acquireSlotFor for a brick whose contents were never written.

@@ -493,6 +493,7 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamSynchronize(ctx->histStream);
     if (ctx->selStream)
         (void)hipStreamSynchronize(ctx->selStream);
+    lvr::destroyPyramid(ctx);
     (void)hipFree(ctx->d_pool);
     (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->pt.d_slots);

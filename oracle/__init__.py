@@ -257,3 +257,45 @@ def tf_lut(rgba):
     out = np.zeros((n, 4), np.uint8)
     lib().orc_tf_lut(p, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
     return out
+
+
+# ------------------------------------------------------------ LOD pyramid
+# Restatement of the raw-volume pyramid (no reference counterpart: the
+# reference serves a raw volume as one brick, RawDataSource.cpp:62-219).
+# Pinned by closed forms in tests/test_oracle_pyramid.py: a constant volume
+# stays constant at every level, the finest level is the file padded with
+# zeros, a 2x2x2 block of known values averages to its rounded mean, and a
+# brick's halo repeats its neighbours' voxels.
+
+def pyramid_levels(volume, depth, root_blocks, block):
+    """volume: numpy (z, y, x) array of the output type. Returns the level
+    grids, level 0 first; level L is root_blocks * 2**L * block per axis."""
+    P = root_blocks * (1 << (depth - 1)) * block
+    finest = np.zeros((P, P, P), volume.dtype)
+    nz, ny, nx = volume.shape
+    finest[:nz, :ny, :nx] = volume
+    levels = [finest]
+    for _ in range(depth - 1):
+        f = levels[0]
+        # children in the order c = dz*4 + dy*2 + dx
+        v = [f[dz::2, dy::2, dx::2] for dz in (0, 1) for dy in (0, 1) for dx in (0, 1)]
+        if np.issubdtype(f.dtype, np.floating):
+            s = ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]))
+            c = (s * np.float32(0.125)).astype(np.float32)
+        else:
+            s = sum(x.astype(np.int64) for x in v)
+            c = ((s + 4) >> 3).astype(f.dtype)
+        levels.insert(0, c)
+    return levels
+
+
+def pyramid_brick(levels, max_block, overlap, node_id):
+    """The brick of node_id: its level's voxels [b*block - overlap,
+    (b+1)*block + overlap) per axis, clamped to the level's extent."""
+    level = node_id & 0xF
+    pos = [(node_id >> (4 + 14 * a)) & 0x3FFF for a in range(3)]
+    block = max_block - 2 * overlap
+    g = levels[level]
+    P = g.shape[0]
+    idx = [np.clip(np.arange(max_block) + p * block - overlap, 0, P - 1) for p in pos]
+    return g[np.ix_(idx[2], idx[1], idx[0])]

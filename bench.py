@@ -516,6 +516,104 @@ def ooc_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def raw_workload(args):
+    """A raw file on disk (1024^3 uint16, written first from the value-noise
+    field) opened as RawDataSource does, its LOD pyramid built in HBM (64^3
+    bricks + 1 halo, depth 5) and every brick cut into the pool; then the
+    1080p orbit-zoom rendered from it."""
+    import tempfile
+    import livre_amd
+    from livre_amd import scenes
+    livre_amd.lib()
+    n = args.raw_size
+    tmpdir = tempfile.mkdtemp(prefix="livre_raw_")
+    path = os.path.join(tmpdir, "volume.raw")
+    # the file: the value-noise field at n^3, generated on the device one
+    # z-slab of bricks at a time and written slab by slab
+    vi_src = livre_amd.volume_info(n, 64, 0, livre_amd.DT_UINT16)
+    noise = livre_amd.noise_params(3, octaves=5, base_cells=8, gain=1.5)
+    nb = n // 64
+    t0 = time.perf_counter()
+    with livre_amd.Context(vi_src, livre_amd.slot_bytes(vi_src) * nb * nb) as gen, \
+            open(path, "wb") as f:
+        for z in range(nb):
+            slab_ids = [scenes.node_id(vi_src.depth - 1, x, y, z)
+                        for y in range(nb) for x in range(nb)]
+            for nid in slab_ids:
+                if gen.resident(nid):
+                    gen.evict(nid)
+            gen.generate_noise(noise, slab_ids)
+            slab = np.empty((64, n, n), np.uint16)
+            for nid in slab_ids:
+                _lv, x, y, _z = scenes.decode_node(nid)
+                slab[:, y * 64:(y + 1) * 64, x * 64:(x + 1) * 64] = gen.download(nid)
+            slab.tofile(f)
+            for nid in slab_ids:
+                gen.evict(nid)
+    t_write = time.perf_counter() - t0
+    log("wrote %s (%.1f GB) in %.1f s" % (path, n ** 3 * 2 / 1e9, t_write))
+    try:
+        with livre_amd.RawSource("raw://%s#%d,%d,%d,uint16" % (path, n, n, n)) as src:
+            vi = src.pyramid_info(64, 1)
+            ids = scenes.all_nodes(vi)
+            ctx = livre_amd.Context(vi, livre_amd.slot_bytes(vi) * len(ids))
+            # the first load builds the levels: file -> HBM, downsampling,
+            # bricks cut into slots
+            t0 = time.perf_counter()
+            ctx.load_pyramid(src, ids)
+            ctx.sync()
+            t_build = time.perf_counter() - t0
+            # again with the levels resident: the cut alone
+            t0 = time.perf_counter()
+            ctx.load_pyramid(src, ids)
+            ctx.sync()
+            t_cut = time.perf_counter() - t0
+            ctx.release_pyramid()
+            brick_bytes = len(ids) * vi.max_block[0] ** 3 * 2
+            log("pyramid: %d bricks, build %.2f s, cut %.3f s" % (len(ids), t_build, t_cut))
+            tf = scenes.piecewise_tf(TF_ENTRIES, seed=4, points=8, max_alpha=0.35)
+            ctx.set_tf(tf, (0.0, 65535.0))
+            ctx.set_timing(True)
+            params = livre_amd.render_params(n, early_exit=0.999)
+            samples, rc_ms, lat = 0, [], []
+            frames = args.steps or 60
+            t0 = time.perf_counter()
+            for t in range(frames):
+                f1 = time.perf_counter()
+                fr = camera(livre_amd, scenes, t * FRAMES // frames)[2]
+                vis = ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1)
+                ctx.render(fr, W, H, livre_amd.order(vi, fr, vis), params)
+                st = ctx.frame_stats()
+                samples += st.samples
+                rc_ms.append(st.raycast_ms)
+                lat.append((time.perf_counter() - f1) * 1e3)
+            t_render = time.perf_counter() - t0
+            ctx.close()
+    finally:
+        os.remove(path)
+        os.rmdir(tmpdir)
+    result = {
+        "metric": "raw volume: LOD pyramid build GB/s; render Gsamples/s",
+        "value": n ** 3 * 2 / t_build / 1e9, "unit": "GB/s (file bytes into a resident "
+                                                     "pyramid)",
+        "n_gpus": 1, "higher_is_better": True, "vs_baseline": None, "dtype": "u16",
+        "data": "synthetic: %d^3 uint16 value noise written to a raw file first" % n,
+        "config": {"workload": "raw://volume.raw#%d,%d,%d,uint16 -> pyramid of 64^3 "
+                               "bricks + 1 halo, depth %d, %d bricks; 1920x1080 "
+                               "orbit-zoom, %d frames" % (n, n, n, vi.depth, len(ids),
+                                                          frames)},
+        "file_write_s": t_write,
+        "build_s": t_build, "cut_s": t_cut,
+        "cut_gb_per_s": brick_bytes / t_cut / 1e9,
+        "bricks": len(ids),
+        "render_gsamples_per_s": samples / t_render / 1e9,
+        "render_fps": frames / t_render,
+        "raycast_ms_mean": float(np.mean(rc_ms)),
+        "frame_p50_ms": float(np.percentile(lat, 50)),
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -523,18 +621,20 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc"),
+    ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc", "raw"),
                     default="render",
                     help="render: the 1080p single-volume frame loop (default); "
                          "batch: uneven requests, plain vs one batched launch; "
                          "serve: 32 sessions under Poisson arrivals; "
-                         "ooc: out-of-core 8192^3 fly-through")
+                         "ooc: out-of-core 8192^3 fly-through; "
+                         "raw: a raw file's LOD pyramid built in HBM, then rendered")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--max-batch", type=int, default=2)
     ap.add_argument("--pool-frac", type=float, default=1.0)
     ap.add_argument("--pool-gb", type=float, default=200.0)
     ap.add_argument("--frames", type=int, default=300)
+    ap.add_argument("--raw-size", type=int, default=1024)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the brick pool, kernels and measurements are "
@@ -550,6 +650,9 @@ def main():
         return
     if args.workload == "ooc":
         ooc_workload(args)
+        return
+    if args.workload == "raw":
+        raw_workload(args)
         return
     if args.steps is None:
         args.steps = FRAMES

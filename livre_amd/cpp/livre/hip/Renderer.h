@@ -5,6 +5,7 @@
  *   livre::Frustum           livre/data/Frustum.h:37-111
  *   livre::Renderer          livre/core/render/Renderer.h:38-109
  *   livre::DataSourcePlugin  livre/data/DataSourcePlugin.h:55-109 (getData)
+ *   livre::RawDataSource     livre/data/RawDataSource.cpp:62-219
  *   livre::RayCastRenderer   livre/eq/render/RayCastRenderer.h:31-74
  *   RenderPipeline::renderSync livre/lib/pipeline/RenderPipeline.cpp:92-158
  * Header-only over the C ABI (include/livre_amd.h); link with liblivre_amd.so.
@@ -175,6 +176,41 @@ protected:
     VolumeInformation _volumeInfo;
 };
 
+/** RawDataSource (livre/data/RawDataSource.cpp:62-219): a raw or NRRD volume
+ * served as one brick, from the reference's raw:// URIs. pyramidInfo() and
+ * BrickPool::loadPyramid() give it levels of detail instead. */
+class RawDataSource : public DataSourcePlugin
+{
+public:
+    explicit RawDataSource(const std::string& uri)
+    {
+        char err[512] = {0};
+        const int rc = lvr_raw_open(uri.c_str(), &_src, err, sizeof err);
+        if (rc != LVR_OK)
+            throw std::runtime_error(std::string("RawDataSource: ") + err);
+        lvr_raw_info ri;
+        lvr_raw_get_info(_src, &ri);
+        _volumeInfo.info = ri.info;
+    }
+    ~RawDataSource() override { lvr_raw_close(_src); }
+    RawDataSource(const RawDataSource&) = delete;
+    RawDataSource& operator=(const RawDataSource&) = delete;
+    bool getData(const NodeId& node, void* out) override
+    {
+        return lvr_raw_brick(_src, node.getId(), out, _volumeInfo.brickBytes()) == LVR_OK;
+    }
+    VolumeInformation pyramidInfo(uint32_t block, uint32_t overlap) const
+    {
+        VolumeInformation vi;
+        check(lvr_pyramid_volume_info(_src, block, overlap, &vi.info), "pyramidInfo");
+        return vi;
+    }
+    const lvr_raw_source* handle() const { return _src; }
+
+private:
+    lvr_raw_source* _src = nullptr;
+};
+
 /** The HBM brick pool of one volume: TexturePool + the texture cache. */
 class BrickPool
 {
@@ -194,6 +230,13 @@ public:
         check(lvr_brick_upload(_ctx, n.getId(), voxels, _vi.brickBytes()), "upload", _ctx);
     }
     void evict(const NodeId& n) { check(lvr_brick_evict(_ctx, n.getId()), "evict", _ctx); }
+    /** Bricks of a raw source's LOD pyramid (the pool's volume information
+     * must be source.pyramidInfo(block, overlap)). */
+    void loadPyramid(const RawDataSource& source, const std::vector<uint64_t>& ids)
+    {
+        check(lvr_pyramid_load(_ctx, source.handle(), ids.data(), int(ids.size())),
+              "loadPyramid", _ctx);
+    }
     uint64_t capacity() const
     {
         uint64_t s = 0;

@@ -5,7 +5,7 @@
  * the histogram to files.
  *
  *   render_frame OUT_PREFIX [n voxels] [block] [width] [height]
- *   render_frame --host-check     (host-side API only, no GPU needed)
+ *   render_frame --host-check [nucleon.raw]  (host-side API only, no GPU)
  *
  * OUT_PREFIX.rgba (w*h*4 bytes, row 0 = bottom), OUT_PREFIX.hist (u64 bins),
  * OUT_PREFIX.txt (bricks, samples, camera matrices).
@@ -72,7 +72,7 @@ void perspective(float fovy, float aspect, float n, float f, float* p)
     p[14] = 2 * f * n / (n - f);
 }
 
-int hostCheck()
+int hostCheck(const std::string& rawPath)
 {
     const livre::NodeId root(0, {0, 0, 0});
     const auto kids = root.getChildren();
@@ -92,6 +92,33 @@ int hostCheck()
     const livre::Frustum fr(mv, proj);
     if (std::fabs(fr.getEyePos()[2] - 1.9f) > 1e-5f)
         return 4;
+    if (!rawPath.empty())
+    {
+        // The reference's raw test volume (41^3 uint8, one brick), through
+        // RawDataSource and against the file's own bytes.
+        livre::RawDataSource raw(rawPath + "#41,41,41");
+        const livre::VolumeInformation& ri = raw.getVolumeInfo();
+        if (ri.info.voxels[0] != 41 || ri.info.data_type != LVR_DT_UINT8 || ri.depth() != 1)
+            return 5;
+        std::vector<uint8_t> brick(ri.brickBytes()), file(ri.brickBytes());
+        std::ifstream in(rawPath, std::ios::binary);
+        in.read(reinterpret_cast<char*>(file.data()), std::streamsize(file.size()));
+        if (!in || !raw.getData(livre::NodeId(0, {0, 0, 0}), brick.data()) || brick != file)
+            return 6;
+        if (raw.pyramidInfo(8, 0).depth() != 4) // 41 -> 64 = 8 * 2^3
+            return 7;
+        bool threw = false;
+        try
+        {
+            livre::RawDataSource bad(rawPath + "#41,41");
+        }
+        catch (const std::runtime_error&)
+        {
+            threw = true;
+        }
+        if (!threw)
+            return 8;
+    }
     std::cout << "host check ok" << std::endl;
     return 0;
 }
@@ -100,7 +127,7 @@ int hostCheck()
 int main(int argc, char** argv)
 {
     if (argc > 1 && std::string(argv[1]) == "--host-check")
-        return hostCheck();
+        return hostCheck(argc > 2 ? argv[2] : "");
     if (argc < 2)
     {
         std::cerr << "usage: render_frame OUT_PREFIX [n] [block] [w] [h]" << std::endl;

@@ -24,8 +24,9 @@ def driver(lvr):
 
 
 def test_host_side_api(driver):
-    res = subprocess.run([driver, "--host-check"], capture_output=True, text=True,
-                         timeout=60)
+    nucleon = os.path.join(ROOT, "tests", "golden", "nucleon.raw")
+    res = subprocess.run([driver, "--host-check", nucleon], capture_output=True,
+                         text=True, timeout=60)
     assert res.returncode == 0, res.stdout + res.stderr
 
 

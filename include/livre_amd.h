@@ -547,6 +547,51 @@ int lvr_brick_pin(lvr_ctx* ctx, const uint64_t* ids, int n, int delta);
 int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins);
 int lvr_histogram_read(lvr_ctx* ctx, uint64_t* out, int bins);
 
+/* ------------------------------------------------- frame grab as JPEG */
+/* FrameGrabber::_encodeJpeg (livre/eq/FrameGrabber.cpp:49-84) compresses the
+ * read-back frame with libjpeg-turbo's tjCompress2: TJPF_BGRA, TJSAMP_444,
+ * quality 100, flags TJXOP_ROT180 (= 6, read as TJFLAG_BOTTOMUP). These entry
+ * points write the same JFIF file on the device: baseline JPEG, 4:4:4, the
+ * accurate integer DCT (tjCompress2's choice at quality >= 96), standard
+ * tables scaled to `quality`. notifyNewImage (FrameGrabber.cpp:86-109) sends
+ * the bytes on; here they land in a host buffer. */
+#define LVR_PIXEL_RGBA 0
+#define LVR_PIXEL_BGRA 1
+typedef struct lvr_jpeg_encoder lvr_jpeg_encoder;
+typedef struct
+{
+    float device_ms; /* kernels of the last encode (HIP events) */
+    int32_t width, height;
+    uint64_t bytes; /* file length of the last encode */
+} lvr_jpeg_stats;
+/* Encoder with its own stream and grow-only device scratch. */
+int lvr_jpeg_encoder_create(int device, lvr_jpeg_encoder** out);
+void lvr_jpeg_encoder_destroy(lvr_jpeg_encoder* enc);
+/* Encode device pixels (4 bytes each, 4-byte aligned, pitch a multiple of 4,
+ * 1 <= width, height <= 16384). bottom_up != 0 takes row height-1 as the
+ * top row (TJFLAG_BOTTOMUP). *size receives the file length; out may be NULL
+ * to learn it (the encode still runs); LVR_ERANGE if cap is too small.
+ * Synchronous: returns once the bytes are in `out`. */
+int lvr_jpeg_encode(lvr_jpeg_encoder* enc, const void* device_pixels, int width, int height,
+                    int pitch_bytes, int pixel_format, int bottom_up, int quality,
+                    uint8_t* out, size_t cap, size_t* size);
+/* The same from host pixels, as FrameGrabber::_encodeJpeg receives them
+ * (FrameGrabber.cpp:49-53: the read-back eq::Image); they are copied to the
+ * encoder's device buffer first. */
+int lvr_jpeg_encode_host(lvr_jpeg_encoder* enc, const void* host_pixels, int width,
+                         int height, int pitch_bytes, int pixel_format, int bottom_up,
+                         int quality, uint8_t* out, size_t cap, size_t* size);
+/* Upper bound of the file length for a width x height image. */
+size_t lvr_jpeg_max_size(int width, int height);
+/* Last error of enc; with enc NULL, why the last create on this thread failed. */
+const char* lvr_jpeg_error(const lvr_jpeg_encoder* enc);
+int lvr_jpeg_get_stats(const lvr_jpeg_encoder* enc, lvr_jpeg_stats* out);
+/* Grab the context's last frame (RGBA8, row 0 = bottom, as lvr_frame_read
+ * returns it) as the reference's frame grabber encodes it; quality 100 in
+ * the reference. Runs on the context's stream after the frame. */
+int lvr_frame_grab_jpeg(lvr_ctx* ctx, int quality, uint8_t* out, size_t cap, size_t* size);
+int lvr_frame_grab_stats(lvr_ctx* ctx, lvr_jpeg_stats* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,6 +34,10 @@ EXPORTS = (
     "lvr_select_visibles_async", "lvr_select_visibles_wait",
     "lvr_raw_open", "lvr_raw_close", "lvr_raw_get_info", "lvr_raw_brick",
     "lvr_pyramid_volume_info", "lvr_pyramid_load", "lvr_pyramid_release",
+    "lvr_jpeg_encoder_create", "lvr_jpeg_encoder_destroy", "lvr_jpeg_encode",
+    "lvr_jpeg_encode_host",
+    "lvr_jpeg_max_size", "lvr_jpeg_error", "lvr_jpeg_get_stats", "lvr_frame_grab_jpeg",
+    "lvr_frame_grab_stats",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
     "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
@@ -282,6 +286,21 @@ def lib():
     L.lvr_stream_get_stats.argtypes = [c.c_void_p, P(StreamStats)]
     L.lvr_brick_pin.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_jpeg_encoder_create.argtypes = [c.c_int, P(c.c_void_p)]
+    L.lvr_jpeg_encoder_destroy.argtypes = [c.c_void_p]
+    L.lvr_jpeg_encoder_destroy.restype = None
+    L.lvr_jpeg_encode.argtypes = [c.c_void_p, c.c_void_p, c.c_int, c.c_int, c.c_int, c.c_int,
+                                  c.c_int, c.c_int, u8p, c.c_size_t, P(c.c_size_t)]
+    L.lvr_jpeg_encode_host.argtypes = [c.c_void_p, c.c_void_p, c.c_int, c.c_int, c.c_int,
+                                       c.c_int, c.c_int, c.c_int, u8p, c.c_size_t,
+                                       P(c.c_size_t)]
+    L.lvr_jpeg_max_size.argtypes = [c.c_int, c.c_int]
+    L.lvr_jpeg_max_size.restype = c.c_size_t
+    L.lvr_jpeg_error.argtypes = [c.c_void_p]
+    L.lvr_jpeg_error.restype = c.c_char_p
+    L.lvr_jpeg_get_stats.argtypes = [c.c_void_p, P(JpegStats)]
+    L.lvr_frame_grab_jpeg.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, P(c.c_size_t)]
+    L.lvr_frame_grab_stats.argtypes = [c.c_void_p, P(JpegStats)]
     L.lvr_frame_stats_get.argtypes = [c.c_void_p, P(FrameStats)]
     L.lvr_frame_stats_back.argtypes = [c.c_void_p, c.c_int, P(FrameStats)]
     _lib = L
@@ -406,6 +425,77 @@ def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
     p.classify = classify
     p.waves_per_simd = waves_per_simd
     return p
+
+
+class JpegStats(ctypes.Structure):
+    _fields_ = [("device_ms", ctypes.c_float), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("bytes", ctypes.c_uint64)]
+
+
+PIXEL_RGBA, PIXEL_BGRA = 0, 1
+
+
+class JpegEncoder:
+    """The frame grabber's JPEG encoder on the device (FrameGrabber::_encodeJpeg)."""
+
+    def __init__(self, device=0):
+        self._h = ctypes.c_void_p()
+        rc = lib().lvr_jpeg_encoder_create(device, ctypes.byref(self._h))
+        if rc != LVR_OK:
+            raise LivreError("lvr_jpeg_encoder_create failed: %s (%d)"
+                             % (lib().lvr_jpeg_error(None).decode(), rc))
+        self._buf = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().lvr_jpeg_encoder_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _run(self, fn, src, width, height, pitch, pixel_format, bottom_up, quality):
+        need = lib().lvr_jpeg_max_size(max(width, 1), max(height, 1))
+        if self._buf is None or self._buf.size < need:
+            self._buf = np.empty(need, np.uint8)
+        size = ctypes.c_size_t()
+        rc = fn(self._h, src, width, height, 4 * width if pitch is None else pitch,
+                pixel_format, int(bool(bottom_up)), quality,
+                self._buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), self._buf.size,
+                ctypes.byref(size))
+        if rc != LVR_OK:
+            raise LivreError("jpeg encode: %s (%d)" % (lib().lvr_jpeg_error(self._h).decode(), rc))
+        return self._buf[:size.value].tobytes()
+
+    def encode(self, pixels, pixel_format=PIXEL_RGBA, bottom_up=True, quality=100, width=None):
+        """JPEG file bytes of an (H, W, 4) uint8 host image (FrameGrabber's
+        input); `width` < W encodes the left columns with the image's pitch."""
+        px = np.ascontiguousarray(pixels, dtype=np.uint8)
+        assert px.ndim == 3 and px.shape[2] == 4
+        h, w = px.shape[:2]
+        return self._run(lib().lvr_jpeg_encode_host, px.ctypes.data_as(ctypes.c_void_p),
+                         w if width is None else width, h, 4 * w, pixel_format, bottom_up,
+                         quality)
+
+    def encode_device(self, device_ptr, width, height, pitch=None, pixel_format=PIXEL_RGBA,
+                      bottom_up=True, quality=100):
+        """JPEG file bytes of width x height 4-byte pixels at device_ptr."""
+        return self._run(lib().lvr_jpeg_encode, ctypes.c_void_p(int(device_ptr)), width, height,
+                         pitch, pixel_format, bottom_up, quality)
+
+    def stats(self):
+        st = JpegStats()
+        lib().lvr_jpeg_get_stats(self._h, ctypes.byref(st))
+        return st
 
 
 class RawSource:
@@ -651,6 +741,28 @@ class Context:
         s = FrameStats()
         self._check(lib().lvr_frame_stats_back(self._h, back, ctypes.byref(s)), "frame_stats")
         return s
+
+    def grab_jpeg(self, quality=100):
+        """The last frame as the reference's frame grabber encodes it."""
+        need = lib().lvr_jpeg_max_size(max(self.width, 1), max(self.height, 1))
+        buf = np.empty(need, np.uint8)
+        size = ctypes.c_size_t()
+        self._check(lib().lvr_frame_grab_jpeg(
+            self._h, quality, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), buf.size,
+            ctypes.byref(size)), "frame_grab_jpeg")
+        return buf[:size.value].tobytes()
+
+    def frame_device_ptr(self):
+        """(device pointer, width, height) of the framebuffer."""
+        p, w, h = ctypes.c_void_p(), ctypes.c_int(), ctypes.c_int()
+        self._check(lib().lvr_frame_device_ptr(self._h, ctypes.byref(p), ctypes.byref(w),
+                                               ctypes.byref(h)), "frame_device_ptr")
+        return p.value, w.value, h.value
+
+    def grab_stats(self):
+        st = JpegStats()
+        self._check(lib().lvr_frame_grab_stats(self._h, ctypes.byref(st)), "frame_grab_stats")
+        return st
 
     def histogram(self, ids, bins, read=True):
         ids, pids = _u64(ids)

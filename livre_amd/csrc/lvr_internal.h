@@ -300,6 +300,7 @@ struct lvr_ctx
     std::vector<int4> hostRanges;
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
     lvr::RawPyramid* pyramid = nullptr; // levels of the last raw source loaded
+    lvr_jpeg_encoder* jpeg = nullptr;   // frame grab encoder on `stream`
     // selection result (counters, then ids), written by the kernel into
     // pinned host memory; one selection in flight
     uint8_t* h_sel = nullptr;
@@ -378,6 +379,8 @@ int32_t slotAndTouch(lvr_ctx* ctx, uint64_t id);
 // Device levels of a raw source's LOD pyramid (raw.hip), freed with the
 // context or by lvr_pyramid_release.
 void destroyPyramid(lvr_ctx* ctx);
+// The frame grab's JPEG encoder (jpeg.hip), freed with the context.
+void destroyJpeg(lvr_ctx* ctx);
 int flushPageTable(lvr_ctx* ctx);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
 // Undo acquireSlotFor for a brick whose contents were never written.

@@ -494,6 +494,7 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     if (ctx->selStream)
         (void)hipStreamSynchronize(ctx->selStream);
     lvr::destroyPyramid(ctx);
+    lvr::destroyJpeg(ctx);
     (void)hipFree(ctx->d_pool);
     (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->pt.d_slots);

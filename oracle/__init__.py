@@ -95,6 +95,10 @@ def lib():
             ctypes.c_float, ctypes.c_float, u64p]
         _lib.orc_default_tf.argtypes = [f32p]
         _lib.orc_tf_lut.argtypes = [f32p, ctypes.c_int, P(ctypes.c_uint8)]
+        _lib.orc_jpeg_encode.argtypes = [
+            P(ctypes.c_uint8), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int, P(ctypes.c_uint8), ctypes.c_size_t,
+            P(ctypes.c_size_t)]
     return _lib
 
 
@@ -299,3 +303,21 @@ def pyramid_brick(levels, max_block, overlap, node_id):
     P = g.shape[0]
     idx = [np.clip(np.arange(max_block) + p * block - overlap, 0, P - 1) for p in pos]
     return g[np.ix_(idx[2], idx[1], idx[0])]
+
+
+def jpeg_encode(pixels, quality=100, bgra=False, bottom_up=True):
+    """JPEG file bytes of an (H, W, 4) uint8 image, as the frame grab writes
+    it (FrameGrabber.cpp:49-84): 4:4:4, bottom_up = row H-1 is the top."""
+    px = np.ascontiguousarray(pixels, dtype=np.uint8)
+    assert px.ndim == 3 and px.shape[2] == 4
+    h, w = px.shape[:2]
+    u8p = ctypes.POINTER(ctypes.c_uint8)
+    size = ctypes.c_size_t(0)
+    rc = lib().orc_jpeg_encode(px.ctypes.data_as(u8p), w, h, 4 * w, int(bgra), int(bottom_up),
+                               quality, None, 0, ctypes.byref(size))
+    assert rc == 0, "orc_jpeg_encode failed (%d)" % rc
+    out = np.empty(size.value, np.uint8)
+    rc = lib().orc_jpeg_encode(px.ctypes.data_as(u8p), w, h, 4 * w, int(bgra), int(bottom_up),
+                               quality, out.ctypes.data_as(u8p), out.size, ctypes.byref(size))
+    assert rc == 0, "orc_jpeg_encode failed (%d)" % rc
+    return out.tobytes()

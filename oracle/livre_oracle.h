@@ -12,6 +12,7 @@
 #ifndef LIVRE_ORACLE_H
 #define LIVRE_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -117,6 +118,15 @@ int orc_histogram(const orc_volume_info* vi, const uint64_t* ids, int n,
 /* TransferFunction1D default LUT and getLUT() truncation. */
 void orc_default_tf(float* rgba256);
 void orc_tf_lut(const float* rgba, int n, uint8_t* lut);
+
+/* Frame grab as JPEG (FrameGrabber.cpp:49-84 through libjpeg-turbo): baseline
+ * 4:4:4, accurate integer DCT, standard tables scaled to `quality`. Pixels are
+ * 4 bytes (RGBA, or BGRA when bgra != 0); bottom_up takes row height-1 first.
+ * *size receives the file length; out may be NULL to query it. Returns 0,
+ * -1 on bad arguments, -2 when cap is too small. */
+int orc_jpeg_encode(const uint8_t* pixels, int width, int height, int pitch,
+                    int bgra, int bottom_up, int quality, uint8_t* out,
+                    size_t cap, size_t* size);
 
 #ifdef __cplusplus
 }

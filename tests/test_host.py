@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def test_library_exports_every_declared_symbol(lvr):
     with open(os.path.join(ROOT, "include", "livre_amd.h")) as f:
         header = f.read()
-    declared = set(re.findall(r"^\s*(?:int|void|double|const char\*)\s+(lvr_\w+)\(",
+    declared = set(re.findall(r"^\s*(?:int|void|double|size_t|const char\*)\s+(lvr_\w+)\(",
                               header, re.M))
     assert declared == set(lvr.EXPORTS)
     L = lvr.lib()

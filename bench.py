@@ -614,6 +614,90 @@ def raw_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def jpeg_workload(args):
+    """Frame grab (FrameGrabber::_encodeJpeg: JPEG quality 100, 4:4:4, rows
+    bottom-up) of the 2048^3 orbit-zoom frames at 1920x1080 and 3840x2160:
+    each frame is rendered, then grabbed into host memory. A grab = encode on
+    the device + copy of the file to the host."""
+    import livre_amd
+    from livre_amd import scenes
+    livre_amd.lib()
+    vi, ctx, _ids, _tf = build_scene(livre_amd, scenes)
+    params = livre_amd.render_params(N_SAMPLES, early_exit=0.999)
+    frames = args.steps or 60
+    res, frames_host = {}, {}
+    for (w, h) in [(W, H), (3840, 2160)]:
+        dev, stage, e2e, size = [], [], [], []
+        for t in range(args.warmup + frames):
+            eye = scenes.orbit_zoom((t * FRAMES // frames) % FRAMES, FRAMES)
+            mv, proj = scenes.camera(eye, w, h, 60.0)
+            fr = livre_amd.frustum(mv, proj)
+            vis = ctx.select_visibles(fr, h, 1.0, 0, vi.depth - 1)
+            ctx.render(fr, w, h, livre_amd.order(vi, fr, vis), params)
+            ctx.sync()
+            t0 = time.perf_counter()
+            jpg = ctx.grab_jpeg(100)
+            t1 = time.perf_counter()
+            if t < args.warmup:
+                continue
+            st = ctx.grab_stats()
+            dev.append(st.device_ms)
+            stage.append((st.dct_ms, st.offsets_ms, st.emit_ms, st.stuff_ms))
+            e2e.append((t1 - t0) * 1e3)
+            size.append(len(jpg))
+            if t == args.warmup:
+                frames_host[(w, h)] = ctx.read_frame()[0]
+        sm = np.mean(np.array(stage), 0)
+        res["%dx%d" % (w, h)] = {
+            "grab_ms_mean": float(np.mean(e2e)), "grab_ms_p50": float(np.percentile(e2e, 50)),
+            "grab_ms_p99": float(np.percentile(e2e, 99)),
+            "device_ms_mean": float(np.mean(dev)),
+            "stage_ms_mean": {"dct": float(sm[0]), "offsets": float(sm[1]),
+                              "emit": float(sm[2]), "stuff": float(sm[3])},
+            "jpeg_bytes_mean": float(np.mean(size)),
+            "grabs_per_s": 1e3 / float(np.mean(e2e)),
+            "device_grabs_per_s": 1e3 / float(np.mean(dev)),
+            "input_gb_per_s": w * h * 4 / (float(np.mean(dev)) * 1e-3) / 1e9}
+        log("%dx%d: grab %.2f ms (device %.3f ms: dct %.3f, offsets %.3f, emit %.3f, "
+            "stuff %.3f), %.2f MB per file" % (w, h, np.mean(e2e), np.mean(dev), sm[0], sm[1],
+                                              sm[2], sm[3], np.mean(size) / 1e6))
+    ctx.close()
+    hd = res["%dx%d" % (W, H)]
+    # the dominant kernel: the DCT stage reads the frame and writes the
+    # quantised coefficients (2 B each) and one code length per block
+    blocks = 3 * ((W + 7) // 8) * ((H + 7) // 8)
+    dct_bytes = W * H * 4 + blocks * (64 * 2 + 4)
+    achieved = dct_bytes / (hd["stage_ms_mean"]["dct"] * 1e-3) / 1e9
+    result = {
+        "metric": "frame grab: JPEG frames/s at 1920x1080 (quality 100, 4:4:4)",
+        "value": hd["grabs_per_s"], "unit": "frames/s", "n_gpus": 1, "steps": frames,
+        "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic: frames of the 2048^3 uint16 value-noise orbit-zoom",
+        "config": {"workload": "render then grab each of %d frames of the orbit-zoom at "
+                               "1920x1080 and at 3840x2160; grab = device encode + file to "
+                               "host memory" % frames},
+        "sizes": res,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
+                     "kernel": "jpegDctKernel",
+                     "bytes_per_launch": "1920x1080x4 + 97200 blocks x (64 x 2 + 4)"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        px = frames_host[(W, H)]
+        t0, n = time.perf_counter(), 0
+        while time.perf_counter() - t0 < args.cpu_seconds or n == 0:
+            oracle.jpeg_encode(px, quality=100)
+            n += 1
+        el = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": n / el, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": "oracle JPEG restatement (scalar C++, one core) on the first timed "
+                      "1920x1080 frame, %d encodes in %.1f s" % (n, el)}
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -621,13 +705,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc", "raw"),
+    ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc", "raw", "jpeg"),
                     default="render",
                     help="render: the 1080p single-volume frame loop (default); "
                          "batch: uneven requests, plain vs one batched launch; "
                          "serve: 32 sessions under Poisson arrivals; "
                          "ooc: out-of-core 8192^3 fly-through; "
-                         "raw: a raw file's LOD pyramid built in HBM, then rendered")
+                         "raw: a raw file's LOD pyramid built in HBM, then rendered; "
+                         "jpeg: frame grab of rendered frames as JPEG")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--max-batch", type=int, default=2)
@@ -653,6 +738,9 @@ def main():
         return
     if args.workload == "raw":
         raw_workload(args)
+        return
+    if args.workload == "jpeg":
+        jpeg_workload(args)
         return
     if args.steps is None:
         args.steps = FRAMES

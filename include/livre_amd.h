@@ -560,7 +560,11 @@ int lvr_histogram_read(lvr_ctx* ctx, uint64_t* out, int bins);
 typedef struct lvr_jpeg_encoder lvr_jpeg_encoder;
 typedef struct
 {
-    float device_ms; /* kernels of the last encode (HIP events) */
+    float device_ms;  /* kernels of the last encode (HIP events), the sum of: */
+    float dct_ms;     /* colour conversion, DCT, quantisation, code lengths */
+    float offsets_ms; /* DC code lengths, bit offsets of the blocks */
+    float emit_ms;    /* Huffman codes into the bit stream */
+    float stuff_ms;   /* 0x00 after each 0xFF, EOI */
     int32_t width, height;
     uint64_t bytes; /* file length of the last encode */
 } lvr_jpeg_stats;

@@ -428,7 +428,9 @@ def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
 
 
 class JpegStats(ctypes.Structure):
-    _fields_ = [("device_ms", ctypes.c_float), ("width", ctypes.c_int32),
+    _fields_ = [("device_ms", ctypes.c_float), ("dct_ms", ctypes.c_float),
+                ("offsets_ms", ctypes.c_float), ("emit_ms", ctypes.c_float),
+                ("stuff_ms", ctypes.c_float), ("width", ctypes.c_int32),
                 ("height", ctypes.c_int32), ("bytes", ctypes.c_uint64)]
 
 

@@ -749,7 +749,7 @@ struct lvr_jpeg_encoder
     uint8_t* d_pixels = nullptr; // host images land here
     size_t pixelCap = 0;
     size_t capBlocks = 0;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipEvent_t ev[5] = {};
     lvr_jpeg_stats stats = {};
     std::string error;
 };
@@ -830,8 +830,8 @@ int encoderInit(lvr_jpeg_encoder* e, int device, hipStream_t stream)
     JPEG_HIP(e, hipHostMalloc((void**)&e->h_outBytes, sizeof(unsigned long long),
                               hipHostMallocMapped | hipHostMallocCoherent));
     JPEG_HIP(e, hipHostGetDevicePointer((void**)&e->d_hostOutBytes, e->h_outBytes, 0));
-    JPEG_HIP(e, hipEventCreate(&e->ev[0]));
-    JPEG_HIP(e, hipEventCreate(&e->ev[1]));
+    for (hipEvent_t& ev : e->ev)
+        JPEG_HIP(e, hipEventCreate(&ev));
     return LVR_OK;
 }
 
@@ -871,6 +871,7 @@ int encode(lvr_jpeg_encoder* e, const void* pixels, int width, int height, int p
                        static_cast<const uint8_t*>(pixels), width, height, pitch,
                        format == LVR_PIXEL_BGRA ? 1 : 0, bottomUp ? 1 : 0, nMcuX, nMcu,
                        e->d_tables, e->d_coef, e->d_blkBits);
+    JPEG_HIP(e, hipEventRecord(e->ev[1], e->stream));
     hipLaunchKernelGGL(lvr::jpegBitsKernel, dim3(nChunks), dim3(256), 0, e->stream, e->d_coef,
                        e->d_blkBits, nBlocks, e->d_tables, e->d_blkOff, e->d_chunkBits);
     hipLaunchKernelGGL(lvr::jpegChunkScanKernel, dim3(1), dim3(256), 0, e->stream,
@@ -878,9 +879,11 @@ int encode(lvr_jpeg_encoder* e, const void* pixels, int width, int height, int p
                        (unsigned long long)words);
     hipLaunchKernelGGL(lvr::jpegClearKernel, dim3(clearGrid), dim3(256), 0, e->stream, e->d_words,
                        e->d_meta);
+    JPEG_HIP(e, hipEventRecord(e->ev[2], e->stream));
     hipLaunchKernelGGL(lvr::jpegEmitKernel, dim3(mcuGrid), dim3(256), 0, e->stream, e->d_coef,
                        nMcu, e->d_tables, e->d_blkOff, e->d_chunkOff, e->d_words,
                        (unsigned long long)words);
+    JPEG_HIP(e, hipEventRecord(e->ev[3], e->stream));
     hipLaunchKernelGGL(lvr::jpegFfCountKernel, dim3(stuffGrid), dim3(256), 0, e->stream,
                        e->d_words, e->d_meta, e->d_ffChunk);
     hipLaunchKernelGGL(lvr::jpegFfScanKernel, dim3(1), dim3(256), 0, e->stream, e->d_ffChunk,
@@ -888,16 +891,21 @@ int encode(lvr_jpeg_encoder* e, const void* pixels, int width, int height, int p
     hipLaunchKernelGGL(lvr::jpegStuffKernel, dim3(stuffGrid), dim3(256), 0, e->stream,
                        e->d_words, e->d_meta, e->d_ffOff, e->d_out);
     JPEG_HIP(e, hipGetLastError());
-    JPEG_HIP(e, hipEventRecord(e->ev[1], e->stream));
+    JPEG_HIP(e, hipEventRecord(e->ev[4], e->stream));
     JPEG_HIP(e, hipStreamSynchronize(e->stream));
     const size_t scanBytes = size_t(*e->h_outBytes);
     if (scanBytes > words * 8 + 16)
         return jfail(e, LVR_EHIP, "jpeg: scan length out of range");
     const size_t total = hdr.size() + scanBytes;
     *size = total;
-    float ms = 0.f;
-    JPEG_HIP(e, hipEventElapsedTime(&ms, e->ev[0], e->ev[1]));
-    e->stats.device_ms = ms;
+    float ms[4] = {};
+    for (int i = 0; i < 4; ++i)
+        JPEG_HIP(e, hipEventElapsedTime(&ms[i], e->ev[i], e->ev[i + 1]));
+    e->stats.dct_ms = ms[0];
+    e->stats.offsets_ms = ms[1];
+    e->stats.emit_ms = ms[2];
+    e->stats.stuff_ms = ms[3];
+    e->stats.device_ms = ms[0] + ms[1] + ms[2] + ms[3];
     e->stats.bytes = total;
     e->stats.width = width;
     e->stats.height = height;

@@ -6,6 +6,7 @@
  *   livre::Renderer          livre/core/render/Renderer.h:38-109
  *   livre::DataSourcePlugin  livre/data/DataSourcePlugin.h:55-109 (getData)
  *   livre::RawDataSource     livre/data/RawDataSource.cpp:62-219
+ *   livre::FrameGrabber      livre/eq/FrameGrabber.cpp:35-110
  *   livre::RayCastRenderer   livre/eq/render/RayCastRenderer.h:31-74
  *   RenderPipeline::renderSync livre/lib/pipeline/RenderPipeline.cpp:92-158
  * Header-only over the C ABI (include/livre_amd.h); link with liblivre_amd.so.
@@ -290,6 +291,54 @@ protected:
 /** Drop-in for RayCastRenderer: same constructor meaning (samples per ray,
  * 0 = derived from the data as onFrameStart does), same order(), the frame
  * rendered by the HIP kernels into an RGBA8 framebuffer on the device. */
+/** FrameGrabber (livre/eq/FrameGrabber.cpp:35-110): frames as the JPEG files
+ * _encodeJpeg writes through tjCompress2 (4:4:4, quality 100, rows taken
+ * bottom-up), encoded on the device. */
+class FrameGrabber
+{
+public:
+    explicit FrameGrabber(int device = 0)
+    {
+        if (lvr_jpeg_encoder_create(device, &_enc) != LVR_OK)
+            throw std::runtime_error(std::string("FrameGrabber: ") + lvr_jpeg_error(nullptr));
+    }
+    ~FrameGrabber() { lvr_jpeg_encoder_destroy(_enc); }
+    FrameGrabber(const FrameGrabber&) = delete;
+    FrameGrabber& operator=(const FrameGrabber&) = delete;
+
+    /** _encodeJpeg(width, height, rawData, dataSize): BGRA host pixels of a
+     * read-back image, row 0 at the bottom. */
+    std::vector<uint8_t> encodeJpeg(uint32_t width, uint32_t height, const uint8_t* rawData,
+                                    int quality = 100)
+    {
+        std::vector<uint8_t> out(lvr_jpeg_max_size(int(width), int(height)));
+        size_t size = 0;
+        if (lvr_jpeg_encode_host(_enc, rawData, int(width), int(height), int(4 * width),
+                                 LVR_PIXEL_BGRA, 1, quality, out.data(), out.size(),
+                                 &size) != LVR_OK)
+            throw std::runtime_error(std::string("encodeJpeg: ") + lvr_jpeg_error(_enc));
+        out.resize(size);
+        return out;
+    }
+
+    /** The pool's last rendered frame, encoded where it lies in HBM. */
+    static std::vector<uint8_t> grab(const BrickPool& pool, int quality = 100)
+    {
+        void* ptr = nullptr;
+        int w = 0, h = 0;
+        check(lvr_frame_device_ptr(pool.ctx(), &ptr, &w, &h), "grab", pool.ctx());
+        std::vector<uint8_t> out(lvr_jpeg_max_size(w, h));
+        size_t size = 0;
+        check(lvr_frame_grab_jpeg(pool.ctx(), quality, out.data(), out.size(), &size), "grab",
+              pool.ctx());
+        out.resize(size);
+        return out;
+    }
+
+private:
+    lvr_jpeg_encoder* _enc = nullptr;
+};
+
 class HipRayCastRenderer : public Renderer
 {
 public:

@@ -7,7 +7,8 @@
  *   render_frame OUT_PREFIX [n voxels] [block] [width] [height]
  *   render_frame --host-check [nucleon.raw]  (host-side API only, no GPU)
  *
- * OUT_PREFIX.rgba (w*h*4 bytes, row 0 = bottom), OUT_PREFIX.hist (u64 bins),
+ * OUT_PREFIX.rgba (w*h*4 bytes, row 0 = bottom), OUT_PREFIX.jpg (the frame grab),
+ * OUT_PREFIX.hist (u64 bins),
  * OUT_PREFIX.txt (bricks, samples, camera matrices).
  */
 #include <livre/hip/Renderer.h>
@@ -174,6 +175,16 @@ int main(int argc, char** argv)
         const std::vector<uint8_t> img = renderer.readFrame(&samples);
         std::ofstream(out + ".rgba", std::ios::binary)
             .write(reinterpret_cast<const char*>(img.data()), std::streamsize(img.size()));
+        // frame grab: from HBM, and from a BGRA host copy as Equalizer hands it over
+        const std::vector<uint8_t> jpg = livre::FrameGrabber::grab(pool);
+        std::vector<uint8_t> bgra(img);
+        for (size_t i = 0; i < bgra.size(); i += 4)
+            std::swap(bgra[i], bgra[i + 2]);
+        livre::FrameGrabber grabber;
+        if (grabber.encodeJpeg(uint32_t(w), uint32_t(h), bgra.data()) != jpg)
+            throw std::runtime_error("frame grab: host and device encodes differ");
+        std::ofstream(out + ".jpg", std::ios::binary)
+            .write(reinterpret_cast<const char*>(jpg.data()), std::streamsize(jpg.size()));
         std::ofstream(out + ".hist", std::ios::binary)
             .write(reinterpret_cast<const char*>(hist.data()),
                    std::streamsize(hist.size() * sizeof(uint64_t)));

@@ -46,6 +46,8 @@ def test_cpp_pipeline_frame_matches_oracle(driver, orc, tmp_path):
     proj = np.array(meta["proj"], np.float32)
     visible = np.array([int(x) for x in meta["visible"]], np.uint64)
     img = np.fromfile(prefix + ".rgba", np.uint8).reshape(h, w, 4)
+    with open(prefix + ".jpg", "rb") as f:
+        assert f.read() == orc.jpeg_encode(img, quality=100)
     hist = np.fromfile(prefix + ".hist", np.uint64)
 
     vi = livre_amd.volume_info(n, block, 1, livre_amd.DT_UINT16)

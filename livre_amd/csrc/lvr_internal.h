@@ -80,11 +80,19 @@ struct BrickEntry
     float bmin[3];
     float bmax[3];
     float ext[3];   // bmax - bmin, rounded as the shader's (aabbMax - aabbMin)
-    float rcp[3];   // 1 / ext, correctly rounded (host division)
+    // 1 / ext, correctly rounded (host division); with kBrickPow2 set,
+    // (1 / ext) * texExt, exact because ext is a power of two
+    float rcp[3];
     int32_t slot;   // pool slot, -1 if not resident
     uint32_t level;
-    uint32_t pad[2];
+    uint32_t flags;
+    uint32_t pad;
 };
+
+// BrickEntry::flags: every extent is a power of two, so the texture position
+// (pos - aabbMin) / ext * texExt is one multiply by rcp, rounded once, exactly
+// as the division and the multiply are.
+constexpr uint32_t kBrickPow2 = 1u;
 
 // Dense device page table over the octree: per level,
 // levelOffset[L] + x + nx * (y + ny * z) holds a slot index or -1.

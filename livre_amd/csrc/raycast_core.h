@@ -169,6 +169,10 @@ __device__ inline float sampleBrick(const T* __restrict__ vox, const FrameUnifor
 // position: (pos - aabbMin) / (aabbMax - aabbMin) correctly rounded, from the
 // host's IEEE reciprocal of the divisor and one FMA correction step
 // (Markstein), then * (texMax - texMin) + texMin.
+// With P2 (the brick's extents are powers of two, BrickEntry::kBrickPow2)
+// the quotient is exact and b.rcp holds (1 / ext) * texExt, so one multiply
+// gives the same rounded product.
+template <bool P2>
 __device__ __forceinline__ void texturePosition(const FrameUniforms& U, const BrickEntry& b,
                                                 const float texExt[3], const float pos[3],
                                                 float tp[3])
@@ -177,20 +181,25 @@ __device__ __forceinline__ void texturePosition(const FrameUniforms& U, const Br
     for (int a = 0; a < 3; ++a)
     {
         const float x = pos[a] - b.bmin[a];
-        const float q = x * b.rcp[a];
-        const float r = __builtin_fmaf(-q, b.ext[a], x);
-        const float quot = __builtin_fmaf(r, b.rcp[a], q);
-        tp[a] = quot * texExt[a] + U.texMin[a];
+        if (P2)
+            tp[a] = x * b.rcp[a] + U.texMin[a];
+        else
+        {
+            const float q = x * b.rcp[a];
+            const float r = __builtin_fmaf(-q, b.ext[a], x);
+            const float quot = __builtin_fmaf(r, b.rcp[a], q);
+            tp[a] = quot * texExt[a] + U.texMin[a];
+        }
     }
 }
 
 // Byte offset in the slot of the GL_NEAREST texel of a sample position.
-template <typename T>
+template <typename T, bool P2>
 __device__ __forceinline__ uint32_t nearestOffset(const FrameUniforms& U, const BrickEntry& b,
                                                   const float texExt[3], const float pos[3])
 {
     float tp[3];
-    texturePosition(U, b, texExt, pos, tp);
+    texturePosition<P2>(U, b, texExt, pos, tp);
     const uint32_t ix = nearestTexel(tp[0] * U.texDimF[0], U.texLastF[0]);
     const uint32_t iy = nearestTexel(tp[1] * U.texDimF[1], U.texLastF[1]);
     const uint32_t iz = nearestTexel(tp[2] * U.texDimF[2], U.texLastF[2]);
@@ -294,7 +303,7 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
 // on a load. Fetches past the last composited sample (end of segment or early
 // exit) are discarded; their texel indices are clamped into the brick like
 // every other.
-template <typename T, int D, bool CLS>
+template <typename T, int D, bool CLS, bool P2>
 __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
                                              const BrickEntry& b, const float texExt[3],
                                              float pos[3], const float step[3], float travel,
@@ -306,14 +315,14 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
         return;
     int guard = 1 << 22;
     T raw[S];
-    raw[0] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+    raw[0] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
 #pragma unroll
     for (int i = 1; i < S; ++i)
     {
         pos[0] = pos[0] + step[0];
         pos[1] = pos[1] + step[1];
         pos[2] = pos[2] + step[2];
-        raw[i] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+        raw[i] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
     }
     for (bool more = true; more;)
     {
@@ -338,7 +347,7 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                const uint32_t next = nearestOffset<T>(U, b, texExt, pos);
+                const uint32_t next = nearestOffset<T, P2>(U, b, texExt, pos);
                 // the oldest voxel has been used before its register takes
                 // the next load; the load stays above this sample's blend
                 __builtin_amdgcn_sched_barrier(0);
@@ -357,7 +366,7 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                raw[0] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+                raw[0] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
             }
         }
     }
@@ -380,7 +389,7 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
 // of the pass. Its extra fetches are clamped into the brick like every other.
 // Per sample this leaves the position step, the texel address and the blend;
 // the table holds exactly what the per-sample classification computes.
-template <typename T, int D>
+template <typename T, int D, bool P2>
 __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
                                            const BrickEntry& b, const float texExt[3],
                                            float pos[3], const float step[3], float travel,
@@ -391,7 +400,7 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
     if (!(travel > 0.0f))
         return;
     int guard = 1 << 22;
-    const uint32_t first = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+    const uint32_t first = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
     // the voxel of sample m (m >= 1) of the segment is raw[(m - 1) % S]; one
     // full register each (two 16-bit voxels packed in one register would
     // have to wait for both loads to merge them)
@@ -402,7 +411,7 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
         pos[0] = pos[0] + step[0];
         pos[1] = pos[1] + step[1];
         pos[2] = pos[2] + step[2];
-        raw[i] = loadSlot<T>(vox, nearestOffset<T>(U, b, texExt, pos));
+        raw[i] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
     }
     float4 cls = tab[first];
     bool alive = true;
@@ -427,7 +436,7 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
             pos[0] = pos[0] + step[0];
             pos[1] = pos[1] + step[1];
             pos[2] = pos[2] + step[2];
-            const uint32_t next = nearestOffset<T>(U, b, texExt, pos);
+            const uint32_t next = nearestOffset<T, P2>(U, b, texExt, pos);
             __builtin_amdgcn_sched_barrier(0);
             raw[(u + S - 1) % S] = loadSlot<T>(vox, next);
             __builtin_amdgcn_sched_barrier(0);
@@ -561,13 +570,28 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         float pos[3] = {rs[0], rs[1], rs[2]};
         const T* vox = reinterpret_cast<const T*>(U.pool + size_t(b.slot) * U.slotStride);
 
+        // the brick's flags are wave-uniform (scalar loads): no divergence
+        const bool p2 = (b.flags & kBrickPow2) != 0;
         if (!LINEAR && CLS)
-            marchTable<T, D>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt, pos,
-                             step, segLen, stepSize, C, tf, nsamp);
+        {
+            if (p2)
+                marchTable<T, D, true>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt,
+                                       pos, step, segLen, stepSize, C, tf, nsamp);
+            else
+                marchTable<T, D, false>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt,
+                                        pos, step, segLen, stepSize, C, tf, nsamp);
+        }
         else if (!LINEAR)
-            marchNearest<T, D, false>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt,
-                                      pos, step, segLen, stepSize, C, tf,
-                                      nsamp);
+        {
+            if (p2)
+                marchNearest<T, D, false, true>(slotResource(vox, uint32_t(U.slotStride)), U, b,
+                                                texExt, pos, step, segLen, stepSize, C, tf,
+                                                nsamp);
+            else
+                marchNearest<T, D, false, false>(slotResource(vox, uint32_t(U.slotStride)), U, b,
+                                                 texExt, pos, step, segLen, stepSize, C, tf,
+                                                 nsamp);
+        }
         else
         {
             // A segment lies inside one brick, so it holds at most
@@ -577,7 +601,10 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
             for (float travel = segLen; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
             {
                 float tp[3];
-                texturePosition(U, b, texExt, pos, tp);
+                if (p2)
+                    texturePosition<true>(U, b, texExt, pos, tp);
+                else
+                    texturePosition<false>(U, b, texExt, pos, tp);
                 ++nsamp;
                 if (C.add(U, tf, sampleBrick<T, true>(vox, U, tp)))
                     break;

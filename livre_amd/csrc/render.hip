@@ -574,10 +574,20 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
         std::memset(&e, 0, sizeof e);
         int4 r = make_int4(1, 1, 0, 0); // empty range
         nodeWorldBox(vi, key, e.bmin, e.bmax);
+        bool pow2 = true;
         for (int a = 0; a < 3; ++a)
         {
             e.ext[a] = e.bmax[a] - e.bmin[a];
             e.rcp[a] = 1.0f / e.ext[a];
+            int ex = 0;
+            pow2 = pow2 && std::isnormal(e.ext[a]) && std::frexp(e.ext[a], &ex) == 0.5f &&
+                   std::isnormal(e.rcp[a] * U->texExt[a]);
+        }
+        if (pow2)
+        {
+            for (int a = 0; a < 3; ++a)
+                e.rcp[a] = e.rcp[a] * U->texExt[a]; // power-of-two scaling: exact
+            e.flags = kBrickPow2;
         }
         e.level = key.level;
         e.slot = slotAndTouch(ctx, ids[b]);

@@ -379,7 +379,10 @@ int lvr_batch_set_graph(lvr_batch* batch, int enable);
 /* Queue one frame of ctx's volume; request receives its index. The TF is
  * n_tf FP32 RGBA entries over [range0, range1] (as lvr_tf_set). Adding after
  * a run starts a new batch. One launch renders all requests, so their
- * samples_per_thread and waves_per_simd must agree (LVR_EINVAL otherwise). */
+ * waves_per_simd must agree, and so must samples_per_thread among requests
+ * of the same voxel type (8-bit, 16-bit, float; LVR_EINVAL otherwise). The
+ * batch pipelines 2 or 3 fetches per ray (1 runs as 2, with the same
+ * samples). */
 int lvr_batch_add(lvr_batch* batch, lvr_ctx* ctx, const lvr_frustum* f,
                   const int viewport[4], const uint64_t* ordered_ids, int n,
                   const lvr_render_params* params, const float* tf_rgba,

@@ -24,7 +24,7 @@ import json
 import math
 import os
 
-from . import RenderParams, lib
+from . import DT_UINT8, DT_UINT16, RenderParams, lib
 
 ARCH = "gfx950"
 SHIPPED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_%s.json" % ARCH)
@@ -43,6 +43,11 @@ def request_class(vi, width, height, params):
     return "%s/dt%d/px%d/rate%+d/%s" % (
         ARCH, vi.data_type, int(round(math.log2(max(1, width * height)))),
         int(round(math.log2(max(rate, 1e-6)))), "linear" if params.linear_filter else "nearest")
+
+
+def _type_class(data_type):
+    """The batch kernel's voxel-type branch: 8-bit, 16-bit or float."""
+    return {DT_UINT8: 0, DT_UINT16: 1}.get(int(data_type), 2)
 
 
 class Tuner:
@@ -97,17 +102,27 @@ class Tuner:
 
     def tune_batch(self, requests):
         """requests: list of (ctx, fr, width, height, ordered, params).
-        Returns one RenderParams per request: the class's tile, and the
-        batch-wide samples per thread and wavefronts per SIMD."""
+        Returns one RenderParams per request: the class's tile, samples per
+        thread by pixel-weighted vote per voxel type (8-bit, 16-bit, float:
+        the batch kernel's branches) and wavefronts per SIMD by vote over
+        the batch."""
         tuned = [self.params_for(*r) for r in requests]
-        for field, nearest_only in (("samples_per_thread", True), ("waves_per_simd", False)):
+
+        def vote(field, group):
             votes = {}
-            for (_ctx, _fr, w, h, _o, _p), t in zip(requests, tuned):
-                if not (nearest_only and t.linear_filter):
+            for (ctx, _fr, w, h, _o, _p), t in zip(requests, tuned):
+                g = group(ctx)
+                if g is not None and not (field == "samples_per_thread" and t.linear_filter):
                     v = getattr(t, field)
-                    votes[v] = votes.get(v, 0) + w * h
-            if votes:
-                best = max(votes.items(), key=lambda kv: kv[1])[0]
-                for t in tuned:
-                    setattr(t, field, best)
+                    votes.setdefault(g, {})
+                    votes[g][v] = votes[g].get(v, 0) + w * h
+            for (ctx, *_r), t in zip(requests, tuned):
+                g = group(ctx)
+                if g in votes:
+                    setattr(t, field, max(votes[g].items(), key=lambda kv: kv[1])[0])
+
+        # samples per thread per voxel type (the batch kernel's branches),
+        # wavefronts per SIMD for the whole launch
+        vote("samples_per_thread", lambda ctx: _type_class(ctx.vi.data_type))
+        vote("waves_per_simd", lambda ctx: 0)
         return tuned

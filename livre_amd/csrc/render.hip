@@ -254,14 +254,24 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
 // Largest transfer function a batched request stages in LDS (per wavefront).
 constexpr int kBatchTf = 256;
 
+// The batch kernel's voxel-type branches (8-bit, 16-bit, float) and the
+// fetch depths it is built with (a depth of 1 runs as 2: the same samples).
+inline int batchTypeClass(int dataType)
+{
+    return dataType == LVR_DT_UINT8 ? 0 : (dataType == LVR_DT_UINT16 ? 1 : 2);
+}
+inline int batchDepth(int depth) { return depth >= 3 ? 3 : 2; }
+
 
 // Persistent batch kernel: the work items of all frames of a batch form one
 // queue. A wavefront looks up the frame of each item in the request table,
 // restages that frame's transfer function in its own LDS slice when the
 // frame changes (tables above kBatchTf entries are read from global memory),
-// and dispatches on the frame's voxel type. Composite steps are counted per
-// frame. A workgroup holds 2 * wavesPerSimd (4 to 16) wavefronts.
-template <int D>
+// and dispatches on the frame's voxel type, each type with its own number of
+// fetches in flight (D8, D16, DF: the autotuner's choice for that type).
+// Composite steps are counted per frame. A workgroup holds 2 * wavesPerSimd
+// (4 to 16) wavefronts.
+template <int D8, int D16, int DF>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void batchKernel(
     const FrameUniforms* __restrict__ reqs, int nReq, int totalItems,
     const BrickEntry* __restrict__ bricks, const int32_t* __restrict__ offsets,
@@ -320,24 +330,24 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         {
         case LVR_DT_UINT8:
             if (U.classify == 1)
-                castTile<uint8_t, false, D, true>(U, tile, lane, bricks, offsets, counts, lists,
+                castTile<uint8_t, false, D8, true>(U, tile, lane, bricks, offsets, counts, lists,
                                                   myTf, nsamp);
             else if (lds)
-                castTile<uint8_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                castTile<uint8_t, false, D8>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
-                castTile<uint8_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                castTile<uint8_t, false, D8>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
             break;
         case LVR_DT_UINT16:
             if (lds)
-                castTile<uint16_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                castTile<uint16_t, false, D16>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
-                castTile<uint16_t, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                castTile<uint16_t, false, D16>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
             break;
         default:
             if (lds)
-                castTile<float, false, D>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                castTile<float, false, DF>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
             else
-                castTile<float, false, D>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                castTile<float, false, DF>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
             break;
         }
     }
@@ -1223,12 +1233,15 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
         b->ranges.resize(nb);
         return rc;
     }
-    if (!b->reqs.empty() && U.depth != b->reqs.front().depth)
-    {
-        b->bricks.resize(nb);
-        b->ranges.resize(nb);
-        return bfail(b, LVR_EINVAL, "samples per thread differs from the batch's");
-    }
+    for (const FrameUniforms& o : b->reqs)
+        if (batchTypeClass(o.dataType) == batchTypeClass(U.dataType) &&
+            batchDepth(o.depth) != batchDepth(U.depth))
+        {
+            b->bricks.resize(nb);
+            b->ranges.resize(nb);
+            return bfail(b, LVR_EINVAL,
+                         "samples per thread differs from the batch's for this voxel type");
+        }
     if (!b->reqs.empty() && U.wavesPerSimd != b->reqs.front().wavesPerSimd)
     {
         b->bricks.resize(nb);
@@ -1321,10 +1334,14 @@ int lvr_batch_run(lvr_batch* b)
     }
 
     // the launch sequence depends on these sizes and buffers only
-    const int depth = b->reqs.front().depth;
+    int depthOf[3] = {2, 2, 2};
+    for (const FrameUniforms& r : b->reqs)
+        depthOf[batchTypeClass(r.dataType)] = batchDepth(r.depth);
+    const int depthCode = (depthOf[0] == 3 ? 4 : 0) + (depthOf[1] == 3 ? 2 : 0) +
+                          (depthOf[2] == 3 ? 1 : 0);
     const int wps = b->reqs.front().wavesPerSimd;
     std::vector<int64_t> key = {nReq,
-                                depth,
+                                depthCode,
                                 wps,
                                 int64_t(b->bricks.size()),
                                 int64_t(b->tf.size()),
@@ -1380,15 +1397,21 @@ int lvr_batch_run(lvr_batch* b)
         const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
         const int blocks = std::max(1, std::min((items + 2 * wps - 1) / (2 * wps), slots));
         const dim3 grid(blocks), block(2 * wps * kWave);
-        if (depth == 1)
-            hipLaunchKernelGGL(batchKernel<1>, grid, block, 0, st, b->d_reqs, nReq, items,
-                               b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);
-        else if (depth == 3)
-            hipLaunchKernelGGL(batchKernel<3>, grid, block, 0, st, b->d_reqs, nReq, items,
-                               b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);
-        else
-            hipLaunchKernelGGL(batchKernel<2>, grid, block, 0, st, b->d_reqs, nReq, items,
-                               b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters);
+#define LVR_BATCH(a, c, d)                                                                   \
+    hipLaunchKernelGGL((batchKernel<a, c, d>), grid, block, 0, st, b->d_reqs, nReq, items,      \
+                       b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters)
+        switch (depthCode)
+        {
+        case 0: LVR_BATCH(2, 2, 2); break;
+        case 1: LVR_BATCH(2, 2, 3); break;
+        case 2: LVR_BATCH(2, 3, 2); break;
+        case 3: LVR_BATCH(2, 3, 3); break;
+        case 4: LVR_BATCH(3, 2, 2); break;
+        case 5: LVR_BATCH(3, 2, 3); break;
+        case 6: LVR_BATCH(3, 3, 2); break;
+        default: LVR_BATCH(3, 3, 3); break;
+        }
+#undef LVR_BATCH
         return hipGetLastError();
     };
 

@@ -60,13 +60,17 @@ def test_cache_hits_persist_and_vote(lvr, tmp_path):
     del t2.table[key]["waves_per_simd"]
     t2.params_for(u8, None, 512, 512, [], p)
     assert u8.calls == 2 and t2.table[key]["waves_per_simd"] == 6
-    # batch: tiles per class, samples per thread and waves per SIMD by
-    # pixel-weighted vote
+    # batch: tiles per class, samples per thread by pixel-weighted vote per
+    # voxel type, waves per SIMD by vote over the batch
+    u8b = FakeCtx(u8.vi, (8, 8, 2, 8))
+    t2.table[autotune.request_class(u8.vi, 2048, 2048, p)] = {
+        "tile": [8, 8], "samples_per_thread": 2, "classify": 0, "waves_per_simd": 8}
     reqs = [(u8, None, 512, 512, [], p), (f32, None, 2048, 2048, [], p),
-            (u8, None, 512, 512, [], p)]
+            (u8, None, 512, 512, [], p), (u8b, None, 2048, 2048, [], p)]
     tuned = t2.tune_batch(reqs)
-    assert [(x.tile_w, x.tile_h) for x in tuned] == [(16, 4), (4, 16), (16, 4)]
-    assert {x.samples_per_thread for x in tuned} == {1}
+    assert [(x.tile_w, x.tile_h) for x in tuned] == [(16, 4), (4, 16), (16, 4), (8, 8)]
+    # u8: 2048^2 pixels voted 2 against 2 x 512^2 for 3; f32 alone keeps 1
+    assert [x.samples_per_thread for x in tuned] == [2, 1, 2, 2]
     assert {x.waves_per_simd for x in tuned} == {8}
     assert p.samples_per_thread == 0 and p.waves_per_simd == 0  # inputs untouched
 

@@ -106,7 +106,8 @@ def test_batch_rejects_what_it_does_not_render(volumes):
         with pytest.raises(livre_amd.LivreError):
             b.add(ctx, fr, 32, 32, s.ids[:4], livre_amd.render_params(32, tile=(5, 5)),
                   scenes.piecewise_tf(256, 1), s.range)
-        # one samples-per-thread value per batch (it selects the kernel)
+        # one samples-per-thread value per voxel type in a batch (it selects
+        # the kernel)
         b.add(ctx, fr, 32, 32, s.ids[:4], livre_amd.render_params(32, samples_per_thread=1),
               scenes.piecewise_tf(256, 1), s.range)
         with pytest.raises(livre_amd.LivreError):

@@ -747,7 +747,9 @@ class Context:
     def grab_jpeg(self, quality=100):
         """The last frame as the reference's frame grabber encodes it."""
         need = lib().lvr_jpeg_max_size(max(self.width, 1), max(self.height, 1))
-        buf = np.empty(need, np.uint8)
+        buf = getattr(self, "_jpeg_buf", None)
+        if buf is None or buf.size < need:  # kept: its pages stay mapped
+            buf = self._jpeg_buf = np.empty(need, np.uint8)
         size = ctypes.c_size_t()
         self._check(lib().lvr_frame_grab_jpeg(
             self._h, quality, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), buf.size,

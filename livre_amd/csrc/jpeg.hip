@@ -748,6 +748,8 @@ struct lvr_jpeg_encoder
     unsigned long long* d_hostOutBytes = nullptr;
     uint8_t* d_pixels = nullptr; // host images land here
     size_t pixelCap = 0;
+    uint8_t* h_scan = nullptr; // pinned landing of the scan (grow-only)
+    size_t hostCap = 0;
     size_t capBlocks = 0;
     hipEvent_t ev[5] = {};
     lvr_jpeg_stats stats = {};
@@ -915,9 +917,22 @@ int encode(lvr_jpeg_encoder* e, const void* pixels, int width, int height, int p
         return jfail(e, LVR_ERANGE, "jpeg: output buffer too small (" + std::to_string(total) +
                                         " bytes needed)");
     std::memcpy(out, hdr.data(), hdr.size());
-    JPEG_HIP(e, hipMemcpyAsync(out + hdr.size(), e->d_out, scanBytes, hipMemcpyDeviceToHost,
+    // the scan comes back by DMA into pinned memory, then one host copy
+    // (a copy into pageable memory would be staged in small pieces)
+    if (scanBytes > e->hostCap)
+    {
+        if (e->h_scan)
+            (void)hipHostFree(e->h_scan);
+        e->h_scan = nullptr;
+        e->hostCap = 0;
+        const size_t cap2 = std::max(scanBytes, size_t(1) << 20) * 2;
+        JPEG_HIP(e, hipHostMalloc((void**)&e->h_scan, cap2, hipHostMallocDefault));
+        e->hostCap = cap2;
+    }
+    JPEG_HIP(e, hipMemcpyAsync(e->h_scan, e->d_out, scanBytes, hipMemcpyDeviceToHost,
                                e->stream));
     JPEG_HIP(e, hipStreamSynchronize(e->stream));
+    std::memcpy(out + hdr.size(), e->h_scan, scanBytes);
     return LVR_OK;
 }
 } // namespace
@@ -960,6 +975,8 @@ extern "C" void lvr_jpeg_encoder_destroy(lvr_jpeg_encoder* e)
         (void)hipFree(e->d_meta);
     if (e->d_pixels)
         (void)hipFree(e->d_pixels);
+    if (e->h_scan)
+        (void)hipHostFree(e->h_scan);
     if (e->h_outBytes)
         (void)hipHostFree(e->h_outBytes);
     for (hipEvent_t ev : e->ev)

@@ -148,29 +148,56 @@ __global__ __launch_bounds__(1024) void scanKernel(const int32_t* __restrict__ c
                                                    unsigned long long* __restrict__ counters,
                                                    int nCounters)
 {
-    __shared__ int32_t partial[1024];
+    // one workgroup: exclusive scan in passes of 4096 counts (4 per thread,
+    // read and written coalesced), wavefront shuffles, then the 16
+    // wavefront totals through LDS; a running carry joins the passes
+    __shared__ int32_t waveTotal[16];
     for (int i = threadIdx.x; i < nCounters; i += blockDim.x)
         counters[i] = 0;
-    const int per = (n + 1023) / 1024;
-    const int begin = threadIdx.x * per;
-    const int end = min(n, begin + per);
-    int s = 0;
-    for (int i = begin; i < end; ++i)
-        s += counts[i];
-    partial[threadIdx.x] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1)
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    int carry = 0;
+    for (int base = 0; base < n; base += 4096)
     {
-        const int v = threadIdx.x >= off ? partial[threadIdx.x - off] : 0;
+        const int i0 = base + 4 * t;
+        int v[4];
+        if (i0 + 3 < n)
+        {
+            const int4 q = *reinterpret_cast<const int4*>(counts + i0);
+            v[0] = q.x;
+            v[1] = q.y;
+            v[2] = q.z;
+            v[3] = q.w;
+        }
+        else
+            for (int k = 0; k < 4; ++k)
+                v[k] = i0 + k < n ? counts[i0 + k] : 0;
+        const int mine = v[0] + v[1] + v[2] + v[3];
+        int inc = mine;
+        for (int d = 1; d < 64; d <<= 1)
+        {
+            const int o = __shfl_up(inc, d, 64);
+            if (lane >= d)
+                inc += o;
+        }
+        if (lane == 63)
+            waveTotal[wave] = inc;
         __syncthreads();
-        partial[threadIdx.x] += v;
+        int before = 0, total = 0;
+        for (int w = 0; w < 16; ++w)
+        {
+            const int x = waveTotal[w];
+            before += w < wave ? x : 0;
+            total += x;
+        }
+        int run = carry + before + inc - mine;
+        for (int k = 0; k < 4; ++k)
+        {
+            if (i0 + k < n)
+                offsets[i0 + k] = run;
+            run += v[k];
+        }
+        carry += total;
         __syncthreads();
-    }
-    int run = threadIdx.x ? partial[threadIdx.x - 1] : 0;
-    for (int i = begin; i < end; ++i)
-    {
-        offsets[i] = run;
-        run += counts[i];
     }
 }
 

@@ -299,10 +299,10 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
         if (need > ctx->histRingBytes[r])
         {
             if (ctx->h_histRing[r])
-                (void)hipHostFree(ctx->h_histRing[r]);
+                ctx->retiredHost.push_back(ctx->h_histRing[r]);
             ctx->h_histRing[r] = nullptr;
+            const size_t cap = std::max(need + need / 4 + 4096, 2 * ctx->histRingBytes[r]);
             ctx->histRingBytes[r] = 0;
-            const size_t cap = need + need / 4 + 4096;
             LVR_HIP(ctx, hipHostMalloc((void**)&ctx->h_histRing[r], cap,
                                        hipHostMallocMapped | hipHostMallocCoherent));
             ctx->histRingBytes[r] = cap;

@@ -228,6 +228,9 @@ struct PinnedRing
     hipEvent_t done[2] = {nullptr, nullptr};
     bool armed[2] = {false, false};
     int next = 0;
+    // outgrown buffers: hipHostFree waits for the whole device, so they are
+    // kept until destroy()
+    std::vector<uint8_t*> retired;
 
     // a buffer of at least `bytes` whose previous copies have completed;
     // *slot names it for release()
@@ -309,6 +312,11 @@ struct lvr_ctx
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
     lvr::RawPyramid* pyramid = nullptr; // levels of the last raw source loaded
     lvr_jpeg_encoder* jpeg = nullptr;   // frame grab encoder on `stream`
+    // device buffers replaced while queued frames may still read them, and
+    // outgrown pinned buffers (hipHostFree would wait for the whole device);
+    // freed where every stream has drained
+    std::vector<void*> retired;
+    std::vector<void*> retiredHost;
     // selection result (counters, then ids), written by the kernel into
     // pinned host memory; one selection in flight
     uint8_t* h_sel = nullptr;

@@ -64,10 +64,10 @@ hipError_t PinnedRing::acquire(size_t bytes, uint8_t** out, int* slot)
     if (bytes > cap[r])
     {
         if (buf[r])
-            (void)hipHostFree(buf[r]);
+            retired.push_back(buf[r]);
         buf[r] = nullptr;
+        const size_t n = std::max(bytes + bytes / 4 + 4096, 2 * cap[r]);
         cap[r] = 0;
-        const size_t n = bytes + bytes / 4 + 4096;
         if ((e = hipHostMalloc((void**)&buf[r], n, hipHostMallocDefault)) != hipSuccess)
             return e;
         cap[r] = n;
@@ -99,6 +99,9 @@ void PinnedRing::destroy()
         cap[r] = 0;
         armed[r] = false;
     }
+    for (uint8_t* b : retired)
+        (void)hipHostFree(b);
+    retired.clear();
 }
 
 int32_t slotAndTouch(lvr_ctx* ctx, uint64_t id)
@@ -495,6 +498,12 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamSynchronize(ctx->selStream);
     lvr::destroyPyramid(ctx);
     lvr::destroyJpeg(ctx);
+    for (void* p : ctx->retired)
+        (void)hipFree(p);
+    ctx->retired.clear();
+    for (void* p : ctx->retiredHost)
+        (void)hipHostFree(p);
+    ctx->retiredHost.clear();
     (void)hipFree(ctx->d_pool);
     (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->pt.d_slots);
@@ -586,6 +595,12 @@ int lvr_ctx_sync(lvr_ctx* ctx)
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
+    for (void* p : ctx->retired)
+        (void)hipFree(p);
+    ctx->retired.clear();
+    for (void* p : ctx->retiredHost)
+        (void)hipHostFree(p);
+    ctx->retiredHost.clear();
     return LVR_OK;
 }
 

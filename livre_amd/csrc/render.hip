@@ -688,17 +688,28 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     return LVR_OK;
 }
 
+// Per-frame device buffers grow without waiting for the frames in flight:
+// the old buffer is retired (frames queued before still read it) and freed
+// at the next point where every stream has drained (lvr_ctx_sync, destroy).
+// Growth at least doubles, so a camera path settles after a few frames.
 template <typename T>
 static int growBuffer(lvr_ctx* ctx, T** ptr, size_t* cap, size_t need)
 {
     if (need <= *cap && *ptr)
         return LVR_OK;
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    (void)hipFree(*ptr);
-    *ptr = nullptr;
+    if (ctx->retired.size() >= 64)
+    {
+        int rc = lvr_ctx_sync(ctx);
+        if (rc)
+            return rc;
+    }
     size_t n = std::max<size_t>(need, 1);
-    n = n + n / 4;
-    LVR_HIP(ctx, hipMalloc((void**)ptr, n * sizeof(T)));
+    n = std::max(n + n / 4, 2 * *cap);
+    T* fresh = nullptr;
+    LVR_HIP(ctx, hipMalloc((void**)&fresh, n * sizeof(T)));
+    if (*ptr)
+        ctx->retired.push_back(*ptr);
+    *ptr = fresh;
     *cap = n;
     return LVR_OK;
 }

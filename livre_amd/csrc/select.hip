@@ -228,7 +228,7 @@ int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f, uint32_t windo
         (void)hipFree(ctx->d_selOut);
         ctx->d_selA = ctx->d_selB = ctx->d_selOut = nullptr;
         if (ctx->h_sel)
-            (void)hipHostFree(ctx->h_sel);
+            ctx->retiredHost.push_back(ctx->h_sel);
         ctx->h_sel = nullptr;
         ctx->selCapacity = 0;
         LVR_HIP(ctx, hipMalloc((void**)&ctx->d_selA, sizeof(uint64_t) * capNeed));

@@ -290,23 +290,45 @@ struct lvr_ctx
     uint32_t* d_frame = nullptr;
     int fbW = 0, fbH = 0;
     size_t fbCapacity = 0;
+    // the current frame's records (aliases into recSets)
     lvr::BrickEntry* d_bricks = nullptr;
-    int brickCapacity = 0;
     int4* d_ranges = nullptr; // conservative tile range per brick
-    size_t rangeCapacity = 0;
+    // the current frame's tile binning (aliases into recSets): ordered brick
+    // lists per tile and, the first level, per super-tile
     int32_t* d_tileCount = nullptr;
     int32_t* d_tileOffset = nullptr;
     int32_t* d_tileList = nullptr;
-    size_t tileCapacity = 0;
-    size_t tileOffsetCapacity = 0;
-    size_t tileListCapacity = 0;
-    // first binning level: ordered brick lists per super-tile
     int32_t* d_superCount = nullptr;
     int32_t* d_superOffset = nullptr;
     int32_t* d_superList = nullptr;
-    size_t superCapacity = 0, superOffsetCapacity = 0, superListCapacity = 0;
+    unsigned long long* curCounters = nullptr; // the current frame's counters
     lvr::FrameUniforms* d_req = nullptr; // the frame as a one-request table
-    size_t reqCapacity = 0;
+    // Two sets of frame records and binning in turn: frame t+1's records
+    // are copied and binned on recStream while frame t ray-casts (the
+    // binning kernels fit beside the persistent ray cast on each CU); a
+    // set's copies wait until the frame that last used it is finished, and
+    // the frame's ray cast waits for its binning.
+    struct RecordSet
+    {
+        lvr::FrameUniforms* req = nullptr;
+        size_t reqCap = 0;
+        lvr::BrickEntry* bricks = nullptr;
+        size_t brickCap = 0;
+        int4* ranges = nullptr;
+        size_t rangeCap = 0;
+        int32_t *tileCount = nullptr, *tileOffset = nullptr, *tileList = nullptr;
+        size_t tileCountCap = 0, tileOffsetCap = 0, tileListCap = 0;
+        int32_t *superCount = nullptr, *superOffset = nullptr, *superList = nullptr;
+        size_t superCountCap = 0, superOffsetCap = 0, superListCap = 0;
+        unsigned long long* counters = nullptr; // [0] samples, [1] work queue
+        size_t countersCap = 0;
+        hipEvent_t ready = nullptr; // binned (recStream)
+        hipEvent_t free = nullptr;  // its frame finished, counters read (stream)
+        bool used = false;
+    };
+    RecordSet recSets[2];
+    int recNext = 0;
+    hipStream_t recStream = nullptr;
     std::vector<lvr::BrickEntry> hostBricks;
     std::vector<int4> hostRanges;
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
@@ -358,9 +380,10 @@ struct lvr_ctx
     {
         uint64_t bricks = 0, entries = 0;
         bool timed = false, histTimed = false, armed = false;
-        // binning start, ray-cast start, ray-cast end (on `stream`),
-        // histogram start and end (on `histStream`)
-        hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        // binning start (recStream), ray-cast start, ray-cast end (on
+        // `stream`), histogram start and end (on `histStream`), binning end
+        // (recStream)
+        hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
         hipEvent_t done = nullptr; // the frame's composite steps are in h_frameSamples
     };
     FrameRecord frames[kFrameSlots];

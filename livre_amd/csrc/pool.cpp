@@ -509,15 +509,23 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->pt.d_slots);
     (void)hipFree(ctx->d_tf);
     (void)hipFree(ctx->d_frame);
-    (void)hipFree(ctx->d_bricks);
-    (void)hipFree(ctx->d_ranges);
-    (void)hipFree(ctx->d_req);
-    (void)hipFree(ctx->d_tileCount);
-    (void)hipFree(ctx->d_tileOffset);
-    (void)hipFree(ctx->d_tileList);
-    (void)hipFree(ctx->d_superCount);
-    (void)hipFree(ctx->d_superOffset);
-    (void)hipFree(ctx->d_superList);
+    if (ctx->recStream)
+    {
+        (void)hipStreamSynchronize(ctx->recStream);
+        (void)hipStreamDestroy(ctx->recStream);
+    }
+    for (auto& rs : ctx->recSets)
+    {
+        void* bufs[] = {rs.req,        rs.bricks,     rs.ranges,    rs.tileCount,
+                        rs.tileOffset, rs.tileList,   rs.superCount, rs.superOffset,
+                        rs.superList,  rs.counters};
+        for (void* p : bufs)
+            (void)hipFree(p);
+        if (rs.ready)
+            (void)hipEventDestroy(rs.ready);
+        if (rs.free)
+            (void)hipEventDestroy(rs.free);
+    }
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_hist);
     for (int r = 0; r < 2; ++r)
@@ -595,6 +603,8 @@ int lvr_ctx_sync(lvr_ctx* ctx)
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->selStream));
+    if (ctx->recStream)
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->recStream));
     for (void* p : ctx->retired)
         (void)hipFree(p);
     ctx->retired.clear();

@@ -882,7 +882,7 @@ static int finishFrame(lvr_ctx* ctx, lvr_ctx::FrameRecord& rec)
     const int slot = int(ctx->frameCount % lvr_ctx::kFrameSlots);
     if (rec.timed)
         LVR_HIP(ctx, hipEventRecord(rec.ev[2], ctx->stream));
-    LVR_HIP(ctx, hipMemcpyAsync(&ctx->h_frameSamples[slot], ctx->d_counters, sizeof(uint64_t),
+    LVR_HIP(ctx, hipMemcpyAsync(&ctx->h_frameSamples[slot], ctx->curCounters, sizeof(uint64_t),
                                 hipMemcpyDeviceToHost, ctx->stream));
     LVR_HIP(ctx, hipEventRecord(rec.done, ctx->stream));
     rec.armed = true;
@@ -922,46 +922,49 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     ctx->fbH = H;
     U.frame = ctx->d_frame;
     U.tf = reinterpret_cast<const float4*>(ctx->d_tf);
-    size_t brickCap = size_t(ctx->brickCapacity);
-    rc = growBuffer(ctx, &ctx->d_bricks, &brickCap, std::max<size_t>(1, size_t(n)));
+    lvr_ctx::RecordSet& rs = ctx->recSets[ctx->recNext];
+    if (!ctx->recStream)
+        LVR_HIP(ctx, hipStreamCreateWithFlags(&ctx->recStream, hipStreamNonBlocking));
+    if (!rs.ready)
+    {
+        LVR_HIP(ctx, hipEventCreateWithFlags(&rs.ready, hipEventDisableTiming));
+        LVR_HIP(ctx, hipEventCreateWithFlags(&rs.free, hipEventDisableTiming));
+    }
+    rc = growBuffer(ctx, &rs.bricks, &rs.brickCap, std::max<size_t>(1, size_t(n)));
     if (rc)
         return rc;
-    size_t rangeCap = ctx->rangeCapacity;
-    rc = growBuffer(ctx, &ctx->d_ranges, &rangeCap, std::max<size_t>(1, size_t(n)));
+    rc = growBuffer(ctx, &rs.ranges, &rs.rangeCap, std::max<size_t>(1, size_t(n)));
     if (rc)
         return rc;
-    ctx->brickCapacity = int(std::min<size_t>(brickCap, rangeCap));
-    ctx->rangeCapacity = rangeCap;
-    rc = growBuffer(ctx, &ctx->d_tileCount, &ctx->tileCapacity, size_t(nTiles));
-    if (rc)
-        return rc;
-    size_t offCap = ctx->tileOffsetCapacity;
-    rc = growBuffer(ctx, &ctx->d_tileOffset, &offCap, size_t(nTiles));
-    if (rc)
-        return rc;
-    ctx->tileOffsetCapacity = offCap;
-    rc = growBuffer(ctx, &ctx->d_tileList, &ctx->tileListCapacity,
-                    size_t(std::max<uint64_t>(listTotal, 1)));
+    rc = growBuffer(ctx, &rs.req, &rs.reqCap, 1);
     if (rc)
         return rc;
     const int nSupers = U.superX * U.superY;
-    rc = growBuffer(ctx, &ctx->d_superCount, &ctx->superCapacity, size_t(nSupers));
-    if (rc)
+    if ((rc = growBuffer(ctx, &rs.tileCount, &rs.tileCountCap, size_t(nTiles))) ||
+        (rc = growBuffer(ctx, &rs.tileOffset, &rs.tileOffsetCap, size_t(nTiles))) ||
+        (rc = growBuffer(ctx, &rs.tileList, &rs.tileListCap,
+                         size_t(std::max<uint64_t>(listTotal, 1)))) ||
+        (rc = growBuffer(ctx, &rs.superCount, &rs.superCountCap, size_t(nSupers))) ||
+        (rc = growBuffer(ctx, &rs.superOffset, &rs.superOffsetCap, size_t(nSupers))) ||
+        (rc = growBuffer(ctx, &rs.superList, &rs.superListCap,
+                         size_t(std::max<uint64_t>(superTotal, 1)))) ||
+        (rc = growBuffer(ctx, &rs.counters, &rs.countersCap, 4)))
         return rc;
-    rc = growBuffer(ctx, &ctx->d_superOffset, &ctx->superOffsetCapacity, size_t(nSupers));
-    if (rc)
-        return rc;
-    rc = growBuffer(ctx, &ctx->d_superList, &ctx->superListCapacity,
-                    size_t(std::max<uint64_t>(superTotal, 1)));
-    if (rc)
-        return rc;
-    size_t reqCap = ctx->reqCapacity;
-    rc = growBuffer(ctx, &ctx->d_req, &reqCap, 1);
-    if (rc)
-        return rc;
-    ctx->reqCapacity = reqCap;
+    ctx->recNext ^= 1;
+    ctx->d_bricks = rs.bricks;
+    ctx->d_ranges = rs.ranges;
+    ctx->d_req = rs.req;
+    ctx->d_tileCount = rs.tileCount;
+    ctx->d_tileOffset = rs.tileOffset;
+    ctx->d_tileList = rs.tileList;
+    ctx->d_superCount = rs.superCount;
+    ctx->d_superOffset = rs.superOffset;
+    ctx->d_superList = rs.superList;
+    ctx->curCounters = rs.counters;
 
-    // uniforms, brick entries and ranges through one pinned staging buffer
+    // uniforms, brick entries and ranges through one pinned staging buffer,
+    // copied on recStream (while the previous frame still ray-casts) once
+    // the ray cast that last read this record set is done
     {
         const size_t bBytes = sizeof(BrickEntry) * size_t(n), rBytes = sizeof(int4) * size_t(n);
         uint8_t* h = nullptr;
@@ -971,16 +974,18 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         std::memcpy(h, &U, sizeof(FrameUniforms));
         std::memcpy(h + sizeof(FrameUniforms), bricks.data(), bBytes);
         std::memcpy(h + sizeof(FrameUniforms) + bBytes, ranges.data(), rBytes);
+        if (rs.used)
+            LVR_HIP(ctx, hipStreamWaitEvent(ctx->recStream, rs.free, 0));
         LVR_HIP(ctx, hipMemcpyAsync(ctx->d_req, h, sizeof(FrameUniforms),
-                                    hipMemcpyHostToDevice, ctx->stream));
+                                    hipMemcpyHostToDevice, ctx->recStream));
         if (n > 0)
         {
             LVR_HIP(ctx, hipMemcpyAsync(ctx->d_bricks, h + sizeof(FrameUniforms), bBytes,
-                                        hipMemcpyHostToDevice, ctx->stream));
+                                        hipMemcpyHostToDevice, ctx->recStream));
             LVR_HIP(ctx, hipMemcpyAsync(ctx->d_ranges, h + sizeof(FrameUniforms) + bBytes,
-                                        rBytes, hipMemcpyHostToDevice, ctx->stream));
+                                        rBytes, hipMemcpyHostToDevice, ctx->recStream));
         }
-        LVR_HIP(ctx, ctx->frameStaging.release(slot, ctx->stream));
+        LVR_HIP(ctx, ctx->frameStaging.release(slot, ctx->recStream));
     }
     // this frame's statistics slot; its previous frame (kFrameSlots ago) has
     // long completed, but its events are only reused once it has
@@ -993,10 +998,15 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     rec.timed = ctx->timing;
     rec.histTimed = false;
     if (rec.timed)
-        LVR_HIP(ctx, hipEventRecord(rec.ev[0], ctx->stream));
-    enqueueBinning(ctx->stream, ctx->d_req, 1, ctx->d_ranges, nSupers, nTiles, ctx->d_superCount,
-                   ctx->d_superOffset, ctx->d_superList, ctx->d_tileCount, ctx->d_tileOffset,
-                   ctx->d_tileList, ctx->d_counters, 2);
+        LVR_HIP(ctx, hipEventRecord(rec.ev[0], ctx->recStream));
+    enqueueBinning(ctx->recStream, ctx->d_req, 1, ctx->d_ranges, nSupers, nTiles,
+                   ctx->d_superCount, ctx->d_superOffset, ctx->d_superList, ctx->d_tileCount,
+                   ctx->d_tileOffset, ctx->d_tileList, ctx->curCounters, 2);
+    LVR_HIP(ctx, hipGetLastError());
+    if (rec.timed)
+        LVR_HIP(ctx, hipEventRecord(rec.ev[5], ctx->recStream));
+    LVR_HIP(ctx, hipEventRecord(rs.ready, ctx->recStream));
+    LVR_HIP(ctx, hipStreamWaitEvent(ctx->stream, rs.ready, 0));
     if (rec.timed)
         LVR_HIP(ctx, hipEventRecord(rec.ev[1], ctx->stream));
     // persistent: two workgroups per CU are resident (the LDS budget of a
@@ -1020,47 +1030,55 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         if (dt == LVR_DT_UINT8)
             launchNearest<uint8_t, true>(blocks, lds, ctx->stream, U, ctx->d_bricks,
                                          ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
-                                         ctx->d_counters);
+                                         ctx->curCounters);
         else
             launchNearest<uint16_t, true>(blocks, lds, ctx->stream, U, ctx->d_bricks,
                                           ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
-                                          ctx->d_counters);
+                                          ctx->curCounters);
         LVR_HIP(ctx, hipGetLastError());
-        return finishFrame(ctx, rec);
+        if ((rc = finishFrame(ctx, rec)))
+            return rc;
+        LVR_HIP(ctx, hipEventRecord(rs.free, ctx->stream));
+        rs.used = true;
+        return LVR_OK;
     }
     switch (dt)
     {
     case LVR_DT_UINT8:
         launchRaycast<uint8_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                               ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                               ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     case LVR_DT_UINT16:
         launchRaycast<uint16_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                                ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                                ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     case LVR_DT_UINT32:
         launchRaycast<uint32_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                                ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                                ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     case LVR_DT_INT8:
         launchRaycast<int8_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                              ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                              ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     case LVR_DT_INT16:
         launchRaycast<int16_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                               ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                               ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     case LVR_DT_INT32:
         launchRaycast<int32_t>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                               ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                               ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     default:
         launchRaycast<float>(blocks, lds, ctx->stream, U, ctx->d_bricks, ctx->d_tileOffset,
-                             ctx->d_tileCount, ctx->d_tileList, ctx->d_counters);
+                             ctx->d_tileCount, ctx->d_tileList, ctx->curCounters);
         break;
     }
     LVR_HIP(ctx, hipGetLastError());
-    return finishFrame(ctx, rec);
+    if ((rc = finishFrame(ctx, rec)))
+        return rc;
+    LVR_HIP(ctx, hipEventRecord(rs.free, ctx->stream));
+    rs.used = true;
+    return LVR_OK;
 }
 
 int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes, uint64_t* samples)
@@ -1078,7 +1096,9 @@ int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes, uint64_t* sam
         LVR_HIP(ctx, hipMemcpyAsync(out_rgba8, ctx->d_frame, need, hipMemcpyDeviceToHost,
                                     ctx->stream));
     }
-    LVR_HIP(ctx, hipMemcpyAsync(ctx->h_counters, ctx->d_counters, sizeof(unsigned long long),
+    LVR_HIP(ctx, hipMemcpyAsync(ctx->h_counters,
+                                ctx->curCounters ? ctx->curCounters : ctx->d_counters,
+                                sizeof(unsigned long long),
                                 hipMemcpyDeviceToHost, ctx->stream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (samples)
@@ -1135,7 +1155,7 @@ int lvr_frame_stats_back(lvr_ctx* ctx, int back, lvr_frame_stats* out)
     out->tile_entries = rec.entries;
     if (rec.timed)
     {
-        LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, rec.ev[0], rec.ev[1]));
+        LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, rec.ev[0], rec.ev[5]));
         LVR_HIP(ctx, hipEventElapsedTime(&out->raycast_ms, rec.ev[1], rec.ev[2]));
         if (rec.histTimed)
         {

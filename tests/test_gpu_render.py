@@ -110,6 +110,34 @@ def test_frame_stats_of_frames_in_flight(blobs_u8):
     ctx.set_timing(False)
 
 
+def test_queued_frames_of_different_sizes(blobs_u8):
+    """Frames of different viewports queued back to back: their records,
+    tile lists and counters alternate between two sets and are binned beside
+    the previous frame's ray cast, and each keeps the samples (and the last
+    its image) of the same frame rendered alone."""
+    s, ctx = blobs_u8
+    ctx.set_tf(scenes.piecewise_tf(256, seed=7), s.range)
+    ctx.set_timing(True)
+    shots = [((1.2, 0.9, 1.4), 96, 80), ((-1.3, 0.5, 1.1), 40, 24),
+             ((0.8, -1.2, 1.3), 130, 97), ((1.5, 0.2, -1.0), 64, 64),
+             ((0.3, 1.6, 0.9), 17, 33)]
+    prepared = []
+    for eye, w, h in shots:
+        mv, proj, fr = s.camera(eye, w, h)
+        prepared.append((fr, w, h, s.visible_ordered(mv, proj, h)))
+    alone = []
+    for fr, w, h, ordered in prepared:
+        ctx.render(fr, w, h, ordered, livre_amd.render_params(64))
+        alone.append(ctx.read_frame())
+    for fr, w, h, ordered in prepared:  # queued without waiting
+        ctx.render(fr, w, h, ordered, livre_amd.render_params(64))
+    for back in range(4):
+        assert ctx.frame_stats(back=back).samples == alone[len(shots) - 1 - back][1]
+    img, smp = ctx.read_frame()
+    assert smp == alone[-1][1] and np.array_equal(img, alone[-1][0])
+    ctx.set_timing(False)
+
+
 def test_camera_inside_volume(blobs_u8):
     s, ctx = blobs_u8
     tf = scenes.piecewise_tf(256, seed=4, max_alpha=0.3)

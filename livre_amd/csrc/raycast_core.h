@@ -208,8 +208,8 @@ __device__ __forceinline__ uint32_t nearestOffset(const FrameUniforms& U, const 
 
 // Classification of one sample value (fragRayCast.glsl:184-204 before the
 // blend): texture(transferFnTex, density) with linear filtering and clamp to
-// edge, then the opacity-corrected alpha. Returns (r*alpha, g*alpha, b*alpha,
-// alpha), the factors the front-to-back blend multiplies by (1 - dst.a).
+// edge, then the opacity-corrected alpha. Returns (r, g, b, alpha): the
+// filtered colour and the corrected alpha the blend weighs it with.
 struct Classify
 {
     float mult, add, nTfF, nTfMax, alphaCorrection, maxA;
@@ -238,7 +238,7 @@ struct Classify
         const float base = 1.0f - fminf(sa, maxA);
         const float alpha =
             1.0f - __builtin_amdgcn_exp2f(alphaCorrection * __builtin_amdgcn_logf(base));
-        return make_float4(sr * alpha, sg * alpha, sb * alpha, alpha);
+        return make_float4(sr, sg, sb, alpha);
     }
 };
 
@@ -255,7 +255,7 @@ __device__ __forceinline__ Classify makeClassify(const FrameUniforms& U)
 }
 
 // Front-to-back "over" of classified samples (fragRayCast.glsl:103-111):
-// dst.rgb += (src.rgb * alpha) * (1 - dst.a); dst.a += alpha * (1 - dst.a).
+// dst.rgb += src.rgb * alpha * (1 - dst.a); dst.a += alpha * (1 - dst.a).
 // Returns true when the ray reached the early-exit opacity.
 struct Composite
 {
@@ -264,11 +264,14 @@ struct Composite
 
     __device__ __forceinline__ bool blend(const FrameUniforms& U, float4 c)
     {
-        // dst += src * (1 - dst.a) contracted to one FMA per channel
+        // the colour weight alpha * (1 - dst.a) once, then one FMA per
+        // channel (the shader's products in another association: colours
+        // move by an ulp; dst.a, and so the early exit, is rounded as before)
         const float rem = 1.0f - acc3;
-        acc0 = __builtin_fmaf(c.x, rem, acc0);
-        acc1 = __builtin_fmaf(c.y, rem, acc1);
-        acc2 = __builtin_fmaf(c.z, rem, acc2);
+        const float wgt = c.w * rem;
+        acc0 = __builtin_fmaf(c.x, wgt, acc0);
+        acc1 = __builtin_fmaf(c.y, wgt, acc1);
+        acc2 = __builtin_fmaf(c.z, wgt, acc2);
         acc3 = __builtin_fmaf(c.w, rem, acc3);
         return acc3 > U.exitA;
     }
@@ -422,9 +425,10 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
         {
             // blend (fragRayCast.glsl:103-111), weight 0 once the ray is done
             const float rem = alive ? 1.0f - C.acc3 : 0.0f;
-            C.acc0 = __builtin_fmaf(cls.x, rem, C.acc0);
-            C.acc1 = __builtin_fmaf(cls.y, rem, C.acc1);
-            C.acc2 = __builtin_fmaf(cls.z, rem, C.acc2);
+            const float wgt = cls.w * rem;
+            C.acc0 = __builtin_fmaf(cls.x, wgt, C.acc0);
+            C.acc1 = __builtin_fmaf(cls.y, wgt, C.acc1);
+            C.acc2 = __builtin_fmaf(cls.z, wgt, C.acc2);
             C.acc3 = __builtin_fmaf(cls.w, rem, C.acc3);
             nsamp += alive ? 1u : 0u;
             travel -= stepSize;

@@ -300,6 +300,30 @@ typedef struct
 int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                const uint64_t* ordered_ids, int n,
                const lvr_render_params* params);
+/* ------------------------------------------------ sort-last compositing */
+/* Sort-last decomposition (SelectVisibles::Impl::visitPost keeps the range
+ * [r0 * N, r1 * N) of the visibles, SelectVisibles.cpp:112-131; Equalizer's
+ * compositor blends the partial frames in depth order, Channel.cpp:464-603).
+ * lvr_render_partial ray-casts ordered_ids, a contiguous range of the frame's
+ * global front-to-back order, like lvr_render, into the context's layer
+ * `layer` (0..15): w * h premultiplied RGBA32F accumulations, row 0 = bottom,
+ * not converted to RGBA8. Sample positions are those of the whole frame
+ * (global-grid snapping), so the layers of consecutive ranges, composited
+ * front first, give the whole frame up to FP rounding and the early exit
+ * each layer takes on its own. */
+int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                       const uint64_t* ordered_ids, int n, const lvr_render_params* params,
+                       int layer);
+/* Device pointer of a layer (w * h float4), e.g. for an exchange between
+ * GPUs; null until the layer is first rendered. */
+int lvr_partial_device_ptr(lvr_ctx* ctx, int layer, void** ptr);
+/* Front-to-back "over" of n (1..16) RGBA32F device images of width x height
+ * (front first) into the context's RGBA8 framebuffer with fragTexCopy's
+ * rounding; read it with lvr_frame_read. kernel_ms, when not NULL, receives
+ * the kernel's time (the call then waits for it). */
+int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int width,
+                         int height, float* kernel_ms);
+
 /* Copy the last frame to the host (w * h * 4 bytes) and return the number of
  * executed composite steps (fragRayCast.glsl:184-204 iterations). */
 int lvr_frame_read(lvr_ctx* ctx, uint8_t* out_rgba8, size_t bytes,

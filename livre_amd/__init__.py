@@ -36,6 +36,7 @@ EXPORTS = (
     "lvr_pyramid_volume_info", "lvr_pyramid_load", "lvr_pyramid_release",
     "lvr_jpeg_encoder_create", "lvr_jpeg_encoder_destroy", "lvr_jpeg_encode",
     "lvr_jpeg_encode_host",
+    "lvr_render_partial", "lvr_partial_device_ptr", "lvr_composite_layers",
     "lvr_jpeg_max_size", "lvr_jpeg_error", "lvr_jpeg_get_stats", "lvr_frame_grab_jpeg",
     "lvr_frame_grab_stats",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
@@ -286,6 +287,11 @@ def lib():
     L.lvr_stream_get_stats.argtypes = [c.c_void_p, P(StreamStats)]
     L.lvr_brick_pin.argtypes = [c.c_void_p, u64p, c.c_int, c.c_int]
     L.lvr_set_timing.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_render_partial.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
+                                     P(RenderParams), c.c_int]
+    L.lvr_partial_device_ptr.argtypes = [c.c_void_p, c.c_int, P(c.c_void_p)]
+    L.lvr_composite_layers.argtypes = [c.c_void_p, P(c.c_void_p), c.c_int, c.c_int, c.c_int,
+                                       P(c.c_float)]
     L.lvr_jpeg_encoder_create.argtypes = [c.c_int, P(c.c_void_p)]
     L.lvr_jpeg_encoder_destroy.argtypes = [c.c_void_p]
     L.lvr_jpeg_encoder_destroy.restype = None
@@ -702,6 +708,30 @@ class Context:
         self._check(lib().lvr_render(self._h, ctypes.byref(fr), vp, pord, len(ordered),
                                      ctypes.byref(params)), "render")
         self.width, self.height = width, height
+
+    def render_partial(self, fr, width, height, ordered, params, layer):
+        """Sort-last partial: ordered (a contiguous range of the frame's
+        order) into RGBA32F layer `layer`."""
+        ordered, pord = _u64(ordered)
+        vp = (ctypes.c_int * 4)(0, 0, width, height)
+        self._check(lib().lvr_render_partial(self._h, ctypes.byref(fr), vp, pord, len(ordered),
+                                             ctypes.byref(params), layer), "render_partial")
+
+    def layer_ptr(self, layer):
+        p = ctypes.c_void_p()
+        self._check(lib().lvr_partial_device_ptr(self._h, layer, ctypes.byref(p)), "layer_ptr")
+        return p.value
+
+    def composite(self, layer_ptrs, width, height, timed=False):
+        """Front-to-back over of RGBA32F device images into the framebuffer;
+        returns the kernel's milliseconds when timed."""
+        arr = (ctypes.c_void_p * len(layer_ptrs))(*[ctypes.c_void_p(p) for p in layer_ptrs])
+        ms = ctypes.c_float(0.0)
+        self._check(lib().lvr_composite_layers(self._h, arr, len(layer_ptrs), width, height,
+                                               ctypes.byref(ms) if timed else None),
+                    "composite")
+        self.width, self.height = width, height
+        return ms.value if timed else None
 
     def autotune(self, fr, w, h, ordered, params, reps=3):
         """Fastest ray tile, samples per thread, classification mode and

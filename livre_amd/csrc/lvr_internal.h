@@ -212,6 +212,8 @@ struct FrameUniforms
     const float4* tf;  // nTf RGBA entries
     const float4* cls; // classified table per raw value (integer NEAREST), or null
     uint32_t* frame;   // RGBA8 output, row 0 = bottom
+    float4* layer;     // or, when set, the unconverted RGBA32F accumulation
+                       // (a sort-last partial), row 0 = bottom
 };
 
 // Pinned staging of the per-frame host records (brick entries, screen
@@ -334,6 +336,11 @@ struct lvr_ctx
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
     lvr::RawPyramid* pyramid = nullptr; // levels of the last raw source loaded
     lvr_jpeg_encoder* jpeg = nullptr;   // frame grab encoder on `stream`
+    // sort-last partial images (RGBA32F), grow-only
+    static constexpr int kMaxLayers = 16;
+    float4* d_layers[kMaxLayers] = {};
+    size_t layerCap[kMaxLayers] = {};
+    hipEvent_t compositeEv[2] = {nullptr, nullptr};
     // device buffers replaced while queued frames may still read them, and
     // outgrown pinned buffers (hipHostFree would wait for the whole device);
     // freed where every stream has drained

@@ -498,6 +498,12 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipStreamSynchronize(ctx->selStream);
     lvr::destroyPyramid(ctx);
     lvr::destroyJpeg(ctx);
+    for (float4* l : ctx->d_layers)
+        if (l)
+            (void)hipFree(l);
+    for (hipEvent_t e : ctx->compositeEv)
+        if (e)
+            (void)hipEventDestroy(e);
     for (void* p : ctx->retired)
         (void)hipFree(p);
     ctx->retired.clear();

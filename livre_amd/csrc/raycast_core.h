@@ -619,7 +619,9 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         }
     }
 
-    if (inside)
+    if (inside && U.layer)
+        U.layer[size_t(py) * U.width + px] = make_float4(C.acc0, C.acc1, C.acc2, C.acc3);
+    else if (inside)
     {
         // fragTexCopy.glsl into the RGBA8 framebuffer: round(clamp(c) * 255)
         const float c[4] = {C.acc0, C.acc1, C.acc2, C.acc3};

@@ -389,6 +389,49 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     }
 }
 
+// Sort-last compositing (eq::Compositor::blendFrames in depth order,
+// Channel.cpp:464-603): front-to-back "over" of n premultiplied RGBA32F
+// layers, front first, then fragTexCopy's RGBA8 rounding. One thread per
+// pixel, each layer read once, coalesced: HBM-bound.
+constexpr int kMaxCompositeLayers = 16;
+struct LayerSet
+{
+    const float4* p[kMaxCompositeLayers];
+};
+
+__global__ __launch_bounds__(256) void compositeKernel(LayerSet L, int n, int npix,
+                                                       uint32_t* __restrict__ out)
+{
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i < npix;
+         i += int(gridDim.x * blockDim.x))
+    {
+        float r = 0.f, g = 0.f, b = 0.f, a = 0.f;
+        for (int k0 = 0; k0 < n; k0 += 4)
+        {
+            // four layers' loads in flight before their blends
+            float4 c[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                c[j] = k0 + j < n ? L.p[k0 + j][i] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+            {
+                const float rem = 1.0f - a;
+                r = __builtin_fmaf(c[j].x, rem, r);
+                g = __builtin_fmaf(c[j].y, rem, g);
+                b = __builtin_fmaf(c[j].z, rem, b);
+                a = __builtin_fmaf(c[j].w, rem, a);
+            }
+        }
+        const float v[4] = {r, g, b, a};
+        uint32_t packed = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            packed |= uint32_t(floorf(fminf(1.0f, fmaxf(0.0f, v[q])) * 255.0f + 0.5f)) << (8 * q);
+        out[i] = packed;
+    }
+}
+
 // Classified table of an 8- or 16-bit volume: entry v = the classification
 // of raw value v, computed by the same code as the in-loop path.
 __global__ __launch_bounds__(256) void classifyKernel(Classify c, const float4* __restrict__ tf,
@@ -890,8 +933,11 @@ static int finishFrame(lvr_ctx* ctx, lvr_ctx::FrameRecord& rec)
     return LVR_OK;
 }
 
-int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
-               const uint64_t* ordered_ids, int n, const lvr_render_params* params)
+// A frame into the RGBA8 framebuffer (layer < 0) or, as a sort-last
+// partial, into the context's RGBA32F layer `layer`.
+static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                       const uint64_t* ordered_ids, int n, const lvr_render_params* params,
+                       int layer)
 {
     using namespace lvr;
     if (!ctx)
@@ -915,12 +961,22 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
     const int nTiles = U.tilesX * U.tilesY;
 
     // device buffers
-    rc = growBuffer(ctx, &ctx->d_frame, &ctx->fbCapacity, size_t(W) * H);
-    if (rc)
-        return rc;
-    ctx->fbW = W;
-    ctx->fbH = H;
-    U.frame = ctx->d_frame;
+    if (layer >= 0)
+    {
+        rc = growBuffer(ctx, &ctx->d_layers[layer], &ctx->layerCap[layer], size_t(W) * H);
+        if (rc)
+            return rc;
+        U.layer = ctx->d_layers[layer];
+    }
+    else
+    {
+        rc = growBuffer(ctx, &ctx->d_frame, &ctx->fbCapacity, size_t(W) * H);
+        if (rc)
+            return rc;
+        ctx->fbW = W;
+        ctx->fbH = H;
+        U.frame = ctx->d_frame;
+    }
     U.tf = reinterpret_cast<const float4*>(ctx->d_tf);
     lvr_ctx::RecordSet& rs = ctx->recSets[ctx->recNext];
     if (!ctx->recStream)
@@ -1078,6 +1134,74 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
         return rc;
     LVR_HIP(ctx, hipEventRecord(rs.free, ctx->stream));
     rs.used = true;
+    return LVR_OK;
+}
+
+int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+               const uint64_t* ordered_ids, int n, const lvr_render_params* params)
+{
+    return renderFrame(ctx, f, viewport, ordered_ids, n, params, -1);
+}
+
+int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                       const uint64_t* ordered_ids, int n, const lvr_render_params* params,
+                       int layer)
+{
+    if (!ctx || layer < 0 || layer >= lvr_ctx::kMaxLayers)
+        return LVR_EINVAL;
+    return renderFrame(ctx, f, viewport, ordered_ids, n, params, layer);
+}
+
+int lvr_partial_device_ptr(lvr_ctx* ctx, int layer, void** ptr)
+{
+    if (!ctx || !ptr || layer < 0 || layer >= lvr_ctx::kMaxLayers)
+        return LVR_EINVAL;
+    *ptr = ctx->d_layers[layer];
+    return LVR_OK;
+}
+
+int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int width, int height,
+                         float* kernel_ms)
+{
+    using namespace lvr;
+    if (!ctx || !layers || n < 1 || n > kMaxCompositeLayers || width < 1 || height < 1 ||
+        width > 16384 || height > 16384)
+        return LVR_EINVAL;
+    int rc = ensureDevice(ctx);
+    if (rc)
+        return rc;
+    LayerSet L = {};
+    for (int i = 0; i < n; ++i)
+    {
+        if (!layers[i])
+            return fail(ctx, LVR_EINVAL, "composite: null layer");
+        L.p[i] = static_cast<const float4*>(layers[i]);
+    }
+    rc = growBuffer(ctx, &ctx->d_frame, &ctx->fbCapacity, size_t(width) * height);
+    if (rc)
+        return rc;
+    ctx->fbW = width;
+    ctx->fbH = height;
+    const int npix = width * height;
+    const int grid = std::min((npix + 255) / 256, 16 * ctx->numCUs);
+    if (kernel_ms)
+    {
+        if (!ctx->compositeEv[0])
+        {
+            LVR_HIP(ctx, hipEventCreate(&ctx->compositeEv[0]));
+            LVR_HIP(ctx, hipEventCreate(&ctx->compositeEv[1]));
+        }
+        LVR_HIP(ctx, hipEventRecord(ctx->compositeEv[0], ctx->stream));
+    }
+    hipLaunchKernelGGL(compositeKernel, dim3(grid), dim3(256), 0, ctx->stream, L, n, npix,
+                       ctx->d_frame);
+    LVR_HIP(ctx, hipGetLastError());
+    if (kernel_ms)
+    {
+        LVR_HIP(ctx, hipEventRecord(ctx->compositeEv[1], ctx->stream));
+        LVR_HIP(ctx, hipEventSynchronize(ctx->compositeEv[1]));
+        LVR_HIP(ctx, hipEventElapsedTime(kernel_ms, ctx->compositeEv[0], ctx->compositeEv[1]));
+    }
     return LVR_OK;
 }
 

@@ -754,6 +754,89 @@ def c1_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def sortlast_workload(args):
+    """One GPU's part of sort-last rendering at the headline shape: each
+    1080p orbit-zoom frame of the 2048^3 volume is split into 8 consecutive
+    ranges of its ordered bricks (the 8 GPUs' shares), each ray-cast into an
+    RGBA32F layer, and the 8 layers composited front to back into RGBA8 (the
+    kernel every GPU runs on its strip after the exchange). The exchange
+    itself (an all-to-all over xGMI) is not run on one GPU."""
+    import livre_amd
+    from livre_amd import scenes
+    livre_amd.lib()
+    vi, ctx, _ids, _tf = build_scene(livre_amd, scenes)
+    params = livre_amd.render_params(N_SAMPLES, early_exit=0.999)
+    ctx.set_timing(True)
+    n_layers = 8
+    frames = args.steps or 30
+    comp_ms, part_max, part_sum, whole_ms, diffs = [], [], [], [], []
+    layers_host = None
+    for t in range(args.warmup + frames):
+        fr = camera(livre_amd, scenes, t * FRAMES // frames)[2]
+        vis = ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1)
+        ordered = livre_amd.order(vi, fr, vis)
+        ctx.render(fr, W, H, ordered, params)
+        whole_st = ctx.frame_stats()
+        whole, _ = ctx.read_frame()
+        parts = np.array_split(np.asarray(ordered), n_layers)
+        rc = []
+        for i, part in enumerate(parts):
+            ctx.render_partial(fr, W, H, part, params, i)
+            rc.append(ctx.frame_stats().raycast_ms)
+        ms = ctx.composite([ctx.layer_ptr(i) for i in range(n_layers)], W, H, timed=True)
+        img, _ = ctx.read_frame()
+        if t < args.warmup:
+            continue
+        d = np.abs(img.astype(np.int32) - whole.astype(np.int32))
+        diffs.append((int(d.max()), float(d.mean())))
+        comp_ms.append(ms)
+        part_max.append(max(rc))
+        part_sum.append(sum(rc))
+        whole_ms.append(whole_st.raycast_ms)
+    comp = float(np.mean(comp_ms))
+    bytes_per = n_layers * W * H * 16 + W * H * 4
+    achieved = bytes_per / (comp * 1e-3) / 1e9
+    result = {
+        "metric": "sort-last: 8-layer composite GB/s at 1920x1080",
+        "value": achieved, "unit": "GB/s", "n_gpus": 1, "steps": frames,
+        "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: frames of the 2048^3 uint16 value-noise orbit-zoom",
+        "config": {"workload": "each frame's ordered bricks in 8 consecutive ranges, each "
+                               "ray-cast into an RGBA32F layer, the 8 layers composited "
+                               "front to back (1920x1080)"},
+        "composite_ms_mean": comp,
+        "partial_raycast_ms_max_mean": float(np.mean(part_max)),
+        "partial_raycast_ms_sum_mean": float(np.mean(part_sum)),
+        "whole_raycast_ms_mean": float(np.mean(whole_ms)),
+        "vs_whole_frame_max_diff": max(x[0] for x in diffs),
+        "vs_whole_frame_mean_diff": float(np.mean([x[1] for x in diffs])),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
+                     "kernel": "compositeKernel",
+                     "bytes_per_launch": "8 x 1920x1080x16 + 1920x1080x4"},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        # the over of the same 8 layers on the host (numpy, float32)
+        rng = np.random.default_rng(0)
+        lay = [(rng.random((H, W, 4), np.float32) * 0.1) for _ in range(n_layers)]
+        t0, n = time.perf_counter(), 0
+        while time.perf_counter() - t0 < min(args.cpu_seconds, 5.0) or n == 0:
+            acc = np.zeros((H, W, 4), np.float32)
+            for l in lay:
+                acc += (np.float32(1) - acc[..., 3:4]) * l
+            out = np.rint(np.clip(acc, 0, 1) * 255).astype(np.uint8)
+            n += 1
+        el = time.perf_counter() - t0
+        result["cpu_baseline"] = {
+            "value": bytes_per * n / el / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "numpy float32 over of 8 1920x1080 RGBA32F layers into RGBA8, "
+                      "%d composites in %.1f s" % (n, el)}
+    ctx.close()
+    print(json.dumps(result), flush=True)
+
+
 def jpeg_workload(args):
     """Frame grab (FrameGrabber::_encodeJpeg: JPEG quality 100, 4:4:4, rows
     bottom-up) of the 2048^3 orbit-zoom frames at 1920x1080 and 3840x2160:
@@ -848,7 +931,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc", "raw", "jpeg",
-                                           "c1"),
+                                           "c1", "sortlast"),
                     default="render",
                     help="render: the 1080p single-volume frame loop (default); "
                          "batch: uneven requests, plain vs one batched launch; "
@@ -856,7 +939,8 @@ def main():
                          "ooc: out-of-core 8192^3 fly-through; "
                          "raw: a raw file's LOD pyramid built in HBM, then rendered; "
                          "jpeg: frame grab of rendered frames as JPEG; "
-                         "c1: the CPU-runnable 256^3 configuration at 1024x1024")
+                         "c1: the CPU-runnable 256^3 configuration at 1024x1024; "
+                         "sortlast: 8 partial layers per frame and their composite")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--max-batch", type=int, default=2)
@@ -888,6 +972,9 @@ def main():
         return
     if args.workload == "c1":
         c1_workload(args)
+        return
+    if args.workload == "sortlast":
+        sortlast_workload(args)
         return
     if args.steps is None:
         args.steps = FRAMES

@@ -406,15 +406,23 @@ __global__ __launch_bounds__(256) void compositeKernel(LayerSet L, int n, int np
          i += int(gridDim.x * blockDim.x))
     {
         float r = 0.f, g = 0.f, b = 0.f, a = 0.f;
-        for (int k0 = 0; k0 < n; k0 += 4)
+        for (int k0 = 0; k0 < n; k0 += 8)
         {
-            // four layers' loads in flight before their blends
-            float4 c[4];
+            // eight layers' loads in flight before their blends; each layer
+            // is read once, so the loads bypass the caches' retention
+            typedef float vec4 __attribute__((ext_vector_type(4)));
+            vec4 v4[8];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                c[j] = k0 + j < n ? L.p[k0 + j][i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int j = 0; j < 8; ++j)
+                v4[j] = k0 + j < n ? __builtin_nontemporal_load(
+                                         reinterpret_cast<const vec4*>(L.p[k0 + j]) + i)
+                                   : vec4{0.f, 0.f, 0.f, 0.f};
+            float4 c[8];
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 8; ++j)
+                c[j] = make_float4(v4[j].x, v4[j].y, v4[j].z, v4[j].w);
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
             {
                 const float rem = 1.0f - a;
                 r = __builtin_fmaf(c[j].x, rem, r);

@@ -319,10 +319,12 @@ int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
 int lvr_partial_device_ptr(lvr_ctx* ctx, int layer, void** ptr);
 /* Front-to-back "over" of n (1..16) RGBA32F device images of width x height
  * (front first) into the context's RGBA8 framebuffer with fragTexCopy's
- * rounding; read it with lvr_frame_read. kernel_ms, when not NULL, receives
- * the kernel's time (the call then waits for it). */
+ * rounding; read it with lvr_frame_read. The kernel runs reps (>= 1) times,
+ * each time with the same result; kernel_ms, when not NULL, receives the
+ * mean time per launch over the reps back-to-back launches (the call then
+ * waits for them). */
 int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int width,
-                         int height, float* kernel_ms);
+                         int height, int reps, float* kernel_ms);
 
 /* Copy the last frame to the host (w * h * 4 bytes) and return the number of
  * executed composite steps (fragRayCast.glsl:184-204 iterations). */

@@ -291,7 +291,7 @@ def lib():
                                      P(RenderParams), c.c_int]
     L.lvr_partial_device_ptr.argtypes = [c.c_void_p, c.c_int, P(c.c_void_p)]
     L.lvr_composite_layers.argtypes = [c.c_void_p, P(c.c_void_p), c.c_int, c.c_int, c.c_int,
-                                       P(c.c_float)]
+                                       c.c_int, P(c.c_float)]
     L.lvr_jpeg_encoder_create.argtypes = [c.c_int, P(c.c_void_p)]
     L.lvr_jpeg_encoder_destroy.argtypes = [c.c_void_p]
     L.lvr_jpeg_encoder_destroy.restype = None
@@ -722,13 +722,15 @@ class Context:
         self._check(lib().lvr_partial_device_ptr(self._h, layer, ctypes.byref(p)), "layer_ptr")
         return p.value
 
-    def composite(self, layer_ptrs, width, height, timed=False):
+    def composite(self, layer_ptrs, width, height, timed=False, reps=1):
         """Front-to-back over of RGBA32F device images into the framebuffer;
-        returns the kernel's milliseconds when timed."""
-        arr = (ctypes.c_void_p * len(layer_ptrs))(*[ctypes.c_void_p(p) for p in layer_ptrs])
+        returns the kernel's mean milliseconds per launch (over reps
+        back-to-back launches) when timed."""
+        arr = (ctypes.c_void_p * max(1, len(layer_ptrs)))(
+            *[ctypes.c_void_p(p) for p in layer_ptrs])
         ms = ctypes.c_float(0.0)
         self._check(lib().lvr_composite_layers(self._h, arr, len(layer_ptrs), width, height,
-                                               ctypes.byref(ms) if timed else None),
+                                               reps, ctypes.byref(ms) if timed else None),
                     "composite")
         self.width, self.height = width, height
         return ms.value if timed else None

@@ -1169,11 +1169,11 @@ int lvr_partial_device_ptr(lvr_ctx* ctx, int layer, void** ptr)
 }
 
 int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int width, int height,
-                         float* kernel_ms)
+                         int reps, float* kernel_ms)
 {
     using namespace lvr;
     if (!ctx || !layers || n < 1 || n > kMaxCompositeLayers || width < 1 || height < 1 ||
-        width > 16384 || height > 16384)
+        width > 16384 || height > 16384 || reps < 1 || reps > 1000)
         return LVR_EINVAL;
     int rc = ensureDevice(ctx);
     if (rc)
@@ -1201,14 +1201,16 @@ int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int wid
         }
         LVR_HIP(ctx, hipEventRecord(ctx->compositeEv[0], ctx->stream));
     }
-    hipLaunchKernelGGL(compositeKernel, dim3(grid), dim3(256), 0, ctx->stream, L, n, npix,
-                       ctx->d_frame);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(compositeKernel, dim3(grid), dim3(256), 0, ctx->stream, L, n, npix,
+                           ctx->d_frame);
     LVR_HIP(ctx, hipGetLastError());
     if (kernel_ms)
     {
         LVR_HIP(ctx, hipEventRecord(ctx->compositeEv[1], ctx->stream));
         LVR_HIP(ctx, hipEventSynchronize(ctx->compositeEv[1]));
         LVR_HIP(ctx, hipEventElapsedTime(kernel_ms, ctx->compositeEv[0], ctx->compositeEv[1]));
+        *kernel_ms /= float(reps);
     }
     return LVR_OK;
 }

@@ -310,7 +310,9 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
  * not converted to RGBA8. Sample positions are those of the whole frame
  * (global-grid snapping), so the layers of consecutive ranges, composited
  * front first, give the whole frame up to FP rounding and the early exit
- * each layer takes on its own. */
+ * each layer takes on its own. The RGBA8 framebuffer is left as it was; the
+ * frame statistics (lvr_frame_stats_get, the samples of lvr_frame_read) are
+ * the partial's. */
 int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                        const uint64_t* ordered_ids, int n, const lvr_render_params* params,
                        int layer);

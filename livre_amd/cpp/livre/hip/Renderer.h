@@ -365,6 +365,28 @@ public:
     }
     void setEarlyExit(float a) { _earlyExit = a; }
 
+    /** Sort-last (SelectVisibles.cpp:112-131, Channel.cpp:464-603): render
+     * the following frames' bricks, a range of the global order, into the
+     * RGBA32F layer `layer` instead of the framebuffer; -1 (the default)
+     * renders whole frames. */
+    void setPartialLayer(int layer) { _layer = layer; }
+    /** Device pointer of a layer, for the exchange between GPUs. */
+    void* layerDevicePtr(int layer) const
+    {
+        void* p = nullptr;
+        check(lvr_partial_device_ptr(_pool.ctx(), layer, &p), "layer", _pool.ctx());
+        return p;
+    }
+    /** Depth-ordered composite of layers (front first) into the framebuffer;
+     * readFrame() then returns it. */
+    void composite(const std::vector<const void*>& layers, const PixelViewport& view)
+    {
+        check(lvr_composite_layers(_pool.ctx(), layers.data(), int(layers.size()), view.w,
+                                   view.h, 1, nullptr),
+              "composite", _pool.ctx());
+        _vp = view;
+    }
+
     NodeIds order(const NodeIds& bricks, const Frustum& frustum) const final
     {
         std::vector<uint64_t> in(bricks.size()), out(bricks.size());
@@ -429,8 +451,14 @@ protected:
                 _visible.push_back(ordered[i]);
         }
         const int vp[4] = {view.x, view.y, view.w, view.h};
-        check(lvr_render(_pool.ctx(), &frustum.raw(), vp, ids.data(), int(ids.size()), &_params),
-              "render", _pool.ctx());
+        if (_layer >= 0)
+            check(lvr_render_partial(_pool.ctx(), &frustum.raw(), vp, ids.data(),
+                                     int(ids.size()), &_params, _layer),
+                  "render partial", _pool.ctx());
+        else
+            check(lvr_render(_pool.ctx(), &frustum.raw(), vp, ids.data(), int(ids.size()),
+                             &_params),
+                  "render", _pool.ctx());
     }
 
     void _onFrameEnd(const Frustum&, const ClipPlanes&, const PixelViewport&,
@@ -444,6 +472,7 @@ private:
     uint32_t _samplesPerRay;
     uint32_t _computed = 0;
     bool _linear = false;
+    int _layer = -1;
     float _earlyExit = 0.999f;
     PixelViewport _vp;
     lvr_render_params _params{};

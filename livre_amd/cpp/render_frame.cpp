@@ -13,7 +13,9 @@
  */
 #include <livre/hip/Renderer.h>
 
+#include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <fstream>
 #include <iostream>
 
@@ -185,23 +187,39 @@ int main(int argc, char** argv)
             throw std::runtime_error("frame grab: host and device encodes differ");
         std::ofstream(out + ".jpg", std::ios::binary)
             .write(reinterpret_cast<const char*>(jpg.data()), std::streamsize(jpg.size()));
+        // sort-last: the ordered visible set in two consecutive ranges, each
+        // ray-cast into a layer, composited front first
+        const livre::NodeIds visible = renderer.getVisibleNodes();
+        const auto mid = visible.begin() + std::ptrdiff_t(visible.size() / 2);
+        const livre::ClipPlanes noPlanes;
+        renderer.setPartialLayer(0);
+        renderer.render(frustum, noPlanes, vp, livre::NodeIds(visible.begin(), mid));
+        renderer.setPartialLayer(1);
+        renderer.render(frustum, noPlanes, vp, livre::NodeIds(mid, visible.end()));
+        renderer.setPartialLayer(-1);
+        renderer.composite({renderer.layerDevicePtr(0), renderer.layerDevicePtr(1)}, vp);
+        const std::vector<uint8_t> sortLast = renderer.readFrame();
+        int sortLastMaxDiff = 0;
+        for (size_t i = 0; i < img.size(); ++i)
+            sortLastMaxDiff = std::max(sortLastMaxDiff, std::abs(int(sortLast[i]) - int(img[i])));
         std::ofstream(out + ".hist", std::ios::binary)
             .write(reinterpret_cast<const char*>(hist.data()),
                    std::streamsize(hist.size() * sizeof(uint64_t)));
         std::ofstream txt(out + ".txt");
         txt.precision(9);
-        txt << "bricks " << renderer.getVisibleNodes().size() << "\nsamples " << samples
+        txt << "bricks " << visible.size() << "\nsamples " << samples
             << "\nwidth " << w << "\nheight " << h << "\nmv";
         for (float v : mv)
             txt << ' ' << v;
         txt << "\nproj";
         for (float v : proj)
             txt << ' ' << v;
+        txt << "\nsortlast_maxdiff " << sortLastMaxDiff;
         txt << "\nvisible";
-        for (const auto& node : renderer.getVisibleNodes())
+        for (const auto& node : visible)
             txt << ' ' << node.getId();
         txt << "\n";
-        std::cout << "rendered " << renderer.getVisibleNodes().size() << " bricks, " << samples
+        std::cout << "rendered " << visible.size() << " bricks, " << samples
                   << " samples" << std::endl;
     }
     catch (const std::exception& e)

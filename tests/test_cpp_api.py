@@ -68,5 +68,6 @@ def test_cpp_pipeline_frame_matches_oracle(driver, orc, tmp_path):
     d = np.abs(img.astype(int) - orc.to_rgba8(ref).astype(int))
     assert d.max() <= 2 and d.mean() <= 0.5
     assert int(meta["samples"][0]) == ref_samples
+    assert int(meta["sortlast_maxdiff"][0]) <= 2  # two layers composited vs whole
     assert np.array_equal(hist, orc.histogram(ovi, sel, [
         livre_amd.noise_brick_host(vi, p, i) for i in sel], 65536))

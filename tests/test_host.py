@@ -219,3 +219,22 @@ def test_host_noise_bricks_follow_the_per_voxel_definition(lvr):
             x, y, z = (int(rng.integers(34)) for _ in range(3))
             g = (pos[0] * 32 - 1 + x, pos[1] * 32 - 1 + y, pos[2] * 32 - 1 + z)
             assert brick[z, y, x] == _noise_voxel_py(p, vi, level, g), (level, x, y, z)
+
+
+def test_jpeg_max_size_bounds_the_worst_block(lvr):
+    # 1664 bits (22 DC + 63 x 26 AC, rounded to 32) per 8x8 block and
+    # component, doubled for byte stuffing, plus the JFIF header and EOI
+    L = lvr.lib()
+    blocks = 3 * ((1920 + 7) // 8) * ((1080 + 7) // 8)
+    assert L.lvr_jpeg_max_size(1920, 1080) == 1024 + blocks * 208 * 2 + 2
+    assert L.lvr_jpeg_max_size(1, 1) == 1024 + 3 * 208 * 2 + 2
+    assert L.lvr_jpeg_max_size(0, 5) == 0 and L.lvr_jpeg_max_size(5, -1) == 0
+
+
+def test_oracle_jpeg_worst_case_fits_the_bound(lvr, orc):
+    # noise at quality 100 is near the worst case per block
+    import numpy as np
+    rng = np.random.default_rng(9)
+    px = rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)
+    px[..., :3] = np.where(rng.random((64, 64, 3)) < 0.5, 0, 255)
+    assert len(orc.jpeg_encode(px, quality=100)) <= lvr.lib().lvr_jpeg_max_size(64, 64)

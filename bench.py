@@ -953,7 +953,9 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU: the brick pool, kernels and measurements are "
-              "single-device; multi-GPU sharding is a design only", file=sys.stderr)
+              "single-device; of multi-GPU sort-last rendering only one GPU's part is "
+              "built (--workload sortlast: partial layers and their composite), not the "
+              "exchange between GPUs", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
         if args.steps is None:

@@ -348,10 +348,23 @@ __global__ __launch_bounds__(256) void jpegDctKernel(const uint8_t* __restrict__
     const int iy = min(my * 8 + r, h - 1);
     const uint8_t* row = px + size_t(bottomUp ? h - 1 - iy : iy) * size_t(pitch);
     int32_t y[8], cb[8], cr[8];
+    // the row's 8 pixels: two 16-byte loads when they lie inside the image
+    // and are 16-byte aligned, else one load per pixel with the edge repeated
+    uint32_t px8[8];
+    const uint8_t* seg = row + 32 * size_t(mx);
+    if (mx * 8 + 8 <= w && (reinterpret_cast<uintptr_t>(seg) & 15) == 0)
+    {
+        const uint4 a = reinterpret_cast<const uint4*>(seg)[0];
+        const uint4 b = reinterpret_cast<const uint4*>(seg)[1];
+        px8[0] = a.x, px8[1] = a.y, px8[2] = a.z, px8[3] = a.w;
+        px8[4] = b.x, px8[5] = b.y, px8[6] = b.z, px8[7] = b.w;
+    }
+    else
+        for (int c = 0; c < 8; ++c)
+            px8[c] = *reinterpret_cast<const uint32_t*>(row + 4 * size_t(min(mx * 8 + c, w - 1)));
     for (int c = 0; c < 8; ++c)
     {
-        const int ix = min(mx * 8 + c, w - 1);
-        const uint32_t p = *reinterpret_cast<const uint32_t*>(row + 4 * size_t(ix));
+        const uint32_t p = px8[c];
         const int32_t c0 = int32_t(p & 0xFF), g = int32_t((p >> 8) & 0xFF),
                       c2 = int32_t((p >> 16) & 0xFF);
         const int32_t R = bgra ? c2 : c0, B = bgra ? c0 : c2;

@@ -50,10 +50,10 @@ def profiled_l2_hit_rate():
     return None
 
 
-def profiled_traffic():
+def profiled_traffic(name="raycast_2048u16_1080p_hbm.txt"):
     """HBM bytes per ray-cast launch from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary (scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(HERE, "profiles", "raycast_2048u16_1080p_hbm.txt")
+    path = os.path.join(HERE, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
@@ -814,7 +814,11 @@ def sortlast_workload(args):
         "vs_whole_frame_max_diff": max(x[0] for x in diffs),
         "vs_whole_frame_mean_diff": float(np.mean([x[1] for x in diffs])),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9), "traffic": None,
+                     "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
+                     "traffic": profiled_traffic("composite_8layer_1080p_hbm.txt"),
+                     "traffic_source": "profiles/composite_8layer_1080p_hbm.txt (rocprofv3 "
+                                       "FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a 1 GiB "
+                                       "copy)",
                      "kernel": "compositeKernel",
                      "bytes_per_launch": "8 x 1920x1080x16 + 1920x1080x4"},
         "cpu_baseline": None,

@@ -209,7 +209,7 @@ struct FrameUniforms
     int32_t dataType;
     int32_t request;   // index in the batch
     const uint8_t* pool;
-    const float4* tf;  // nTf RGBA entries
+    const float4* tf;  // nTf + 2 RGBA entries: tf[0], the nTf entries, tf[nTf - 1]
     const float4* cls; // classified table per raw value (integer NEAREST), or null
     uint32_t* frame;   // RGBA8 output, row 0 = bottom
     float4* layer;     // or, when set, the unconverted RGBA32F accumulation
@@ -279,7 +279,7 @@ struct lvr_ctx
     // transfer function
     float* d_tf = nullptr; // float4 entries
     int nTf = 0;
-    int tfCapacity = 0;
+    int tfCapacity = 0; // entries of d_tf (the table and its two edge copies)
     uint64_t tfGeneration = 0; // bumped by every transfer-function upload
     // classified table (r*a, g*a, b*a, a) per raw value of 8/16-bit volumes
     // for the current TF, data range and sampling rate

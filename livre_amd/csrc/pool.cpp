@@ -789,16 +789,23 @@ static int tfUpload(lvr_ctx* ctx, const std::vector<float>& rgba, int n,
     int rc = lvr::ensureDevice(ctx);
     if (rc)
         return rc;
-    if (n > ctx->tfCapacity)
+    // on the device the table carries a copy of its first entry before it
+    // and of its last after it (the layout the classification reads)
+    std::vector<float> padded(size_t(n + 2) * 4);
+    std::copy(rgba.begin(), rgba.begin() + 4, padded.begin());
+    std::copy(rgba.begin(), rgba.begin() + size_t(n) * 4, padded.begin() + 4);
+    std::copy(rgba.begin() + size_t(n - 1) * 4, rgba.begin() + size_t(n) * 4,
+              padded.begin() + size_t(n + 1) * 4);
+    if (n + 2 > ctx->tfCapacity)
     {
         LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
         (void)hipFree(ctx->d_tf);
         ctx->d_tf = nullptr;
-        LVR_HIP(ctx, hipMalloc((void**)&ctx->d_tf, size_t(n) * 4 * sizeof(float)));
-        ctx->tfCapacity = n;
+        LVR_HIP(ctx, hipMalloc((void**)&ctx->d_tf, padded.size() * sizeof(float)));
+        ctx->tfCapacity = n + 2;
     }
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    LVR_HIP(ctx, hipMemcpyAsync(ctx->d_tf, rgba.data(), size_t(n) * 4 * sizeof(float),
+    LVR_HIP(ctx, hipMemcpyAsync(ctx->d_tf, padded.data(), padded.size() * sizeof(float),
                                 hipMemcpyHostToDevice, ctx->stream));
     LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->nTf = n;

@@ -218,14 +218,17 @@ struct Classify
     {
         const float density = value * mult + add;
         // the two entries are floor(u) and floor(u) + 1 clamped to
-        // [0, nTf - 1] (fmed3 + truncation, as for the voxel)
+        // [0, nTf - 1]. The table carries a copy of its first entry before
+        // it and of its last after it (tf[0] and tf[nTf + 1]; entry j of the
+        // function is tf[j + 1]), so floor(u) clamped to [-1, nTf - 1] picks
+        // both with one index: below the range both reads give entry 0,
+        // above it both give entry nTf - 1, exactly as clamping each index.
         const float u = density * nTfF - 0.5f;
         const float fl = floorf(u);
         const float w = u - fl;
-        const int ia = int(__builtin_amdgcn_fmed3f(fl, 0.0f, nTfMax));
-        const int ib = int(__builtin_amdgcn_fmed3f(fl + 1.0f, 0.0f, nTfMax));
-        const float4 ca = tf[ia];
-        const float4 cb = tf[ib];
+        const int i = int(__builtin_amdgcn_fmed3f(fl, -1.0f, nTfMax));
+        const float4 ca = tf[i + 1];
+        const float4 cb = tf[i + 2];
         // the linear filter as a + w (b - a), one FMA per channel (a GLSL
         // compiler may contract the same way; the texel and TF indices above
         // stay exact, only colour roundings move by an ulp)

@@ -250,7 +250,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
     const float4* table = sTf;
     if (!CLS)
     {
-        for (int i = threadIdx.x; i < U.nTf; i += blockDim.x)
+        for (int i = threadIdx.x; i < U.nTf + 2; i += blockDim.x)
             sTf[i] = U.tf[i];
     }
     else if (sizeof(T) == 1)
@@ -278,8 +278,9 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         atomicAdd(&counters[0], s);
 }
 
-// Largest transfer function a batched request stages in LDS (per wavefront).
-constexpr int kBatchTf = 256;
+// LDS entries per wavefront for a batched request's transfer function: a
+// 256-entry table with its two edge copies.
+constexpr int kBatchTf = 258;
 
 // The batch kernel's voxel-type branches (8-bit, 16-bit, float) and the
 // fetch depths it is built with (a depth of 1 runs as 2: the same samples).
@@ -339,10 +340,10 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
                 for (int i = lane; i < 256; i += kWave)
                     myTf[i] = c(U.tf, float(i));
             }
-            else if (U.nTf <= kBatchTf)
+            else if (U.nTf + 2 <= kBatchTf)
             {
                 // this wavefront's slice only: ordered by the wave barrier
-                for (int i = lane; i < U.nTf; i += kWave)
+                for (int i = lane; i < U.nTf + 2; i += kWave)
                     myTf[i] = U.tf[i];
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -352,7 +353,7 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         const int tile = itemToTile(U, item - U.itemBase);
         if (tile < 0)
             continue;
-        const bool lds = U.nTf <= kBatchTf;
+        const bool lds = U.nTf + 2 <= kBatchTf;
         switch (U.dataType)
         {
         case LVR_DT_UINT8:
@@ -1078,7 +1079,7 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     const int perBlock = U.wavesPerSimd * 2;
     const int blocks =
         std::max(1, std::min((U.nItems + perBlock - 1) / perBlock, 2 * ctx->numCUs));
-    size_t lds = size_t(U.nTf) * sizeof(float4);
+    size_t lds = size_t(U.nTf + 2) * sizeof(float4);
     const int dt = ctx->vi.data_type;
     if (!U.linear && U.classify == 1 && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16))
     {
@@ -1443,10 +1444,15 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
     U.request = int32_t(b->reqs.size());
     if (std::find(b->ctxs.begin(), b->ctxs.end(), ctx) == b->ctxs.end())
         b->ctxs.push_back(ctx);
+    // the table with a copy of its first entry before it and of its last
+    // after it (the layout the classification reads)
     b->tfOffset.push_back(b->tf.size());
-    for (int i = 0; i < n_tf; ++i)
+    for (int j = -1; j <= n_tf; ++j)
+    {
+        const int i = std::min(std::max(j, 0), n_tf - 1);
         b->tf.push_back(make_float4(tf_rgba[4 * i], tf_rgba[4 * i + 1], tf_rgba[4 * i + 2],
                                     tf_rgba[4 * i + 3]));
+    }
     b->frameOffset.push_back(b->pixels);
     b->outputs.push_back(nullptr);
     b->pixels += size_t(U.width) * U.height;

@@ -187,6 +187,28 @@ def test_noise_volumes_fp32_tf(lvr, orc, dt):
         ctx.close()
 
 
+@pytest.mark.parametrize("dt,linear", [(livre_amd.DT_UINT16, False),
+                                       (livre_amd.DT_UINT8, True),
+                                       (livre_amd.DT_FLOAT, False)])
+def test_data_range_narrower_than_the_volume(lvr, orc, dt, linear):
+    """Densities below 0 and above 1 take the first and the last transfer
+    function entry (clamp to edge, fragRayCast.glsl:184-204). Both edge
+    entries differ sharply from their neighbours, so reading a neighbour
+    instead of the edge entry shows far above the tolerance."""
+    s = Scene(dt, n=64, block=16, overlap=1, source="noise", seed=13)
+    lo, hi = s.range
+    s.range = (lo + 0.35 * (hi - lo), lo + 0.65 * (hi - lo))
+    tf = scenes.piecewise_tf(256, seed=14, max_alpha=0.4).copy()
+    tf[0] = (1.0, 0.0, 0.0, 0.05)
+    tf[-1] = (0.0, 0.0, 1.0, 0.3)
+    ctx = s.context()
+    try:
+        _check_frame(s, ctx, (-1.4, 0.8, 1.5), 80, 64, tf, 64, linear=linear,
+                     rgba8_tf=False)
+    finally:
+        ctx.close()
+
+
 def test_trilinear_against_fp32_oracle(lvr, orc):
     s = Scene(livre_amd.DT_UINT16, n=64, block=16, overlap=1, source="noise", seed=13)
     ctx = s.context()

@@ -235,7 +235,11 @@ int lvr_pyramid_release(lvr_ctx* ctx);
 /* ----------------------------------------------------- transfer function */
 /* FP32 RGBA transfer function of n entries over the data range
  * [range0, range1] (RayCastRenderer.cpp:161-193, 280-285). Sampled with
- * linear filtering and clamp-to-edge, as the GL 1D texture is. */
+ * linear filtering and clamp-to-edge, as the GL 1D texture is. As in the
+ * reference, the range is used only when range1 > 0 and range1 > range0;
+ * otherwise frames map the volume type's full range (numeric_limits min and
+ * max, RayCastRenderer.cpp:236-285; for float volumes FLT_MIN and FLT_MAX).
+ * Float histograms need a valid range. */
 int lvr_tf_set(lvr_ctx* ctx, const float* rgba, int n, float range0,
                float range1);
 /* RGBA8 LUT as TransferFunction1D::getLUT produces (TransferFunction1D.cpp:
@@ -270,9 +274,17 @@ int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
               const uint64_t* ids, int n, uint64_t* out);
 
 /* ---------------------------------------------------------------- render */
+/* nSamplesPerRay of a frame (RayCastRenderer::Impl::onFrameStart,
+ * RayCastRenderer.cpp:216-234): `requested` when above 0, otherwise
+ * max(maxVoxelDim / 2^(depth - maxLOD - 1), 512) truncated, maxLOD being the
+ * finest level among the n rendered bricks. Host only. */
+int lvr_samples_per_ray(const lvr_volume_info* info, const uint64_t* ids, int n,
+                        int32_t requested, int32_t* out);
+
 typedef struct
 {
-    int32_t n_samples_per_ray;   /* nSamplesPerRay (RayCastRenderer.cpp:216-234) */
+    int32_t n_samples_per_ray;   /* nSamplesPerRay; 0 = lvr_samples_per_ray's rule
+                                    over the frame's bricks (RayCastRenderer.cpp:216-234) */
     int32_t max_samples_per_ray; /* opacity reference, 32 in the reference */
     float early_exit;            /* EARLY_EXIT, 0.999 (fragRayCast.glsl:12) */
     int32_t linear_filter;       /* 0: GL_NEAREST (reference default), 1: trilinear */
@@ -317,11 +329,22 @@ int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
                        const uint64_t* ordered_ids, int n, const lvr_render_params* params,
                        int layer);
 /* Device pointer of a layer (w * h float4), e.g. for an exchange between
- * GPUs; null until the layer is first rendered. */
+ * GPUs; null until the layer is first rendered. The pointer stays valid until
+ * a partial render into the same layer at a larger viewport replaces the
+ * buffer (the old one is freed at the next lvr_ctx_sync). */
 int lvr_partial_device_ptr(lvr_ctx* ctx, int layer, void** ptr);
+/* Copy a layer to host memory: width * height premultiplied RGBA32F pixels,
+ * row 0 = bottom (bytes >= width * height * 16, LVR_ERANGE otherwise;
+ * out_rgba32f may be NULL to learn the size). Synchronous. No reference
+ * counterpart: Equalizer reads partial frames back inside its compositor. */
+int lvr_partial_read(lvr_ctx* ctx, int layer, float* out_rgba32f, size_t bytes, int* width,
+                     int* height);
 /* Front-to-back "over" of n (1..16) RGBA32F device images of width x height
  * (front first) into the context's RGBA8 framebuffer with fragTexCopy's
- * rounding; read it with lvr_frame_read. The kernel runs reps (>= 1) times,
+ * rounding; read it with lvr_frame_read. A layer of this context must have
+ * been rendered at exactly width x height, and a layer buffer that has since
+ * been replaced is rejected (LVR_EINVAL); other device pointers must hold
+ * width * height float4. The kernel runs reps (>= 1) times,
  * each time with the same result; kernel_ms, when not NULL, receives the
  * mean time per launch over the reps back-to-back launches (the call then
  * waits for them). */

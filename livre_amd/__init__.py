@@ -36,7 +36,8 @@ EXPORTS = (
     "lvr_pyramid_volume_info", "lvr_pyramid_load", "lvr_pyramid_release",
     "lvr_jpeg_encoder_create", "lvr_jpeg_encoder_destroy", "lvr_jpeg_encode",
     "lvr_jpeg_encode_host",
-    "lvr_render_partial", "lvr_partial_device_ptr", "lvr_composite_layers",
+    "lvr_render_partial", "lvr_partial_device_ptr", "lvr_partial_read", "lvr_composite_layers",
+    "lvr_samples_per_ray",
     "lvr_jpeg_max_size", "lvr_jpeg_error", "lvr_jpeg_get_stats", "lvr_frame_grab_jpeg",
     "lvr_frame_grab_stats",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
@@ -225,6 +226,8 @@ def lib():
                                             c.c_uint32, c.c_uint32]
     L.lvr_select_visibles_wait.argtypes = [c.c_void_p, u64p, c.c_int, P(c.c_int)]
     L.lvr_order.argtypes = [P(VolumeInfo), P(Frustum), u64p, c.c_int, u64p]
+    L.lvr_samples_per_ray.argtypes = [P(VolumeInfo), u64p, c.c_int, c.c_int32,
+                                      P(c.c_int32)]
     L.lvr_render.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
                              P(RenderParams)]
     L.lvr_frame_read.argtypes = [c.c_void_p, u8p, c.c_size_t, u64p]
@@ -290,6 +293,8 @@ def lib():
     L.lvr_render_partial.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
                                      P(RenderParams), c.c_int]
     L.lvr_partial_device_ptr.argtypes = [c.c_void_p, c.c_int, P(c.c_void_p)]
+    L.lvr_partial_read.argtypes = [c.c_void_p, c.c_int, f32p, c.c_size_t, P(c.c_int),
+                                   P(c.c_int)]
     L.lvr_composite_layers.argtypes = [c.c_void_p, P(c.c_void_p), c.c_int, c.c_int, c.c_int,
                                        c.c_int, P(c.c_float)]
     L.lvr_jpeg_encoder_create.argtypes = [c.c_int, P(c.c_void_p)]
@@ -412,6 +417,17 @@ def field_brick_host(vi, params, node_id):
     if rc != LVR_OK:
         raise LivreError("field brick failed (%d)" % rc)
     return out
+
+
+def samples_per_ray(vi, ids, requested=0):
+    """nSamplesPerRay of a frame of these bricks: `requested`, or for 0 the
+    reference's rule from the finest rendered level."""
+    ids, pids = _u64(ids)
+    out = ctypes.c_int32()
+    if lib().lvr_samples_per_ray(ctypes.byref(vi), pids, len(ids), requested,
+                                 ctypes.byref(out)) != 0:
+        raise LivreError("samples_per_ray: invalid arguments")
+    return out.value
 
 
 def render_params(n_samples_per_ray, early_exit=0.999, linear=False,
@@ -721,6 +737,18 @@ class Context:
         p = ctypes.c_void_p()
         self._check(lib().lvr_partial_device_ptr(self._h, layer, ctypes.byref(p)), "layer_ptr")
         return p.value
+
+    def read_layer(self, layer):
+        """A sort-last layer as an (h, w, 4) float32 array (premultiplied
+        RGBA32F, row 0 = bottom)."""
+        w, h = ctypes.c_int(), ctypes.c_int()
+        self._check(lib().lvr_partial_read(self._h, layer, None, 0, ctypes.byref(w),
+                                           ctypes.byref(h)), "partial_read")
+        out = np.zeros((h.value, w.value, 4), np.float32)
+        self._check(lib().lvr_partial_read(
+            self._h, layer, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), out.nbytes,
+            ctypes.byref(w), ctypes.byref(h)), "partial_read")
+        return out
 
     def composite(self, layer_ptrs, width, height, timed=False, reps=1):
         """Front-to-back over of RGBA32F device images into the framebuffer;

@@ -340,6 +340,8 @@ struct lvr_ctx
     static constexpr int kMaxLayers = 16;
     float4* d_layers[kMaxLayers] = {};
     size_t layerCap[kMaxLayers] = {};
+    // viewport of each layer's last partial render (0 x 0: never rendered)
+    int layerW[kMaxLayers] = {}, layerH[kMaxLayers] = {};
     hipEvent_t compositeEv[2] = {nullptr, nullptr};
     // device buffers replaced while queued frames may still read them, and
     // outgrown pinned buffers (hipHostFree would wait for the whole device);

@@ -236,4 +236,33 @@ int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
         out[i] = v[size_t(i)].second;
     return LVR_OK;
 }
+
+int lvr_samples_per_ray(const lvr_volume_info* info, const uint64_t* ids, int n,
+                        int32_t requested, int32_t* out)
+{
+    if (!info || !out || n < 0 || (n > 0 && !ids) || requested < 0 || info->depth < 1)
+        return LVR_EINVAL;
+    if (requested > 0)
+    {
+        *out = requested;
+        return LVR_OK;
+    }
+    // RayCastRenderer::Impl::onFrameStart (RayCastRenderer.cpp:216-234): the
+    // finest level among the rendered bricks, its voxel count along the
+    // longest axis, at least minSamplesPerRay = 512; the float is stored to
+    // an unsigned integer (truncated)
+    uint32_t maxLod = 0;
+    for (int i = 0; i < n; ++i)
+    {
+        const uint32_t level = lvr::decodeNode(ids[i]).level;
+        if (level >= info->depth)
+            return LVR_EINVAL;
+        maxLod = std::max(maxLod, level);
+    }
+    const float maxVoxelDim =
+        float(std::max(info->voxels[0], std::max(info->voxels[1], info->voxels[2])));
+    const float maxVoxelsAtLod = maxVoxelDim / float(1u << (info->depth - maxLod - 1));
+    *out = int32_t(uint32_t(std::max(maxVoxelsAtLod, 512.0f)));
+    return LVR_OK;
+}
 }

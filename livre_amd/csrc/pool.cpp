@@ -7,6 +7,7 @@
 // The page table maps a NodeId to its slot on the device, so kernels can
 // resolve bricks without the host.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "lvr_internal.h"
@@ -212,7 +213,6 @@ void* genScratch(lvr_ctx* ctx, size_t bytes)
         return nullptr;
     }
     (void)hipFree(ctx->d_gen);
-    (void)hipFree(ctx->d_cls);
     ctx->d_gen = nullptr;
     ctx->genCapacity = 0;
     const size_t cap = std::max<size_t>(bytes, size_t(1) << 20);
@@ -512,6 +512,7 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     ctx->retiredHost.clear();
     (void)hipFree(ctx->d_pool);
     (void)hipFree(ctx->d_gen);
+    (void)hipFree(ctx->d_cls);
     (void)hipFree(ctx->pt.d_slots);
     (void)hipFree(ctx->d_tf);
     (void)hipFree(ctx->d_frame);
@@ -821,8 +822,8 @@ int lvr_tf_set(lvr_ctx* ctx, const float* rgba, int n, float range0,
 {
     if (!ctx || !rgba || n < 1)
         return LVR_EINVAL;
-    if (!(range1 > range0))
-        return lvr::fail(ctx, LVR_EINVAL, "empty data range");
+    if (!std::isfinite(range0) || !std::isfinite(range1))
+        return lvr::fail(ctx, LVR_EINVAL, "data range not finite");
     std::vector<float> v(rgba, rgba + size_t(n) * 4);
     return tfUpload(ctx, v, n, range0, range1);
 }
@@ -832,8 +833,8 @@ int lvr_tf_set_rgba8(lvr_ctx* ctx, const uint8_t* lut, int n, float range0,
 {
     if (!ctx || !lut || n < 1)
         return LVR_EINVAL;
-    if (!(range1 > range0))
-        return lvr::fail(ctx, LVR_EINVAL, "empty data range");
+    if (!std::isfinite(range0) || !std::isfinite(range1))
+        return lvr::fail(ctx, LVR_EINVAL, "data range not finite");
     std::vector<float> v(size_t(n) * 4);
     for (size_t i = 0; i < v.size(); ++i)
         v[i] = float(lut[i]) / 255.0f;

@@ -17,8 +17,10 @@
 //      records with scalar loads, composite in FP32 registers, exit early
 //      per ray and per wavefront (ballot) and store RGBA8 directly (no
 //      RGBA32F round trip, fragTexCopy.glsl:11-15).
-// The per-sample arithmetic follows the shader expression by expression with
-// floating-point contraction off (raycast_core.h).
+// The per-sample arithmetic follows the shader expression by expression
+// (raycast_core.h): sample positions, texel indices, TF indices and the
+// early-exit opacity are rounded exactly as written; the linear filter of the
+// TF and the colour blend use explicit FMAs, which moves colours by an ulp.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -553,9 +555,13 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     const int W = viewport[2], H = viewport[3];
     if (W <= 0 || H <= 0 || W > 16384 || H > 16384)
         return fail(ctx, LVR_EINVAL, "invalid viewport");
-    if (params->n_samples_per_ray <= 0 || params->n_clip_planes < 0 ||
+    if (params->n_samples_per_ray < 0 || params->n_clip_planes < 0 ||
         params->n_clip_planes > 6)
         return fail(ctx, LVR_EINVAL, "invalid render parameters");
+    // 0 samples per ray: the reference's rule from the finest rendered level
+    int32_t nSamples = 0;
+    if (lvr_samples_per_ray(&ctx->vi, ids, n, params->n_samples_per_ray, &nSamples) != LVR_OK)
+        return fail(ctx, LVR_EINVAL, "node id outside the octree");
     int tw = params->tile_w, th = params->tile_h;
     if (tw == 0 && th == 0)
     {
@@ -595,7 +601,7 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
         U->texMax[i] = ov + sz;
         U->texDim[i] = int32_t(vi.max_block[i]);
     }
-    U->nSamples = params->n_samples_per_ray;
+    U->nSamples = nSamples;
     U->maxSamples = params->max_samples_per_ray;
     for (int i = 0; i < 3; ++i)
     {
@@ -975,6 +981,8 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
         rc = growBuffer(ctx, &ctx->d_layers[layer], &ctx->layerCap[layer], size_t(W) * H);
         if (rc)
             return rc;
+        ctx->layerW[layer] = W;
+        ctx->layerH[layer] = H;
         U.layer = ctx->d_layers[layer];
     }
     else
@@ -1169,6 +1177,32 @@ int lvr_partial_device_ptr(lvr_ctx* ctx, int layer, void** ptr)
     return LVR_OK;
 }
 
+int lvr_partial_read(lvr_ctx* ctx, int layer, float* out_rgba32f, size_t bytes, int* width,
+                     int* height)
+{
+    if (!ctx || layer < 0 || layer >= lvr_ctx::kMaxLayers)
+        return LVR_EINVAL;
+    if (!ctx->d_layers[layer] || ctx->layerW[layer] <= 0)
+        return lvr::fail(ctx, LVR_EINVAL, "layer never rendered");
+    int rc = lvr::ensureDevice(ctx);
+    if (rc)
+        return rc;
+    const int W = ctx->layerW[layer], H = ctx->layerH[layer];
+    if (width)
+        *width = W;
+    if (height)
+        *height = H;
+    if (!out_rgba32f)
+        return LVR_OK;
+    const size_t need = size_t(W) * size_t(H) * sizeof(float4);
+    if (bytes < need)
+        return lvr::fail(ctx, LVR_ERANGE, "layer buffer too small");
+    LVR_HIP(ctx, hipMemcpyAsync(out_rgba32f, ctx->d_layers[layer], need, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return LVR_OK;
+}
+
 int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int width, int height,
                          int reps, float* kernel_ms)
 {
@@ -1184,6 +1218,18 @@ int lvr_composite_layers(lvr_ctx* ctx, const void* const* layers, int n, int wid
     {
         if (!layers[i])
             return fail(ctx, LVR_EINVAL, "composite: null layer");
+        // the context's own layers are checked against the viewport they were
+        // rendered at; a buffer a layer has outgrown is no longer valid
+        for (int k = 0; k < lvr_ctx::kMaxLayers; ++k)
+            if (layers[i] == ctx->d_layers[k] &&
+                (ctx->layerW[k] != width || ctx->layerH[k] != height))
+                return fail(ctx, LVR_EINVAL,
+                            "composite: layer " + std::to_string(k) + " holds " +
+                                std::to_string(ctx->layerW[k]) + " x " +
+                                std::to_string(ctx->layerH[k]) + " pixels");
+        for (void* p : ctx->retired)
+            if (layers[i] == p)
+                return fail(ctx, LVR_EINVAL, "composite: layer buffer was replaced");
         L.p[i] = static_cast<const float4*>(layers[i]);
     }
     rc = growBuffer(ctx, &ctx->d_frame, &ctx->fbCapacity, size_t(width) * height);

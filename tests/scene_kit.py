@@ -58,3 +58,26 @@ class Scene:
 def diff_stats(gpu_rgba8, ref_rgba8):
     d = np.abs(gpu_rgba8.astype(np.int32) - ref_rgba8.astype(np.int32))
     return int(d.max()), float(d.mean())
+
+
+def fp64_error(gpu_rgba8, ref32_rgba, ref64_rgba, label, samples=None):
+    """Error of a frame against the FP64 transcription of the shader, printed
+    beside the FP32 restatement's own error against FP64 (the precision the
+    reference shader computes in). Returns (max, mean, channels above 2/255)
+    of the frame. The FP64 bound: mean within 0.5/255, and at most 1e-4 of
+    the channels beyond 2/255 (FP32 rounding flips a texel or a sample at a
+    few pixels, in the FP32 restatement as much as in the kernel)."""
+    import oracle
+    ref64 = oracle.to_rgba8(ref64_rgba).astype(np.int32)
+    d = np.abs(gpu_rgba8.astype(np.int32) - ref64)
+    d32 = np.abs(oracle.to_rgba8(ref32_rgba).astype(np.int32) - ref64)
+    over, over32 = int((d > 2).sum()), int((d32 > 2).sum())
+    extra = ""
+    if samples is not None:
+        extra = " | samples kernel %d fp32 %d fp64 %d" % samples
+    print("FP64ERR %s max %d mean %.6f over2 %d of %d | fp32 oracle vs fp64: max %d mean %.6f "
+          "over2 %d%s" % (label, d.max(), d.mean(), over, d.size, d32.max(), d32.mean(), over32,
+                          extra), flush=True)
+    assert d.mean() <= 0.5 and over <= 1e-4 * d.size, (label, d.max(), d.mean(), over)
+    return int(d.max()), float(d.mean()), over
+

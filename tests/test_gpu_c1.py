@@ -9,7 +9,10 @@ Render: 1024x1024, 60 degree FOV, 8 cameras on a radius-2 orbit, 256 samples
 per unit (1 per voxel), early exit 0.99, SSE 1 px. Checked: visible NodeId
 sets and their order (bit-exact), the 256-bin histogram of each rendered set
 (bit-exact), the RGBA8 frames (within 2/255 max and 0.5/255 mean of the FP32
-oracle; identical so far) and the composite-step counts."""
+oracle; identical so far) and the composite-step counts. Each frame is also
+compared with the FP64 transcription of the shader (mean within 0.5/255,
+at most 1e-4 of the channels beyond 2/255), and the error is printed per
+camera beside the FP32 restatement's own (lines starting FP64ERR)."""
 import numpy as np
 import pytest
 
@@ -17,7 +20,7 @@ import oracle
 import livre_amd
 from livre_amd import scenes
 
-from scene_kit import diff_stats
+from scene_kit import diff_stats, fp64_error
 
 pytestmark = pytest.mark.gpu
 
@@ -63,6 +66,11 @@ def test_c1_orbit_frame(c1, k):
           ref_samples, flush=True)
     assert mx <= 2 and mean <= 0.5
     assert abs(int(samples) - int(ref_samples)) <= 1e-4 * ref_samples
+    # the FP64 transcription of the shader on the same bricks
+    ref64, samples64 = oracle.render(ovi, cam, ordered, [bricks[int(i)] for i in ordered],
+                                     lut.astype(np.float32) / np.float32(255.0),
+                                     oracle.render_params(256, 0.99, (0.0, 255.0)), "f64")
+    fp64_error(img, ref, ref64, "c1 camera %d" % k, (samples, ref_samples, samples64))
     # histogram of the rendered set
     got = ctx.histogram(ordered, 256)
     want = oracle.histogram(ovi, ordered, [bricks[int(i)] for i in ordered], 256, (0.0, 255.0))

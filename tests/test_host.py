@@ -7,6 +7,7 @@ import re
 import numpy as np
 import pytest
 
+import livre_amd
 from livre_amd import scenes
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -238,3 +239,20 @@ def test_oracle_jpeg_worst_case_fits_the_bound(lvr, orc):
     px = rng.integers(0, 256, (64, 64, 4), dtype=np.uint8)
     px[..., :3] = np.where(rng.random((64, 64, 3)) < 0.5, 0, 255)
     assert len(orc.jpeg_encode(px, quality=100)) <= lvr.lib().lvr_jpeg_max_size(64, 64)
+
+
+def test_samples_per_ray_rule_host(lvr):
+    """RayCastRenderer.cpp:216-234, host side: maxVoxelDim / 2^(depth - maxLOD - 1), at least 512."""
+    vi = livre_amd.volume_info(2048, 64, 1, livre_amd.DT_UINT16)
+    assert vi.depth == 6
+    nid = scenes.node_id
+    assert livre_amd.samples_per_ray(vi, [nid(5, 3, 4, 5)]) == 2048
+    assert livre_amd.samples_per_ray(vi, [nid(2, 0, 0, 0), nid(4, 1, 1, 1)]) == 1024
+    assert livre_amd.samples_per_ray(vi, [nid(1, 0, 0, 0), nid(2, 1, 1, 1)]) == 512
+    assert livre_amd.samples_per_ray(vi, []) == 512
+    assert livre_amd.samples_per_ray(vi, [nid(5, 0, 0, 0)], requested=300) == 300
+    small = livre_amd.volume_info([300, 200, 100], 32, 1, livre_amd.DT_UINT8)
+    top = small.depth - 1
+    assert livre_amd.samples_per_ray(small, [nid(top, 0, 0, 0)]) == 512
+    with pytest.raises(livre_amd.LivreError):
+        livre_amd.samples_per_ray(vi, [nid(6, 0, 0, 0)])

@@ -1005,6 +1005,7 @@ def main():
     # (the context keeps the last four frames'). A frame's latency runs from
     # the queueing of its selection to the moment its completion is observed.
     lat, samples, bricks, entries, rc_ms, bin_ms, hist_ms = [], 0, [], [], [], [], []
+    host_ms = []
     alg_bytes = []
     last = args.warmup + args.steps - 1
 
@@ -1040,6 +1041,7 @@ def main():
         rc_ms.append(st.raycast_ms)
         bin_ms.append(st.bin_ms)
         hist_ms.append(st.histogram_ms)
+        host_ms.append(st.host_ms)
         alg_bytes.append(st.bricks * st.brick_bytes + W * H * 4 + TF_ENTRIES * 16)
         if i % 20 == 0:
             log("frame %d: %d bricks, %.2f Gsamples, raycast %.2f ms, frame %.2f ms"
@@ -1090,6 +1092,7 @@ def main():
         "raycast_ms_mean": float(rc.mean()),
         "bin_ms_mean": float(np.mean(bin_ms)),
         "histogram_ms_mean": float(np.mean(hist_ms)),
+        "render_call_host_ms_mean": float(np.mean(host_ms)),
         "kernel_gsamples_per_s": samples / (rc.sum() * 1e-3) / 1e9,
         "bandwidth_ceiling_gsamples_per_s": ceiling,
         "l2_hit_rate": profiled_l2_hit_rate(),

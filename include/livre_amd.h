@@ -306,9 +306,13 @@ typedef struct
 /* Render one frame of the ordered bricks (fragRayCast.glsl:113-207 per brick,
  * RayCastRenderer.cpp:467-571) into the context framebuffer, followed by
  * the RGBA32F -> RGBA8 copy (fragTexCopy.glsl:11-15). viewport = {x, y, w, h};
- * only w and h are used (the frame covers the whole viewport). Bricks that
- * are not resident are skipped as the reference skips invalid textures.
- * Asynchronous on the context stream. */
+ * only w and h are used (the frame covers the whole viewport). The host sends
+ * each brick's NodeId and page-table index; the device builds the brick
+ * records and reads each brick's slot from the device page table when the
+ * frame's GPU work starts (the reference looks each texture up by NodeId when
+ * it draws the brick, RayCastRenderer.cpp:510-522). Bricks that are not
+ * resident then are skipped, as the reference skips a missing texture
+ * (RayCastRenderer.cpp:517-522). Asynchronous on the context stream. */
 int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                const uint64_t* ordered_ids, int n,
                const lvr_render_params* params);
@@ -371,6 +375,8 @@ typedef struct
     float bin_ms;          /* binning kernels */
     float raycast_ms;      /* ray-cast kernel */
     float histogram_ms;    /* histogram kernels of the last lvr_histogram */
+    float host_ms;         /* host time of the lvr_render call: preparation of the
+                              brick keys and screen ranges, staging, launches */
 } lvr_frame_stats;
 int lvr_set_timing(lvr_ctx* ctx, int enable);
 /* Statistics of the last rendered frame (waits for it). */

@@ -82,7 +82,7 @@ class FrameStats(ctypes.Structure):
     _fields_ = [("samples", ctypes.c_uint64), ("bricks", ctypes.c_uint64),
                 ("tile_entries", ctypes.c_uint64), ("brick_bytes", ctypes.c_uint64),
                 ("bin_ms", ctypes.c_float), ("raycast_ms", ctypes.c_float),
-                ("histogram_ms", ctypes.c_float)]
+                ("histogram_ms", ctypes.c_float), ("host_ms", ctypes.c_float)]
 
 
 class LivreError(RuntimeError):

@@ -48,14 +48,13 @@ size_t bytesPerVoxel(int dataType);
 // World box of a node, as internalNodeToLODNode computes it
 // (livre/data/DataSourcePlugin.cpp:40-71), in FP32 with the same order of
 // operations. Usable on host and device.
-__host__ __device__ inline void nodeWorldBox(const lvr_volume_info& vi,
-                                             const NodeKey& k, float* mn,
-                                             float* mx)
+__host__ __device__ inline void nodeWorldBox(const float* worldSize, const uint32_t* rootBlocks,
+                                             const NodeKey& k, float* mn, float* mx)
 {
     const uint32_t scale = 1u << k.level;
-    const uint32_t bx = vi.root_blocks[0] * scale;
-    const uint32_t by = vi.root_blocks[1] * scale;
-    const uint32_t bz = vi.root_blocks[2] * scale;
+    const uint32_t bx = rootBlocks[0] * scale;
+    const uint32_t by = rootBlocks[1] * scale;
+    const uint32_t bz = rootBlocks[2] * scale;
     uint32_t bmax = bx;
     if (by > bmax)
         bmax = by;
@@ -68,22 +67,41 @@ __host__ __device__ inline void nodeWorldBox(const lvr_volume_info& vi,
     {
         const float cmin = float(pos[i]) / denom;
         const float cmax = float(pos[i] + 1u) / denom;
-        const float half = vi.world_size[i] * 0.5f;
-        mn[i] = cmin * vi.world_size[i] - half;
-        mx[i] = cmax * vi.world_size[i] - half;
+        const float half = worldSize[i] * 0.5f;
+        mn[i] = cmin * worldSize[i] - half;
+        mx[i] = cmax * worldSize[i] - half;
     }
 }
 
-// Per-brick record of one frame, in the order of the ordered list. 64 bytes.
+__host__ __device__ inline void nodeWorldBox(const lvr_volume_info& vi, const NodeKey& k,
+                                             float* mn, float* mx)
+{
+    nodeWorldBox(vi.world_size, vi.root_blocks, k, mn, mx);
+}
+
+// What the host sends per brick of a frame: the NodeId and its page-table
+// index (-1 for an id outside the octree). 16 bytes.
+struct BrickKey
+{
+    uint64_t id;
+    int32_t page;
+    uint32_t pad;
+};
+
+// Per-brick record of one frame, in the order of the ordered list, built on
+// the device from the BrickKey (brickRecordsKernel) when the frame's GPU work
+// starts; the slot comes from the device page table at that point, so a
+// brick evicted after the host prepared the frame (or a batch replayed after
+// an eviction) reads -1 and is skipped. 64 bytes.
 struct BrickEntry
 {
     float bmin[3];
     float bmax[3];
     float ext[3];   // bmax - bmin, rounded as the shader's (aabbMax - aabbMin)
-    // 1 / ext, correctly rounded (host division); with kBrickPow2 set,
-    // (1 / ext) * texExt, exact because ext is a power of two
+    // 1 / ext, correctly rounded; with kBrickPow2 set, (1 / ext) * texExt,
+    // exact because ext is a power of two
     float rcp[3];
-    int32_t slot;   // pool slot, -1 if not resident
+    int32_t slot;   // pool slot from the device page table, -1 if not resident
     uint32_t level;
     uint32_t flags;
     uint32_t pad;
@@ -94,8 +112,11 @@ struct BrickEntry
 // as the division and the multiply are.
 constexpr uint32_t kBrickPow2 = 1u;
 
-// Dense device page table over the octree: per level,
-// levelOffset[L] + x + nx * (y + ny * z) holds a slot index or -1.
+// Dense page table over the octree: per level,
+// levelOffset[L] + x + nx * (y + ny * z) holds a slot index or -1. The host
+// copy is authoritative; entries changed since the last frame are listed in
+// `dirty` and written into the device copy by a scatter kernel on the render
+// stream before the next launch that reads it (applyPageTable).
 struct PageTable
 {
     std::vector<uint64_t> levelOffset; // depth + 1 entries
@@ -103,7 +124,14 @@ struct PageTable
     int32_t* d_slots = nullptr;
     std::vector<int32_t> h_slots;
     uint64_t size = 0;
-    uint64_t dirtyLo = ~0ull, dirtyHi = 0; // host entries not yet on the device
+    std::vector<uint32_t> dirty;   // page indices changed on the host
+    std::vector<uint8_t> isDirty;  // per page: listed in `dirty`
+    // fixed staging of (page, slot) pairs, allocated with the context
+    static constexpr int kUpdateChunk = 32768;
+    int2* h_update = nullptr;  // pinned
+    int2* d_update = nullptr;
+    hipEvent_t updateDone = nullptr; // last scatter that read h_update
+    bool updateArmed = false;
 };
 
 struct PoolEntry
@@ -208,7 +236,11 @@ struct FrameUniforms
     int32_t nItems;    // work items (16 per 4x4 super-tile)
     int32_t dataType;
     int32_t request;   // index in the batch
+    // volume shape for the device-built brick records
+    float worldSize[3];
+    uint32_t rootBlocks[3];
     const uint8_t* pool;
+    const int32_t* pageTable; // device page table of the volume: slot or -1
     const float4* tf;  // nTf + 2 RGBA entries: tf[0], the nTf entries, tf[nTf - 1]
     const float4* cls; // classified table per raw value (integer NEAREST), or null
     uint32_t* frame;   // RGBA8 output, row 0 = bottom
@@ -264,6 +296,10 @@ struct lvr_ctx
     uint64_t nSlots = 0;
     uint8_t* d_pool = nullptr;
     lvr::PageTable pt;
+    // recorded after the last page-table scatter, on the stream it ran on:
+    // record passes on other streams wait for it before they read the table
+    hipEvent_t tableReady = nullptr;
+    bool tableRecorded = false;
     std::vector<int32_t> freeSlots;
     std::unordered_map<uint64_t, lvr::PoolEntry> resident;
     lvr::DenseLru lru; // over page indices
@@ -314,6 +350,8 @@ struct lvr_ctx
     {
         lvr::FrameUniforms* req = nullptr;
         size_t reqCap = 0;
+        lvr::BrickKey* keys = nullptr;
+        size_t keyCap = 0;
         lvr::BrickEntry* bricks = nullptr;
         size_t brickCap = 0;
         int4* ranges = nullptr;
@@ -331,7 +369,7 @@ struct lvr_ctx
     RecordSet recSets[2];
     int recNext = 0;
     hipStream_t recStream = nullptr;
-    std::vector<lvr::BrickEntry> hostBricks;
+    std::vector<lvr::BrickKey> hostBricks;
     std::vector<int4> hostRanges;
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
     lvr::RawPyramid* pyramid = nullptr; // levels of the last raw source loaded
@@ -388,6 +426,7 @@ struct lvr_ctx
     struct FrameRecord
     {
         uint64_t bricks = 0, entries = 0;
+        float hostMs = 0.f; // host time of the render call
         bool timed = false, histTimed = false, armed = false;
         // binning start (recStream), ray-cast start, ray-cast end (on
         // `stream`), histogram start and end (on `histStream`), binning end
@@ -429,7 +468,11 @@ int32_t slotAndTouch(lvr_ctx* ctx, uint64_t id);
 void destroyPyramid(lvr_ctx* ctx);
 // The frame grab's JPEG encoder (jpeg.hip), freed with the context.
 void destroyJpeg(lvr_ctx* ctx);
-int flushPageTable(lvr_ctx* ctx);
+// Write the page-table entries changed on the host into the device table on
+// stream `st`, and make `st` wait for the last such update wherever it ran:
+// kernels queued on `st` afterwards read the current table. Only the brick
+// record pass reads the table, so updates never race a ray cast.
+int applyPageTable(lvr_ctx* ctx, hipStream_t st);
 int acquireSlotFor(lvr_ctx* ctx, uint64_t id, int32_t* slot);
 // Undo acquireSlotFor for a brick whose contents were never written.
 int releaseSlot(lvr_ctx* ctx, uint64_t id);

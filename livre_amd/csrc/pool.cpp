@@ -4,8 +4,11 @@
 // The pool is one hipMalloc arena of fixed-size slots (one slot per brick of
 // maximumBlockSize^3 voxels, 256-byte aligned), the HBM counterpart of
 // TexturePool's one-3D-texture-per-brick scheme (TexturePool.cpp:90-126).
-// The page table maps a NodeId to its slot on the device, so kernels can
-// resolve bricks without the host.
+// The page table maps a NodeId to its slot. The ray-cast kernels read the
+// device copy when they draw a brick, as the reference looks each brick's
+// texture up by NodeId at draw time (RayCastRenderer.cpp:510-522); the host
+// copy is authoritative and its changes reach the device through
+// applyPageTable, ordered on the render stream between launches.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -127,24 +130,62 @@ static int setPage(lvr_ctx* ctx, uint64_t id, int32_t slot)
     if (!pageIndex(ctx, id, &idx))
         return fail(ctx, LVR_EINVAL, "node id outside the octree");
     ctx->pt.h_slots[idx] = slot;
-    ctx->pt.dirtyLo = std::min(ctx->pt.dirtyLo, idx);
-    ctx->pt.dirtyHi = std::max(ctx->pt.dirtyHi, idx + 1);
+    if (!ctx->pt.isDirty[idx])
+    {
+        ctx->pt.isDirty[idx] = 1;
+        ctx->pt.dirty.push_back(uint32_t(idx));
+    }
     return LVR_OK;
 }
 
-// Copy the dirty range of the host page table to the device on the render
-// stream, before any kernel that resolves bricks through it.
-int flushPageTable(lvr_ctx* ctx)
+// table[page] = slot for each listed pair
+__global__ __launch_bounds__(256) void pageScatterKernel(const int2* __restrict__ upd, int n,
+                                                         int32_t* __restrict__ table)
 {
-    if (ctx->pt.dirtyLo >= ctx->pt.dirtyHi)
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        table[upd[i].x] = upd[i].y;
+}
+
+int applyPageTable(lvr_ctx* ctx, hipStream_t st)
+{
+    PageTable& pt = ctx->pt;
+    if (pt.dirty.empty())
+    {
+        // the last update may have run on another stream
+        if (ctx->tableRecorded)
+            LVR_HIP(ctx, hipStreamWaitEvent(st, ctx->tableReady, 0));
         return LVR_OK;
-    const uint64_t lo = ctx->pt.dirtyLo, hi = ctx->pt.dirtyHi;
-    LVR_HIP(ctx, hipMemcpyAsync(ctx->pt.d_slots + lo, &ctx->pt.h_slots[lo],
-                                (hi - lo) * sizeof(int32_t),
-                                hipMemcpyHostToDevice, ctx->stream));
-    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->pt.dirtyLo = ~0ull;
-    ctx->pt.dirtyHi = 0;
+    }
+    if (ctx->tableRecorded)
+        LVR_HIP(ctx, hipStreamWaitEvent(st, ctx->tableReady, 0));
+    size_t done = 0;
+    while (done < pt.dirty.size())
+    {
+        const int cnt = int(std::min<size_t>(PageTable::kUpdateChunk, pt.dirty.size() - done));
+        // the pinned pairs are rewritten only once the copy that last read
+        // them has completed
+        if (pt.updateArmed)
+            LVR_HIP(ctx, hipEventSynchronize(pt.updateDone));
+        pt.updateArmed = false;
+        for (int i = 0; i < cnt; ++i)
+        {
+            const uint32_t idx = pt.dirty[done + size_t(i)];
+            pt.h_update[i] = make_int2(int(idx), pt.h_slots[idx]);
+            pt.isDirty[idx] = 0;
+        }
+        LVR_HIP(ctx, hipMemcpyAsync(pt.d_update, pt.h_update, sizeof(int2) * size_t(cnt),
+                                    hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(pageScatterKernel, dim3((cnt + 255) / 256), dim3(256), 0, st,
+                           pt.d_update, cnt, pt.d_slots);
+        LVR_HIP(ctx, hipGetLastError());
+        LVR_HIP(ctx, hipEventRecord(pt.updateDone, st));
+        pt.updateArmed = true;
+        done += size_t(cnt);
+    }
+    pt.dirty.clear();
+    LVR_HIP(ctx, hipEventRecord(ctx->tableReady, st));
+    ctx->tableRecorded = true;
     return LVR_OK;
 }
 
@@ -451,6 +492,7 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
     if (total > uint64_t(INT32_MAX))
         return bail(LVR_EINVAL);
     ctx->pt.h_slots.assign(total, -1);
+    ctx->pt.isDirty.assign(total, 0);
     ctx->lru.init(size_t(total));
     if (hipMalloc((void**)&ctx->pt.d_slots, total * sizeof(int32_t)) !=
         hipSuccess)
@@ -458,6 +500,13 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
     if (hipMemcpy(ctx->pt.d_slots, ctx->pt.h_slots.data(),
                   total * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess)
         return bail(LVR_EHIP);
+    if (hipHostMalloc((void**)&ctx->pt.h_update, sizeof(int2) * lvr::PageTable::kUpdateChunk,
+                      hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&ctx->pt.d_update, sizeof(int2) * lvr::PageTable::kUpdateChunk) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&ctx->pt.updateDone, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->tableReady, hipEventDisableTiming) != hipSuccess)
+        return bail(LVR_ENOMEM);
 
     ctx->freeSlots.reserve(ctx->nSlots);
     for (int64_t s = int64_t(ctx->nSlots) - 1; s >= 0; --s)
@@ -514,6 +563,13 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     (void)hipFree(ctx->d_gen);
     (void)hipFree(ctx->d_cls);
     (void)hipFree(ctx->pt.d_slots);
+    (void)hipFree(ctx->pt.d_update);
+    if (ctx->pt.h_update)
+        (void)hipHostFree(ctx->pt.h_update);
+    if (ctx->pt.updateDone)
+        (void)hipEventDestroy(ctx->pt.updateDone);
+    if (ctx->tableReady)
+        (void)hipEventDestroy(ctx->tableReady);
     (void)hipFree(ctx->d_tf);
     (void)hipFree(ctx->d_frame);
     if (ctx->recStream)
@@ -523,7 +579,7 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     }
     for (auto& rs : ctx->recSets)
     {
-        void* bufs[] = {rs.req,        rs.bricks,     rs.ranges,    rs.tileCount,
+        void* bufs[] = {rs.req,        rs.bricks,     rs.keys,      rs.ranges,    rs.tileCount,
                         rs.tileOffset, rs.tileList,   rs.superCount, rs.superOffset,
                         rs.superList,  rs.counters};
         for (void* p : bufs)

@@ -527,6 +527,12 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         const BrickEntry b = bricks[bi];
         if (!alive)
             continue;
+        // the slot the record pass read from the device page table; -1 for
+        // a brick that is not resident is skipped, as the reference skips a
+        // missing texture (RayCastRenderer.cpp:517-522)
+        const int slot = b.slot;
+        if (slot < 0)
+            continue;
         if (C.acc3 > U.exitA) // imageLoad + discard (fragRayCast.glsl:115-117)
         {
             alive = false;
@@ -575,7 +581,7 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         const float step[3] = {seg[0] / segLen * stepSize, seg[1] / segLen * stepSize,
                                seg[2] / segLen * stepSize};
         float pos[3] = {rs[0], rs[1], rs[2]};
-        const T* vox = reinterpret_cast<const T*>(U.pool + size_t(b.slot) * U.slotStride);
+        const T* vox = reinterpret_cast<const T*>(U.pool + size_t(slot) * U.slotStride);
 
         // the brick's flags are wave-uniform (scalar loads): no divergence
         const bool p2 = (b.flags & kBrickPow2) != 0;
@@ -637,6 +643,46 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         }
         U.frame[size_t(py) * U.width + px] = packed;
     }
+}
+
+// The per-brick record of a frame from its key (the device half of frame
+// preparation): the pool slot from the device page table (the reference looks
+// each brick's texture up by NodeId when it draws it, RayCastRenderer.cpp:
+// 510-522), the world box as internalNodeToLODNode computes it
+// (DataSourcePlugin.cpp:40-71), its extent, the correctly rounded reciprocal
+// of the extent, and for power-of-two extents that reciprocal times the
+// texture extent (exact) with kBrickPow2 set.
+__device__ inline BrickEntry makeBrickEntry(const FrameUniforms& U, const BrickKey& key)
+{
+    BrickEntry e;
+    const NodeKey k = decodeNode(key.id);
+    nodeWorldBox(U.worldSize, U.rootBlocks, k, e.bmin, e.bmax);
+    bool pow2 = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        e.ext[a] = e.bmax[a] - e.bmin[a];
+        e.rcp[a] = 1.0f / e.ext[a];
+        // a normal power of two: biased exponent in [1, 254], mantissa 0;
+        // the scaled reciprocal must stay normal too
+        const uint32_t u = __float_as_uint(e.ext[a]);
+        const uint32_t ex = (u >> 23) & 0xFFu;
+        const uint32_t sc = (__float_as_uint(e.rcp[a] * U.texExt[a]) >> 23) & 0xFFu;
+        pow2 = pow2 && (u & 0x807FFFFFu) == 0u && ex != 0u && ex != 0xFFu && sc != 0u &&
+               sc != 0xFFu;
+    }
+    e.flags = 0u;
+    if (pow2)
+    {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            e.rcp[a] = e.rcp[a] * U.texExt[a];
+        e.flags = kBrickPow2;
+    }
+    e.slot = key.page >= 0 ? U.pageTable[key.page] : -1;
+    e.level = k.level;
+    e.pad = 0u;
+    return e;
 }
 
 __device__ inline unsigned long long waveSum(unsigned long long s)

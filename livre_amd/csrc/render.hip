@@ -22,6 +22,7 @@
 // early-exit opacity are rounded exactly as written; the linear filter of the
 // TF and the colour blend use explicit FMAs, which moves colours by an ulp.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -223,6 +224,43 @@ static void enqueueBinning(hipStream_t st, const FrameUniforms* reqs, int nReq,
                        nCounters);
     hipLaunchKernelGGL(tileBinKernel<true>, dim3(tBlocks), dim3(256), 0, st, reqs, nReq, ranges,
                        tiles, sOffset, sCount, sList, tOffset, tList);
+}
+
+__device__ inline int findByBrick(const FrameUniforms* __restrict__ reqs, int nReq, int brick)
+{
+    int lo = 0, hi = nReq - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (reqs[mid].brickBase <= brick)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Brick records of every request from the keys the host sent (one thread
+// per brick; requests are sorted by brickBase).
+__global__ __launch_bounds__(256) void brickRecordsKernel(const FrameUniforms* __restrict__ reqs,
+                                                          int nReq,
+                                                          const BrickKey* __restrict__ keys,
+                                                          int nBricks,
+                                                          BrickEntry* __restrict__ out)
+{
+    const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= nBricks)
+        return;
+    const FrameUniforms& U = reqs[findByBrick(reqs, nReq, i)];
+    out[i] = makeBrickEntry(U, keys[i]);
+}
+
+static void enqueueRecords(hipStream_t st, const FrameUniforms* reqs, int nReq,
+                           const BrickKey* keys, int nBricks, BrickEntry* out)
+{
+    if (nBricks > 0)
+        hipLaunchKernelGGL(brickRecordsKernel, dim3((nBricks + 255) / 256), dim3(256), 0, st,
+                           reqs, nReq, keys, nBricks, out);
 }
 
 __device__ inline int nextItem(unsigned long long* counter, int lane)
@@ -547,7 +585,7 @@ static void dataRange(int dt, float tf0, float tf1, float* r0, float* r1)
 static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                         const uint64_t* ids, int n, const lvr_render_params* params,
                         int nTf, float tf0, float tf1, FrameUniforms* U,
-                        std::vector<BrickEntry>& bricks, std::vector<int4>& ranges,
+                        std::vector<BrickKey>& bricks, std::vector<int4>& ranges,
                         uint64_t* listTotal, uint64_t* superTotal, uint64_t* onScreen)
 {
     if (!f || !viewport || !params || n < 0 || (n > 0 && !ids))
@@ -647,6 +685,12 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     }
     U->dataType = vi.data_type;
     U->pool = ctx->d_pool;
+    U->pageTable = ctx->pt.d_slots;
+    for (int a = 0; a < 3; ++a)
+    {
+        U->worldSize[a] = vi.world_size[a];
+        U->rootBlocks[a] = vi.root_blocks[a];
+    }
     U->nItems = ((U->tilesX + kSuper - 1) / kSuper) * ((U->tilesY + kSuper - 1) / kSuper) *
                 kSuper * kSuper;
     U->brickBase = int32_t(bricks.size());
@@ -665,38 +709,28 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
         const NodeKey key = decodeNode(ids[b]);
         if (key.level >= vi.depth)
             return fail(ctx, LVR_EINVAL, "node id outside the octree");
-        BrickEntry e;
-        std::memset(&e, 0, sizeof e);
+        // the key goes to the device, which builds the brick's record
+        // (brickRecordsKernel) and reads its slot from the page table at
+        // draw time; the host keeps the screen range and the residency test
+        BrickKey e;
+        e.id = ids[b];
+        e.pad = 0;
+        uint64_t page = 0;
+        e.page = pageIndex(ctx, ids[b], &page) ? int32_t(page) : -1;
         int4 r = make_int4(1, 1, 0, 0); // empty range
-        nodeWorldBox(vi, key, e.bmin, e.bmax);
-        bool pow2 = true;
-        for (int a = 0; a < 3; ++a)
-        {
-            e.ext[a] = e.bmax[a] - e.bmin[a];
-            e.rcp[a] = 1.0f / e.ext[a];
-            int ex = 0;
-            pow2 = pow2 && std::isnormal(e.ext[a]) && std::frexp(e.ext[a], &ex) == 0.5f &&
-                   std::isnormal(e.rcp[a] * U->texExt[a]);
-        }
-        if (pow2)
-        {
-            for (int a = 0; a < 3; ++a)
-                e.rcp[a] = e.rcp[a] * U->texExt[a]; // power-of-two scaling: exact
-            e.flags = kBrickPow2;
-        }
-        e.level = key.level;
-        e.slot = slotAndTouch(ctx, ids[b]);
-        if (e.slot >= 0)
+        float bmin[3], bmax[3];
+        nodeWorldBox(vi, key, bmin, bmax);
+        if (e.page >= 0 && slotAndTouch(ctx, ids[b]) >= 0)
         {
             // clip coordinates of the 8 corners: the min corner plus the
             // projected edge vectors
             double base[4], edge[3][4];
             for (int i = 0; i < 4; ++i)
             {
-                base[i] = mvp[i] * e.bmin[0] + mvp[4 + i] * e.bmin[1] + mvp[8 + i] * e.bmin[2] +
+                base[i] = mvp[i] * bmin[0] + mvp[4 + i] * bmin[1] + mvp[8 + i] * bmin[2] +
                           mvp[12 + i];
                 for (int a = 0; a < 3; ++a)
-                    edge[a][i] = mvp[4 * a + i] * (double(e.bmax[a]) - double(e.bmin[a]));
+                    edge[a][i] = mvp[4 * a + i] * (double(bmax[a]) - double(bmin[a]));
             }
             double mnx = 1e300, mny = 1e300, mxx = -1e300, mxy = -1e300;
             bool full = false;
@@ -784,7 +818,7 @@ struct lvr_batch
     lvr_ctx* errCtx = nullptr; // error sink for the shared helpers
 
     std::vector<lvr::FrameUniforms> reqs;
-    std::vector<lvr::BrickEntry> bricks;
+    std::vector<lvr::BrickKey> bricks;
     std::vector<int4> ranges;
     std::vector<float4> tf;
     std::vector<size_t> tfOffset, frameOffset;
@@ -795,6 +829,8 @@ struct lvr_batch
     size_t reqCap = 0;
     lvr::BrickEntry* d_bricks = nullptr;
     size_t brickCap = 0;
+    lvr::BrickKey* d_keys = nullptr;
+    size_t keyCap = 0;
     int4* d_ranges = nullptr;
     size_t rangeCap = 0;
     float4* d_tf = nullptr;
@@ -957,13 +993,14 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     using namespace lvr;
     if (!ctx)
         return LVR_EINVAL;
+    const auto hostStart = std::chrono::steady_clock::now();
     if (!ctx->tfSet)
         return fail(ctx, LVR_EINVAL, "no transfer function set");
     int rc = ensureDevice(ctx);
     if (rc)
         return rc;
     FrameUniforms U;
-    std::vector<BrickEntry>& bricks = ctx->hostBricks;
+    std::vector<BrickKey>& bricks = ctx->hostBricks;
     std::vector<int4>& ranges = ctx->hostRanges;
     bricks.clear();
     ranges.clear();
@@ -1006,6 +1043,9 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     rc = growBuffer(ctx, &rs.bricks, &rs.brickCap, std::max<size_t>(1, size_t(n)));
     if (rc)
         return rc;
+    rc = growBuffer(ctx, &rs.keys, &rs.keyCap, std::max<size_t>(1, size_t(n)));
+    if (rc)
+        return rc;
     rc = growBuffer(ctx, &rs.ranges, &rs.rangeCap, std::max<size_t>(1, size_t(n)));
     if (rc)
         return rc;
@@ -1035,11 +1075,11 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     ctx->d_superList = rs.superList;
     ctx->curCounters = rs.counters;
 
-    // uniforms, brick entries and ranges through one pinned staging buffer,
+    // uniforms, brick keys and ranges through one pinned staging buffer,
     // copied on recStream (while the previous frame still ray-casts) once
     // the ray cast that last read this record set is done
     {
-        const size_t bBytes = sizeof(BrickEntry) * size_t(n), rBytes = sizeof(int4) * size_t(n);
+        const size_t bBytes = sizeof(BrickKey) * size_t(n), rBytes = sizeof(int4) * size_t(n);
         uint8_t* h = nullptr;
         int slot = 0;
         LVR_HIP(ctx, ctx->frameStaging.acquire(sizeof(FrameUniforms) + bBytes + rBytes, &h,
@@ -1053,7 +1093,7 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
                                     hipMemcpyHostToDevice, ctx->recStream));
         if (n > 0)
         {
-            LVR_HIP(ctx, hipMemcpyAsync(ctx->d_bricks, h + sizeof(FrameUniforms), bBytes,
+            LVR_HIP(ctx, hipMemcpyAsync(rs.keys, h + sizeof(FrameUniforms), bBytes,
                                         hipMemcpyHostToDevice, ctx->recStream));
             LVR_HIP(ctx, hipMemcpyAsync(ctx->d_ranges, h + sizeof(FrameUniforms) + bBytes,
                                         rBytes, hipMemcpyHostToDevice, ctx->recStream));
@@ -1072,6 +1112,11 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     rec.histTimed = false;
     if (rec.timed)
         LVR_HIP(ctx, hipEventRecord(rec.ev[0], ctx->recStream));
+    // page-table changes since the last frame, then the records that read it
+    rc = applyPageTable(ctx, ctx->recStream);
+    if (rc)
+        return rc;
+    enqueueRecords(ctx->recStream, ctx->d_req, 1, rs.keys, n, ctx->d_bricks);
     enqueueBinning(ctx->recStream, ctx->d_req, 1, ctx->d_ranges, nSupers, nTiles,
                    ctx->d_superCount, ctx->d_superOffset, ctx->d_superList, ctx->d_tileCount,
                    ctx->d_tileOffset, ctx->d_tileList, ctx->curCounters, 2);
@@ -1109,6 +1154,8 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
                                           ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
                                           ctx->curCounters);
         LVR_HIP(ctx, hipGetLastError());
+        rec.hostMs = std::chrono::duration<float, std::milli>(
+                         std::chrono::steady_clock::now() - hostStart).count();
         if ((rc = finishFrame(ctx, rec)))
             return rc;
         LVR_HIP(ctx, hipEventRecord(rs.free, ctx->stream));
@@ -1147,6 +1194,9 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
         break;
     }
     LVR_HIP(ctx, hipGetLastError());
+    rec.hostMs = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() -
+                                                          hostStart)
+                     .count();
     if ((rc = finishFrame(ctx, rec)))
         return rc;
     LVR_HIP(ctx, hipEventRecord(rs.free, ctx->stream));
@@ -1334,6 +1384,7 @@ int lvr_frame_stats_back(lvr_ctx* ctx, int back, lvr_frame_stats* out)
     out->samples = ctx->h_frameSamples[slot];
     out->bricks = rec.bricks;
     out->tile_entries = rec.entries;
+    out->host_ms = rec.hostMs;
     if (rec.timed)
     {
         LVR_HIP(ctx, hipEventElapsedTime(&out->bin_ms, rec.ev[0], rec.ev[5]));
@@ -1388,6 +1439,7 @@ void lvr_batch_destroy(lvr_batch* b)
         (void)hipGraphExecDestroy(b->exec);
     (void)hipFree(b->d_reqs);
     (void)hipFree(b->d_bricks);
+    (void)hipFree(b->d_keys);
     (void)hipFree(b->d_ranges);
     (void)hipFree(b->d_tf);
     (void)hipFree(b->d_frames);
@@ -1523,6 +1575,7 @@ int lvr_batch_run(lvr_batch* b)
     int rc = LVR_OK;
     if ((rc = bgrow(b, &b->d_reqs, &b->reqCap, size_t(nReq), &grew)) ||
         (rc = bgrow(b, &b->d_bricks, &b->brickCap, std::max<size_t>(1, b->bricks.size()), &grew)) ||
+        (rc = bgrow(b, &b->d_keys, &b->keyCap, std::max<size_t>(1, b->bricks.size()), &grew)) ||
         (rc = bgrow(b, &b->d_ranges, &b->rangeCap, std::max<size_t>(1, b->ranges.size()), &grew)) ||
         (rc = bgrow(b, &b->d_tf, &b->tfCap, b->tf.size(), &grew)) ||
         (rc = bgrow(b, &b->d_frames, &b->frameCap, b->pixels, &grew)) ||
@@ -1594,6 +1647,7 @@ int lvr_batch_run(lvr_batch* b)
                                 items,
                                 int64_t(reinterpret_cast<intptr_t>(b->d_reqs)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_bricks)),
+                                int64_t(reinterpret_cast<intptr_t>(b->d_keys)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_ranges)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_tf)),
                                 int64_t(reinterpret_cast<intptr_t>(b->d_counts)),
@@ -1608,7 +1662,7 @@ int lvr_batch_run(lvr_batch* b)
     // copied before the launch sequence
     {
         const size_t qB = sizeof(FrameUniforms) * size_t(nReq);
-        const size_t bB = sizeof(BrickEntry) * b->bricks.size();
+        const size_t bB = sizeof(BrickKey) * b->bricks.size();
         const size_t rB = sizeof(int4) * b->ranges.size();
         const size_t tB = sizeof(float4) * b->tf.size();
         uint8_t* h = nullptr;
@@ -1621,7 +1675,7 @@ int lvr_batch_run(lvr_batch* b)
         LVR_BHIP(b, hipMemcpyAsync(b->d_reqs, h, qB, hipMemcpyHostToDevice, b->stream));
         if (bB)
         {
-            LVR_BHIP(b, hipMemcpyAsync(b->d_bricks, h + qB, bB, hipMemcpyHostToDevice,
+            LVR_BHIP(b, hipMemcpyAsync(b->d_keys, h + qB, bB, hipMemcpyHostToDevice,
                                        b->stream));
             LVR_BHIP(b, hipMemcpyAsync(b->d_ranges, h + qB + bB, rB, hipMemcpyHostToDevice,
                                        b->stream));
@@ -1632,7 +1686,14 @@ int lvr_batch_run(lvr_batch* b)
         LVR_BHIP(b, b->staging.release(slot, b->stream));
     }
 
+    // the pools' page-table changes reach the device before the record pass
+    // of the launch reads the tables
+    for (lvr_ctx* c : b->ctxs)
+        if ((rc = applyPageTable(c, b->stream)))
+            return bfail(b, rc, c->error);
+
     auto enqueue = [&](hipStream_t st) -> hipError_t {
+        enqueueRecords(st, b->d_reqs, nReq, b->d_keys, int(b->bricks.size()), b->d_bricks);
         enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,
                        b->d_soffsets, b->d_slists, b->d_counts, b->d_offsets, b->d_lists,
                        b->d_counters, nReq + 2);

@@ -114,3 +114,51 @@ def test_batch_rejects_what_it_does_not_render(volumes):
             b.add(ctx, fr, 32, 32, s.ids[:4],
                   livre_amd.render_params(32, samples_per_thread=3),
                   scenes.piecewise_tf(256, 1), s.range)
+
+
+def test_replay_reads_slots_from_the_page_table(lvr, orc):
+    """The batch's brick records carry page-table indices, not slots: the
+    kernel reads each brick's slot from the device page table when it draws
+    it. A replay after one of the request's bricks was evicted skips that
+    brick (the oracle's frame without it); after the brick came back into
+    another slot, the replay reads it from there (the first frame again)."""
+    s = Scene(livre_amd.DT_UINT8, n=64, block=16, overlap=1, source="blobs", seed=8)
+    slot = livre_amd.slot_bytes(s.vi)
+    ctx = s.context(pool_bytes=slot * (len(s.ids) + 4))
+    try:
+        w, h = 96, 72
+        tf = scenes.piecewise_tf(256, seed=12, max_alpha=0.5)
+        mv, proj, fr = s.camera((1.3, 0.7, 1.2), w, h)
+        ordered = s.visible_ordered(mv, proj, h)
+        params = livre_amd.render_params(64)
+        with livre_amd.Batch() as b:
+            b.add(ctx, fr, w, h, ordered, params, tf, s.range)
+            b.run()
+            b.wait()
+            first, s1 = b.read(0)
+            ctx.set_tf(tf, s.range)
+            ctx.render(fr, w, h, ordered, params)
+            assert np.array_equal(ctx.read_frame()[0], first)
+            # a brick near the front of the order that covers many pixels
+            m = int(ordered[len(ordered) // 4])
+            q = int(next(i for i in s.ids if int(i) not in set(int(x) for x in ordered)))
+            ctx.evict(m)
+            b.run()  # replay: same keys and ranges, m now reads -1
+            b.wait()
+            without, s2 = b.read(0)
+            ref, ref_s = s.oracle_render(fr, w, h, ordered, tf, 64, missing=(m,))
+            mx, mean = diff_stats(without, oracle.to_rgba8(ref))
+            print("evicted", m, "samples", s1, s2, ref_s, "max", mx, "mean", mean, flush=True)
+            assert s2 == ref_s and s2 < s1
+            assert mx <= 2 and mean <= 0.5
+            assert not np.array_equal(without, first)
+            # m comes back in q's slot, q in m's old one
+            ctx.evict(q)
+            ctx.upload([m], s.bricks[m][None])
+            ctx.upload([q], s.bricks[q][None])
+            b.run()
+            b.wait()
+            again, s3 = b.read(0)
+            assert s3 == s1 and np.array_equal(again, first)
+    finally:
+        ctx.close()

@@ -316,6 +316,25 @@ typedef struct
 int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                const uint64_t* ordered_ids, int n,
                const lvr_render_params* params);
+/* lvr_render without allocating: no reference counterpart. For callers that
+ * cannot allocate device or pinned memory during a frame (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it). The same
+ * frame as lvr_render, written to out_rgba8 (caller device memory of
+ * w * h * 4 bytes, 4-byte aligned, row 0 = bottom) instead of the context's
+ * framebuffer; out_samples (caller device memory, 8-byte aligned, or NULL)
+ * receives the composite-step count. The call allocates nothing: the kernels'
+ * scratch is one static device array of the library (about 26 MB per device,
+ * held until the process ends) and the host staging is fixed at
+ * lvr_ctx_create. Capacity per call: 32768 bricks, 262144 ray tiles (4096 x
+ * 4096 pixels at 8 x 8 rays), 4M (tile, brick) pairs; beyond it LVR_ERANGE.
+ * Every call on one device shares that array, which keeps the last call's
+ * contents: calls that overlap on one device (contexts on different streams)
+ * race, so serialise them. Asynchronous on the context's render stream; no
+ * frame statistics are recorded. */
+int lvr_render_noalloc(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                       const uint64_t* ordered_ids, int n, const lvr_render_params* params,
+                       void* out_rgba8, uint64_t* out_samples);
+
 /* ------------------------------------------------ sort-last compositing */
 /* Sort-last decomposition (SelectVisibles::Impl::visitPost keeps the range
  * [r0 * N, r1 * N) of the visibles, SelectVisibles.cpp:112-131; Equalizer's

@@ -37,7 +37,7 @@ EXPORTS = (
     "lvr_jpeg_encoder_create", "lvr_jpeg_encoder_destroy", "lvr_jpeg_encode",
     "lvr_jpeg_encode_host",
     "lvr_render_partial", "lvr_partial_device_ptr", "lvr_partial_read", "lvr_composite_layers",
-    "lvr_samples_per_ray",
+    "lvr_samples_per_ray", "lvr_render_noalloc",
     "lvr_jpeg_max_size", "lvr_jpeg_error", "lvr_jpeg_get_stats", "lvr_frame_grab_jpeg",
     "lvr_frame_grab_stats",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
@@ -230,6 +230,8 @@ def lib():
                                       P(c.c_int32)]
     L.lvr_render.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
                              P(RenderParams)]
+    L.lvr_render_noalloc.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
+                                     P(RenderParams), c.c_void_p, c.c_void_p]
     L.lvr_frame_read.argtypes = [c.c_void_p, u8p, c.c_size_t, u64p]
     L.lvr_frame_device_ptr.argtypes = [c.c_void_p, P(c.c_void_p), P(c.c_int),
                                        P(c.c_int)]
@@ -724,6 +726,15 @@ class Context:
         self._check(lib().lvr_render(self._h, ctypes.byref(fr), vp, pord, len(ordered),
                                      ctypes.byref(params)), "render")
         self.width, self.height = width, height
+
+    def render_noalloc(self, fr, width, height, ordered, params, out_ptr, samples_ptr=None):
+        """lvr_render into caller device memory (out_ptr: width * height * 4
+        bytes; samples_ptr: 8 bytes or None), allocating nothing."""
+        ordered, pord = _u64(ordered)
+        vp = (ctypes.c_int * 4)(0, 0, width, height)
+        self._check(lib().lvr_render_noalloc(self._h, ctypes.byref(fr), vp, pord, len(ordered),
+                                             ctypes.byref(params), ctypes.c_void_p(out_ptr),
+                                             ctypes.c_void_p(samples_ptr)), "render_noalloc")
 
     def render_partial(self, fr, width, height, ordered, params, layer):
         """Sort-last partial: ordered (a contiguous range of the frame's

@@ -372,6 +372,14 @@ struct lvr_ctx
     std::vector<lvr::BrickKey> hostBricks;
     std::vector<int4> hostRanges;
     lvr::PinnedRing frameStaging; // the frame's records on their way to d_*
+    // lvr_render_noalloc: device address of the library's static scratch
+    // array on this context's device (null if it could not be taken), the
+    // fixed pinned staging of one frame's uniforms, keys and ranges, and the
+    // event of the last copy that read that staging
+    uint8_t* d_scratch = nullptr;
+    uint8_t* h_noalloc = nullptr;
+    hipEvent_t noallocCopied = nullptr;
+    bool noallocArmed = false;
     lvr::RawPyramid* pyramid = nullptr; // levels of the last raw source loaded
     lvr_jpeg_encoder* jpeg = nullptr;   // frame grab encoder on `stream`
     // sort-last partial images (RGBA32F), grow-only
@@ -485,6 +493,10 @@ uint64_t evictableSlots(const lvr_ctx* ctx);
 int waitPoolReaders(lvr_ctx* ctx, hipStream_t st);
 // Device scratch of at least `bytes` owned by the context (nullptr + error).
 void* genScratch(lvr_ctx* ctx, size_t bytes);
+// lvr_render_noalloc set-up, called by lvr_ctx_create with the context's
+// device current: the device address of the static scratch array (taken
+// once per device) and the context's fixed pinned staging.
+int noallocSetup(lvr_ctx* ctx);
 // Batch internals used by the pool and the server.
 hipEvent_t batchDoneEvent(lvr_batch* b);
 void batchForgetCtx(lvr_batch* b, lvr_ctx* ctx);

@@ -508,6 +508,9 @@ int lvr_ctx_create(const lvr_volume_info* info, size_t pool_bytes, int device,
         hipEventCreateWithFlags(&ctx->tableReady, hipEventDisableTiming) != hipSuccess)
         return bail(LVR_ENOMEM);
 
+    rc = lvr::noallocSetup(ctx);
+    if (rc)
+        return bail(rc);
     ctx->freeSlots.reserve(ctx->nSlots);
     for (int64_t s = int64_t(ctx->nSlots) - 1; s >= 0; --s)
         ctx->freeSlots.push_back(int32_t(s));
@@ -608,6 +611,12 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
         (void)hipHostFree(ctx->h_staging);
     if (ctx->h_counters)
         (void)hipHostFree(ctx->h_counters);
+    if (ctx->noallocArmed)
+        (void)hipEventSynchronize(ctx->noallocCopied);
+    if (ctx->h_noalloc)
+        (void)hipHostFree(ctx->h_noalloc);
+    if (ctx->noallocCopied)
+        (void)hipEventDestroy(ctx->noallocCopied);
     ctx->frameStaging.destroy();
     if (ctx->h_sel)
         (void)hipHostFree(ctx->h_sel);

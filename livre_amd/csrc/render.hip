@@ -25,6 +25,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
 #include "raycast_core.h"
@@ -491,6 +493,20 @@ __global__ __launch_bounds__(256) void classifyKernel(Classify c, const float4* 
         out[v] = c(tf, float(v));
 }
 
+// The classification constants of a frame, as makeClassify forms them on
+// the device.
+inline Classify makeClassifyHost(const FrameUniforms& U)
+{
+    Classify c;
+    c.mult = U.mult;
+    c.add = U.add;
+    c.nTfF = U.nTfF;
+    c.nTfMax = U.nTfLastF;
+    c.alphaCorrection = U.alphaCorrection;
+    c.maxA = 1.0f - 1.0f / 256.0f;
+    return c;
+}
+
 template <typename T, bool LINEAR, int D, bool CLS>
 void launchRaycastAs(int blocks, size_t lds, hipStream_t st, const FrameUniforms& U,
                      const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
@@ -533,6 +549,38 @@ void launchRaycast(int blocks, size_t lds, hipStream_t st, const FrameUniforms& 
         launchNearest<T, false>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);
 }
 } // namespace
+
+// ------------------------------------------------ no-allocation frames
+// lvr_render_noalloc keeps every buffer its kernels use beyond the inputs and
+// the output in one static device array of fixed capacity: the frame's
+// uniforms, the brick keys and records, the screen ranges, the two levels of
+// tile binning, the counters and the classified table. Capacity per call:
+constexpr int kNaBricks = 32768;                  // bricks of the ordered list
+constexpr int kNaTiles = 262144;                  // ray tiles (4096 x 4096 at 8 x 8)
+constexpr int kNaSupers = 8192;                   // binning super-tiles
+constexpr uint64_t kNaTileEntries = 4ull << 20;   // (tile, brick) pairs
+constexpr uint64_t kNaSuperEntries = 1ull << 20;  // (super-tile, brick) pairs
+constexpr int kNaCls = 65536;                     // classified-table entries
+constexpr size_t naUp(size_t x)
+{
+    return (x + 255) & ~size_t(255);
+}
+constexpr size_t kNaOffU = 0;
+constexpr size_t kNaOffCounters = kNaOffU + naUp(sizeof(FrameUniforms));
+constexpr size_t kNaOffKeys = kNaOffCounters + 256;
+constexpr size_t kNaOffBricks = kNaOffKeys + naUp(sizeof(BrickKey) * kNaBricks);
+constexpr size_t kNaOffRanges = kNaOffBricks + naUp(sizeof(BrickEntry) * kNaBricks);
+constexpr size_t kNaOffTileCount = kNaOffRanges + naUp(sizeof(int4) * kNaBricks);
+constexpr size_t kNaOffTileOffset = kNaOffTileCount + naUp(4u * kNaTiles);
+constexpr size_t kNaOffSuperCount = kNaOffTileOffset + naUp(4u * kNaTiles);
+constexpr size_t kNaOffSuperOffset = kNaOffSuperCount + naUp(4u * kNaSupers);
+constexpr size_t kNaOffTileList = kNaOffSuperOffset + naUp(4u * kNaSupers);
+constexpr size_t kNaOffSuperList = kNaOffTileList + naUp(4u * kNaTileEntries);
+constexpr size_t kNaOffCls = kNaOffSuperList + naUp(4u * kNaSuperEntries);
+constexpr size_t kNaScratchBytes = kNaOffCls + naUp(sizeof(float4) * kNaCls);
+// pinned host staging per context: uniforms, keys, ranges
+constexpr size_t kNaStagingBytes =
+    sizeof(FrameUniforms) + (sizeof(BrickKey) + sizeof(int4)) * size_t(kNaBricks);
 
 // Data range rule of RayCastRenderer::Impl::onFrameStart
 // (RayCastRenderer.cpp:236-285): the type's full range, replaced by the
@@ -807,6 +855,43 @@ static int growBuffer(lvr_ctx* ctx, T** ptr, size_t* cap, size_t need)
 }
 } // namespace lvr
 
+// The scratch of lvr_render_noalloc (about 26 MB, held by every device the
+// library runs on until the process ends). In host code this name is only a
+// stand-in for the symbol: kernels get the device address that
+// lvr::noallocSetup takes with hipGetSymbolAddress on each device.
+static __device__ __align__(256) uint8_t scratch[lvr::kNaScratchBytes];
+
+namespace lvr
+{
+int noallocSetup(lvr_ctx* ctx)
+{
+    // the device address of `scratch`, once per device
+    static std::mutex lock;
+    static std::unordered_map<int, uint8_t*> perDevice;
+    {
+        std::lock_guard<std::mutex> g(lock);
+        auto it = perDevice.find(ctx->device);
+        if (it == perDevice.end())
+        {
+            void* p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return hipFail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol,
+                               "hipGetSymbolAddress(scratch)");
+            it = perDevice.emplace(ctx->device, static_cast<uint8_t*>(p)).first;
+        }
+        ctx->d_scratch = it->second;
+    }
+    // host side of a no-allocation frame: its keys and ranges go into
+    // vectors of fixed capacity and through a fixed pinned staging buffer
+    ctx->hostBricks.reserve(size_t(kNaBricks));
+    ctx->hostRanges.reserve(size_t(kNaBricks));
+    LVR_HIP(ctx, hipHostMalloc((void**)&ctx->h_noalloc, kNaStagingBytes, hipHostMallocDefault));
+    LVR_HIP(ctx, hipEventCreateWithFlags(&ctx->noallocCopied, hipEventDisableTiming));
+    return LVR_OK;
+}
+} // namespace lvr
+
 // --------------------------------------------------------------- batches
 struct lvr_batch
 {
@@ -954,13 +1039,7 @@ static int ensureClassified(lvr_ctx* ctx, const lvr::FrameUniforms& U, int nValu
         return LVR_OK;
     if (!ctx->d_cls)
         LVR_HIP(ctx, hipMalloc((void**)&ctx->d_cls, sizeof(float4) * 65536));
-    Classify c;
-    c.mult = U.mult;
-    c.add = U.add;
-    c.nTfF = U.nTfF;
-    c.nTfMax = U.nTfLastF;
-    c.alphaCorrection = U.alphaCorrection;
-    c.maxA = 1.0f - 1.0f / 256.0f;
+    const Classify c = makeClassifyHost(U);
     hipLaunchKernelGGL(classifyKernel, dim3((nValues + 255) / 256), dim3(256), 0, ctx->stream, c,
                        U.tf, nValues, ctx->d_cls);
     LVR_HIP(ctx, hipGetLastError());
@@ -1208,6 +1287,150 @@ int lvr_render(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                const uint64_t* ordered_ids, int n, const lvr_render_params* params)
 {
     return renderFrame(ctx, f, viewport, ordered_ids, n, params, -1);
+}
+
+int lvr_render_noalloc(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
+                       const uint64_t* ordered_ids, int n, const lvr_render_params* params,
+                       void* out_rgba8, uint64_t* out_samples)
+{
+    using namespace lvr;
+    if (!ctx)
+        return LVR_EINVAL;
+    if (!out_rgba8 || reinterpret_cast<uintptr_t>(out_rgba8) % 4u ||
+        reinterpret_cast<uintptr_t>(out_samples) % 8u)
+        return fail(ctx, LVR_EINVAL, "output must be 4-byte (frame) and 8-byte (samples) aligned");
+    if (!ctx->tfSet)
+        return fail(ctx, LVR_EINVAL, "no transfer function set");
+    if (!ctx->d_scratch || !ctx->h_noalloc)
+        return fail(ctx, LVR_EHIP, "no-allocation scratch unavailable on this device");
+    if (n > kNaBricks)
+        return fail(ctx, LVR_ERANGE, "no-allocation render: more than 32768 bricks");
+    int rc = ensureDevice(ctx);
+    if (rc)
+        return rc;
+    FrameUniforms U;
+    std::vector<BrickKey>& keys = ctx->hostBricks; // capacity reserved at set-up
+    std::vector<int4>& ranges = ctx->hostRanges;
+    keys.clear();
+    ranges.clear();
+    uint64_t listTotal = 0, superTotal = 0, onScreen = 0;
+    rc = prepareFrame(ctx, f, viewport, ordered_ids, n, params, ctx->nTf, ctx->range0,
+                      ctx->range1, &U, keys, ranges, &listTotal, &superTotal, &onScreen);
+    if (rc)
+        return rc;
+    const int nTiles = U.tilesX * U.tilesY, nSupers = U.superX * U.superY;
+    if (nTiles > kNaTiles || nSupers > kNaSupers)
+        return fail(ctx, LVR_ERANGE, "no-allocation render: viewport above 262144 ray tiles");
+    if (listTotal > kNaTileEntries || superTotal > kNaSuperEntries)
+        return fail(ctx, LVR_ERANGE,
+                    "no-allocation render: more than 4M (tile, brick) pairs to bin");
+    uint8_t* d = ctx->d_scratch;
+    FrameUniforms* dU = reinterpret_cast<FrameUniforms*>(d + kNaOffU);
+    unsigned long long* dCounters = reinterpret_cast<unsigned long long*>(d + kNaOffCounters);
+    BrickKey* dKeys = reinterpret_cast<BrickKey*>(d + kNaOffKeys);
+    BrickEntry* dBricks = reinterpret_cast<BrickEntry*>(d + kNaOffBricks);
+    int4* dRanges = reinterpret_cast<int4*>(d + kNaOffRanges);
+    int32_t* dTileCount = reinterpret_cast<int32_t*>(d + kNaOffTileCount);
+    int32_t* dTileOffset = reinterpret_cast<int32_t*>(d + kNaOffTileOffset);
+    int32_t* dSuperCount = reinterpret_cast<int32_t*>(d + kNaOffSuperCount);
+    int32_t* dSuperOffset = reinterpret_cast<int32_t*>(d + kNaOffSuperOffset);
+    int32_t* dTileList = reinterpret_cast<int32_t*>(d + kNaOffTileList);
+    int32_t* dSuperList = reinterpret_cast<int32_t*>(d + kNaOffSuperList);
+    float4* dCls = reinterpret_cast<float4*>(d + kNaOffCls);
+
+    U.frame = static_cast<uint32_t*>(out_rgba8);
+    U.layer = nullptr;
+    U.tf = reinterpret_cast<const float4*>(ctx->d_tf);
+    const int dt = ctx->vi.data_type;
+    const bool table = !U.linear && U.classify == 1 && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16);
+    if (table)
+        U.cls = dCls;
+
+    // uniforms, keys and ranges through the fixed pinned staging, once the
+    // previous call's copies have read it; every stage runs on the render
+    // stream in order
+    if (ctx->noallocArmed)
+        LVR_HIP(ctx, hipEventSynchronize(ctx->noallocCopied));
+    ctx->noallocArmed = false;
+    const size_t kB = sizeof(BrickKey) * size_t(n), rB = sizeof(int4) * size_t(n);
+    uint8_t* h = ctx->h_noalloc;
+    std::memcpy(h, &U, sizeof(FrameUniforms));
+    std::memcpy(h + sizeof(FrameUniforms), keys.data(), kB);
+    std::memcpy(h + sizeof(FrameUniforms) + kB, ranges.data(), rB);
+    LVR_HIP(ctx, hipMemcpyAsync(dU, h, sizeof(FrameUniforms), hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (n > 0)
+    {
+        LVR_HIP(ctx, hipMemcpyAsync(dKeys, h + sizeof(FrameUniforms), kB,
+                                    hipMemcpyHostToDevice, ctx->stream));
+        LVR_HIP(ctx, hipMemcpyAsync(dRanges, h + sizeof(FrameUniforms) + kB, rB,
+                                    hipMemcpyHostToDevice, ctx->stream));
+    }
+    LVR_HIP(ctx, hipEventRecord(ctx->noallocCopied, ctx->stream));
+    ctx->noallocArmed = true;
+    rc = applyPageTable(ctx, ctx->stream);
+    if (rc)
+        return rc;
+    enqueueRecords(ctx->stream, dU, 1, dKeys, n, dBricks);
+    // the second scan clears the samples and work-queue counters
+    enqueueBinning(ctx->stream, dU, 1, dRanges, nSupers, nTiles, dSuperCount, dSuperOffset,
+                   dSuperList, dTileCount, dTileOffset, dTileList, dCounters, 2);
+    LVR_HIP(ctx, hipGetLastError());
+    const int perBlock = U.wavesPerSimd * 2;
+    const int blocks =
+        std::max(1, std::min((U.nItems + perBlock - 1) / perBlock, 2 * ctx->numCUs));
+    size_t lds = size_t(U.nTf + 2) * sizeof(float4);
+    if (table)
+    {
+        const int nValues = dt == LVR_DT_UINT8 ? 256 : 65536;
+        Classify c = makeClassifyHost(U);
+        hipLaunchKernelGGL(classifyKernel, dim3((nValues + 255) / 256), dim3(256), 0,
+                           ctx->stream, c, U.tf, nValues, dCls);
+        lds = dt == LVR_DT_UINT8 ? 256 * sizeof(float4) : 0;
+        if (dt == LVR_DT_UINT8)
+            launchNearest<uint8_t, true>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                         dTileCount, dTileList, dCounters);
+        else
+            launchNearest<uint16_t, true>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                          dTileCount, dTileList, dCounters);
+    }
+    else
+        switch (dt)
+        {
+        case LVR_DT_UINT8:
+            launchRaycast<uint8_t>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                   dTileCount, dTileList, dCounters);
+            break;
+        case LVR_DT_UINT16:
+            launchRaycast<uint16_t>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                    dTileCount, dTileList, dCounters);
+            break;
+        case LVR_DT_UINT32:
+            launchRaycast<uint32_t>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                    dTileCount, dTileList, dCounters);
+            break;
+        case LVR_DT_INT8:
+            launchRaycast<int8_t>(blocks, lds, ctx->stream, U, dBricks, dTileOffset, dTileCount,
+                                  dTileList, dCounters);
+            break;
+        case LVR_DT_INT16:
+            launchRaycast<int16_t>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                   dTileCount, dTileList, dCounters);
+            break;
+        case LVR_DT_INT32:
+            launchRaycast<int32_t>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
+                                   dTileCount, dTileList, dCounters);
+            break;
+        default:
+            launchRaycast<float>(blocks, lds, ctx->stream, U, dBricks, dTileOffset, dTileCount,
+                                 dTileList, dCounters);
+            break;
+        }
+    LVR_HIP(ctx, hipGetLastError());
+    if (out_samples)
+        LVR_HIP(ctx, hipMemcpyAsync(out_samples, dCounters, sizeof(uint64_t),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    return LVR_OK;
 }
 
 int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],

@@ -268,6 +268,29 @@ int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f,
                               uint32_t max_lod);
 int lvr_select_visibles_wait(lvr_ctx* ctx, uint64_t* out, int cap, int* n_out);
 
+/* The selection with everything SelectVisibles takes (SelectVisibles.cpp:
+ * 30-41): clip planes, and the sort-last range [range0, range1) of the
+ * DFS-ordered visibles that visitPost keeps (lines 109-131: indices
+ * [range0 * N, range1 * N), the products in float, truncated). A node wholly
+ * in the half-space a clip plane removes (n . x + d < 0 at the box corner
+ * furthest along n, the half-space fragRayCast.glsl:152-167 clips away) is
+ * culled with its subtree, as _clipPlanes.isOutside(worldBox) culls it
+ * (line 69). n_out receives the count of the range. */
+typedef struct
+{
+    uint32_t window_height;
+    float sse;
+    uint32_t min_lod, max_lod;
+    int32_t n_clip_planes;  /* 0..6 */
+    float clip_planes[24];  /* normal xyz, d */
+    float range0, range1;   /* 0 <= range0 <= range1 <= 1; {0, 1} keeps all */
+} lvr_select_params;
+int lvr_select_visibles_ex(lvr_ctx* ctx, const lvr_frustum* f, const lvr_select_params* params,
+                           uint64_t* out, int cap, int* n_out);
+/* The same in two steps (lvr_select_visibles_wait returns the range). */
+int lvr_select_visibles_ex_async(lvr_ctx* ctx, const lvr_frustum* f,
+                                 const lvr_select_params* params);
+
 /* RayCastRenderer::order (RayCastRenderer.cpp:57-81, 153-159): front to
  * back by |MV * brick centre|, with libstdc++ std::sort on the given order. */
 int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,

@@ -31,7 +31,8 @@ EXPORTS = (
     "lvr_field_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
-    "lvr_select_visibles_async", "lvr_select_visibles_wait",
+    "lvr_select_visibles_async", "lvr_select_visibles_wait", "lvr_select_visibles_ex",
+    "lvr_select_visibles_ex_async",
     "lvr_raw_open", "lvr_raw_close", "lvr_raw_get_info", "lvr_raw_brick",
     "lvr_pyramid_volume_info", "lvr_pyramid_load", "lvr_pyramid_release",
     "lvr_jpeg_encoder_create", "lvr_jpeg_encoder_destroy", "lvr_jpeg_encode",
@@ -135,6 +136,24 @@ class RenderParams(ctypes.Structure):
                 ("waves_per_simd", ctypes.c_int32)]
 
 
+class SelectParams(ctypes.Structure):
+    _fields_ = [("window_height", ctypes.c_uint32), ("sse", ctypes.c_float),
+                ("min_lod", ctypes.c_uint32), ("max_lod", ctypes.c_uint32),
+                ("n_clip_planes", ctypes.c_int32), ("clip_planes", ctypes.c_float * 24),
+                ("range0", ctypes.c_float), ("range1", ctypes.c_float)]
+
+
+def select_params(window_height, sse, min_lod, max_lod, clip_planes=(), rng=(0.0, 1.0)):
+    p = SelectParams()
+    p.window_height, p.sse, p.min_lod, p.max_lod = window_height, sse, min_lod, max_lod
+    p.n_clip_planes = len(clip_planes)
+    for i, pl in enumerate(clip_planes):
+        for j in range(4):
+            p.clip_planes[4 * i + j] = pl[j]
+    p.range0, p.range1 = rng
+    return p
+
+
 class TuneEntry(ctypes.Structure):
     _fields_ = [("tile_w", ctypes.c_int32), ("tile_h", ctypes.c_int32),
                 ("samples_per_thread", ctypes.c_int32), ("classify", ctypes.c_int32),
@@ -225,6 +244,9 @@ def lib():
     L.lvr_select_visibles_async.argtypes = [c.c_void_p, P(Frustum), c.c_uint32, c.c_float,
                                             c.c_uint32, c.c_uint32]
     L.lvr_select_visibles_wait.argtypes = [c.c_void_p, u64p, c.c_int, P(c.c_int)]
+    L.lvr_select_visibles_ex.argtypes = [c.c_void_p, P(Frustum), P(SelectParams), u64p, c.c_int,
+                                         P(c.c_int)]
+    L.lvr_select_visibles_ex_async.argtypes = [c.c_void_p, P(Frustum), P(SelectParams)]
     L.lvr_order.argtypes = [P(VolumeInfo), P(Frustum), u64p, c.c_int, u64p]
     L.lvr_samples_per_ray.argtypes = [P(VolumeInfo), u64p, c.c_int, c.c_int32,
                                       P(c.c_int32)]
@@ -703,6 +725,17 @@ class Context:
             self._h, ctypes.byref(fr), window_height, sse, min_lod, max_lod,
             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap,
             ctypes.byref(n)), "select_visibles")
+        return out[:n.value].copy()
+
+    def select_visibles_ex(self, fr, params, cap=None):
+        """Selection with clip planes and a sort-last range (select_params)."""
+        cap = cap or self.node_count()
+        out = np.zeros(cap, np.uint64)
+        n = ctypes.c_int()
+        self._check(lib().lvr_select_visibles_ex(
+            self._h, ctypes.byref(fr), ctypes.byref(params),
+            out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap, ctypes.byref(n)),
+            "select_visibles_ex")
         return out[:n.value].copy()
 
     def select_visibles_async(self, fr, window_height, sse, min_lod, max_lod):

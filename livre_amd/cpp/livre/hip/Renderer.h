@@ -1,5 +1,9 @@
 /*
- * C++ face of the MI355X render path, mirroring the reference's interfaces:
+ * C++ face of the MI355X render path, in namespace livre::hip so that it can
+ * sit in a translation unit beside Livre's own livre:: classes. It mirrors
+ * the reference's interfaces for callers outside Livre (LivreIntegration.h
+ * binds the same library to Livre's own livre::Renderer, DataSource and
+ * DataSourcePlugin):
  *   livre::NodeId            livre/data/NodeId.h:31-160
  *   livre::VolumeInformation livre/data/VolumeInformation.h:42-111
  *   livre::Frustum           livre/data/Frustum.h:37-111
@@ -29,6 +33,8 @@
 #include <vector>
 
 namespace livre
+{
+namespace hip
 {
 typedef uint64_t Identifier;
 
@@ -212,6 +218,112 @@ private:
     lvr_raw_source* _src = nullptr;
 };
 
+/** The procedural volumes as a DataSource URI (no reference counterpart; the
+ * reference's mem:// source, MemoryDataSource.cpp:63-98, is the template):
+ *   procedural://noise?seed=3&octaves=5&cells=8&gain=1.5#2048,2048,2048,64,uint16
+ *   procedural://blobs?seed=1&count=64&noise=0.1#256,256,256,32,uint8
+ *   procedural://sines?seed=7&count=8#1536,1536,1536,48,float
+ * The fragment gives the voxels per axis, the brick size (without the
+ * 1-voxel overlap) and the type (uint8, uint16 or float). */
+struct ProceduralSpec
+{
+    enum Kind
+    {
+        NOISE,
+        BLOBS,
+        SINES
+    } kind = NOISE;
+    std::array<uint32_t, 3> voxels{{0, 0, 0}};
+    uint32_t block = 0;
+    DataType type = DT_UINT8;
+    uint64_t seed = 0;
+    int octaves = 5, cells = 8, count = 64;
+    float gain = 1.5f, noise = 0.0f;
+    lvr_noise_params noiseParams{}; // the remap table of a noise source
+
+    /** From the URI's host ("noise", "blobs", "sines"), query (key, value)
+     * pairs and fragment; throws std::runtime_error on a malformed spec. */
+    static ProceduralSpec parse(const std::string& host,
+                                const std::vector<std::pair<std::string, std::string>>& query,
+                                const std::string& fragment)
+    {
+        ProceduralSpec p;
+        if (host == "noise")
+            p.kind = NOISE;
+        else if (host == "blobs")
+            p.kind = BLOBS;
+        else if (host == "sines")
+            p.kind = SINES;
+        else
+            throw std::runtime_error("procedural source: unknown kind '" + host + "'");
+        std::vector<std::string> f;
+        size_t b = 0;
+        for (size_t e; (e = fragment.find(',', b)) != std::string::npos; b = e + 1)
+            f.push_back(fragment.substr(b, e - b));
+        f.push_back(fragment.substr(b));
+        if (f.size() != 5)
+            throw std::runtime_error("procedural source: fragment must be nx,ny,nz,block,type");
+        try
+        {
+            for (int i = 0; i < 3; ++i)
+                p.voxels[size_t(i)] = uint32_t(std::stoul(f[size_t(i)]));
+            p.block = uint32_t(std::stoul(f[3]));
+            for (const auto& kv : query)
+            {
+                if (kv.first == "seed")
+                    p.seed = std::stoull(kv.second);
+                else if (kv.first == "octaves")
+                    p.octaves = std::stoi(kv.second);
+                else if (kv.first == "cells")
+                    p.cells = std::stoi(kv.second);
+                else if (kv.first == "count")
+                    p.count = std::stoi(kv.second);
+                else if (kv.first == "gain")
+                    p.gain = std::stof(kv.second);
+                else if (kv.first == "noise")
+                    p.noise = std::stof(kv.second);
+                else
+                    throw std::runtime_error("procedural source: unknown parameter '" +
+                                             kv.first + "'");
+            }
+        }
+        catch (const std::logic_error&)
+        {
+            throw std::runtime_error("procedural source: malformed number");
+        }
+        if (f[4] == "uint8")
+            p.type = DT_UINT8;
+        else if (f[4] == "uint16")
+            p.type = DT_UINT16;
+        else if (f[4] == "float")
+            p.type = DT_FLOAT;
+        else
+            throw std::runtime_error("procedural source: type must be uint8, uint16 or float");
+        if (p.block == 0 || p.voxels[0] == 0 || p.voxels[1] == 0 || p.voxels[2] == 0)
+            throw std::runtime_error("procedural source: empty volume or brick");
+        if (p.kind == NOISE &&
+            lvr_noise_params_init(&p.noiseParams, p.seed, p.octaves, p.cells, p.gain) != LVR_OK)
+            throw std::runtime_error("procedural source: invalid noise parameters");
+        return p;
+    }
+
+    VolumeInformation volumeInfo() const { return VolumeInformation(voxels, block, 1, type); }
+
+    /** One brick, x fastest (DataSourcePlugin::getData): `out` holds
+     * volumeInfo().brickBytes() bytes. */
+    int brick(const VolumeInformation& vi, uint64_t nodeId, void* out) const
+    {
+        if (kind == NOISE)
+            return lvr_noise_brick_host(&vi.info, &noiseParams, nodeId, out, vi.brickBytes());
+        lvr_field_params fp{};
+        fp.kind = kind == BLOBS ? LVR_FIELD_BLOBS : LVR_FIELD_SINES;
+        fp.count = count;
+        fp.seed = seed;
+        fp.noise = noise;
+        return lvr_field_brick_host(&vi.info, &fp, nodeId, out, vi.brickBytes());
+    }
+};
+
 /** The HBM brick pool of one volume: TexturePool + the texture cache. */
 class BrickPool
 {
@@ -364,6 +476,27 @@ public:
               "tf", _pool.ctx());
     }
     void setEarlyExit(float a) { _earlyExit = a; }
+    /** 0 = derived from the rendered bricks (RayCastRenderer.cpp:216-234). */
+    void setSamplesPerRay(uint32_t n) { _samplesPerRay = n; }
+    void setLinearFiltering(bool on) { _linear = on; }
+
+    /** The three frame stages on an already ordered list, for a caller whose
+     * own Renderer::render ran order() (LivreIntegration.h). */
+    void frameStart(const Frustum& f, const ClipPlanes& c, const PixelViewport& v,
+                    const NodeIds& ordered)
+    {
+        _onFrameStart(f, c, v, ordered);
+    }
+    void frameRender(const Frustum& f, const ClipPlanes& c, const PixelViewport& v,
+                     const NodeIds& ordered)
+    {
+        _onFrameRender(f, c, v, ordered);
+    }
+    void frameEnd(const Frustum& f, const ClipPlanes& c, const PixelViewport& v,
+                  const NodeIds& ordered)
+    {
+        _onFrameEnd(f, c, v, ordered);
+    }
 
     /** Sort-last (SelectVisibles.cpp:112-131, Channel.cpp:464-603): render
      * the following frames' bricks, a range of the global order, into the
@@ -417,17 +550,15 @@ protected:
                        const NodeIds& bricks) override
     {
         _vp = view;
-        _computed = _samplesPerRay;
-        if (_samplesPerRay == 0) // RayCastRenderer.cpp:216-234
-        {
-            uint32_t maxLOD = 0;
-            for (const NodeId& n : bricks)
-                maxLOD = std::max(maxLOD, n.getLevel());
-            const lvr_volume_info& vi = _pool.volumeInfo().info;
-            const float maxVoxelDim = float(std::max(vi.voxels[0], std::max(vi.voxels[1], vi.voxels[2])));
-            const float atLOD = maxVoxelDim / float(1u << (vi.depth - maxLOD - 1));
-            _computed = uint32_t(std::max(atLOD, 512.0f));
-        }
+        // RayCastRenderer.cpp:216-234: 0 derives it from the finest level
+        std::vector<uint64_t> ids(bricks.size());
+        for (size_t i = 0; i < bricks.size(); ++i)
+            ids[i] = bricks[i].getId();
+        int32_t computed = 0;
+        check(lvr_samples_per_ray(&_pool.volumeInfo().info, ids.data(), int(ids.size()),
+                                  int32_t(_samplesPerRay), &computed),
+              "samples per ray");
+        _computed = uint32_t(computed);
         std::memset(&_params, 0, sizeof _params);
         _params.n_samples_per_ray = int32_t(_computed);
         _params.max_samples_per_ray = 32;
@@ -539,4 +670,5 @@ private:
     BrickPool& _pool;
     std::vector<uint64_t> _vis; // visible-set buffer, one entry per octree node
 };
+} // namespace hip
 } // namespace livre

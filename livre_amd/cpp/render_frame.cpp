@@ -1,6 +1,6 @@
 /*
  * render_frame: renders one frame of a procedural volume through the C++
- * drop-in classes (livre::HipRayCastRenderer, livre::RenderPipeline), the
+ * drop-in classes (livre::hip::HipRayCastRenderer, livre::hip::RenderPipeline), the
  * way Livre's Channel drives RayCastRenderer, and writes the RGBA8 frame and
  * the histogram to files.
  *
@@ -21,15 +21,15 @@
 
 namespace
 {
-class NoiseSource : public livre::DataSourcePlugin
+class NoiseSource : public livre::hip::DataSourcePlugin
 {
 public:
     NoiseSource(uint32_t n, uint32_t block, uint64_t seed)
     {
-        _volumeInfo = livre::VolumeInformation({n, n, n}, block, 1, livre::DT_UINT16);
-        livre::check(lvr_noise_params_init(&_params, seed, 4, 4, 1.5f), "noise");
+        _volumeInfo = livre::hip::VolumeInformation({n, n, n}, block, 1, livre::hip::DT_UINT16);
+        livre::hip::check(lvr_noise_params_init(&_params, seed, 4, 4, 1.5f), "noise");
     }
-    bool getData(const livre::NodeId& node, void* out) override
+    bool getData(const livre::hip::NodeId& node, void* out) override
     {
         return lvr_noise_brick_host(&_volumeInfo.info, &_params, node.getId(), out,
                                     _volumeInfo.brickBytes()) == LVR_OK;
@@ -77,43 +77,43 @@ void perspective(float fovy, float aspect, float n, float f, float* p)
 
 int hostCheck(const std::string& rawPath)
 {
-    const livre::NodeId root(0, {0, 0, 0});
+    const livre::hip::NodeId root(0, {0, 0, 0});
     const auto kids = root.getChildren();
-    if (kids.size() != 8 || kids[1].getId() != livre::NodeId(1, {0, 0, 1}).getId() ||
-        kids[4].getId() != livre::NodeId(1, {1, 0, 0}).getId() ||
-        kids[7].getParent() != root || livre::NodeId().isValid())
+    if (kids.size() != 8 || kids[1].getId() != livre::hip::NodeId(1, {0, 0, 1}).getId() ||
+        kids[4].getId() != livre::hip::NodeId(1, {1, 0, 0}).getId() ||
+        kids[7].getParent() != root || livre::hip::NodeId().isValid())
         return 1;
-    if (livre::NodeId(1, {1, 0, 0}).getId() != 17u) // lodSelection.cpp:78
+    if (livre::hip::NodeId(1, {1, 0, 0}).getId() != 17u) // lodSelection.cpp:78
         return 2;
-    const livre::VolumeInformation vi({2048, 2048, 2048}, 64, 0, livre::DT_UINT8);
+    const livre::hip::VolumeInformation vi({2048, 2048, 2048}, 64, 0, livre::hip::DT_UINT8);
     if (vi.depth() != 6) // volumeInformation.cpp:63
         return 3;
     float mv[16], proj[16];
     const float eye[3] = {0.4f, 0.3f, 1.9f};
     lookAt(eye, mv);
     perspective(60.f, 1.f, 0.9f, 3.0f, proj);
-    const livre::Frustum fr(mv, proj);
+    const livre::hip::Frustum fr(mv, proj);
     if (std::fabs(fr.getEyePos()[2] - 1.9f) > 1e-5f)
         return 4;
     if (!rawPath.empty())
     {
         // The reference's raw test volume (41^3 uint8, one brick), through
         // RawDataSource and against the file's own bytes.
-        livre::RawDataSource raw(rawPath + "#41,41,41");
-        const livre::VolumeInformation& ri = raw.getVolumeInfo();
+        livre::hip::RawDataSource raw(rawPath + "#41,41,41");
+        const livre::hip::VolumeInformation& ri = raw.getVolumeInfo();
         if (ri.info.voxels[0] != 41 || ri.info.data_type != LVR_DT_UINT8 || ri.depth() != 1)
             return 5;
         std::vector<uint8_t> brick(ri.brickBytes()), file(ri.brickBytes());
         std::ifstream in(rawPath, std::ios::binary);
         in.read(reinterpret_cast<char*>(file.data()), std::streamsize(file.size()));
-        if (!in || !raw.getData(livre::NodeId(0, {0, 0, 0}), brick.data()) || brick != file)
+        if (!in || !raw.getData(livre::hip::NodeId(0, {0, 0, 0}), brick.data()) || brick != file)
             return 6;
         if (raw.pyramidInfo(8, 0).depth() != 4) // 41 -> 64 = 8 * 2^3
             return 7;
         bool threw = false;
         try
         {
-            livre::RawDataSource bad(rawPath + "#41,41");
+            livre::hip::RawDataSource bad(rawPath + "#41,41");
         }
         catch (const std::runtime_error&)
         {
@@ -121,6 +121,66 @@ int hostCheck(const std::string& rawPath)
         }
         if (!threw)
             return 8;
+    }
+    {
+        // procedural:// URIs: the parsed source serves the bricks of the
+        // library's host sources with the same parameters
+        using Spec = livre::hip::ProceduralSpec;
+        const Spec noise = Spec::parse("noise", {{"seed", "3"}, {"octaves", "4"}, {"cells", "4"},
+                                                 {"gain", "1.5"}},
+                                       "40,40,40,8,uint16");
+        const livre::hip::VolumeInformation nvi = noise.volumeInfo();
+        if (nvi.info.voxels[0] != 40 || nvi.info.max_block[0] != 10 ||
+            nvi.info.data_type != LVR_DT_UINT16)
+            return 9;
+        lvr_noise_params np;
+        livre::hip::check(lvr_noise_params_init(&np, 3, 4, 4, 1.5f), "noise");
+        const uint64_t id = livre::hip::NodeId(2, {1, 2, 3}).getId();
+        std::vector<uint8_t> a(nvi.brickBytes()), b(nvi.brickBytes());
+        if (noise.brick(nvi, id, a.data()) != LVR_OK ||
+            lvr_noise_brick_host(&nvi.info, &np, id, b.data(), b.size()) != LVR_OK || a != b)
+            return 10;
+        const Spec blobs =
+            Spec::parse("blobs", {{"seed", "1"}, {"count", "16"}, {"noise", "0.1"}},
+                        "32,32,32,16,uint8");
+        lvr_field_params fp{};
+        fp.kind = LVR_FIELD_BLOBS;
+        fp.count = 16;
+        fp.seed = 1;
+        fp.noise = 0.1f;
+        const livre::hip::VolumeInformation bvi = blobs.volumeInfo();
+        std::vector<uint8_t> c(bvi.brickBytes()), d(bvi.brickBytes());
+        if (blobs.brick(bvi, 0, c.data()) != LVR_OK ||
+            lvr_field_brick_host(&bvi.info, &fp, 0, d.data(), d.size()) != LVR_OK || c != d)
+            return 11;
+        const char* bad[][2] = {{"noise", "40,40,40,8"}, {"fog", "40,40,40,8,uint8"},
+                                {"noise", "40,40,40,8,int64"}, {"sines", "0,40,40,8,float"},
+                                {"noise", "40,x,40,8,uint8"}};
+        for (const auto& bd : bad)
+        {
+            bool threw = false;
+            try
+            {
+                Spec::parse(bd[0], {}, bd[1]);
+            }
+            catch (const std::runtime_error&)
+            {
+                threw = true;
+            }
+            if (!threw)
+                return 12;
+        }
+        bool threw = false;
+        try
+        {
+            Spec::parse("noise", {{"colour", "red"}}, "40,40,40,8,uint8");
+        }
+        catch (const std::runtime_error&)
+        {
+            threw = true;
+        }
+        if (!threw)
+            return 13;
     }
     std::cout << "host check ok" << std::endl;
     return 0;
@@ -144,11 +204,11 @@ int main(int argc, char** argv)
     try
     {
         NoiseSource source(n, block, 7);
-        const livre::VolumeInformation& vi = source.getVolumeInfo();
+        const livre::hip::VolumeInformation& vi = source.getVolumeInfo();
         uint64_t slot = 0;
-        livre::check(lvr_slot_bytes(&vi.info, &slot), "slot bytes");
-        livre::BrickPool pool(vi, size_t(slot) * 4096);
-        livre::HipRayCastRenderer renderer(pool, n);
+        livre::hip::check(lvr_slot_bytes(&vi.info, &slot), "slot bytes");
+        livre::hip::BrickPool pool(vi, size_t(slot) * 4096);
+        livre::hip::HipRayCastRenderer renderer(pool, n);
         std::vector<float> tf(256 * 4);
         for (int i = 0; i < 256; ++i)
         {
@@ -164,11 +224,11 @@ int main(int argc, char** argv)
         lookAt(eye, mv);
         const float d = std::sqrt(eye[0] * eye[0] + eye[1] * eye[1] + eye[2] * eye[2]);
         perspective(60.f, float(w) / float(h), std::max(1e-4f, d - 1.f), d + 1.f, proj);
-        const livre::Frustum frustum(mv, proj);
-        livre::RenderPipeline pipeline(source, pool);
-        livre::RenderPipeline::Params params;
+        const livre::hip::Frustum frustum(mv, proj);
+        livre::hip::RenderPipeline pipeline(source, pool);
+        livre::hip::RenderPipeline::Params params;
         params.sse = 1.0f;
-        livre::PixelViewport vp;
+        livre::hip::PixelViewport vp;
         vp.w = w;
         vp.h = h;
         std::vector<uint64_t> hist;
@@ -178,24 +238,24 @@ int main(int argc, char** argv)
         std::ofstream(out + ".rgba", std::ios::binary)
             .write(reinterpret_cast<const char*>(img.data()), std::streamsize(img.size()));
         // frame grab: from HBM, and from a BGRA host copy as Equalizer hands it over
-        const std::vector<uint8_t> jpg = livre::FrameGrabber::grab(pool);
+        const std::vector<uint8_t> jpg = livre::hip::FrameGrabber::grab(pool);
         std::vector<uint8_t> bgra(img);
         for (size_t i = 0; i < bgra.size(); i += 4)
             std::swap(bgra[i], bgra[i + 2]);
-        livre::FrameGrabber grabber;
+        livre::hip::FrameGrabber grabber;
         if (grabber.encodeJpeg(uint32_t(w), uint32_t(h), bgra.data()) != jpg)
             throw std::runtime_error("frame grab: host and device encodes differ");
         std::ofstream(out + ".jpg", std::ios::binary)
             .write(reinterpret_cast<const char*>(jpg.data()), std::streamsize(jpg.size()));
         // sort-last: the ordered visible set in two consecutive ranges, each
         // ray-cast into a layer, composited front first
-        const livre::NodeIds visible = renderer.getVisibleNodes();
+        const livre::hip::NodeIds visible = renderer.getVisibleNodes();
         const auto mid = visible.begin() + std::ptrdiff_t(visible.size() / 2);
-        const livre::ClipPlanes noPlanes;
+        const livre::hip::ClipPlanes noPlanes;
         renderer.setPartialLayer(0);
-        renderer.render(frustum, noPlanes, vp, livre::NodeIds(visible.begin(), mid));
+        renderer.render(frustum, noPlanes, vp, livre::hip::NodeIds(visible.begin(), mid));
         renderer.setPartialLayer(1);
-        renderer.render(frustum, noPlanes, vp, livre::NodeIds(mid, visible.end()));
+        renderer.render(frustum, noPlanes, vp, livre::hip::NodeIds(mid, visible.end()));
         renderer.setPartialLayer(-1);
         renderer.composite({renderer.layerDevicePtr(0), renderer.layerDevicePtr(1)}, vp);
         const std::vector<uint8_t> sortLast = renderer.readFrame();

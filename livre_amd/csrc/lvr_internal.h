@@ -399,6 +399,7 @@ struct lvr_ctx
     uint8_t* h_sel = nullptr;
     hipEvent_t selDone = nullptr;
     bool selPending = false;
+    float selRange[2] = {0.f, 1.f}; // sort-last share of the pending selection
     unsigned long long* d_counters = nullptr; // [0] samples, [1] work queue
     unsigned long long* h_counters = nullptr; // pinned
 

@@ -31,6 +31,8 @@ struct SelectUniforms
     float sse;
     uint32_t height, minLod, maxLod;
     int32_t capFrontier, capOut;
+    int32_t nClip;
+    float clip[6][4]; // normal xyz, d: the kept half-space is n . x + d >= 0
 };
 
 __device__ inline float planeDot(const float* p, const float* v)
@@ -61,6 +63,16 @@ __device__ int visitNode(const SelectUniforms& U, uint64_t id)
         float vmin[3], vmax[3];
         nearFar(mn, mx, U.planes[p], vmin, vmax);
         if (planeDot(U.planes[p], vmax) < 0.f)
+            return 0;
+    }
+    // _clipPlanes.isOutside(worldBox) (SelectVisibles.cpp:69): the box lies
+    // wholly in the half-space the shader clips away (fragRayCast.glsl:152-
+    // 167 keeps n . x + d >= 0), tested at its corner furthest along n
+    for (int p = 0; p < U.nClip; ++p)
+    {
+        float vmin[3], vmax[3];
+        nearFar(mn, mx, U.clip[p], vmin, vmax);
+        if (planeDot(U.clip[p], vmax) < 0.f)
             return 0;
     }
     const float* np = U.planes[4];
@@ -203,12 +215,17 @@ uint64_t dfsKey(const lvr_volume_info& vi, uint64_t id)
 
 extern "C" {
 
-int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f, uint32_t window_height,
-                              float sse, uint32_t min_lod, uint32_t max_lod)
+int lvr_select_visibles_ex_async(lvr_ctx* ctx, const lvr_frustum* f,
+                                 const lvr_select_params* sp)
 {
     using namespace lvr;
-    if (!ctx || !f || window_height == 0)
+    if (!ctx || !f || !sp || sp->window_height == 0 || sp->n_clip_planes < 0 ||
+        sp->n_clip_planes > 6 || !(sp->range0 >= 0.f) || !(sp->range1 <= 1.f) ||
+        !(sp->range0 <= sp->range1))
         return LVR_EINVAL;
+    const uint32_t window_height = sp->window_height, min_lod = sp->min_lod,
+                   max_lod = sp->max_lod;
+    const float sse = sp->sse;
     int rc = ensureDevice(ctx);
     if (rc)
         return rc;
@@ -262,6 +279,12 @@ int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f, uint32_t windo
     U.maxLod = max_lod;
     U.capFrontier = int(std::min<size_t>(capNeed, 1u << 30));
     U.capOut = U.capFrontier;
+    U.nClip = sp->n_clip_planes;
+    for (int p = 0; p < 6; ++p)
+        for (int j = 0; j < 4; ++j)
+            U.clip[p][j] = p < U.nClip ? sp->clip_planes[4 * p + j] : 0.f;
+    ctx->selRange[0] = sp->range0;
+    ctx->selRange[1] = sp->range1;
     uint8_t* dHost = nullptr;
     LVR_HIP(ctx, hipHostGetDevicePointer((void**)&dHost, ctx->h_sel, 0));
     hipLaunchKernelGGL(selectKernel, dim3(1), dim3(1024), 0, ctx->selStream, U,
@@ -287,8 +310,14 @@ int lvr_select_visibles_wait(lvr_ctx* ctx, uint64_t* out, int cap, int* n_out)
     if (counts[1])
         return fail(ctx, LVR_ENOMEM, "selection frontier overflow");
     const int n = counts[0];
-    *n_out = n;
-    if (n > cap)
+    // SelectVisibles::Impl::visitPost (SelectVisibles.cpp:109-131): the
+    // sort-last share [range0 * N, range1 * N) of the DFS-ordered visibles,
+    // the products formed in float and truncated as the reference does
+    const size_t start = size_t(ctx->selRange[0] * float(n));
+    const size_t end = std::min(size_t(ctx->selRange[1] * float(n)), size_t(n));
+    const int kept = end > start ? int(end - start) : 0;
+    *n_out = kept;
+    if (kept > cap)
         return fail(ctx, LVR_ERANGE, "visible set larger than the output buffer");
     const uint64_t* ids = reinterpret_cast<const uint64_t*>(ctx->h_sel + sizeof(uint64_t));
     std::vector<std::pair<uint64_t, uint64_t>> keyed(static_cast<size_t>(n)), tmp;
@@ -299,9 +328,30 @@ int lvr_select_visibles_wait(lvr_ctx* ctx, uint64_t* out, int cap, int* n_out)
     radixSortByKey(keyed, tmp);
     if (hasEqualKeys(keyed))
         std::sort(keyed.begin(), keyed.end());
-    for (int i = 0; i < n; ++i)
-        out[i] = keyed[size_t(i)].second;
+    for (int i = 0; i < kept; ++i)
+        out[i] = keyed[start + size_t(i)].second;
     return LVR_OK;
+}
+
+int lvr_select_visibles_ex(lvr_ctx* ctx, const lvr_frustum* f, const lvr_select_params* sp,
+                           uint64_t* out, int cap, int* n_out)
+{
+    if (!ctx || !f || !sp || !n_out || cap < 0 || (cap > 0 && !out))
+        return LVR_EINVAL;
+    const int rc = lvr_select_visibles_ex_async(ctx, f, sp);
+    return rc ? rc : lvr_select_visibles_wait(ctx, out, cap, n_out);
+}
+
+int lvr_select_visibles_async(lvr_ctx* ctx, const lvr_frustum* f, uint32_t window_height,
+                              float sse, uint32_t min_lod, uint32_t max_lod)
+{
+    lvr_select_params sp{};
+    sp.window_height = window_height;
+    sp.sse = sse;
+    sp.min_lod = min_lod;
+    sp.max_lod = max_lod;
+    sp.range1 = 1.f;
+    return lvr_select_visibles_ex_async(ctx, f, &sp);
 }
 
 int lvr_select_visibles(lvr_ctx* ctx, const lvr_frustum* f,

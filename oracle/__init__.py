@@ -82,7 +82,7 @@ def lib():
         _lib.orc_select_visibles.argtypes = [
             P(VolumeInfo), f32p, f32p, ctypes.c_uint32, ctypes.c_float,
             ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
-            u64p, ctypes.c_int, P(ctypes.c_int)]
+            ctypes.c_int, f32p, u64p, ctypes.c_int, P(ctypes.c_int)]
         _lib.orc_order.argtypes = [P(VolumeInfo), f32p, u64p, ctypes.c_int, u64p]
         for name in ("orc_render_f32", "orc_render_f64"):
             getattr(_lib, name).argtypes = [
@@ -140,14 +140,19 @@ def frustum_info(mv, proj):
 
 
 def select_visibles(vi, mv, proj, height, sse, min_lod, max_lod, rng=(0.0, 1.0),
-                    cap=1 << 22):
+                    cap=1 << 22, clip_planes=()):
     out = np.zeros(cap, np.uint64)
     n = ctypes.c_int(0)
     mv, pmv = _f32(mv)
     proj, pproj = _f32(proj)
+    planes = np.zeros((max(1, len(clip_planes)), 4), np.float32)
+    for i, pl in enumerate(clip_planes):
+        planes[i] = pl
+    planes, pplanes = _f32(planes)
     rc = lib().orc_select_visibles(
         ctypes.byref(vi), pmv, pproj, height, sse, min_lod, max_lod, rng[0],
-        rng[1], out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap,
+        rng[1], len(clip_planes), pplanes,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap,
         ctypes.byref(n))
     if rc != 0:
         raise RuntimeError("selection overflow")

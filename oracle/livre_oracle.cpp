@@ -12,7 +12,10 @@
  *   orc_fill_volume_info   tests/data/volumeInformation.cpp:57-106,
  *                          tests/data/dataSource.cpp:46 (golden fixtures)
  *   orc_select_visibles    tests/data/lodSelection.cpp:70-207 (seven golden
- *                          NodeId sets, including FP32 knife-edge nodes)
+ *                          NodeId sets, including FP32 knife-edge nodes);
+ *                          clip planes by brute force over the box corners
+ *                          and the march's own clipping (Lexis' isOutside
+ *                          itself is not vendored: parity unpinned there)
  *   orc_order              brute-force property (non-decreasing eye distance)
  *   orc_render_*           closed forms on tiny scenes (empty TF, constant
  *                          volume series, brick-split invariance, exit
@@ -267,6 +270,8 @@ struct Selector
     uint32_t height;
     float sse;
     uint32_t minLod, maxLod;
+    int nClip = 0;
+    const float* clip = nullptr; /* n_clip planes: normal xyz, d */
     std::vector<uint64_t> visibles;
 
     /* SelectVisibles::Impl::isLODVisible (SelectVisibles.cpp:45-64) */
@@ -288,7 +293,20 @@ struct Selector
         const Node n = decode(id);
         const Box box = world_box(*vi, n);
         if (!in_frustum(f, box))
-            return false; /* no clip planes in this restatement's callers */
+            return false;
+        /* _clipPlanes.isOutside(worldBox) (SelectVisibles.cpp:69): Lexis'
+         * ClipPlanes is not under the reference tree, so the rule restated
+         * here is the half-space fragRayCast.glsl:152-167 clips away: the
+         * box is outside a plane when even its corner furthest along the
+         * normal has n . x + d < 0. */
+        for (int c = 0; c < nClip; ++c)
+        {
+            const float* cp = clip + 4 * c;
+            float vmin[3], vmax[3];
+            near_far(box, cp, vmin, vmax);
+            if (plane_dot(cp, vmax) < 0.f)
+                return false;
+        }
 
         float vmin[3], vmax[3];
         near_far(box, f.planes[4], vmin, vmax);
@@ -827,10 +845,13 @@ int orc_frustum_info_get(const float mv[16], const float proj[16],
 int orc_select_visibles(const orc_volume_info* vi, const float mv[16],
                         const float proj[16], uint32_t window_height,
                         float sse, uint32_t min_lod, uint32_t max_lod,
-                        float range0, float range1, uint64_t* out, int cap,
+                        float range0, float range1, int n_clip,
+                        const float* clip_planes, uint64_t* out, int cap,
                         int* n_out)
 {
     Selector s;
+    s.nClip = n_clip;
+    s.clip = clip_planes;
     s.vi = vi;
     s.f = make_frustum(mv, proj);
     s.height = window_height;

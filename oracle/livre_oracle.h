@@ -86,7 +86,8 @@ int orc_frustum_info_get(const float mv[16], const float proj[16],
 int orc_select_visibles(const orc_volume_info* vi, const float mv[16],
                         const float proj[16], uint32_t window_height,
                         float sse, uint32_t min_lod, uint32_t max_lod,
-                        float range0, float range1, uint64_t* out, int cap,
+                        float range0, float range1, int n_clip,
+                        const float* clip_planes, uint64_t* out, int cap,
                         int* n_out);
 
 int orc_order(const orc_volume_info* vi, const float mv[16],

@@ -71,3 +71,33 @@ def test_cpp_pipeline_frame_matches_oracle(driver, orc, tmp_path):
     assert int(meta["sortlast_maxdiff"][0]) <= 2  # two layers composited vs whole
     assert np.array_equal(hist, orc.histogram(ovi, sel, [
         livre_amd.noise_brick_host(vi, p, i) for i in sel], 65536))
+
+
+def _compile_only(src, extra_include=None):
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROOT, "livre_amd", "cpp"),
+           "-I", os.path.join(ROOT, "tests", "cpp")]
+    return subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror"]
+                          + inc + [src], capture_output=True, text=True, timeout=120)
+
+
+def test_livre_integration_compiles_beside_livre_declarations():
+    """livre/hip/LivreIntegration.h and livre/hip/Renderer.h in one
+    translation unit with the declarations of Livre's own classes: no name is
+    redefined, livre::hip::RayCastRenderer is a livre::Renderer with the
+    reference's constructor and update(), ProceduralDataSource is a
+    livre::DataSourcePlugin with the plugin statics."""
+    res = _compile_only(os.path.join(ROOT, "tests", "cpp", "livre_adapter_check.cpp"))
+    assert res.returncode == 0, res.stderr
+
+
+def test_integration_snippet_compiles(tmp_path):
+    """The C++ snippet of INTEGRATION.md marked as compiled."""
+    text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    marker = "<!-- compiled by tests/test_cpp_api.py -->"
+    assert marker in text
+    block = text.split(marker, 1)[1]
+    code = block.split("```cpp", 1)[1].split("```", 1)[0]
+    src = tmp_path / "snippet.cpp"
+    src.write_text('#include "livre_signatures.h"\n' + code)
+    res = _compile_only(str(src))
+    assert res.returncode == 0, res.stderr

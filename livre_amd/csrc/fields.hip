@@ -424,8 +424,13 @@ int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p, const uint6
             hipLaunchKernelGGL(lvr::fieldBricksKernel<float>, grid, dim3(256), 0, ctx->stream,
                                f.shape, d_blobs, d_waves, d_geo, d_slots, ctx->d_pool,
                                ctx->slotStride, ctx->layout);
+        if (hipGetLastError() != hipSuccess)
+            rc = lvr::fail(ctx, LVR_EHIP, "field source kernel");
+        // each brick's histogram into the cache as it lands in its slot
+        if (rc == LVR_OK)
+            rc = lvr::cacheBrickHistograms(ctx, node_ids + base, slots.data(), cnt, ctx->stream);
         // the host vectors and the scratch are reused by the next chunk
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+        if (rc == LVR_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = lvr::fail(ctx, LVR_EHIP, "field source kernel");
     }
     return rc;

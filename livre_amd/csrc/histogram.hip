@@ -5,12 +5,15 @@
 // exact and independent of the order of the atomics.
 //
 // The reference caches one histogram per brick and sums the cached
-// histograms of the rendered set every frame. Here the device keeps the
-// running sum of the last set instead: a new frame adds the bricks that
-// entered the set and subtracts those that left it (two's-complement u64
-// adds, exact modulo 2^64, and the true result is never negative), and
-// recomputes from scratch when that is cheaper or when any brick of the
-// pool changed since.
+// histograms of the rendered set every frame. Here each pool slot has a cache
+// row of u32 counts, binned once for the brick it holds (histogramCached),
+// and the device keeps the running sum of the last set: a new frame adds the
+// rows of the bricks that entered the set and subtracts those that left it
+// (two's-complement u64 adds, exact modulo 2^64, and the true result is never
+// negative), and sums afresh when that is cheaper or a leaving brick's row is
+// gone. When the rows would take more than half the pool's bytes, bricks are
+// binned straight into the running sum instead (entering and leaving bricks
+// every frame; from scratch when any brick of the pool changed).
 //
 // Bricks are grouped by level on the host; a workgroup takes a run of bricks
 // of one level and one part of the bin range (all 256 bins for 8-bit and
@@ -98,12 +101,16 @@ __device__ inline uint32_t interiorBits(int tc, int lo, int hi)
     return m;
 }
 
-template <typename T>
+// TO_CACHE = false: counts of the job's bricks, scaled and signed, added to
+// the u64 result `out`. TO_CACHE = true (one brick per job): the brick's
+// unscaled counts of the job's bin part stored into its row of the per-slot
+// cache (row = slot, `bins` u32 per row), zeros included.
+template <typename T, bool TO_CACHE>
 __global__ __launch_bounds__(1024) void histKernel(
     const HistJob* __restrict__ jobs, const int32_t* __restrict__ slots,
     const uint8_t* __restrict__ pool, size_t slotStride, int M, int ov, uint32_t tileStride,
     int binsPerJob, int bins, float r0, float perBin, BrickLayout L,
-    unsigned long long* __restrict__ out)
+    unsigned long long* __restrict__ out, uint32_t* __restrict__ cache)
 {
     extern __shared__ uint32_t sHist[];
     const HistJob job = jobs[blockIdx.x];
@@ -153,6 +160,13 @@ __global__ __launch_bounds__(1024) void histKernel(
         }
     }
     __syncthreads();
+    if (TO_CACHE)
+    {
+        uint32_t* row = cache + size_t(slots[job.firstBrick]) * size_t(bins) + job.binLo;
+        for (int i = threadIdx.x; i < binsPerJob; i += blockDim.x)
+            row[i] = sHist[i];
+        return;
+    }
     for (int i = threadIdx.x; i < binsPerJob; i += blockDim.x)
     {
         const uint32_t c = sHist[i];
@@ -164,6 +178,51 @@ __global__ __launch_bounds__(1024) void histKernel(
     }
 }
 
+// A term of the cached sum: the row of `slot`, scaled by 2^shift, added or
+// (negate) subtracted.
+struct SumTerm
+{
+    int32_t slot;
+    int32_t shift;
+    int32_t negate;
+    int32_t pad;
+};
+
+// out[bin] += sum over a chunk of terms of +-(cache[slot][bin] << shift):
+// blockIdx.x picks 4 * blockDim.x consecutive bins (one 16-byte load per
+// term and thread, coalesced over the row), blockIdx.y a chunk of terms;
+// the chunk's sum goes to `out` with one atomic per non-zero bin. Integer
+// arithmetic modulo 2^64: exact in any order.
+constexpr int kSumTermsPerBlock = 32;
+__global__ __launch_bounds__(256) void histSumKernel(const SumTerm* __restrict__ terms, int nTerms,
+                                                     const uint32_t* __restrict__ cache, int bins,
+                                                     unsigned long long* __restrict__ out)
+{
+    const int b0 = (int(blockIdx.x) * int(blockDim.x) + int(threadIdx.x)) * 4;
+    if (b0 >= bins)
+        return;
+    const int t0 = int(blockIdx.y) * kSumTermsPerBlock;
+    const int t1 = min(nTerms, t0 + kSumTermsPerBlock);
+    unsigned long long acc[4] = {0ull, 0ull, 0ull, 0ull};
+    for (int t = t0; t < t1; ++t)
+    {
+        const SumTerm term = terms[t];
+        const uint4 c =
+            *reinterpret_cast<const uint4*>(cache + size_t(term.slot) * size_t(bins) + b0);
+        const uint32_t v[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            const unsigned long long x = (unsigned long long)v[k] << term.shift;
+            acc[k] += term.negate ? 0ull - x : x;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (acc[k])
+            atomicAdd(&out[b0 + k], acc[k]);
+}
+
 __global__ void clearKernel(unsigned long long* p, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,6 +230,332 @@ __global__ void clearKernel(unsigned long long* p, int n)
         p[i] = 0;
 }
 } // namespace
+
+// The per-slot cache is used when its rows (bins x u32 per pool slot) take at
+// most half the pool's bytes: 1 KiB per slot for 8-bit and float volumes,
+// 256 KiB per slot for 16-bit ones.
+static bool cacheFits(const lvr_ctx* ctx, int bins)
+{
+    const uint64_t rows = uint64_t(ctx->nSlots) * uint64_t(bins) * sizeof(uint32_t);
+    return rows <= uint64_t(ctx->nSlots) * uint64_t(ctx->slotStride) / 2;
+}
+
+// Bins per job part and a tile stride near 0.618 of the tile count, coprime
+// to it (consecutive threads take tiles far apart in the brick).
+static void jobShape(const lvr_ctx* ctx, int bins, int* halves, uint32_t* tileStride)
+{
+    *halves = bins > kMaxLdsBins ? bins / kMaxLdsBins : 1;
+    const uint32_t nT = ctx->layout.tiles[0] * ctx->layout.tiles[1] * ctx->layout.tiles[2];
+    uint32_t st = std::max<uint32_t>(1, uint32_t(double(nT) * 0.6180339887));
+    while (std::gcd(st, nT) != 1)
+        ++st;
+    *tileStride = st;
+}
+
+// The cache rows of every pool slot, allocated on first use.
+static int ensureHistCache(lvr_ctx* ctx, int bins)
+{
+    if (ctx->d_histCache && ctx->histCacheBins == bins)
+        return LVR_OK;
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    LVR_HIP(ctx, hipStreamSynchronize(ctx->uploadStream));
+    (void)hipFree(ctx->d_histCache);
+    ctx->d_histCache = nullptr;
+    ctx->histCacheBins = 0;
+    LVR_HIP(ctx, hipMalloc((void**)&ctx->d_histCache,
+                           size_t(ctx->nSlots) * size_t(bins) * sizeof(uint32_t)));
+    ctx->histCacheBins = bins;
+    ctx->histCachedId.assign(size_t(ctx->nSlots), ~0ull);
+    ctx->histRowVersion.assign(size_t(ctx->nSlots), 0u);
+    ctx->histValid = false;
+    return LVR_OK;
+}
+
+// The next pinned, device-mapped job buffer of at least `need` bytes, once
+// the kernels that read it last have finished; *dev is its device address.
+static int histRing(lvr_ctx* ctx, size_t need, uint8_t** host, uint8_t** dev, int* which)
+{
+    const int r = ctx->histRingNext;
+    ctx->histRingNext ^= 1;
+    if (!ctx->histRingDone[r])
+        LVR_HIP(ctx, hipEventCreateWithFlags(&ctx->histRingDone[r], hipEventDisableTiming));
+    else
+        LVR_HIP(ctx, hipEventSynchronize(ctx->histRingDone[r]));
+    if (need > ctx->histRingBytes[r])
+    {
+        if (ctx->h_histRing[r])
+            ctx->retiredHost.push_back(ctx->h_histRing[r]);
+        ctx->h_histRing[r] = nullptr;
+        const size_t cap = std::max(need + need / 4 + 4096, 2 * ctx->histRingBytes[r]);
+        ctx->histRingBytes[r] = 0;
+        LVR_HIP(ctx, hipHostMalloc((void**)&ctx->h_histRing[r], cap,
+                                   hipHostMallocMapped | hipHostMallocCoherent));
+        ctx->histRingBytes[r] = cap;
+    }
+    *host = ctx->h_histRing[r];
+    LVR_HIP(ctx, hipHostGetDevicePointer((void**)dev, ctx->h_histRing[r], 0));
+    *which = r;
+    return LVR_OK;
+}
+
+// HistogramObject per brick and HistogramFilter's sum (HistogramObject.cpp:
+// 140-224, HistogramFilter.cpp:78-149): each brick of the set is binned once
+// into its slot's cache row (unscaled counts of its interior voxels), the
+// first time it is counted after entering that slot; the frame's histogram is
+// the sum of the rows scaled by 8^(depth - level - 1). The running sum of the
+// previous set is updated by the rows of the bricks that entered (+) and
+// left (-) when every leaving brick's row is still its own and that is the
+// cheaper sum; otherwise it is summed afresh.
+static int histogramCached(lvr_ctx* ctx, std::vector<uint64_t>& next, int bins, float r0,
+                           float r1)
+{
+    const int dt = ctx->vi.data_type;
+    const int depth = int(ctx->vi.depth);
+    int rc0 = ensureHistCache(ctx, bins);
+    if (rc0)
+        return rc0;
+    if (dt == LVR_DT_FLOAT && (r0 != ctx->histCacheR0 || r1 != ctx->histCacheR1))
+    {
+        // float rows depend on the bin range (the TF range)
+        std::fill(ctx->histCachedId.begin(), ctx->histCachedId.end(), ~0ull);
+        ctx->histValid = false;
+    }
+    ctx->histCacheR0 = r0;
+    ctx->histCacheR1 = r1;
+    if (bins > ctx->histBins)
+    {
+        LVR_HIP(ctx, hipStreamSynchronize(ctx->histStream));
+        (void)hipFree(ctx->d_hist);
+        ctx->d_hist = nullptr;
+        ctx->histValid = false;
+        LVR_HIP(ctx, hipMalloc((void**)&ctx->d_hist, sizeof(unsigned long long) * bins));
+        ctx->histBins = bins;
+    }
+
+    std::vector<int32_t> nextSlots(next.size());
+    for (size_t i = 0; i < next.size(); ++i)
+        nextSlots[i] = slotOf(ctx, next[i]);
+    // entering and leaving bricks, with the slot each was (or will be)
+    // counted from
+    std::vector<std::pair<uint64_t, int32_t>> added, removed;
+    bool delta = ctx->histValid && ctx->histLast == bins &&
+                 ctx->histSetSlots.size() == ctx->histSet.size() &&
+                 ctx->histSetVersions.size() == ctx->histSet.size();
+    // a row still holds the counts the running sum took from it
+    auto same = [&](size_t j) {
+        const size_t slot = size_t(ctx->histSetSlots[j]);
+        return ctx->histCachedId[slot] == ctx->histSet[j] &&
+               ctx->histRowVersion[slot] == ctx->histSetVersions[j];
+    };
+    std::vector<size_t> removedAt;
+    if (delta)
+    {
+        size_t i = 0, j = 0;
+        const std::vector<uint64_t>& prev = ctx->histSet;
+        while (i < next.size() || j < prev.size())
+        {
+            if (j == prev.size() || (i < next.size() && next[i] < prev[j]))
+            {
+                added.emplace_back(next[i], nextSlots[i]);
+                ++i;
+            }
+            else if (i == next.size() || prev[j] < next[i])
+            {
+                removed.emplace_back(prev[j], ctx->histSetSlots[j]);
+                removedAt.push_back(j);
+                ++j;
+            }
+            else
+            {
+                // the same brick in both sets: unchanged unless it moved slot;
+                // a brick rewritten since has lost the row its counts in the
+                // running sum came from, so the sum starts afresh
+                if (nextSlots[i] != ctx->histSetSlots[j])
+                {
+                    added.emplace_back(next[i], nextSlots[i]);
+                    removed.emplace_back(prev[j], ctx->histSetSlots[j]);
+                    removedAt.push_back(j);
+                }
+                else if (!same(j))
+                    delta = false;
+                ++i;
+                ++j;
+            }
+        }
+        for (size_t j : removedAt)
+            delta = delta && same(j);
+        delta = delta && added.size() + removed.size() < next.size();
+    }
+    if (!delta)
+    {
+        added.clear();
+        removed.clear();
+        for (size_t i = 0; i < next.size(); ++i)
+            added.emplace_back(next[i], nextSlots[i]);
+    }
+
+    // rows to bin: bricks counted now whose slot does not hold their counts
+    int halves = 1;
+    uint32_t tileStride = 1;
+    jobShape(ctx, bins, &halves, &tileStride);
+    const int binsPerJob = bins / halves;
+    std::vector<int32_t> binSlots;
+    std::vector<HistJob> jobs;
+    for (const auto& a : added)
+        if (ctx->histCachedId[size_t(a.second)] != a.first)
+        {
+            ctx->histCachedId[size_t(a.second)] = a.first;
+            ++ctx->histRowVersion[size_t(a.second)];
+            const int first = int(binSlots.size());
+            binSlots.push_back(a.second);
+            for (int h = 0; h < halves; ++h)
+                jobs.push_back({first, 1, 0, h * binsPerJob, 0, {0, 0, 0}});
+        }
+    std::vector<SumTerm> terms;
+    terms.reserve(added.size() + removed.size());
+    for (const auto& a : added)
+        terms.push_back({a.second, 3 * (depth - int(decodeNode(a.first).level) - 1), 0, 0});
+    for (const auto& r : removed)
+        terms.push_back({r.second, 3 * (depth - int(decodeNode(r.first).level) - 1), 1, 0});
+
+    ctx->histLast = bins;
+    lvr_ctx::FrameRecord* rec =
+        ctx->timing && ctx->frameCount > 0
+            ? &ctx->frames[(ctx->frameCount - 1) % lvr_ctx::kFrameSlots]
+            : nullptr;
+    if (rec && !rec->timed)
+        rec = nullptr;
+    if (rec)
+        LVR_HIP(ctx, hipEventRecord(rec->ev[3], ctx->histStream));
+    if (!delta)
+        hipLaunchKernelGGL(clearKernel, dim3((bins + 255) / 256), dim3(256), 0,
+                           ctx->histStream, ctx->d_hist, bins);
+    if (!jobs.empty() || !terms.empty())
+    {
+        const size_t jobBytes = sizeof(HistJob) * jobs.size();
+        const size_t slotBytes = (sizeof(int32_t) * binSlots.size() + 15) & ~size_t(15);
+        const size_t termBytes = sizeof(SumTerm) * terms.size();
+        uint8_t *h = nullptr, *d = nullptr;
+        int r = 0;
+        int rc = histRing(ctx, jobBytes + slotBytes + termBytes, &h, &d, &r);
+        if (rc)
+            return rc;
+        std::memcpy(h, jobs.data(), jobBytes);
+        std::memcpy(h + jobBytes, binSlots.data(), sizeof(int32_t) * binSlots.size());
+        std::memcpy(h + jobBytes + slotBytes, terms.data(), termBytes);
+        if (!jobs.empty())
+        {
+            const int M = int(ctx->vi.max_block[0]);
+            const int ov = int(ctx->vi.overlap[0]);
+            const size_t lds = sizeof(uint32_t) * binsPerJob;
+            const float perBin = (r1 - r0) / float(bins - 1);
+            const HistJob* dj = reinterpret_cast<const HistJob*>(d);
+            const int32_t* ds = reinterpret_cast<const int32_t*>(d + jobBytes);
+            auto launch = [&](auto kernel) {
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+                hipLaunchKernelGGL(kernel, dim3(unsigned(jobs.size())), dim3(1024), lds,
+                                   ctx->histStream, dj, ds, ctx->d_pool, ctx->slotStride, M, ov,
+                                   tileStride, binsPerJob, bins, r0, perBin, ctx->layout,
+                                   ctx->d_hist, ctx->d_histCache);
+            };
+            if (dt == LVR_DT_UINT8)
+                launch(&histKernel<uint8_t, true>);
+            else if (dt == LVR_DT_UINT16)
+                launch(&histKernel<uint16_t, true>);
+            else
+                launch(&histKernel<float, true>);
+            LVR_HIP(ctx, hipGetLastError());
+        }
+        if (!terms.empty())
+        {
+            const int threads = std::min(256, bins / 4);
+            const dim3 grid(unsigned((bins / 4 + threads - 1) / threads),
+                            unsigned((terms.size() + kSumTermsPerBlock - 1) / kSumTermsPerBlock));
+            hipLaunchKernelGGL(histSumKernel, grid, dim3(unsigned(threads)), 0, ctx->histStream,
+                               reinterpret_cast<const SumTerm*>(d + jobBytes + slotBytes),
+                               int(terms.size()), ctx->d_histCache, bins, ctx->d_hist);
+            LVR_HIP(ctx, hipGetLastError());
+        }
+        LVR_HIP(ctx, hipEventRecord(ctx->histRingDone[r], ctx->histStream));
+    }
+    if (rec)
+    {
+        LVR_HIP(ctx, hipEventRecord(rec->ev[4], ctx->histStream));
+        rec->histTimed = true;
+    }
+    LVR_HIP(ctx, hipEventRecord(ctx->histDone, ctx->histStream));
+    std::vector<uint32_t> versions(nextSlots.size());
+    for (size_t i = 0; i < nextSlots.size(); ++i)
+        versions[i] = ctx->histRowVersion[size_t(nextSlots[i])];
+    ctx->histSet.swap(next);
+    ctx->histSetSlots.swap(nextSlots);
+    ctx->histSetVersions.swap(versions);
+    ctx->histValid = true;
+    ctx->histGeneration = ctx->poolGeneration;
+    ctx->histRange0 = r0;
+    ctx->histRange1 = r1;
+    ctx->histDeltaBricks = delta ? added.size() + removed.size() : next.size();
+    ctx->histBinned = binSlots.size();
+    return LVR_OK;
+}
+int cacheBrickHistograms(lvr_ctx* ctx, const uint64_t* ids, const int32_t* slots, int n,
+                         hipStream_t st)
+{
+    const int dt = ctx->vi.data_type;
+    // float rows depend on the TF range, known only when a frame is counted
+    if (n <= 0 || (dt != LVR_DT_UINT8 && dt != LVR_DT_UINT16))
+        return LVR_OK;
+    const int bins = dt == LVR_DT_UINT8 ? 256 : 65536;
+    const lvr_volume_info& vi = ctx->vi;
+    if (!cacheFits(ctx, bins) || vi.overlap[0] != vi.overlap[1] ||
+        vi.overlap[0] != vi.overlap[2] || vi.max_block[0] != vi.max_block[1] ||
+        vi.max_block[0] != vi.max_block[2])
+        return LVR_OK;
+    int rc = ensureHistCache(ctx, bins);
+    if (rc)
+        return rc;
+    int halves = 1;
+    uint32_t tileStride = 1;
+    jobShape(ctx, bins, &halves, &tileStride);
+    const int binsPerJob = bins / halves;
+    std::vector<HistJob> jobs;
+    jobs.reserve(size_t(n) * size_t(halves));
+    for (int i = 0; i < n; ++i)
+    {
+        ctx->histCachedId[size_t(slots[i])] = ids[i];
+        ++ctx->histRowVersion[size_t(slots[i])];
+        for (int h = 0; h < halves; ++h)
+            jobs.push_back({i, 1, 0, h * binsPerJob, 0, {0, 0, 0}});
+    }
+    const size_t jobBytes = sizeof(HistJob) * jobs.size();
+    uint8_t *h = nullptr, *d = nullptr;
+    int r = 0;
+    rc = histRing(ctx, jobBytes + sizeof(int32_t) * size_t(n), &h, &d, &r);
+    if (rc)
+        return rc;
+    std::memcpy(h, jobs.data(), jobBytes);
+    std::memcpy(h + jobBytes, slots, sizeof(int32_t) * size_t(n));
+    const size_t lds = sizeof(uint32_t) * binsPerJob;
+    auto launch = [&](auto kernel) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+        hipLaunchKernelGGL(kernel, dim3(unsigned(jobs.size())), dim3(1024), lds, st,
+                           reinterpret_cast<const HistJob*>(d),
+                           reinterpret_cast<const int32_t*>(d + jobBytes), ctx->d_pool,
+                           ctx->slotStride, int(vi.max_block[0]), int(vi.overlap[0]), tileStride,
+                           binsPerJob, bins, 0.f, 1.f, ctx->layout, ctx->d_hist,
+                           ctx->d_histCache);
+    };
+    if (dt == LVR_DT_UINT8)
+        launch(&histKernel<uint8_t, true>);
+    else
+        launch(&histKernel<uint16_t, true>);
+    LVR_HIP(ctx, hipGetLastError());
+    LVR_HIP(ctx, hipEventRecord(ctx->histRingDone[r], st));
+    return LVR_OK;
+}
 } // namespace lvr
 
 extern "C" {
@@ -216,6 +601,9 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
 
     const float r0 = dt == LVR_DT_FLOAT ? ctx->range0 : 0.f;
     const float r1 = dt == LVR_DT_FLOAT ? ctx->range1 : 0.f;
+    if (cacheFits(ctx, bins))
+        return histogramCached(ctx, next, bins, r0, r1);
+    ctx->histBinned = 0;
     bool delta = ctx->histValid && ctx->histLast == bins &&
                  ctx->histGeneration == ctx->poolGeneration &&
                  ctx->histRange0 == r0 && ctx->histRange1 == r1;
@@ -329,14 +717,14 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
                                       hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
             hipLaunchKernelGGL(kernel, grid, block, lds, ctx->histStream, dj, ds, ctx->d_pool,
                                ctx->slotStride, M, ov, tileStride, binsPerJob, bins, r0, perBin,
-                               ctx->layout, ctx->d_hist);
+                               ctx->layout, ctx->d_hist, nullptr);
         };
         if (dt == LVR_DT_UINT8)
-            launch(&histKernel<uint8_t>);
+            launch(&histKernel<uint8_t, false>);
         else if (dt == LVR_DT_UINT16)
-            launch(&histKernel<uint16_t>);
+            launch(&histKernel<uint16_t, false>);
         else
-            launch(&histKernel<float>);
+            launch(&histKernel<float, false>);
         LVR_HIP(ctx, hipGetLastError());
         LVR_HIP(ctx, hipEventRecord(ctx->histRingDone[r], ctx->histStream));
     }
@@ -348,6 +736,7 @@ int lvr_histogram(lvr_ctx* ctx, const uint64_t* ids, int n, int bins)
     LVR_HIP(ctx, hipGetLastError());
     LVR_HIP(ctx, hipEventRecord(ctx->histDone, ctx->histStream));
     ctx->histSet.swap(next);
+    ctx->histSetSlots.clear(); // counted from the pool, not from cached rows
     ctx->histValid = true;
     ctx->histGeneration = ctx->poolGeneration;
     ctx->histRange0 = r0;

@@ -419,6 +419,19 @@ struct lvr_ctx
     uint64_t histGeneration = 0;
     float histRange0 = 0.f, histRange1 = 0.f;
     uint64_t histDeltaBricks = 0;
+    // per-slot histogram cache (HistogramObject per brick): u32 counts of
+    // the interior voxels of the brick in the slot, unscaled; histCachedId[s]
+    // is the brick whose counts slot s holds (~0: none); histSetSlots gives
+    // the slot each brick of histSet was counted from
+    uint32_t* d_histCache = nullptr;
+    int histCacheBins = 0;
+    bool histCacheTried = false;
+    std::vector<uint64_t> histCachedId;
+    std::vector<uint32_t> histRowVersion; // bumped whenever a row is (re)binned
+    std::vector<int32_t> histSetSlots;
+    std::vector<uint32_t> histSetVersions; // row version each was counted with
+    float histCacheR0 = 0.f, histCacheR1 = 0.f;
+    uint64_t histBinned = 0; // bricks binned into the cache by the last call
     // bumped whenever brick contents change (upload, generation, eviction)
     uint64_t poolGeneration = 0;
 
@@ -492,6 +505,12 @@ uint64_t evictableSlots(const lvr_ctx* ctx);
 // Order slot reuse after every batch still reading the pool: stream `st`
 // waits for them, or the host does when st is null.
 int waitPoolReaders(lvr_ctx* ctx, hipStream_t st);
+// HistogramObject per brick, computed as the bricks land in their slots:
+// bin the bricks ids[0..n) just written into slots[0..n) (a host array) into
+// the per-slot histogram cache, on stream st after the writes. A no-op where
+// the cache is not used (float data, rows above half the pool's bytes).
+int cacheBrickHistograms(lvr_ctx* ctx, const uint64_t* ids, const int32_t* slots, int n,
+                         hipStream_t st);
 // Device scratch of at least `bytes` owned by the context (nullptr + error).
 void* genScratch(lvr_ctx* ctx, size_t bytes);
 // lvr_render_noalloc set-up, called by lvr_ctx_create with the context's

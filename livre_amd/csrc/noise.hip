@@ -509,9 +509,13 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
             hipLaunchKernelGGL(lvr::noiseBricksKernel<float>, grid, dim3(256), 0,
                                ctx->stream, s, d_remap, d_geo, d_slots,
                                ctx->d_pool, ctx->slotStride, ctx->layout);
+        if (hipGetLastError() != hipSuccess)
+            rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");
+        // each brick's histogram into the cache as it lands in its slot
+        if (rc == LVR_OK)
+            rc = lvr::cacheBrickHistograms(ctx, node_ids + base, slots.data(), cnt, ctx->stream);
         // the host vectors and the scratch are reused by the next chunk
-        if (hipGetLastError() != hipSuccess ||
-            hipStreamSynchronize(ctx->stream) != hipSuccess)
+        if (rc == LVR_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");
     }
     return rc;

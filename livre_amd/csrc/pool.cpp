@@ -295,7 +295,10 @@ static int acquireSlot(lvr_ctx* ctx, uint64_t id, int32_t* slot)
     auto it = ctx->resident.find(id);
     if (it != ctx->resident.end())
     {
+        // the brick's voxels are about to be rewritten in place
         *slot = it->second.slot;
+        if (!ctx->histCachedId.empty())
+            ctx->histCachedId[size_t(*slot)] = ~0ull;
         touch(ctx, id);
         return LVR_OK;
     }
@@ -307,6 +310,8 @@ static int acquireSlot(lvr_ctx* ctx, uint64_t id, int32_t* slot)
     }
     *slot = ctx->freeSlots.back();
     ctx->freeSlots.pop_back();
+    if (!ctx->histCachedId.empty())
+        ctx->histCachedId[size_t(*slot)] = ~0ull; // its cached counts are stale
     ctx->lru.pushBack(int32_t(idx), id);
     PoolEntry e;
     e.slot = *slot;
@@ -594,6 +599,7 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     }
     (void)hipFree(ctx->d_counters);
     (void)hipFree(ctx->d_hist);
+    (void)hipFree(ctx->d_histCache);
     for (int r = 0; r < 2; ++r)
     {
         if (ctx->h_histRing[r])
@@ -776,6 +782,10 @@ int lvr_brick_upload_many(lvr_ctx* ctx, const uint64_t* node_ids, int n,
         LVR_HIP(ctx, hipMemcpyAsync(ctx->d_stagingSlots, slots, sizeof(int32_t) * cnt,
                                     hipMemcpyHostToDevice, ctx->uploadStream));
         rc = lvr::convertLayout(ctx, cnt, true, ctx->uploadStream);
+        if (rc)
+            return rc;
+        // each brick's histogram into the cache as it lands in its slot
+        rc = lvr::cacheBrickHistograms(ctx, node_ids + base, slots, cnt, ctx->uploadStream);
         if (rc)
             return rc;
     }

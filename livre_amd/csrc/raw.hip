@@ -782,8 +782,13 @@ int lvr_pyramid_load(lvr_ctx* ctx, const lvr_raw_source* src, const uint64_t* no
                                ctx->slotStride, ctx->layout);
             break;
         }
+        if (hipGetLastError() != hipSuccess)
+            rc = fail(ctx, LVR_EHIP, "pyramid bricks");
+        // each brick's histogram into the cache as it lands in its slot
+        if (rc == LVR_OK)
+            rc = cacheBrickHistograms(ctx, node_ids + base, slots.data(), cnt, ctx->stream);
         // the host vectors and the scratch are reused by the next chunk
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+        if (rc == LVR_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = fail(ctx, LVR_EHIP, "pyramid bricks");
     }
     return rc;

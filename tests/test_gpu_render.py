@@ -291,6 +291,50 @@ def test_histogram_incremental_sequence(lvr, orc):
         ctx.close()
 
 
+@pytest.mark.parametrize("dt,n,block,bins", [(livre_amd.DT_UINT8, 64, 16, 256),
+                                              (livre_amd.DT_UINT16, 128, 64, 65536)])
+def test_histogram_cache_sequence(lvr, orc, dt, n, block, bins):
+    """The per-slot cache of brick histograms (HistogramObject per brick):
+    frames over changing sets, a brick rewritten in place, a brick evicted and
+    reloaded into another slot, a brick of the previous set evicted (its row
+    is gone: the sum starts afresh); every frame equals the oracle's count."""
+    s = Scene(dt, n=n, block=block, overlap=1,
+              source="blobs" if dt == livre_amd.DT_UINT8 else "noise", seed=37)
+    slot = livre_amd.slot_bytes(s.vi)
+    assert slot * 2 >= bins * 4  # the cache's rows fit: its path is the one taken
+    ctx = s.context(pool_bytes=slot * (len(s.ids) + 2))
+    try:
+        prev = None
+        for k in range(9):
+            eye = scenes.orbit(k, 12, radius=1.5 + 0.12 * k, height=0.3 - 0.05 * k)
+            mv, proj, fr = s.camera(eye, 96, 96)
+            ids = s.visible_ordered(mv, proj, 96, sse=0.8)
+            if k == 3:  # new voxels for a brick of the set, in its slot
+                nid = int(ids[len(ids) // 2])
+                s.bricks[nid] = (s.bricks[nid] // 2).astype(s.bricks[nid].dtype)
+                ctx.upload([nid], s.bricks[nid][None])
+            if k == 5:  # a brick of the set moves to another slot
+                nid = int(ids[0])
+                other = int(next(i for i in s.ids if int(i) not in set(int(x) for x in ids)))
+                ctx.evict(nid)
+                ctx.evict(other)
+                ctx.upload([nid], s.bricks[nid][None])
+                ctx.upload([other], s.bricks[other][None])
+            if k == 7 and prev is not None:  # a brick of the previous set leaves the pool
+                gone = int(prev[-1])
+                ctx.evict(gone)
+                ids = np.array([i for i in ids if int(i) != gone], np.uint64)
+            got = ctx.histogram(ids, bins)
+            # bricks not resident are skipped (the histogram cache cannot
+            # load them)
+            here = [int(i) for i in ids if ctx.resident(int(i))]
+            ref = oracle.histogram(s.ovi, here, [s.bricks[i] for i in here], bins, s.range)
+            assert np.array_equal(got, ref), k
+            prev = ids
+    finally:
+        ctx.close()
+
+
 def test_selection_bit_exact_against_goldens(lvr, orc):
     import json
     import os

@@ -306,8 +306,9 @@ int lvr_samples_per_ray(const lvr_volume_info* info, const uint64_t* ids, int n,
 
 typedef struct
 {
-    int32_t n_samples_per_ray;   /* nSamplesPerRay; 0 = lvr_samples_per_ray's rule
-                                    over the frame's bricks (RayCastRenderer.cpp:216-234) */
+    int32_t n_samples_per_ray;   /* nSamplesPerRay, at most 1048576; 0 =
+                                    lvr_samples_per_ray's rule over the frame's
+                                    bricks (RayCastRenderer.cpp:216-234) */
     int32_t max_samples_per_ray; /* opacity reference, 32 in the reference */
     float early_exit;            /* EARLY_EXIT, 0.999 (fragRayCast.glsl:12) */
     int32_t linear_filter;       /* 0: GL_NEAREST (reference default), 1: trilinear */
@@ -315,7 +316,9 @@ typedef struct
     float clip_planes[24];       /* normal xyz, d */
     int32_t tile_w, tile_h;      /* wavefront ray tile (w * h = 64); 0, 0 = 8 x 8 */
     int32_t samples_per_thread;  /* voxel fetches in flight per ray, 1..3 (NEAREST);
-                                    0 = 2 */
+                                    0 = 2. With 2 and per-sample classification a
+                                    ray may run one brick ahead of the others of
+                                    its wavefront tile; identical images always */
     int32_t classify;            /* 8/16-bit NEAREST: 1 = classification tabulated
                                     per raw value (one table read per sample),
                                     2 = computed per sample from the TF,

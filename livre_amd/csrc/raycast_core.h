@@ -167,30 +167,39 @@ __device__ inline float sampleBrick(const T* __restrict__ vox, const FrameUnifor
 
 // calcTexturePositionFromAABBPos (fragRayCast.glsl:72-76) for a sample
 // position: (pos - aabbMin) / (aabbMax - aabbMin) correctly rounded, from the
-// host's IEEE reciprocal of the divisor and one FMA correction step
-// (Markstein), then * (texMax - texMin) + texMin.
+// IEEE reciprocal of the divisor and one FMA correction step (Markstein),
+// then * (texMax - texMin) + texMin.
 // With P2 (the brick's extents are powers of two, BrickEntry::kBrickPow2)
-// the quotient is exact and b.rcp holds (1 / ext) * texExt, so one multiply
+// the quotient is exact and rcp holds (1 / ext) * texExt, so one multiply
 // gives the same rounded product.
+template <bool P2>
+__device__ __forceinline__ void texturePositionOf(const FrameUniforms& U, const float bmin[3],
+                                                  const float rcp[3], const float ext[3],
+                                                  const float texExt[3], const float pos[3],
+                                                  float tp[3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const float x = pos[a] - bmin[a];
+        if (P2)
+            tp[a] = x * rcp[a] + U.texMin[a];
+        else
+        {
+            const float q = x * rcp[a];
+            const float r = __builtin_fmaf(-q, ext[a], x);
+            const float quot = __builtin_fmaf(r, rcp[a], q);
+            tp[a] = quot * texExt[a] + U.texMin[a];
+        }
+    }
+}
+
 template <bool P2>
 __device__ __forceinline__ void texturePosition(const FrameUniforms& U, const BrickEntry& b,
                                                 const float texExt[3], const float pos[3],
                                                 float tp[3])
 {
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-    {
-        const float x = pos[a] - b.bmin[a];
-        if (P2)
-            tp[a] = x * b.rcp[a] + U.texMin[a];
-        else
-        {
-            const float q = x * b.rcp[a];
-            const float r = __builtin_fmaf(-q, b.ext[a], x);
-            const float quot = __builtin_fmaf(r, b.rcp[a], q);
-            tp[a] = quot * texExt[a] + U.texMin[a];
-        }
-    }
+    texturePositionOf<P2>(U, b.bmin, b.rcp, b.ext, texExt, pos, tp);
 }
 
 // Byte offset in the slot of the GL_NEAREST texel of a sample position.
@@ -296,9 +305,7 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
 
 // The march of one ray through one brick segment with GL_NEAREST fetches
 // (fragRayCast.glsl:184-204): `travel` starts at the segment length and
-// drops by stepSize per composite step; a segment lies inside one brick, so it
-// holds at most diagonal * nSamples steps, and the guard only bounds the loop
-// should `travel - stepSize` ever round back to `travel`.
+// drops by stepSize per composite step (it ends: see kMaxSegment).
 //
 // D samples per thread are in the pipeline: while sample k is classified and
 // composited, the voxels of samples k+1 .. k+D-1 are being fetched. The fetch
@@ -319,7 +326,6 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
     constexpr int S = D > 1 ? D - 1 : 1;
     if (!(travel > 0.0f))
         return;
-    int guard = 1 << 22;
     T raw[S];
     raw[0] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
 #pragma unroll
@@ -336,7 +342,6 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
         for (int s = 0; s < S; ++s)
         {
             travel -= stepSize;
-            --guard;
             float4 cls = make_float4(0.f, 0.f, 0.f, 0.f);
             float value = 0.f;
             if (CLS)
@@ -362,7 +367,7 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
             }
             ++nsamp;
             const bool opaque = CLS ? C.blend(U, cls) : C.add(U, tf, value);
-            if (opaque || !(travel > 0.0f && guard > 0))
+            if (opaque || !(travel > 0.0f))
             {
                 more = false;
                 break;
@@ -405,7 +410,6 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
     constexpr int S = D + 1;
     if (!(travel > 0.0f))
         return;
-    int guard = 1 << 22;
     const uint32_t first = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
     // the voxel of sample m (m >= 1) of the segment is raw[(m - 1) % S]; one
     // full register each (two 16-bit voxels packed in one register would
@@ -435,9 +439,8 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
             C.acc3 = __builtin_fmaf(cls.w, rem, C.acc3);
             nsamp += alive ? 1u : 0u;
             travel -= stepSize;
-            --guard;
             // bitwise: a short-circuit would put branches in the pass
-            alive = alive & !(C.acc3 > U.exitA) & (travel > 0.0f) & (guard > 0);
+            alive = alive & !(C.acc3 > U.exitA) & (travel > 0.0f);
             cls = tab[raw[u]];
             __builtin_amdgcn_sched_barrier(0);
             pos[0] = pos[0] + step[0];
@@ -453,23 +456,29 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
     }
 }
 
-// One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
-// ordered list, composited in registers, then the RGBA8 store. `tf` is the
-// transfer function (LDS or global, resolved at the call site), or with CLS
-// the classified table indexed by the raw voxel value.
-template <typename T, bool LINEAR, int D, bool CLS = false>
-__device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int lane,
-                                         const BrickEntry* __restrict__ bricks,
-                                         const int32_t* __restrict__ offsets,
-                                         const int32_t* __restrict__ counts,
-                                         const int32_t* __restrict__ lists,
-                                         const float4* tf, uint32_t& nsamp)
-{
-    const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
-    const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
-    const bool inside = px < U.width && py < U.height;
+// Longest segment a march takes, in world units. A brick box lies within 3
+// world units per axis (the octree's root blocks cover less than three times
+// the volume's longest axis, which is 1), so a real segment is shorter than
+// 6 and is never cut; the bound keeps every march finite for any input:
+// with at most kMaxSamplesPerRay samples per ray (the host rejects more) the
+// step is at least 2^-20, more than half an ulp of every travel up to 16, so
+// `travel -= stepSize` decreases strictly and a segment takes at most 2^24
+// steps.
+constexpr float kMaxSegment = 16.0f;
+constexpr int32_t kMaxSamplesPerRay = 1 << 20;
 
-    // pixel ray (fragRayCast.glsl:121-140)
+// A pixel's ray (fragRayCast.glsl:121-140) and what every brick fragment of
+// it recomputes identically: the direction, 1 / dir with zero components
+// replaced by EPSILON (intersectBox), the near-plane distance along the ray,
+// the view cosine of the exit depth and the entry into the volume's box.
+struct PixelRay
+{
+    float dir[3], inv[3];
+    float tNearPlane, viewCos, tnearGlobal;
+};
+
+__device__ __forceinline__ void pixelRay(const FrameUniforms& U, int px, int py, PixelRay& R)
+{
     const float ndcx = (float(px) + 0.5f) / float(U.width) * 2.0f - 1.0f;
     const float ndcy = (float(py) + 0.5f) / float(U.height) * 2.0f - 1.0f;
     const float* P = U.invProj;
@@ -486,7 +495,10 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
     wp[0] = M[0] * ep[0] + M[4] * ep[1] + M[8] * ep[2] + M[12] * 1.0f;
     wp[1] = M[1] * ep[0] + M[5] * ep[1] + M[9] * ep[2] + M[13] * 1.0f;
     wp[2] = M[2] * ep[0] + M[6] * ep[1] + M[10] * ep[2] + M[14] * 1.0f;
-    float dir[3] = {wp[0] - U.eye[0], wp[1] - U.eye[1], wp[2] - U.eye[2]};
+    float* dir = R.dir;
+    dir[0] = wp[0] - U.eye[0];
+    dir[1] = wp[1] - U.eye[1];
+    dir[2] = wp[2] - U.eye[2];
     {
         const float l = sqrtf(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
         dir[0] = dir[0] / l;
@@ -498,16 +510,112 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         const float l = sqrtf(ep[0] * ep[0] + ep[1] * ep[1] + ep[2] * ep[2]);
         nz = ep[2] / l;
     }
-    const float tNearPlane = -U.nearPlane / nz;
-    const float viewCos = -nz;
-    // 1 / r.Dir with zero components replaced by EPSILON, as intersectBox
-    // computes it for every brick of this ray (same value each time)
-    float inv[3];
+    R.tNearPlane = -U.nearPlane / nz;
+    R.viewCos = -nz;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
-        inv[i] = 1.0f / (dir[i] == 0.f ? 0.0000000001f : dir[i]);
-    float tnearGlobal, tfarGlobal;
-    intersectBoxInv(U.eye, inv, U.gmin, U.gmax, tnearGlobal, tfarGlobal);
+        R.inv[i] = 1.0f / (dir[i] == 0.f ? 0.0000000001f : dir[i]);
+    float tfarGlobal;
+    intersectBoxInv(U.eye, R.inv, U.gmin, U.gmax, R.tnearGlobal, tfarGlobal);
+}
+
+// The segment one brick's back-face fragment marches for a ray
+// (fragRayCast.glsl:141-183): the fragment exists where the exit point lies
+// between the near and far clip planes; the entry is moved to the near plane
+// and onto the ray's global sample grid, then cut by the clip planes. Returns
+// false when the brick has no fragment for the ray or the march takes no
+// step; else the start, the step vector and the travel (the segment length,
+// at most kMaxSegment).
+__device__ __forceinline__ bool brickSegment(const FrameUniforms& U, const PixelRay& R,
+                                             const BrickEntry& b, float rs[3], float step[3],
+                                             float& travel)
+{
+    const float* dir = R.dir;
+    const float stepSize = U.stepSize;
+    float tnear, tfar;
+    intersectBoxInv(U.eye, R.inv, b.bmin, b.bmax, tnear, tfar);
+    const float exitDepth = tfar * R.viewCos;
+    if (!(exitDepth >= U.nearPlane && exitDepth <= U.farPlane))
+        return false;
+    if (tnear < R.tNearPlane)
+        tnear = R.tNearPlane;
+    {
+        // GLSL mod(x, y) = x - y * floor(x / y)
+        const float x = tnear - R.tnearGlobal;
+        const float residu = x - stepSize * floorf(x / stepSize);
+        if (residu > 0.0f)
+            tnear += stepSize - residu;
+    }
+    if (tnear > tfar)
+        return false;
+    for (int c = 0; c < U.nClip; ++c)
+    {
+        const float* cp = U.clip + 4 * c;
+        float rn = dir[0] * cp[0] + dir[1] * cp[1] + dir[2] * cp[2];
+        if (rn == 0.f)
+            rn = 0.0000000001f;
+        const float t =
+            -((cp[0] * U.eye[0] + cp[1] * U.eye[1] + cp[2] * U.eye[2]) + cp[3]) / rn;
+        if (rn > 0.0f)
+            tnear = fmaxf(tnear, t);
+        else
+            tfar = fminf(tfar, t);
+    }
+    if (tnear > tfar)
+        return false;
+    rs[0] = U.eye[0] + dir[0] * tnear;
+    rs[1] = U.eye[1] + dir[1] * tnear;
+    rs[2] = U.eye[2] + dir[2] * tnear;
+    const float re[3] = {U.eye[0] + dir[0] * tfar, U.eye[1] + dir[1] * tfar,
+                         U.eye[2] + dir[2] * tfar};
+    const float seg[3] = {re[0] - rs[0], re[1] - rs[1], re[2] - rs[2]};
+    const float segLen = sqrtf(seg[0] * seg[0] + seg[1] * seg[1] + seg[2] * seg[2]);
+    step[0] = seg[0] / segLen * stepSize;
+    step[1] = seg[1] / segLen * stepSize;
+    step[2] = seg[2] / segLen * stepSize;
+    travel = segLen > kMaxSegment ? kMaxSegment : segLen;
+    return travel > 0.0f; // false for an empty segment (and a NaN one)
+}
+
+// fragTexCopy.glsl into the RGBA8 framebuffer, round(clamp(c) * 255), or the
+// unconverted RGBA32F accumulation into a sort-last layer.
+__device__ __forceinline__ void storePixel(const FrameUniforms& U, int px, int py,
+                                           const Composite& C)
+{
+    if (U.layer)
+    {
+        U.layer[size_t(py) * U.width + px] = make_float4(C.acc0, C.acc1, C.acc2, C.acc3);
+        return;
+    }
+    const float c[4] = {C.acc0, C.acc1, C.acc2, C.acc3};
+    uint32_t packed = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+    {
+        const float v = fminf(1.0f, fmaxf(0.0f, c[i]));
+        packed |= uint32_t(floorf(v * 255.0f + 0.5f)) << (8 * i);
+    }
+    U.frame[size_t(py) * U.width + px] = packed;
+}
+
+// One wavefront tile of rays: fragRayCast.glsl for each brick of the tile's
+// ordered list, composited in registers, then the RGBA8 store. `tf` is the
+// transfer function (LDS or global, resolved at the call site), or with CLS
+// the classified table indexed by the raw voxel value. The wavefront marches
+// one brick at a time; castTileWindow below lets lanes run one brick ahead.
+template <typename T, bool LINEAR, int D, bool CLS = false>
+__device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int lane,
+                                         const BrickEntry* __restrict__ bricks,
+                                         const int32_t* __restrict__ offsets,
+                                         const int32_t* __restrict__ counts,
+                                         const int32_t* __restrict__ lists,
+                                         const float4* tf, uint32_t& nsamp)
+{
+    const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
+    const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
+    const bool inside = px < U.width && py < U.height;
+    PixelRay R;
+    pixelRay(U, px, py, R);
 
     const float stepSize = U.stepSize;
     const float* texExt = U.texExt;
@@ -538,49 +646,9 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
             alive = false;
             continue;
         }
-        float tnear, tfar;
-        intersectBoxInv(U.eye, inv, b.bmin, b.bmax, tnear, tfar);
-        // a back-face fragment exists where the exit point lies between the
-        // near and far clip planes
-        const float exitDepth = tfar * viewCos;
-        if (!(exitDepth >= U.nearPlane && exitDepth <= U.farPlane))
+        float pos[3], step[3], segLen;
+        if (!brickSegment(U, R, b, pos, step, segLen))
             continue;
-        if (tnear < tNearPlane)
-            tnear = tNearPlane;
-        {
-            // GLSL mod(x, y) = x - y * floor(x / y)
-            const float x = tnear - tnearGlobal;
-            const float residu = x - stepSize * floorf(x / stepSize);
-            if (residu > 0.0f)
-                tnear += stepSize - residu;
-        }
-        if (tnear > tfar)
-            continue;
-        for (int c = 0; c < U.nClip; ++c)
-        {
-            const float* cp = U.clip + 4 * c;
-            float rn = dir[0] * cp[0] + dir[1] * cp[1] + dir[2] * cp[2];
-            if (rn == 0.f)
-                rn = 0.0000000001f;
-            const float t =
-                -((cp[0] * U.eye[0] + cp[1] * U.eye[1] + cp[2] * U.eye[2]) + cp[3]) / rn;
-            if (rn > 0.0f)
-                tnear = fmaxf(tnear, t);
-            else
-                tfar = fminf(tfar, t);
-        }
-        if (tnear > tfar)
-            continue;
-
-        const float rs[3] = {U.eye[0] + dir[0] * tnear, U.eye[1] + dir[1] * tnear,
-                             U.eye[2] + dir[2] * tnear};
-        const float re[3] = {U.eye[0] + dir[0] * tfar, U.eye[1] + dir[1] * tfar,
-                             U.eye[2] + dir[2] * tfar};
-        const float seg[3] = {re[0] - rs[0], re[1] - rs[1], re[2] - rs[2]};
-        const float segLen = sqrtf(seg[0] * seg[0] + seg[1] * seg[1] + seg[2] * seg[2]);
-        const float step[3] = {seg[0] / segLen * stepSize, seg[1] / segLen * stepSize,
-                               seg[2] / segLen * stepSize};
-        float pos[3] = {rs[0], rs[1], rs[2]};
         const T* vox = reinterpret_cast<const T*>(U.pool + size_t(slot) * U.slotStride);
 
         // the brick's flags are wave-uniform (scalar loads): no divergence
@@ -607,11 +675,7 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         }
         else
         {
-            // A segment lies inside one brick, so it holds at most
-            // diagonal * nSamples steps; the guard only bounds the loop should
-            // `travel - stepSize` ever round back to `travel`.
-            int guard = 1 << 22;
-            for (float travel = segLen; travel > 0.0f && guard > 0; travel -= stepSize, --guard)
+            for (float travel = segLen; travel > 0.0f; travel -= stepSize)
             {
                 float tp[3];
                 if (p2)
@@ -627,22 +691,243 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
             }
         }
     }
+    if (inside)
+        storePixel(U, px, py, C);
+}
 
-    if (inside && U.layer)
-        U.layer[size_t(py) * U.width + px] = make_float4(C.acc0, C.acc1, C.acc2, C.acc3);
-    else if (inside)
-    {
-        // fragTexCopy.glsl into the RGBA8 framebuffer: round(clamp(c) * 255)
-        const float c[4] = {C.acc0, C.acc1, C.acc2, C.acc3};
-        uint32_t packed = 0;
+// The brick a lane of castTileWindow marches, in registers: its box minimum
+// and extent, its slot, and the texture-position reciprocal in both forms:
+// rcpP, BrickEntry::rcp of a power-of-two brick ((1 / ext) * texExt, read by
+// the P2 march), and rcpG, 1 / ext correctly rounded (read by the general
+// march; for a power-of-two brick its quotient is then exact and equals the
+// P2 product bit for bit). A window marches with the P2 form when both of
+// its bricks are powers of two, so a lane always finds the form it needs.
+struct LaneBrick
+{
+    float bmin[3], ext[3], rcpG[3], rcpP[3];
+    const uint8_t* vox;
+};
+
+// 1 / ext for the bricks of a window (wave-uniform): BrickEntry::rcp, or
+// for a power-of-two brick (normal, with a normal reciprocal: kBrickPow2)
+// the reciprocal 2^-e of ext = 2^e, exact, by negating the exponent.
+__device__ __forceinline__ void generalReciprocal(const BrickEntry& b, float rcpG[3])
+{
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-        {
-            const float v = fminf(1.0f, fmaxf(0.0f, c[i]));
-            packed |= uint32_t(floorf(v * 255.0f + 0.5f)) << (8 * i);
-        }
-        U.frame[size_t(py) * U.width + px] = packed;
+    for (int a = 0; a < 3; ++a)
+        rcpG[a] = (b.flags & kBrickPow2)
+                      ? __uint_as_float(0x7F000000u - __float_as_uint(b.ext[a]))
+                      : b.rcp[a];
+}
+
+__device__ __forceinline__ void laneBrick(const BrickEntry& b, const float rcpG[3],
+                                          const uint8_t* vox, LaneBrick& L)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        L.bmin[a] = b.bmin[a];
+        L.ext[a] = b.ext[a];
+        L.rcpG[a] = rcpG[a];
+        L.rcpP[a] = b.rcp[a];
     }
+    L.vox = vox;
+}
+
+template <typename T, bool P2>
+__device__ __forceinline__ T laneFetch(const FrameUniforms& U, const LaneBrick& L,
+                                       const float pos[3])
+{
+    float tp[3];
+    texturePositionOf<P2>(U, L.bmin, P2 ? L.rcpP : L.rcpG, L.ext, U.texExt, pos, tp);
+    const uint32_t ix = nearestTexel(tp[0] * U.texDimF[0], U.texLastF[0]);
+    const uint32_t iy = nearestTexel(tp[1] * U.texDimF[1], U.texLastF[1]);
+    const uint32_t iz = nearestTexel(tp[2] * U.texDimF[2], U.texLastF[2]);
+    // the texel indices are clamped into the brick, so the offset stays
+    // inside the slot
+    return *reinterpret_cast<const T*>(L.vox + tiledOffsetDev<T>(U, ix, iy, iz));
+}
+
+// A lane of castTileWindow. The wavefront holds two bricks of the tile's
+// list: A, the oldest one a lane still marches, and B, the next one with a
+// segment for some lane (list positions kA < kB; `seg` is the position of
+// the brick the lane marches). A lane marches while travel > 0 and it is not
+// opaque; it has a B segment while travelB > 0 (set up for every lane when B
+// was chosen). A lane that ends its A segment goes on with its B segment at
+// once; one that ends its B segment, or has none, waits. When no lane
+// marches A any more, B becomes A and the next brick becomes B. Each lane so
+// composites its bricks in list order, sample by sample exactly as castTile
+// does; only the wavefront's schedule differs: the lanes of a tile no longer
+// wait for the longest segment of every brick.
+struct WindowLane
+{
+    float pos[3], step[3], travel;
+    float posB[3], stepB[3], travelB;
+    int32_t seg;
+    LaneBrick L;
+};
+
+// The march of the lanes that are marching, until none marches A:
+// GL_NEAREST fetches with one in flight per lane (the voxel of the next
+// sample is fetched before the current one is classified and blended).
+template <typename T, bool P2>
+__device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4* tf,
+                                            Composite& C, WindowLane& W, T& raw, int kA,
+                                            int kB, const BrickEntry& bB, const float rcpGB[3],
+                                            const uint8_t* voxB, uint32_t& nsamp)
+{
+    const float stepSize = U.stepSize;
+    for (;;)
+    {
+        // uniform exit: the lanes still here march A or B
+        if (__ballot(W.seg == kA) == 0)
+            break;
+        const float value = float(raw);
+        W.travel -= stepSize;
+        W.pos[0] = W.pos[0] + W.step[0];
+        W.pos[1] = W.pos[1] + W.step[1];
+        W.pos[2] = W.pos[2] + W.step[2];
+        bool cont = W.travel > 0.0f;
+        // a lane marching B has travelB == 0 (its B segment is taken), so
+        // travelB > 0 means an A lane with a B segment
+        if (!cont && W.travelB > 0.0f)
+        {
+            // the A segment ended: the B segment starts with this fetch
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                W.pos[a] = W.posB[a];
+                W.step[a] = W.stepB[a];
+            }
+            W.travel = W.travelB;
+            W.travelB = 0.0f;
+            W.seg = kB;
+            laneBrick(bB, rcpGB, voxB, W.L);
+            cont = true;
+        }
+        const T next = laneFetch<T, P2>(U, W.L, W.pos);
+        __builtin_amdgcn_sched_barrier(0);
+        raw = next;
+        ++nsamp;
+        const bool opaque = C.add(U, tf, value);
+        if (opaque || !cont)
+            break;
+    }
+}
+
+// castTile with the two-brick window (WindowLane) for the GL_NEAREST march
+// with per-sample classification: identical samples and frames, fewer idle
+// lanes per wavefront step.
+template <typename T>
+__device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile, int lane,
+                                               const BrickEntry* __restrict__ bricks,
+                                               const int32_t* __restrict__ offsets,
+                                               const int32_t* __restrict__ counts,
+                                               const int32_t* __restrict__ lists,
+                                               const float4* tf, uint32_t& nsamp)
+{
+    const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
+    const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
+    const bool inside = px < U.width && py < U.height;
+    PixelRay R;
+    pixelRay(U, px, py, R);
+    Composite C;
+    C.cls = makeClassify(U);
+    C.acc0 = C.acc1 = C.acc2 = C.acc3 = 0.f;
+
+    WindowLane W;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        W.pos[a] = W.step[a] = W.posB[a] = W.stepB[a] = 0.f;
+        W.L.bmin[a] = W.L.ext[a] = W.L.rcpG[a] = W.L.rcpP[a] = 0.f;
+    }
+    W.travel = W.travelB = 0.f;
+    W.seg = -1;
+    W.L.vox = U.pool;
+    T raw = T(0);
+
+    const int gtile = U.tileBase + tile;
+    const int beg = __builtin_amdgcn_readfirstlane(offsets[gtile]);
+    const int cnt = __builtin_amdgcn_readfirstlane(counts[gtile]);
+    BrickEntry bB; // wave-uniform
+    bB.flags = kBrickPow2;
+    float rcpGB[3] = {0.f, 0.f, 0.f};
+    const uint8_t* voxB = U.pool;
+    int kA = -1, kB = -1;
+    bool p2B = true;
+    int k = 0;
+    for (;;)
+    {
+        // a ray ends when it is opaque (imageLoad + discard,
+        // fragRayCast.glsl:115-117)
+        const bool alive = inside && !(C.acc3 > U.exitA);
+        if (__ballot(alive) == 0)
+            break;
+        // the lanes still marching are in B: it becomes A
+        kA = kB;
+        const bool p2A = p2B;
+        // B: the next brick of the list with a segment for some lane; bricks
+        // that are not resident are skipped (RayCastRenderer.cpp:517-522)
+        bool found = false;
+        while (k < cnt)
+        {
+            const int bi = __builtin_amdgcn_readfirstlane(lists[beg + k]);
+            ++k;
+            const BrickEntry cand = bricks[bi];
+            if (cand.slot < 0)
+                continue;
+            W.travelB = 0.0f;
+            if (alive)
+                brickSegment(U, R, cand, W.posB, W.stepB, W.travelB);
+            if (__ballot(W.travelB > 0.0f) != 0)
+            {
+                bB = cand;
+                kB = k - 1;
+                found = true;
+                break;
+            }
+        }
+        const bool marching = alive && W.travel > 0.0f;
+        if (!found)
+        {
+            W.travelB = 0.0f;
+            if (__ballot(marching) == 0)
+                break;
+        }
+        else
+        {
+            voxB = U.pool + size_t(bB.slot) * U.slotStride;
+            p2B = (bB.flags & kBrickPow2) != 0;
+            generalReciprocal(bB, rcpGB);
+        }
+        // the march is built for power-of-two bricks or for any brick
+        const bool p2 = p2A && p2B;
+        // waiting lanes start their B segment now
+        if (alive && !marching && W.travelB > 0.0f)
+        {
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                W.pos[a] = W.posB[a];
+                W.step[a] = W.stepB[a];
+            }
+            W.travel = W.travelB;
+            W.travelB = 0.0f;
+            W.seg = kB;
+            laneBrick(bB, rcpGB, voxB, W.L);
+            raw = p2 ? laneFetch<T, true>(U, W.L, W.pos) : laneFetch<T, false>(U, W.L, W.pos);
+        }
+        if (alive && W.travel > 0.0f)
+        {
+            if (p2)
+                marchWindow<T, true>(U, tf, C, W, raw, kA, kB, bB, rcpGB, voxB, nsamp);
+            else
+                marchWindow<T, false>(U, tf, C, W, raw, kA, kB, bB, rcpGB, voxB, nsamp);
+        }
+    }
+    if (inside)
+        storePixel(U, px, py, C);
 }
 
 // The per-brick record of a frame from its key (the device half of frame

@@ -280,8 +280,14 @@ __device__ inline int nextItem(unsigned long long* counter, int lane)
 // through the whole volume), so pulling them dynamically keeps every CU busy
 // to the end. Up to 8 wavefronts per SIMD fit in <= 64 VGPRs; fewer keep
 // fewer voxel lines in flight per CU, which the 32 KB L1 holds better.
+// The windowed march (castTileWindow) keeps two bricks' segments per lane and
+// needs about 80 VGPRs, at most 6 wavefronts per SIMD; the others fit 8.
+template <bool LINEAR, int D, bool CLS>
+constexpr int kRaycastWavesPerEu = (!LINEAR && !CLS && D == 2) ? 4 : 8;
+
 template <typename T, bool LINEAR, int D, bool CLS>
-__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void raycastKernel(
+__global__ __launch_bounds__(1024) __attribute__((
+    amdgpu_waves_per_eu(kRaycastWavesPerEu<LINEAR, D, CLS>))) void raycastKernel(
     const FrameUniforms U, const BrickEntry* __restrict__ bricks,
     const int32_t* __restrict__ offsets, const int32_t* __restrict__ counts,
     const int32_t* __restrict__ lists, unsigned long long* __restrict__ counters)
@@ -311,7 +317,11 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         if (item >= U.nItems)
             break;
         const int tile = itemToTile(U, item);
-        if (tile >= 0)
+        if (tile < 0)
+            continue;
+        if (!LINEAR && !CLS && D == 2)
+            castTileWindow<T>(U, tile, lane, bricks, offsets, counts, lists, table, nsamp);
+        else
             castTile<T, LINEAR, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, table,
                                         nsamp);
     }
@@ -648,6 +658,9 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     int32_t nSamples = 0;
     if (lvr_samples_per_ray(&ctx->vi, ids, n, params->n_samples_per_ray, &nSamples) != LVR_OK)
         return fail(ctx, LVR_EINVAL, "node id outside the octree");
+    // the march's termination bound (kMaxSegment) holds up to 2^20
+    if (nSamples > kMaxSamplesPerRay)
+        return fail(ctx, LVR_EINVAL, "more than 1048576 samples per ray");
     int tw = params->tile_w, th = params->tile_h;
     if (tw == 0 && th == 0)
     {

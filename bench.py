@@ -37,31 +37,31 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def profiled_l2_hit_rate():
-    """L2 (TCC) hit rate of the ray cast from the committed counter summary."""
-    path = os.path.join(HERE, "profiles", "raycast_2048u16_1080p_counters.txt")
-    try:
-        with open(path) as f:
-            for line in f:
-                if "L2 (TCC) hit rate" in line:
-                    return float(line.split("hit rate")[1].split(";")[0])
-    except (OSError, ValueError):
-        pass
-    return None
-
-
-def profiled_traffic(name="raycast_2048u16_1080p_hbm.txt"):
-    """HBM bytes per ray-cast launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (scaled by the copy-kernel calibration), or None."""
+def profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/,
+    or None when the summary or the line is absent."""
     path = os.path.join(HERE, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":")[1])
-    except OSError:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except (OSError, ValueError, IndexError):
         pass
     return None
+
+
+HBM_PROFILE = "raycast_2048u16_1080p_hbm.txt"
+COUNTER_PROFILE = "raycast_2048u16_1080p_counters.txt"
+
+
+def schedule(steps):
+    """Frame indices of the timed steps: the 120-frame orbit-zoom path
+    sampled evenly (stride 120 / steps), so any --steps covers the whole
+    path; above 120 steps the path repeats."""
+    if steps >= FRAMES:
+        return [i % FRAMES for i in range(steps)]
+    return [(i * FRAMES) // steps for i in range(steps)]
 
 
 def build_scene(livre_amd, scenes):
@@ -95,46 +95,54 @@ def frame(livre_amd, vi, ctx, fr, params):
 
 
 def cpu_baseline(livre_amd, scenes, vi, ctx, tf, target_s):
-    """The FP32 restatement (oracle) on the host cores, frame 0: selection,
-    order and the per-pixel march over a band of rows around the centre,
-    sized to about target_s seconds."""
+    """The FP32 restatement (oracle) on the host cores over frames 0, 30, 60
+    and 90 of the orbit path: per frame, selection, order and the per-pixel
+    march over a band of rows around the centre sized to about target_s / 4
+    seconds, plus the frame's histogram as a sum of cached per-brick ones."""
     import oracle
     ovi = oracle.volume_info([2048] * 3, [64] * 3, [1] * 3, oracle.DT_UINT16)
-    mv, proj, fr = camera(livre_amd, scenes, 0)
-    cam = oracle.camera(list(fr.inv_proj), list(fr.inv_mv), list(fr.eye),
-                        fr.near_plane, fr.far_plane, W, H)
-    t0 = time.perf_counter()
-    vis = oracle.select_visibles(ovi, mv, proj, H, 1.0, 0, ovi.depth - 1)
-    ordered = oracle.order(ovi, mv, vis)
-    t_sel = time.perf_counter() - t0
-    bricks = [ctx.download(i) for i in ordered]
     threads = os.cpu_count() or 1
-    rows, elapsed, samples = 8, 0.0, 0
-    while True:
-        y0 = H // 2 - rows // 2
-        p = oracle.render_params(N_SAMPLES, 0.999, (0.0, 65535.0),
-                                 window=(0, y0, W, y0 + rows))
+    total_samples, total_s, parts = 0, 0.0, []
+    for t in (0, 30, 60, 90):
+        mv, proj, fr = camera(livre_amd, scenes, t)
+        cam = oracle.camera(list(fr.inv_proj), list(fr.inv_mv), list(fr.eye),
+                            fr.near_plane, fr.far_plane, W, H)
         t0 = time.perf_counter()
-        _img, samples = oracle.render(ovi, cam, ordered, bricks, tf, p, "f32", threads)
-        elapsed = time.perf_counter() - t0
-        if elapsed >= target_s / 4 or rows >= H:
-            break
-        rows = min(H, int(rows * max(2.0, target_s / max(elapsed, 1e-3))))
-    frac = rows / H
-    # the frame's histogram with warm caches (HistogramFilter.cpp:78-149):
-    # each rendered brick's HistogramObject is binned once, when the brick
-    # is loaded, and a frame sums the cached ones; timed here as the sum of
-    # one 65536-bin histogram per rendered brick (32 distinct ones binned by
-    # the oracle, reused)
-    nh = min(len(ordered), 32)
-    per = [oracle.histogram(ovi, [ordered[i]], [bricks[i]], BINS, (0.0, 65535.0))
-           for i in range(nh)]
-    t0 = time.perf_counter()
-    acc = np.zeros(BINS, np.uint64)
-    for i in range(len(ordered)):
-        acc += per[i % max(nh, 1)]
-    t_hist = time.perf_counter() - t0
-    value = samples / (elapsed + (t_sel + t_hist) * frac) / 1e9
+        vis = oracle.select_visibles(ovi, mv, proj, H, 1.0, 0, ovi.depth - 1)
+        ordered = oracle.order(ovi, mv, vis)
+        t_sel = time.perf_counter() - t0
+        bricks = [ctx.download(i) for i in ordered]
+        rows, elapsed, samples = 8, 0.0, 0
+        while True:
+            y0 = H // 2 - rows // 2
+            p = oracle.render_params(N_SAMPLES, 0.999, (0.0, 65535.0),
+                                     window=(0, y0, W, y0 + rows))
+            t0 = time.perf_counter()
+            _img, samples = oracle.render(ovi, cam, ordered, bricks, tf, p, "f32", threads)
+            elapsed = time.perf_counter() - t0
+            if elapsed >= target_s / 16 or rows >= H:
+                break
+            rows = min(H, int(rows * max(2.0, target_s / 4 / max(elapsed, 1e-3))))
+        frac = rows / H
+        # the frame's histogram with warm caches (HistogramFilter.cpp:78-149):
+        # each rendered brick's HistogramObject is binned once, when the brick
+        # is loaded, and a frame sums the cached ones; timed here as the sum
+        # of one 65536-bin histogram per rendered brick (32 distinct ones
+        # binned by the oracle, reused)
+        nh = min(len(ordered), 32)
+        per = [oracle.histogram(ovi, [ordered[i]], [bricks[i]], BINS, (0.0, 65535.0))
+               for i in range(nh)]
+        t0 = time.perf_counter()
+        acc = np.zeros(BINS, np.uint64)
+        for i in range(len(ordered)):
+            acc += per[i % max(nh, 1)]
+        t_hist = time.perf_counter() - t0
+        del bricks
+        spent = elapsed + (t_sel + t_hist) * frac
+        total_samples += samples
+        total_s += spent
+        parts.append("frame %d rows %d..%d (%d samples, %.1f s)"
+                     % (t, H // 2 - rows // 2, H // 2 - rows // 2 + rows, samples, spent))
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -142,23 +150,71 @@ def cpu_baseline(livre_amd, scenes, vi, ctx, tf, target_s):
                           if ln.startswith("model name")), "")
     except OSError:
         pass
-    return {"value": value, "unit": "Gsamples/s", "cores": threads, "kind": "port",
-            "cpu_model": model,
-            "sample": "frame 0 of the orbit path: oracle selection + order + FP32 "
-                      "march of rows %d..%d (%d x %d px, %d samples, %.1f s) + the frame's "
-                      "65536-bin histogram as the sum of %d cached per-brick histograms "
-                      "(%.2f s); selection and histogram charged by the band's share of "
-                      "the frame"
-                      % (H // 2 - rows // 2, H // 2 - rows // 2 + rows, W, rows,
-                         samples, elapsed, len(ordered), t_hist)}
+    return {"value": total_samples / total_s / 1e9, "unit": "Gsamples/s", "cores": threads,
+            "kind": "port", "cpu_model": model,
+            "sample": "frames 0, 30, 60, 90 of the orbit path, a band of rows around the "
+                      "centre of each: oracle selection + order + FP32 march + the frame's "
+                      "65536-bin histogram as the sum of cached per-brick histograms, "
+                      "selection and histogram charged by the band's share of the frame; "
+                      + "; ".join(parts)}
+
+
+def batching_1080p(livre_amd, scenes, vi, ctx, tf, reps):
+    """16 concurrent 1080p sessions of the 2048^3 volume (cameras 7.5 frames
+    apart on the orbit path), rendered the plain way (lvr_render per session,
+    queued back to back, no wait per request) and as one lvr_batch launch,
+    both with the default launch shape. Selection and ordering are done
+    before either timed region."""
+    sess = []
+    for k in range(16):
+        fr = camera(livre_amd, scenes, (k * FRAMES) // 16)[2]
+        vis = ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1)
+        sess.append((fr, livre_amd.order(vi, fr, vis)))
+    params = livre_amd.render_params(N_SAMPLES)
+    samples = 0
+    for fr, ordered in sess:  # warm-up, and the samples of one pass
+        ctx.render(fr, W, H, ordered, params)
+        samples += ctx.samples()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for fr, ordered in sess:
+            ctx.render(fr, W, H, ordered, params)
+        ctx.sync()
+    t_plain = time.perf_counter() - t0
+    with livre_amd.Batch() as b:
+        def batched():
+            for fr, ordered in sess:
+                b.add(ctx, fr, W, H, ordered, params, tf, (0.0, 65535.0))
+            b.run()
+            return b.wait()
+        batched()  # warm-up and graph capture
+        t0 = time.perf_counter()
+        batch_samples = sum(batched().samples for _ in range(reps))
+        t_batch = time.perf_counter() - t0
+        fr, ordered = sess[5]
+        ctx.render(fr, W, H, ordered, params)
+        img, smp = ctx.read_frame()
+        bimg, bsmp = b.read(5)
+        same = smp == bsmp and bool(np.array_equal(img, bimg))
+    plain = samples * reps / t_plain / 1e9
+    batched_rate = batch_samples / t_batch / 1e9
+    log("16 sessions at 1080p: plain %.1f Gsamples/s, batched %.1f Gsamples/s (x%.2f)"
+        % (plain, batched_rate, batched_rate / plain))
+    return {"sessions": 16, "reps": reps, "plain_gsamples_per_s": plain,
+            "batched_gsamples_per_s": batched_rate, "ratio": batched_rate / plain,
+            "batched_equals_plain": same,
+            "plain": "lvr_render per session, queued back to back, one sync per pass",
+            "batched": "one lvr_batch launch per pass (graph replay)",
+            "shape": "default (8x8 tiles, 2 fetches in flight, 4 waves/SIMD) for both"}
 
 
 def batch_workload(args):
-    """64 uneven requests over four volumes: the plain renderer (one
-    lvr_render + wait per request, default launch shape) against one
-    persistent batched launch (autotuned launch shape, graph-replayed), then
-    16 concurrent 1080p sessions of the 2048^3 volume the same two ways.
-    Selection and ordering are done before either timed region."""
+    """64 uneven requests over four volumes: the plain renderer (lvr_render
+    per request, queued without a wait per request) against one persistent
+    batched launch (graph-replayed), both with the autotuned launch shape of
+    each request, then 16 concurrent 1080p sessions of the 2048^3 volume the
+    same two ways. Selection and ordering are done before either timed
+    region."""
     import livre_amd
     from livre_amd import autotune, scenes, workloads
     livre_amd.lib()
@@ -168,13 +224,25 @@ def batch_workload(args):
         log("%s: %d bricks generated in %.1f s" % (name, len(vols[name].ids),
                                                     vols[name].gen_s))
 
-    def plain_pass(items):
+    def plain_count(items, tuned):
         total = 0
-        for ctx, fr, w, h, ordered, params, tf, rng in items:
+        for (ctx, fr, w, h, ordered, _p, tf, rng), tp in zip(items, tuned):
             ctx.set_tf(tf, rng)
-            ctx.render(fr, w, h, ordered, params)
+            ctx.render(fr, w, h, ordered, tp)
             total += ctx.samples()  # waits for the frame
         return total
+
+    def plain_pass(items, tuned):
+        # queued: no wait per request; lvr_tf_set waits for the frames of its
+        # own context in flight, so a context's TF changes only when it must
+        last_tf = {}
+        for (ctx, fr, w, h, ordered, _p, tf, rng), tp in zip(items, tuned):
+            if last_tf.get(id(ctx)) is not tf:
+                ctx.set_tf(tf, rng)
+                last_tf[id(ctx)] = tf
+            ctx.render(fr, w, h, ordered, tp)
+        for ctx in {id(it[0]): it[0] for it in items}.values():
+            ctx.sync()
 
     def batch_pass(batch, items, tuned):
         for (ctx, fr, w, h, ordered, _p, tf, rng), tp in zip(items, tuned):
@@ -183,14 +251,15 @@ def batch_workload(args):
         return batch.wait()
 
     def compare(items, reps, label):
-        plain_pass(items)  # warm-up
-        t0 = time.perf_counter()
-        plain_samples = sum(plain_pass(items) for _ in range(reps))
-        t_plain = time.perf_counter() - t0
         tuner = autotune.Tuner()
         t0 = time.perf_counter()
         tuned = tuner.tune_batch([(c, f, w, h, o, p) for c, f, w, h, o, p, _t, _r in items])
         t_tune = time.perf_counter() - t0
+        plain_samples = plain_count(items, tuned) * reps  # also the warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plain_pass(items, tuned)
+        t_plain = time.perf_counter() - t0
         with livre_amd.Batch() as b0, livre_amd.Batch() as b1:
             pair = (b0, b1)
             for batch in pair:  # warm-up and graph capture of both
@@ -219,7 +288,7 @@ def batch_workload(args):
             for i in range(0, len(items), max(1, len(items) // 8)):
                 ctx, fr, w, h, ordered, params, tf, rng = items[i]
                 ctx.set_tf(tf, rng)
-                ctx.render(fr, w, h, ordered, params)
+                ctx.render(fr, w, h, ordered, tuned[i])
                 img, smp = ctx.read_frame()
                 bimg, bsmp = batch.read(i)
                 same = same and smp == bsmp and bool(np.array_equal(img, bimg))
@@ -273,8 +342,9 @@ def batch_workload(args):
                                "(48^3 bricks) / 2048^3 u16 noise, 256-entry TF each, rates "
                                "{0.5,1,2} samples/voxel, sse 1; plus 16 x 1080p sessions of "
                                "the 2048^3 volume (4096-entry TF)",
-                   "plain": "lvr_render + wait per request, default 8x8 tile, 2 samples "
-                            "per thread",
+                   "plain": "lvr_render per request, queued (no wait per request; a "
+                            "context's TF upload waits for its frames in flight), autotuned "
+                            "launch shape per request",
                    "batched": "one lvr_batch launch per pass (graph replay), two batches "
                               "in turn so the host assembles the next pass during the "
                               "current one, autotuned tiles and samples per thread"},
@@ -285,10 +355,14 @@ def batch_workload(args):
 
 
 def serve_workload(args):
+    print(json.dumps(serve_measure(args, args.rates, args.duration)), flush=True)
+
+
+def serve_measure(args, rates, duration):
     """32 concurrent sessions over the five volumes (720p..4K viewports,
     random-walk cameras, own TFs, 1 sample per voxel) served by the
-    continuous-batching loop; Poisson arrivals at each rate for a fixed
-    duration. Latency = completion - arrival of every rendered frame."""
+    continuous-batching loop; Poisson arrivals at each rate for `duration`
+    seconds. Latency = completion - arrival of every rendered frame."""
     import livre_amd
     from livre_amd import workloads
     livre_amd.lib()
@@ -334,8 +408,8 @@ def serve_workload(args):
         collect(srv.poll())
         log("warm-up: 32 frames in %.2f s, %d bricks generated"
             % (time.perf_counter() - t0, srv.stats().bricks_generated))
-        for rate in args.rates:
-            arrivals = workloads.poisson_arrivals(rate, args.duration, len(sessions), seed=rate)
+        for rate in rates:
+            arrivals = workloads.poisson_arrivals(rate, duration, len(sessions), seed=rate)
             before = srv.stats()
             done = []
             t0 = srv.now_ms()
@@ -408,14 +482,14 @@ def serve_workload(args):
         "config": {"workload": "32 sessions, viewports {720p,1080p,1440p,4K}, volumes "
                                "256^3 u8 / 512^3 u8 / 1024^3 u16 / 1536^3 f32 / 2048^3 u16, "
                                "random-walk cameras, 256-entry TFs, 1 sample/voxel, sse 1; "
-                               "Poisson arrivals, %.0f s per rate" % args.duration,
+                               "Poisson arrivals, %.0f s per rate" % duration,
                    "server": "continuous batching, 2 launches in flight, max %d requests "
                              "per launch, newest request per session wins" % args.max_batch,
                    "budget_bytes": budget, "pool_bytes": pools,
                    "pool_fraction_of_pyramid": args.pool_frac},
         "rates": result_rates, "served_equals_alone": same, "admitted_sessions": st.admitted,
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def ooc_workload(args):
@@ -815,10 +889,11 @@ def sortlast_workload(args):
         "vs_whole_frame_mean_diff": float(np.mean([x[1] for x in diffs])),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
-                     "traffic": profiled_traffic("composite_8layer_1080p_hbm.txt"),
+                     "traffic": profile_value("composite_8layer_1080p_hbm.txt",
+                                              "traffic_bytes_per_launch"),
                      "traffic_source": "profiles/composite_8layer_1080p_hbm.txt (rocprofv3 "
-                                       "FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a 1 GiB "
-                                       "copy)",
+                                       "FETCH_SIZE and WRITE_SIZE per dispatch, each scaled "
+                                       "by the same counter on a 1 GiB device copy)",
                      "kernel": "compositeKernel",
                      "bytes_per_launch": "8 x 1920x1080x16 + 1920x1080x4"},
         "cpu_baseline": None,
@@ -929,6 +1004,26 @@ def jpeg_workload(args):
     print(json.dumps(result), flush=True)
 
 
+def hbmcopy_workload(args):
+    """A plain device-to-device copy of 1 GiB, four times: the reference for
+    the HBM traffic counters (FETCH_SIZE / WRITE_SIZE of a dispatch that reads
+    and writes exactly 1 GiB) and the copy bandwidth the part reaches."""
+    import torch
+    n = 1 << 30
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    print(json.dumps({"metric": "device copy GB/s (read + write)", "value": 4 * 2 * n / el / 1e9,
+                      "unit": "GB/s", "n_gpus": 1, "bytes_per_copy": n, "copies": 4,
+                      "higher_is_better": True}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -936,8 +1031,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra-legs", action="store_true",
+                    help="render leg only: no 1080p batching ratio, no serving leg")
     ap.add_argument("--workload", choices=("render", "batch", "serve", "ooc", "raw", "jpeg",
-                                           "c1", "sortlast"),
+                                           "c1", "sortlast", "hbmcopy"),
                     default="render",
                     help="render: the 1080p single-volume frame loop (default); "
                          "batch: uneven requests, plain vs one batched launch; "
@@ -946,7 +1043,8 @@ def main():
                          "raw: a raw file's LOD pyramid built in HBM, then rendered; "
                          "jpeg: frame grab of rendered frames as JPEG; "
                          "c1: the CPU-runnable 256^3 configuration at 1024x1024; "
-                         "sortlast: 8 partial layers per frame and their composite")
+                         "sortlast: 8 partial layers per frame and their composite; "
+                         "hbmcopy: a 1 GiB device copy (HBM counter reference)")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
     ap.add_argument("--max-batch", type=int, default=2)
@@ -984,6 +1082,9 @@ def main():
     if args.workload == "sortlast":
         sortlast_workload(args)
         return
+    if args.workload == "hbmcopy":
+        hbmcopy_workload(args)
+        return
     if args.steps is None:
         args.steps = FRAMES
 
@@ -994,8 +1095,10 @@ def main():
     params = livre_amd.render_params(N_SAMPLES, early_exit=0.999)
     ctx.set_timing(True)
 
-    for t in range(args.warmup):
-        frame(livre_amd, vi, ctx, camera(livre_amd, scenes, t)[2], params)
+    frames = schedule(args.steps)
+    for i in range(args.warmup):
+        frame(livre_amd, vi, ctx, camera(livre_amd, scenes, frames[i % len(frames)])[2],
+              params)
     ctx.sync()
 
     # Frames are pipelined as RenderPipeline::renderAsync does: while frame t
@@ -1007,31 +1110,31 @@ def main():
     lat, samples, bricks, entries, rc_ms, bin_ms, hist_ms = [], 0, [], [], [], [], []
     host_ms = []
     alg_bytes = []
-    last = args.warmup + args.steps - 1
+    last = args.steps - 1
 
-    def select(t):
-        fr = camera(livre_amd, scenes, t)[2]
+    def select(i):
+        fr = camera(livre_amd, scenes, frames[i])[2]
         ctx.select_visibles_async(fr, H, 1.0, 0, vi.depth - 1)
         return time.perf_counter(), fr
 
-    def submit(pending, t):
+    def submit(pending, i):
         f0, fr = pending
         vis = ctx.select_visibles_wait()
-        nxt = select(t + 1) if t < last else None
+        nxt = select(i + 1) if i < last else None
         ordered = livre_amd.order(vi, fr, vis)
         ctx.render(fr, W, H, ordered, params)
         ctx.histogram(ordered, BINS, read=False)
         return f0, nxt
 
     t_start = time.perf_counter()
-    f0, pending = submit(select(args.warmup), args.warmup)
+    f0, pending = submit(select(0), 0)
     starts = [f0]
     for i in range(args.steps):
-        t = args.warmup + i
-        if t < last:
-            f0, pending = submit(pending, t + 1)
+        t = frames[i]
+        if i < last:
+            f0, pending = submit(pending, i + 1)
             starts.append(f0)
-            st = ctx.frame_stats(back=1)  # waits for frame t only
+            st = ctx.frame_stats(back=1)  # waits for step i only
         else:
             st = ctx.frame_stats()
         lat.append((time.perf_counter() - starts[i]) * 1e3)
@@ -1054,11 +1157,17 @@ def main():
     sync_lat = []
     for i in range(args.steps):
         f1 = time.perf_counter()
-        frame(livre_amd, vi, ctx, camera(livre_amd, scenes, args.warmup + i)[2], params)
+        frame(livre_amd, vi, ctx, camera(livre_amd, scenes, frames[i])[2], params)
         sync_lat.append((time.perf_counter() - f1) * 1e3)
 
     rc = np.array(rc_ms)
+    # algorithmic bytes of a launch over its duration, averaged over launches
     achieved = float(np.mean(np.array(alg_bytes) / (rc * 1e-3))) / 1e9
+    # HBM bytes per algorithmic byte, measured over the same sampled path
+    ratio = profile_value(HBM_PROFILE, "traffic_per_algorithmic_byte")
+    traffic = ratio * float(np.mean(alg_bytes)) if ratio is not None else None
+    valu_frac = profile_value(COUNTER_PROFILE, "valu_issue_fraction")
+    lanes = profile_value(COUNTER_PROFILE, "active_lanes_per_valu")
     # samples per second if every algorithmic byte moved at the HBM peak
     ceiling = samples / (float(np.sum(alg_bytes)) / HBM_PEAK) / 1e9
     result = {
@@ -1076,9 +1185,12 @@ def main():
         "data": "synthetic: procedural 5-octave value noise (seed 3), generated on the "
                 "device into the brick pool before timing",
         "config": {"workload": "2048^3 uint16, 64^3 bricks + 1 halo, depth 6, all 37449 "
-                               "bricks resident; 1920x1080, 120-frame orbit-zoom, 2048 "
-                               "samples/unit, FP32 4096-entry TF, sse 1, 65536-bin "
-                               "histogram",
+                               "bricks resident; 1920x1080, %d frames sampled evenly from "
+                               "the 120-frame orbit-zoom (frames %s), 2048 samples/unit, "
+                               "FP32 4096-entry TF, sse 1, 65536-bin histogram"
+                               % (args.steps, "0..119" if args.steps >= FRAMES else
+                                  ", ".join(map(str, frames[:3])) +
+                                  (", ..., %d" % frames[-1] if len(frames) > 3 else "")),
                    "frame": "select (device) + order + bin + raycast + histogram; the "
                             "next frame is ordered and queued while the current one "
                             "ray-casts, its selection queued one frame ahead"},
@@ -1095,18 +1207,27 @@ def main():
         "render_call_host_ms_mean": float(np.mean(host_ms)),
         "kernel_gsamples_per_s": samples / (rc.sum() * 1e-3) / 1e9,
         "bandwidth_ceiling_gsamples_per_s": ceiling,
-        "l2_hit_rate": profiled_l2_hit_rate(),
+        "l2_hit_rate": profile_value(COUNTER_PROFILE, "l2_hit_rate"),
         "bricks_mean": float(np.mean(bricks)),
         "tile_entries_mean": float(np.mean(entries)),
+        # priced against HBM (the path has no matrix work); the counters say
+        # what limits it instead: VALU issue, not bandwidth
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / (HBM_PEAK / 1e9),
-                     "traffic": profiled_traffic(),
-                     "traffic_source": "profiles/raycast_2048u16_1080p_hbm.txt (rocprofv3 "
-                                       "FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a 1 GiB "
-                                       "copy)",
-                     "kernel": "raycastKernel<unsigned short>",
+                     "traffic": traffic,
+                     "traffic_source": "mean algorithmic bytes of the timed launches x "
+                                       "HBM bytes per algorithmic byte from profiles/"
+                                       + HBM_PROFILE + " (rocprofv3 FETCH_SIZE and "
+                                       "WRITE_SIZE per dispatch over the same sampled path, "
+                                       "each scaled by the same counter on a 1 GiB copy)",
+                     "kernel": "raycastKernel<unsigned short, NEAREST, 2 fetches in flight>",
                      "bytes_per_launch": "rendered bricks x 574992 B + 1920x1080x4 + "
-                                         "4096x16"},
+                                         "4096x16",
+                     "limiter": "VALU issue: %s of the SIMDs' VALU issue cycles, %s of 64 "
+                                "lanes active (profiles/%s)"
+                                % ("%.2f" % valu_frac if valu_frac is not None else "?",
+                                   "%.1f" % lanes if lanes is not None else "?",
+                                   COUNTER_PROFILE)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
@@ -1115,7 +1236,19 @@ def main():
                                                   args.cpu_seconds)
         except Exception as e:  # reported, never silently replaced
             result["cpu_baseline"] = {"error": repr(e)}
+    if not args.no_extra_legs:
+        result["batching_1080p"] = batching_1080p(livre_amd, scenes, vi, ctx, tf, reps=3)
     ctx.close()
+    if not args.no_extra_legs:
+        # the metric's serving latency: 32 sessions at 200 and 400 requests/s
+        srv = serve_measure(args, (200, 400), 2.0)
+        result["serving"] = {
+            "config": srv["config"]["workload"] + "; " + srv["config"]["server"],
+            "served_equals_alone": srv["served_equals_alone"],
+            "rates": [{k: r[k] for k in ("rate_req_per_s", "requests", "rendered",
+                                         "superseded", "failed", "p50_ms", "p99_ms",
+                                         "max_ms", "fps", "gsamples_per_s")}
+                      for r in srv["rates"]]}
     print(json.dumps(result), flush=True)
 
 

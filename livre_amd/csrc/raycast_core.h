@@ -930,6 +930,22 @@ __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile,
         storePixel(U, px, py, C);
 }
 
+// The GL_NEAREST march with per-sample classification at a fetch depth D:
+// depth 2 through the two-brick window, 1 and 3 brick by brick.
+template <typename T, int D>
+__device__ __forceinline__ void castNearestTile(const FrameUniforms& U, int tile, int lane,
+                                                const BrickEntry* __restrict__ bricks,
+                                                const int32_t* __restrict__ offsets,
+                                                const int32_t* __restrict__ counts,
+                                                const int32_t* __restrict__ lists,
+                                                const float4* tf, uint32_t& nsamp)
+{
+    if (D == 2)
+        castTileWindow<T>(U, tile, lane, bricks, offsets, counts, lists, tf, nsamp);
+    else
+        castTile<T, false, D>(U, tile, lane, bricks, offsets, counts, lists, tf, nsamp);
+}
+
 // The per-brick record of a frame from its key (the device half of frame
 // preparation): the pool slot from the device page table (the reference looks
 // each brick's texture up by NodeId when it draws it, RayCastRenderer.cpp:

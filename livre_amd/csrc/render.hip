@@ -319,8 +319,8 @@ __global__ __launch_bounds__(1024) __attribute__((
         const int tile = itemToTile(U, item);
         if (tile < 0)
             continue;
-        if (!LINEAR && !CLS && D == 2)
-            castTileWindow<T>(U, tile, lane, bricks, offsets, counts, lists, table, nsamp);
+        if (!LINEAR && !CLS)
+            castNearestTile<T, D>(U, tile, lane, bricks, offsets, counts, lists, table, nsamp);
         else
             castTile<T, LINEAR, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, table,
                                         nsamp);
@@ -351,6 +351,7 @@ inline int batchDepth(int depth) { return depth >= 3 ? 3 : 2; }
 // fetches in flight (D8, D16, DF: the autotuner's choice for that type).
 // Composite steps are counted per frame. A workgroup holds 2 * wavesPerSimd
 // (4 to 16) wavefronts.
+// The batch kernel marches brick by brick (castTile) at every depth.
 template <int D8, int D16, int DF>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void batchKernel(
     const FrameUniforms* __restrict__ reqs, int nReq, int totalItems,

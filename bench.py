@@ -251,6 +251,8 @@ def batch_workload(args):
         return batch.wait()
 
     def compare(items, reps, label):
+        for ctx, _f, _w, _h, _o, _p, tf, rng in items:  # a class the table lacks is timed
+            ctx.set_tf(tf, rng)
         tuner = autotune.Tuner()
         t0 = time.perf_counter()
         tuned = tuner.tune_batch([(c, f, w, h, o, p) for c, f, w, h, o, p, _t, _r in items])

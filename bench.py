@@ -831,12 +831,14 @@ def c1_workload(args):
 
 
 def sortlast_workload(args):
-    """One GPU's part of sort-last rendering at the headline shape: each
-    1080p orbit-zoom frame of the 2048^3 volume is split into 8 consecutive
-    ranges of its ordered bricks (the 8 GPUs' shares), each ray-cast into an
-    RGBA32F layer, and the 8 layers composited front to back into RGBA8 (the
-    kernel every GPU runs on its strip after the exchange). The exchange
-    itself (an all-to-all over xGMI) is not run on one GPU."""
+    """One GPU's part of sort-last rendering at the headline shape: the
+    bricks of each 1080p orbit-zoom frame of the 2048^3 volume are split by
+    the static decomposition into the 8 octants of the block grid (the 8
+    GPUs' shares: a brick's owner never changes with the camera), each part
+    ray-cast into an RGBA32F layer, and the 8 layers composited in the
+    octants' front-to-back order for the eye into RGBA8 (the kernel every GPU
+    runs on its strip after the exchange). The exchange itself (an
+    all-to-all over xGMI) is not run on one GPU."""
     import livre_amd
     from livre_amd import scenes
     livre_amd.lib()
@@ -846,19 +848,24 @@ def sortlast_workload(args):
     n_layers = 8
     frames = args.steps or 30
     comp_ms, part_max, part_sum, whole_ms, diffs = [], [], [], [], []
-    layers_host = None
+    owner, sizes = {}, []
     for t in range(args.warmup + frames):
         fr = camera(livre_amd, scenes, t * FRAMES // frames)[2]
         vis = ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1)
-        ordered = livre_amd.order(vi, fr, vis)
+        ordered = np.asarray(livre_amd.order(vi, fr, vis))
         ctx.render(fr, W, H, ordered, params)
         whole_st = ctx.frame_stats()
         whole, _ = ctx.read_frame()
-        parts = np.array_split(np.asarray(ordered), n_layers)
+        part_of = livre_amd.sortlast_parts(vi, ordered, n_layers)
+        for nid, p in zip(ordered, part_of):
+            if owner.setdefault(int(nid), int(p)) != p:
+                raise RuntimeError("a brick changed owner")
+        order = livre_amd.sortlast_order(vi, list(fr.eye), n_layers)
         rc = []
-        for i, part in enumerate(parts):
-            ctx.render_partial(fr, W, H, part, params, i)
+        for i, p in enumerate(order):
+            ctx.render_partial(fr, W, H, ordered[part_of == p], params, i)
             rc.append(ctx.frame_stats().raycast_ms)
+        sizes.append([int((part_of == p).sum()) for p in range(n_layers)])
         ms = ctx.composite([ctx.layer_ptr(i) for i in range(n_layers)], W, H, timed=True,
                            reps=10)
         img, _ = ctx.read_frame()
@@ -879,9 +886,13 @@ def sortlast_workload(args):
         "warmup": args.warmup, "higher_is_better": True, "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: frames of the 2048^3 uint16 value-noise orbit-zoom",
-        "config": {"workload": "each frame's ordered bricks in 8 consecutive ranges, each "
-                               "ray-cast into an RGBA32F layer, the 8 layers composited "
-                               "front to back (1920x1080)"},
+        "config": {"workload": "each frame's ordered bricks split into the 8 octants of the "
+                               "block grid (static decomposition: owners fixed across the "
+                               "orbit), each part ray-cast into an RGBA32F layer, the 8 "
+                               "layers composited in the octants' front-to-back order for "
+                               "the eye (1920x1080)"},
+        "bricks_per_part_mean": np.mean(np.array(sizes), axis=0).round(1).tolist(),
+        "distinct_bricks_owned": len(owner),
         "composite_ms_mean": comp,
         "composite_timing": "HIP events around 10 back-to-back launches per frame",
         "partial_raycast_ms_max_mean": float(np.mean(part_max)),

@@ -365,18 +365,39 @@ int lvr_render_noalloc(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
 /* Sort-last decomposition (SelectVisibles::Impl::visitPost keeps the range
  * [r0 * N, r1 * N) of the visibles, SelectVisibles.cpp:112-131; Equalizer's
  * compositor blends the partial frames in depth order, Channel.cpp:464-603).
- * lvr_render_partial ray-casts ordered_ids, a contiguous range of the frame's
- * global front-to-back order, like lvr_render, into the context's layer
- * `layer` (0..15): w * h premultiplied RGBA32F accumulations, row 0 = bottom,
- * not converted to RGBA8. Sample positions are those of the whole frame
- * (global-grid snapping), so the layers of consecutive ranges, composited
- * front first, give the whole frame up to FP rounding and the early exit
- * each layer takes on its own. The RGBA8 framebuffer is left as it was; the
- * frame statistics (lvr_frame_stats_get, the samples of lvr_frame_read) are
- * the partial's. */
+ * lvr_render_partial ray-casts ordered_ids, a part of the frame's global
+ * front-to-back order kept in that order (a contiguous range of it, or the
+ * bricks of one part of lvr_sortlast_parts), like lvr_render, into the
+ * context's layer `layer` (0..15): w * h premultiplied RGBA32F accumulations,
+ * row 0 = bottom, not converted to RGBA8. Sample positions are those of the
+ * whole frame (global-grid snapping), so the layers of consecutive ranges,
+ * composited front first, or the layers of the parts, composited in
+ * lvr_sortlast_order, give the whole frame up to FP rounding and the early
+ * exit each layer takes on its own. The RGBA8 framebuffer is left as it was;
+ * the frame statistics (lvr_frame_stats_get, the samples of lvr_frame_read)
+ * are the partial's. */
 int lvr_render_partial(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],
                        const uint64_t* ordered_ids, int n, const lvr_render_params* params,
                        int layer);
+/* Static sort-last decomposition (SelectVisibles.cpp:112-131 built with
+ * LIVRE_STATIC_DECOMPOSITION: a node belongs to one channel whatever the
+ * camera, by its own position, NodeId.cpp:129-138). Here the parts are the
+ * halves of the octree's block grid split at its middle along x, then y,
+ * then z: n_parts = 1, 2, 4 or 8 (8: the level-1 octants). part_of[i] is
+ * the part of ids[i] (bit 2 x, bit 1 y, bit 0 z of the halves used, 1 = the
+ * high half), so a brick keeps its part, and a GPU its bricks, as the
+ * camera moves. A node that straddles a split (the root, or a level-0 node
+ * of a root grid with an odd block count along a split axis) gets -1: it is
+ * a frame's only node when it is visible with a power-of-two root grid.
+ * LVR_EINVAL for another n_parts or an id outside the octree. */
+int lvr_sortlast_parts(const lvr_volume_info* info, const uint64_t* ids, int n, int n_parts,
+                       int32_t* part_of);
+/* The front-to-back order of the parts for an eye position (world space):
+ * at each split the half holding the eye first, x before y before z (the
+ * traversal of the splits as a BSP tree), so every ray meets the layers in
+ * this order. order receives n_parts part indices. */
+int lvr_sortlast_order(const lvr_volume_info* info, const float eye[3], int n_parts,
+                       int32_t* order);
 /* Device pointer of a layer (w * h float4), e.g. for an exchange between
  * GPUs; null until the layer is first rendered. The pointer stays valid until
  * a partial render into the same layer at a larger viewport replaces the

@@ -38,7 +38,7 @@ EXPORTS = (
     "lvr_jpeg_encoder_create", "lvr_jpeg_encoder_destroy", "lvr_jpeg_encode",
     "lvr_jpeg_encode_host",
     "lvr_render_partial", "lvr_partial_device_ptr", "lvr_partial_read", "lvr_composite_layers",
-    "lvr_samples_per_ray", "lvr_render_noalloc",
+    "lvr_samples_per_ray", "lvr_render_noalloc", "lvr_sortlast_parts", "lvr_sortlast_order",
     "lvr_jpeg_max_size", "lvr_jpeg_error", "lvr_jpeg_get_stats", "lvr_frame_grab_jpeg",
     "lvr_frame_grab_stats",
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
@@ -250,6 +250,8 @@ def lib():
     L.lvr_order.argtypes = [P(VolumeInfo), P(Frustum), u64p, c.c_int, u64p]
     L.lvr_samples_per_ray.argtypes = [P(VolumeInfo), u64p, c.c_int, c.c_int32,
                                       P(c.c_int32)]
+    L.lvr_sortlast_parts.argtypes = [P(VolumeInfo), u64p, c.c_int, c.c_int, P(c.c_int32)]
+    L.lvr_sortlast_order.argtypes = [P(VolumeInfo), P(c.c_float), c.c_int, P(c.c_int32)]
     L.lvr_render.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
                              P(RenderParams)]
     L.lvr_render_noalloc.argtypes = [c.c_void_p, P(Frustum), P(c.c_int), u64p, c.c_int,
@@ -452,6 +454,27 @@ def samples_per_ray(vi, ids, requested=0):
                                  ctypes.byref(out)) != 0:
         raise LivreError("samples_per_ray: invalid arguments")
     return out.value
+
+
+def sortlast_parts(vi, ids, n_parts):
+    """Static sort-last part (0..n_parts-1, or -1 for a node straddling a
+    split) of every id: the halves of the block grid along x, y, z."""
+    ids, pids = _u64(ids)
+    out = np.zeros(len(ids), np.int32)
+    if lib().lvr_sortlast_parts(ctypes.byref(vi), pids, len(ids), n_parts,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))) != 0:
+        raise LivreError("sortlast_parts: invalid arguments")
+    return out
+
+
+def sortlast_order(vi, eye, n_parts):
+    """Front-to-back order of the static parts for an eye position."""
+    e = (ctypes.c_float * 3)(*[float(x) for x in eye])
+    out = np.zeros(n_parts, np.int32)
+    if lib().lvr_sortlast_order(ctypes.byref(vi), e, n_parts,
+                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))) != 0:
+        raise LivreError("sortlast_order: invalid arguments")
+    return out
 
 
 def render_params(n_samples_per_ray, early_exit=0.999, linear=False,

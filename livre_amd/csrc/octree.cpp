@@ -237,6 +237,87 @@ int lvr_order(const lvr_volume_info* info, const lvr_frustum* f,
     return LVR_OK;
 }
 
+static int splitAxes(int nParts)
+{
+    switch (nParts)
+    {
+    case 1: return 0;
+    case 2: return 1;
+    case 4: return 2;
+    case 8: return 3;
+    default: return -1;
+    }
+}
+
+int lvr_sortlast_parts(const lvr_volume_info* info, const uint64_t* ids, int n, int n_parts,
+                       int32_t* part_of)
+{
+    const int axes = splitAxes(n_parts);
+    if (!info || n < 0 || (n > 0 && (!ids || !part_of)) || axes < 0 || info->depth < 1)
+        return LVR_EINVAL;
+    for (int i = 0; i < n; ++i)
+    {
+        const lvr::NodeKey k = lvr::decodeNode(ids[i]);
+        if (k.level >= info->depth)
+            return LVR_EINVAL;
+        // blocks per axis at the node's level; the split is at half of them:
+        // low half if 2 (pos + 1) <= blocks, high half if 2 pos >= blocks
+        const uint32_t pos[3] = {k.x, k.y, k.z};
+        int32_t part = 0;
+        for (int a = 0; a < axes; ++a)
+        {
+            const uint64_t blocks = uint64_t(info->root_blocks[a]) << k.level;
+            const uint64_t p = pos[a];
+            int bit;
+            if (2 * (p + 1) <= blocks)
+                bit = 0;
+            else if (2 * p >= blocks)
+                bit = 1;
+            else
+            {
+                part = -1;
+                break;
+            }
+            part = (part << 1) | bit;
+        }
+        part_of[i] = part;
+    }
+    return LVR_OK;
+}
+
+int lvr_sortlast_order(const lvr_volume_info* info, const float eye[3], int n_parts,
+                       int32_t* order)
+{
+    const int axes = splitAxes(n_parts);
+    if (!info || !eye || !order || axes < 0 || info->depth < 1)
+        return LVR_EINVAL;
+    // the split planes in world space: the minimum corner of the level-1
+    // node at the middle block position (internalNodeToLODNode's box)
+    lvr::NodeKey mid;
+    mid.level = 1;
+    mid.x = info->root_blocks[0];
+    mid.y = info->root_blocks[1];
+    mid.z = info->root_blocks[2];
+    mid.t = 0;
+    float mn[3], mx[3];
+    lvr::nodeWorldBox(*info, mid, mn, mx);
+    int nearBit[3];
+    for (int a = 0; a < 3; ++a)
+        nearBit[a] = eye[a] < mn[a] ? 0 : 1;
+    // BSP traversal: near half before far half at every split, x outermost
+    for (int r = 0; r < n_parts; ++r)
+    {
+        int32_t part = 0;
+        for (int a = 0; a < axes; ++a)
+        {
+            const int far = (r >> (axes - 1 - a)) & 1;
+            part = (part << 1) | (far ? 1 - nearBit[a] : nearBit[a]);
+        }
+        order[r] = part;
+    }
+    return LVR_OK;
+}
+
 int lvr_samples_per_ray(const lvr_volume_info* info, const uint64_t* ids, int n,
                         int32_t requested, int32_t* out)
 {

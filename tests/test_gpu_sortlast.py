@@ -3,8 +3,11 @@ consecutive ranges (SelectVisibles.cpp:112-131), each ray-cast into an RGBA32F
 layer (lvr_render_partial), the layers composited front to back
 (lvr_composite_layers, Channel.cpp:464-603). Checked against the frame
 rendered whole, against the oracle's render of each range, and against the
-"over" of the oracle's layers. Tolerance: RGBA8 within 2/255 max and 0.5/255
-mean; one layer reproduces the whole frame exactly."""
+"over" of the oracle's layers. The static decomposition (lvr_sortlast_parts:
+the halves of the block grid, a brick's part fixed whatever the camera) is
+checked the same way for 2, 4 and 8 parts from cameras on every side, its
+layers composited in lvr_sortlast_order. Tolerance: RGBA8 within 2/255 max
+and 0.5/255 mean; one layer reproduces the whole frame exactly."""
 import numpy as np
 import pytest
 
@@ -64,6 +67,38 @@ def test_layers_composite_to_the_whole_frame(noise_u16, k):
         one, _ = ctx.read_frame()
         mx, mean = diff_stats(one, oracle.to_rgba8(ref_layers[i].reshape(-1, 4)).reshape(h, w, 4))
         assert mx <= 2 and mean <= 0.5, (k, i, mx, mean)
+
+
+@pytest.mark.parametrize("n_parts", [2, 4, 8])
+def test_static_parts_composite_to_the_whole_frame(noise_u16, n_parts):
+    s, ctx = noise_u16
+    w, h = 120, 90
+    tf = scenes.piecewise_tf(256, seed=9, max_alpha=0.4)
+    ctx.set_tf(tf, s.range)
+    params = livre_amd.render_params(64)
+    owner = {}
+    for eye in ((1.4, 0.7, 1.2), (-1.1, -0.6, 1.5), (0.2, 1.6, -0.9), (-0.9, 0.3, -1.7)):
+        mv, proj, fr = s.camera(eye, w, h)
+        ordered = np.asarray(s.visible_ordered(mv, proj, h))
+        ctx.render(fr, w, h, ordered, params)
+        whole, whole_smp = ctx.read_frame()
+        parts = livre_amd.sortlast_parts(s.vi, ordered, n_parts)
+        assert (parts >= 0).all()
+        for nid, p in zip(ordered, parts):  # a brick's part never changes
+            assert owner.setdefault(int(nid), int(p)) == p
+        order = livre_amd.sortlast_order(s.vi, list(fr.eye), n_parts)
+        smp = 0
+        for i, p in enumerate(order):
+            ctx.render_partial(fr, w, h, ordered[parts == p], params, i)
+            smp += ctx.read_frame()[1]
+        ctx.composite([ctx.layer_ptr(i) for i in range(n_parts)], w, h)
+        img, _ = ctx.read_frame()
+        mx, mean = diff_stats(img, whole)
+        print("static parts", n_parts, "eye", eye, "sizes",
+              [int((parts == p).sum()) for p in range(n_parts)], "max", mx, "mean", mean,
+              "samples", smp, whole_smp, flush=True)
+        assert mx <= 2 and mean <= 0.5, (n_parts, eye, mx, mean)
+        assert smp >= whole_smp
 
 
 def test_empty_range_and_errors(noise_u16):

@@ -1060,7 +1060,9 @@ def main():
                          "hbmcopy: a 1 GiB device copy (HBM counter reference)")
     ap.add_argument("--rates", type=int, nargs="+", default=[50, 100, 200, 400])
     ap.add_argument("--duration", type=float, default=3.0)
-    ap.add_argument("--max-batch", type=int, default=2)
+    ap.add_argument("--max-batch", type=int, default=1,
+                    help="requests per serving launch; one request per launch runs the "
+                         "single-frame kernel (two-brick window)")
     ap.add_argument("--pool-frac", type=float, default=1.0)
     ap.add_argument("--pool-gb", type=float, default=200.0)
     ap.add_argument("--frames", type=int, default=300)

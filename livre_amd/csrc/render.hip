@@ -1929,6 +1929,15 @@ int lvr_batch_run(lvr_batch* b)
         if ((rc = applyPageTable(c, b->stream)))
             return bfail(b, rc, c->error);
 
+    // A batch of one request with per-sample NEAREST classification at two
+    // fetches in flight runs the single-frame kernel, which marches through
+    // the two-brick window; its uniforms go to the kernel by value, so this
+    // launch sequence is not captured as a graph.
+    const FrameUniforms& U0 = b->reqs.front();
+    const bool single = nReq == 1 && !U0.linear && U0.depth == 2 &&
+                        !(U0.classify == 1 && (U0.dataType == LVR_DT_UINT8 ||
+                                               U0.dataType == LVR_DT_UINT16));
+
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         enqueueRecords(st, b->d_reqs, nReq, b->d_keys, int(b->bricks.size()), b->d_bricks);
         enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,
@@ -1938,6 +1947,27 @@ int lvr_batch_run(lvr_batch* b)
         // may use
         const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
         const int blocks = std::max(1, std::min((items + 2 * wps - 1) / (2 * wps), slots));
+        if (single)
+        {
+            const size_t lds = size_t(U0.nTf + 2) * sizeof(float4);
+            switch (U0.dataType)
+            {
+#define LVR_SINGLE(T)                                                                       \
+    launchRaycastAs<T, false, 2, false>(blocks, lds, st, U0, b->d_bricks, b->d_offsets,      \
+                                        b->d_counts, b->d_lists, b->d_counters)
+            case LVR_DT_UINT8: LVR_SINGLE(uint8_t); break;
+            case LVR_DT_UINT16: LVR_SINGLE(uint16_t); break;
+            case LVR_DT_UINT32: LVR_SINGLE(uint32_t); break;
+            case LVR_DT_INT8: LVR_SINGLE(int8_t); break;
+            case LVR_DT_INT16: LVR_SINGLE(int16_t); break;
+            case LVR_DT_INT32: LVR_SINGLE(int32_t); break;
+            default: LVR_SINGLE(float); break;
+            }
+#undef LVR_SINGLE
+            // the request's composite steps where the batch keeps them
+            return hipMemcpyAsync(b->d_counters + 2, b->d_counters, sizeof(unsigned long long),
+                                  hipMemcpyDeviceToDevice, st);
+        }
         const dim3 grid(blocks), block(2 * wps * kWave);
 #define LVR_BATCH(a, c, d)                                                                   \
     hipLaunchKernelGGL((batchKernel<a, c, d>), grid, block, 0, st, b->d_reqs, nReq, items,      \
@@ -1958,7 +1988,7 @@ int lvr_batch_run(lvr_batch* b)
     };
 
     LVR_BHIP(b, hipEventRecord(b->ev[0], b->stream));
-    if (b->useGraph)
+    if (b->useGraph && !single)
     {
         if (!b->exec || key != b->graphKey)
         {

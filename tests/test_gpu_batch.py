@@ -96,6 +96,33 @@ def test_batch_without_graph_and_oracle_tolerance(volumes, orc):
             assert smp == ref_smp
 
 
+@pytest.mark.parametrize("vol,classify", [(1, 0), (2, 0), (0, 2)])
+def test_single_request_batch(volumes, vol, classify):
+    """A batch of one request with per-sample classification runs the
+    single-frame kernel (no graph capture); run again (a replay of the same
+    request) it gives the same frame; both equal lvr_render's frame."""
+    s, ctx = volumes[vol]
+    w, h = 88, 64
+    mv, proj, fr = s.camera((1.2, -0.8, 1.5), w, h)
+    ordered = s.visible_ordered(mv, proj, h)
+    tf = scenes.piecewise_tf(256, seed=40 + vol, max_alpha=0.5)
+    params = livre_amd.render_params(96, classify=classify)
+    ctx.set_tf(tf, s.range)
+    ctx.render(fr, w, h, ordered, params)
+    ref, ref_smp = ctx.read_frame()
+    with livre_amd.Batch() as b:
+        b.add(ctx, fr, w, h, ordered, params, tf, s.range)
+        b.run()
+        st = b.wait()
+        img, smp = b.read(0)
+        assert st.graph_captures == 0 and st.samples == smp == ref_smp
+        assert np.array_equal(img, ref)
+        b.run()
+        st = b.wait()
+        again, smp2 = b.read(0)
+        assert st.graph_captures == 0 and smp2 == ref_smp and np.array_equal(again, ref)
+
+
 def test_batch_rejects_what_it_does_not_render(volumes):
     s, ctx = volumes[0]
     mv, proj, fr = s.camera((1.5, 0.2, 1.1), 32, 32)

@@ -599,6 +599,7 @@ int lvr_server_read(lvr_server* server, int session, uint8_t* out_rgba8,
                     size_t bytes, uint64_t* samples);
 int lvr_server_get_stats(lvr_server* server, lvr_server_stats* out);
 
+
 /* --------------------------------------------------------- out-of-core */
 /* DataSourcePlugin::getData as a callback (DataSourcePlugin.h:81): write the
  * brick of node_id, bytes = maximumBlockSize^3 * bytesPerVoxel, x fastest;
@@ -721,6 +722,39 @@ int lvr_jpeg_get_stats(const lvr_jpeg_encoder* enc, lvr_jpeg_stats* out);
  * the reference. Runs on the context's stream after the frame. */
 int lvr_frame_grab_jpeg(lvr_ctx* ctx, int quality, uint8_t* out, size_t cap, size_t* size);
 int lvr_frame_grab_stats(lvr_ctx* ctx, lvr_jpeg_stats* out);
+
+/* ---------------------------------------------------- session front end */
+/* The messages of a session as Livre's ZeroEQ communicator handles them
+ * (livre/eq/zeroeq/communicator.cpp); the transport (ZeroEQ publish /
+ * subscribe, the HTTP server) is not part of this library. */
+/* LookOut → modelview (onCamera, communicator.cpp:106-133): the client's
+ * view matrix (meters, the data's frame) to the renderer's (Livre space):
+ * the camera position -R^-1 t scaled by meter_to_data_unit, taken through
+ * data_to_livre (VolumeInformation::dataToLivreTransform; NULL = identity),
+ * then -R (that) as the new translation. Matrices are 16 floats,
+ * column-major. LVR_EINVAL for a singular rotation or a zero ratio. */
+int lvr_lookout_to_modelview(const float lookout[16], const float data_to_livre[16],
+                             float meter_to_data_unit, float out_modelview[16]);
+/* Modelview → LookOut (_createLookOut, communicator.cpp:257-283), the
+ * inverse conversion, for publishing the camera. */
+int lvr_modelview_to_lookout(const float modelview[16], const float data_to_livre[16],
+                             float meter_to_data_unit, float out_lookout[16]);
+/* TransferFunction1D message (the communicator subscribes to the render
+ * settings' transfer function, communicator.cpp:236): the session's TF and
+ * data range for the requests staged from now on. */
+int lvr_server_set_tf(lvr_server* server, int session, const float* tf_rgba, int n_tf,
+                      float range0, float range1);
+/* VolumeRendererParameters and ClipPlanes messages (communicator.cpp:
+ * 235-237): the session's render parameters (NEAREST, up to 6 clip planes)
+ * for the requests staged from now on. */
+int lvr_server_set_params(lvr_server* server, int session, const lvr_render_params* params);
+/* ImageJPEG request (communicator.cpp:197-200 → Config::renderJPEG,
+ * Config.cpp:217-240, and FrameGrabber): the session's last finished frame as
+ * a JPEG file, encoded on the device from the session's framebuffer (rows
+ * bottom-up, as lvr_frame_grab_jpeg; *size receives the length; LVR_ERANGE
+ * if cap is too small, out may be NULL to learn it). */
+int lvr_server_grab_jpeg(lvr_server* server, int session, lvr_jpeg_encoder* enc, int quality,
+                         uint8_t* out, size_t cap, size_t* size);
 
 #ifdef __cplusplus
 }

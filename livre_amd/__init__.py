@@ -50,7 +50,8 @@ EXPORTS = (
     "lvr_server_now_ms", "lvr_server_create", "lvr_server_destroy", "lvr_server_last_error",
     "lvr_server_set_coalesce", "lvr_server_add_volume", "lvr_server_open", "lvr_server_close",
     "lvr_server_submit", "lvr_server_step", "lvr_server_drain", "lvr_server_poll",
-    "lvr_server_read", "lvr_server_get_stats",
+    "lvr_server_read", "lvr_server_get_stats", "lvr_server_set_tf", "lvr_server_set_params",
+    "lvr_server_grab_jpeg", "lvr_lookout_to_modelview", "lvr_modelview_to_lookout",
     "lvr_noise_source", "lvr_stream_create", "lvr_stream_destroy", "lvr_stream_last_error",
     "lvr_stream_request", "lvr_stream_request_ex", "lvr_stream_pump", "lvr_stream_wait", "lvr_stream_rendering_set",
     "lvr_rendering_set", "lvr_stream_get_stats", "lvr_brick_pin",
@@ -299,6 +300,15 @@ def lib():
     L.lvr_server_poll.argtypes = [c.c_void_p, P(ServerFrame), c.c_int, P(c.c_int)]
     L.lvr_server_read.argtypes = [c.c_void_p, c.c_int, u8p, c.c_size_t, u64p]
     L.lvr_server_get_stats.argtypes = [c.c_void_p, P(ServerStats)]
+    L.lvr_server_set_tf.argtypes = [c.c_void_p, c.c_int, P(c.c_float), c.c_int, c.c_float,
+                                    c.c_float]
+    L.lvr_server_set_params.argtypes = [c.c_void_p, c.c_int, P(RenderParams)]
+    L.lvr_server_grab_jpeg.argtypes = [c.c_void_p, c.c_int, c.c_void_p, c.c_int,
+                                       P(c.c_uint8), c.c_size_t, P(c.c_size_t)]
+    L.lvr_lookout_to_modelview.argtypes = [P(c.c_float), P(c.c_float), c.c_float,
+                                           P(c.c_float)]
+    L.lvr_modelview_to_lookout.argtypes = [P(c.c_float), P(c.c_float), c.c_float,
+                                           P(c.c_float)]
     L.lvr_stream_create.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int, c.c_int,
                                     P(c.c_void_p)]
     L.lvr_stream_destroy.argtypes = [c.c_void_p]
@@ -1097,6 +1107,57 @@ class Server:
         st = ServerStats()
         self._check(lib().lvr_server_get_stats(self._h, ctypes.byref(st)), "stats")
         return st
+
+    def set_tf(self, session, tf, data_range):
+        """A session's transfer function message: requests staged from now
+        on use it."""
+        tf, ptf = _f32(tf)
+        self._check(lib().lvr_server_set_tf(self._h, session, ptf, tf.size // 4,
+                                            data_range[0], data_range[1]), "set_tf")
+
+    def set_params(self, session, params):
+        """A session's render parameters / clip planes message."""
+        self._check(lib().lvr_server_set_params(self._h, session, ctypes.byref(params)),
+                    "set_params")
+
+    def grab_jpeg(self, session, encoder, quality=100):
+        """ImageJPEG of the session's last finished frame, encoded on the
+        device from its framebuffer."""
+        w, h = self.sizes[session]
+        cap = lib().lvr_jpeg_max_size(w, h)
+        buf = np.empty(cap, np.uint8)
+        size = ctypes.c_size_t()
+        self._check(lib().lvr_server_grab_jpeg(
+            self._h, session, encoder._h, quality,
+            buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), cap, ctypes.byref(size)),
+            "grab_jpeg")
+        return buf[:size.value].tobytes()
+
+
+def lookout_to_modelview(lookout, data_to_livre=None, meter_to_data_unit=1.0):
+    """Livre's modelview (column-major 16 floats) from a client LookOut
+    matrix (the communicator's onCamera)."""
+    src, psrc = _f32(np.asarray(lookout, np.float32).reshape(16))
+    d, pd = (None, None) if data_to_livre is None else _f32(
+        np.asarray(data_to_livre, np.float32).reshape(16))
+    out = np.zeros(16, np.float32)
+    if lib().lvr_lookout_to_modelview(psrc, pd, meter_to_data_unit,
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+        raise LivreError("lookout_to_modelview: invalid matrix")
+    return out
+
+
+def modelview_to_lookout(modelview, data_to_livre=None, meter_to_data_unit=1.0):
+    """The client LookOut (column-major 16 floats) of a Livre modelview (the
+    communicator's _createLookOut)."""
+    src, psrc = _f32(np.asarray(modelview, np.float32).reshape(16))
+    d, pd = (None, None) if data_to_livre is None else _f32(
+        np.asarray(data_to_livre, np.float32).reshape(16))
+    out = np.zeros(16, np.float32)
+    if lib().lvr_modelview_to_lookout(psrc, pd, meter_to_data_unit,
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))) != 0:
+        raise LivreError("modelview_to_lookout: invalid matrix")
+    return out
 
 
 def rendering_set(visible, resident):

@@ -14,7 +14,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "liblivre_amd.so")
 SOURCES = ["octree.cpp", "pool.cpp", "noise.hip", "render.hip", "histogram.hip",
            "select.hip", "fields.hip", "autotune.cpp",
-           "server.cpp", "stream.cpp", "raw.hip", "jpeg.hip"]
+           "server.cpp", "stream.cpp", "raw.hip", "jpeg.hip", "session.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LIVRE_AMD_ARCH", "gfx950")
 

@@ -23,6 +23,7 @@
 // Every frame is rendered by the same kernels as a frame rendered alone, so
 // its pixels and sample count equal the standalone result.
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <deque>
@@ -666,6 +667,52 @@ int lvr_server_read(lvr_server* s, int session, uint8_t* out_rgba8, size_t bytes
     }
     if (samples)
         *samples = ss.lastSamples;
+    return LVR_OK;
+}
+
+int lvr_server_set_tf(lvr_server* s, int session, const float* tf_rgba, int n_tf,
+                      float range0, float range1)
+{
+    if (!s || session < 0 || size_t(session) >= s->sessions.size() || !tf_rgba || n_tf < 1 ||
+        n_tf > 8192 || !std::isfinite(range0) || !std::isfinite(range1))
+        return LVR_EINVAL;
+    Session& ss = s->sessions[size_t(session)];
+    if (ss.state == 0)
+        return sfail(s, LVR_EINVAL, "session closed");
+    ss.tf.assign(tf_rgba, tf_rgba + 4 * size_t(n_tf));
+    ss.r0 = range0;
+    ss.r1 = range1;
+    return LVR_OK;
+}
+
+int lvr_server_set_params(lvr_server* s, int session, const lvr_render_params* params)
+{
+    if (!s || session < 0 || size_t(session) >= s->sessions.size() || !params ||
+        params->linear_filter || params->n_clip_planes < 0 || params->n_clip_planes > 6 ||
+        params->n_samples_per_ray < 0)
+        return LVR_EINVAL;
+    Session& ss = s->sessions[size_t(session)];
+    if (ss.state == 0)
+        return sfail(s, LVR_EINVAL, "session closed");
+    ss.params = *params;
+    return LVR_OK;
+}
+
+int lvr_server_grab_jpeg(lvr_server* s, int session, lvr_jpeg_encoder* enc, int quality,
+                         uint8_t* out, size_t cap, size_t* size)
+{
+    if (!s || !enc || session < 0 || size_t(session) >= s->sessions.size() || !size)
+        return LVR_EINVAL;
+    const Session& ss = s->sessions[size_t(session)];
+    if (ss.state != 2 || !ss.hasFrame)
+        return sfail(s, LVR_ENOENT, "session has no finished frame");
+    // the finished frame is never the one a launch writes; its launch has
+    // been retired, so its pixels are complete
+    const uint32_t* px = ss.d_frames + size_t(ss.shownBuf) * ss.width * ss.height;
+    const int rc = lvr_jpeg_encode(enc, px, ss.width, ss.height, ss.width * 4, LVR_PIXEL_RGBA,
+                                   1, quality, out, cap, size);
+    if (rc)
+        return sfail(s, rc, "session frame grab");
     return LVR_OK;
 }
 

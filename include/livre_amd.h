@@ -325,7 +325,9 @@ typedef struct
                                     0 = automatic (table for 8-bit data);
                                     identical images either way */
     int32_t waves_per_simd;      /* wavefronts per SIMD the ray cast keeps
-                                    resident: 2 to 8; 0 = 4. More hide
+                                    resident: 2 to 8; 0 = 5 for NEAREST
+                                    with 2 samples per thread and per-sample
+                                    classification, 4 otherwise. More hide
                                     more latency, fewer thrash the L1 less */
 } lvr_render_params;
 
@@ -456,7 +458,7 @@ int lvr_frame_stats_back(lvr_ctx* ctx, int back, lvr_frame_stats* out);
 
 /* Autotuner of the launch shape (no reference counterpart: the GL path has
  * no tunable launch). Renders the given frame with every ray tile in
- * {8x8, 16x4, 4x16, 32x2, 2x32}, every waves_per_simd in {4, 6, 8}, for
+ * {8x8, 16x4, 4x16, 32x2, 2x32}, every waves_per_simd in {4, 5, 6, 8}, for
  * NEAREST every samples_per_thread in {1, 2, 3}, and for 8/16-bit data both
  * classify modes; times the ray-cast kernel with HIP events (median of reps)
  * and writes the fastest tile_w, tile_h, samples_per_thread, classify and

@@ -15,7 +15,7 @@
 namespace
 {
 const int kTiles[][2] = {{8, 8}, {16, 4}, {4, 16}, {32, 2}, {2, 32}};
-const int kWavesPerSimd[] = {4, 6, 8};
+const int kWavesPerSimd[] = {4, 5, 6, 8};
 } // namespace
 
 extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4],

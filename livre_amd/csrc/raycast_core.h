@@ -734,9 +734,37 @@ __device__ __forceinline__ void laneBrick(const BrickEntry& b, const float rcpG[
     L.vox = vox;
 }
 
-template <typename T, bool P2>
+// The byte offset of a voxel in a slot is a sum of one term per axis
+// (tiledOffsetDev); with TAB the three terms come from per-axis tables in LDS
+// (offsetTables) instead of being computed per sample.
+template <typename T, bool TAB>
+__device__ __forceinline__ uint32_t slotOffset(const FrameUniforms& U, const uint32_t* tab,
+                                               uint32_t ix, uint32_t iy, uint32_t iz)
+{
+    if (TAB)
+        return tab[ix] + tab[uint32_t(U.texDim[0]) + iy] +
+               tab[uint32_t(U.texDim[0] + U.texDim[1]) + iz];
+    return tiledOffsetDev<T>(U, ix, iy, iz);
+}
+
+// The per-axis terms of tiledOffsetDev for every texel index of the brick:
+// texDim[0] + texDim[1] + texDim[2] words, filled by the workgroup.
+template <typename T>
+__device__ inline void offsetTables(const FrameUniforms& U, uint32_t* tab)
+{
+    constexpr uint32_t lb = sizeof(T) == 1 ? 0u : (sizeof(T) == 2 ? 1u : 2u);
+    const int n0 = U.texDim[0], n1 = U.texDim[1], n2 = U.texDim[2];
+    for (int i = int(threadIdx.x); i < n0 + n1 + n2; i += int(blockDim.x))
+    {
+        const int a = i < n0 ? 0 : (i < n0 + n1 ? 1 : 2);
+        const uint32_t x = uint32_t(i - (a == 0 ? 0 : (a == 1 ? n0 : n0 + n1)));
+        tab[i] = __umul24(x >> 2, U.tileK[a]) + (x << (lb + 2u * uint32_t(a)));
+    }
+}
+
+template <typename T, bool P2, bool TAB>
 __device__ __forceinline__ T laneFetch(const FrameUniforms& U, const LaneBrick& L,
-                                       const float pos[3])
+                                       const uint32_t* tab, const float pos[3])
 {
     float tp[3];
     texturePositionOf<P2>(U, L.bmin, P2 ? L.rcpP : L.rcpG, L.ext, U.texExt, pos, tp);
@@ -745,7 +773,7 @@ __device__ __forceinline__ T laneFetch(const FrameUniforms& U, const LaneBrick& 
     const uint32_t iz = nearestTexel(tp[2] * U.texDimF[2], U.texLastF[2]);
     // the texel indices are clamped into the brick, so the offset stays
     // inside the slot
-    return *reinterpret_cast<const T*>(L.vox + tiledOffsetDev<T>(U, ix, iy, iz));
+    return *reinterpret_cast<const T*>(L.vox + slotOffset<T, TAB>(U, tab, ix, iy, iz));
 }
 
 // A lane of castTileWindow. The wavefront holds two bricks of the tile's
@@ -770,11 +798,12 @@ struct WindowLane
 // The march of the lanes that are marching, until none marches A:
 // GL_NEAREST fetches with one in flight per lane (the voxel of the next
 // sample is fetched before the current one is classified and blended).
-template <typename T, bool P2>
+template <typename T, bool P2, bool TAB>
 __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4* tf,
-                                            Composite& C, WindowLane& W, T& raw, int kA,
-                                            int kB, const BrickEntry& bB, const float rcpGB[3],
-                                            const uint8_t* voxB, uint32_t& nsamp)
+                                            const uint32_t* tab, Composite& C, WindowLane& W,
+                                            T& raw, int kA, int kB, const BrickEntry& bB,
+                                            const float rcpGB[3], const uint8_t* voxB,
+                                            uint32_t& nsamp)
 {
     const float stepSize = U.stepSize;
     for (;;)
@@ -805,7 +834,7 @@ __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4
             laneBrick(bB, rcpGB, voxB, W.L);
             cont = true;
         }
-        const T next = laneFetch<T, P2>(U, W.L, W.pos);
+        const T next = laneFetch<T, P2, TAB>(U, W.L, tab, W.pos);
         __builtin_amdgcn_sched_barrier(0);
         raw = next;
         ++nsamp;
@@ -818,13 +847,14 @@ __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4
 // castTile with the two-brick window (WindowLane) for the GL_NEAREST march
 // with per-sample classification: identical samples and frames, fewer idle
 // lanes per wavefront step.
-template <typename T>
+template <typename T, bool TAB>
 __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile, int lane,
                                                const BrickEntry* __restrict__ bricks,
                                                const int32_t* __restrict__ offsets,
                                                const int32_t* __restrict__ counts,
                                                const int32_t* __restrict__ lists,
-                                               const float4* tf, uint32_t& nsamp)
+                                               const float4* tf, const uint32_t* tab,
+                                               uint32_t& nsamp)
 {
     const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
     const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
@@ -916,14 +946,17 @@ __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile,
             W.travelB = 0.0f;
             W.seg = kB;
             laneBrick(bB, rcpGB, voxB, W.L);
-            raw = p2 ? laneFetch<T, true>(U, W.L, W.pos) : laneFetch<T, false>(U, W.L, W.pos);
+            raw = p2 ? laneFetch<T, true, TAB>(U, W.L, tab, W.pos)
+                     : laneFetch<T, false, TAB>(U, W.L, tab, W.pos);
         }
         if (alive && W.travel > 0.0f)
         {
             if (p2)
-                marchWindow<T, true>(U, tf, C, W, raw, kA, kB, bB, rcpGB, voxB, nsamp);
+                marchWindow<T, true, TAB>(U, tf, tab, C, W, raw, kA, kB, bB, rcpGB, voxB,
+                                          nsamp);
             else
-                marchWindow<T, false>(U, tf, C, W, raw, kA, kB, bB, rcpGB, voxB, nsamp);
+                marchWindow<T, false, TAB>(U, tf, tab, C, W, raw, kA, kB, bB, rcpGB, voxB,
+                                           nsamp);
         }
     }
     if (inside)
@@ -932,16 +965,17 @@ __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile,
 
 // The GL_NEAREST march with per-sample classification at a fetch depth D:
 // depth 2 through the two-brick window, 1 and 3 brick by brick.
-template <typename T, int D>
+template <typename T, int D, bool TAB = false>
 __device__ __forceinline__ void castNearestTile(const FrameUniforms& U, int tile, int lane,
                                                 const BrickEntry* __restrict__ bricks,
                                                 const int32_t* __restrict__ offsets,
                                                 const int32_t* __restrict__ counts,
                                                 const int32_t* __restrict__ lists,
-                                                const float4* tf, uint32_t& nsamp)
+                                                const float4* tf, uint32_t& nsamp,
+                                                const uint32_t* tab = nullptr)
 {
     if (D == 2)
-        castTileWindow<T>(U, tile, lane, bricks, offsets, counts, lists, tf, nsamp);
+        castTileWindow<T, TAB>(U, tile, lane, bricks, offsets, counts, lists, tf, tab, nsamp);
     else
         castTile<T, false, D>(U, tile, lane, bricks, offsets, counts, lists, tf, nsamp);
 }

@@ -296,10 +296,16 @@ __global__ __launch_bounds__(1024) __attribute__((
     // (16-bit tables stay in global memory)
     extern __shared__ float4 sTf[];
     const float4* table = sTf;
+    // the windowed march reads the slot offsets of its texels from per-axis
+    // tables placed after the transfer function (rayLds)
+    constexpr bool kTab = !LINEAR && !CLS && D == 2;
+    uint32_t* offTab = reinterpret_cast<uint32_t*>(sTf + (U.nTf + 2));
     if (!CLS)
     {
         for (int i = threadIdx.x; i < U.nTf + 2; i += blockDim.x)
             sTf[i] = U.tf[i];
+        if (kTab)
+            offsetTables<T>(U, offTab);
     }
     else if (sizeof(T) == 1)
     {
@@ -320,7 +326,8 @@ __global__ __launch_bounds__(1024) __attribute__((
         if (tile < 0)
             continue;
         if (!LINEAR && !CLS)
-            castNearestTile<T, D>(U, tile, lane, bricks, offsets, counts, lists, table, nsamp);
+            castNearestTile<T, D, kTab>(U, tile, lane, bricks, offsets, counts, lists, table,
+                                        nsamp, offTab);
         else
             castTile<T, LINEAR, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, table,
                                         nsamp);
@@ -328,6 +335,15 @@ __global__ __launch_bounds__(1024) __attribute__((
     const unsigned long long s = waveSum(nsamp);
     if (lane == 0 && s)
         atomicAdd(&counters[0], s);
+}
+
+// Dynamic LDS of a single-frame ray cast without a classified table: the
+// transfer function with its edge copies, then the per-axis slot-offset
+// tables of the windowed march.
+inline size_t rayLds(const FrameUniforms& U)
+{
+    return size_t(U.nTf + 2) * sizeof(float4) +
+           sizeof(uint32_t) * size_t(U.texDim[0] + U.texDim[1] + U.texDim[2]);
 }
 
 // LDS entries per wavefront for a batched request's transfer function: a
@@ -675,8 +691,8 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
         return fail(ctx, LVR_EINVAL, "samples per thread must be 1, 2 or 3");
     if (params->classify < 0 || params->classify > 2)
         return fail(ctx, LVR_EINVAL, "classify must be 0, 1 or 2");
-    const int wps = params->waves_per_simd == 0 ? 4 : params->waves_per_simd;
-    if (wps < 2 || wps > 8)
+    const int wps = params->waves_per_simd;
+    if (wps != 0 && (wps < 2 || wps > 8))
         return fail(ctx, LVR_EINVAL, "waves per SIMD must be 2 to 8");
 
     std::memset(U, 0, sizeof *U);
@@ -729,11 +745,18 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     U->nBricks = n;
     U->linear = params->linear_filter ? 1 : 0;
     U->depth = depth;
-    U->wavesPerSimd = wps;
     // automatic: a 256-entry table sits in LDS beside the march; a 65536-entry
     // one is a dependent global gather per sample, slower than computing
     U->classify = params->classify != 0 ? params->classify
                                         : (ctx->vi.data_type == LVR_DT_UINT8 ? 1 : 2);
+    // the two-brick window (NEAREST, two fetches per ray, classification
+    // computed per sample) holds 80 VGPRs, room for six wavefronts per SIMD;
+    // five measured fastest on the 2048^3 uint16 orbit. The other variants
+    // are built for 64 VGPRs and run best at four.
+    const bool window = !U->linear && depth == 2 &&
+                        !(U->classify == 1 && (ctx->vi.data_type == LVR_DT_UINT8 ||
+                                               ctx->vi.data_type == LVR_DT_UINT16));
+    U->wavesPerSimd = wps != 0 ? wps : (window ? 5 : 4);
     U->nClip = params->n_clip_planes;
     std::memcpy(U->clip, params->clip_planes, sizeof U->clip);
     U->slotStride = ctx->slotStride;
@@ -917,6 +940,7 @@ struct lvr_batch
     lvr_ctx* errCtx = nullptr; // error sink for the shared helpers
 
     std::vector<lvr::FrameUniforms> reqs;
+    bool wpsDefault = false; // the first request left waves_per_simd to the library
     std::vector<lvr::BrickKey> bricks;
     std::vector<int4> ranges;
     std::vector<float4> tf;
@@ -1225,7 +1249,7 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     const int perBlock = U.wavesPerSimd * 2;
     const int blocks =
         std::max(1, std::min((U.nItems + perBlock - 1) / perBlock, 2 * ctx->numCUs));
-    size_t lds = size_t(U.nTf + 2) * sizeof(float4);
+    size_t lds = rayLds(U);
     const int dt = ctx->vi.data_type;
     if (!U.linear && U.classify == 1 && (dt == LVR_DT_UINT8 || dt == LVR_DT_UINT16))
     {
@@ -1393,7 +1417,7 @@ int lvr_render_noalloc(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     const int perBlock = U.wavesPerSimd * 2;
     const int blocks =
         std::max(1, std::min((U.nItems + perBlock - 1) / perBlock, 2 * ctx->numCUs));
-    size_t lds = size_t(U.nTf + 2) * sizeof(float4);
+    size_t lds = rayLds(U);
     if (table)
     {
         const int nValues = dt == LVR_DT_UINT8 ? 256 : 65536;
@@ -1770,6 +1794,11 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
             return bfail(b, LVR_EINVAL,
                          "samples per thread differs from the batch's for this voxel type");
         }
+    // the batch kernel runs four wavefronts per SIMD unless told otherwise,
+    // whatever single-frame variant a request would take alone
+    const bool wpsDefault = params->waves_per_simd == 0;
+    if (wpsDefault)
+        U.wavesPerSimd = 4;
     if (!b->reqs.empty() && U.wavesPerSimd != b->reqs.front().wavesPerSimd)
     {
         b->bricks.resize(nb);
@@ -1794,6 +1823,8 @@ int lvr_batch_add(lvr_batch* b, lvr_ctx* ctx, const lvr_frustum* f, const int vi
     b->listTotal += listTotal;
     b->superTotal += superTotal;
     b->onScreen += onScreen;
+    if (b->reqs.empty())
+        b->wpsDefault = wpsDefault;
     b->reqs.push_back(U);
     if (request)
         *request = U.request;
@@ -1937,6 +1968,11 @@ int lvr_batch_run(lvr_batch* b)
     const bool single = nReq == 1 && !U0.linear && U0.depth == 2 &&
                         !(U0.classify == 1 && (U0.dataType == LVR_DT_UINT8 ||
                                                U0.dataType == LVR_DT_UINT16));
+    // with the single-frame kernel's own default occupancy when the request
+    // left it to the library
+    FrameUniforms U1 = U0;
+    if (single && b->wpsDefault)
+        U1.wavesPerSimd = 5;
 
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         enqueueRecords(st, b->d_reqs, nReq, b->d_keys, int(b->bricks.size()), b->d_bricks);
@@ -1949,11 +1985,13 @@ int lvr_batch_run(lvr_batch* b)
         const int blocks = std::max(1, std::min((items + 2 * wps - 1) / (2 * wps), slots));
         if (single)
         {
-            const size_t lds = size_t(U0.nTf + 2) * sizeof(float4);
+            const int sblocks = std::max(
+                1, std::min((items + 2 * U1.wavesPerSimd - 1) / (2 * U1.wavesPerSimd), slots));
+            const size_t lds = rayLds(U1);
             switch (U0.dataType)
             {
 #define LVR_SINGLE(T)                                                                       \
-    launchRaycastAs<T, false, 2, false>(blocks, lds, st, U0, b->d_bricks, b->d_offsets,      \
+    launchRaycastAs<T, false, 2, false>(sblocks, lds, st, U1, b->d_bricks, b->d_offsets,     \
                                         b->d_counts, b->d_lists, b->d_counters)
             case LVR_DT_UINT8: LVR_SINGLE(uint8_t); break;
             case LVR_DT_UINT16: LVR_SINGLE(uint16_t); break;

@@ -89,10 +89,10 @@ def test_autotune_candidates_render_the_same_frame(lvr):
         ctx.render(fr, 96, 80, ordered, base)
         ref, ref_smp = ctx.read_frame()
         tuned, cands = ctx.autotune(fr, 96, 80, ordered, base, reps=2)
-        assert len(cands) == 90
+        assert len(cands) == 120
         assert {(c[0], c[1]) for c in cands} == {(8, 8), (16, 4), (4, 16), (32, 2), (2, 32)}
         assert {c[2] for c in cands} == {1, 2, 3} and {c[3] for c in cands} == {1, 2}
-        assert {c[4] for c in cands} == {4, 6, 8}
+        assert {c[4] for c in cands} == {4, 5, 6, 8}
         best = min(cands, key=lambda c: c[5])
         assert (tuned.tile_w, tuned.tile_h, tuned.samples_per_thread, tuned.classify,
                 tuned.waves_per_simd) == best[:5]

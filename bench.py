@@ -495,25 +495,25 @@ def serve_measure(args, rates, duration):
 
 
 def ooc_workload(args):
-    """8192^3 uint8 (512 GB logical) from the host value-noise DataSource,
-    64^3 bricks, depth 8, pool capped at --pool-gb; bricks stream through
+    """8192^3 uint8 (512 GB logical) from the host blobs-plus-value-noise
+    DataSource, 64^3 bricks, depth 8, pool capped at --pool-gb; bricks stream through
     pinned memory on the upload stream while frames render the coarser-LOD
     rendering set; 3840x2160, 300-frame fly-through."""
     import livre_amd
     from livre_amd import scenes, workloads
     livre_amd.lib()
     W4, H4 = 3840, 2160
-    vi = livre_amd.volume_info(8192, 64, 1, livre_amd.DT_UINT8)
+    vi = workloads.c4_volume()
     slot = livre_amd.slot_bytes(vi)
     n_slots = int(args.pool_gb * 1e9) // slot
     ctx = livre_amd.Context(vi, slot * n_slots)
-    noise = livre_amd.noise_params(8, octaves=5, base_cells=8, gain=1.5)
+    source = workloads.c4_source(vi)
     tf = scenes.piecewise_tf(256, seed=8, points=6, max_alpha=0.3)
     ctx.set_tf(tf, (0.0, 255.0))
     ctx.set_timing(True)
     threads = max(1, min(14, (os.cpu_count() or 2) - 2))
     rows = []
-    with livre_amd.Stream(ctx, noise, threads=threads, staging_bricks=256) as st:
+    with livre_amd.Stream(ctx, source, threads=threads, staging_bricks=256) as st:
         t0 = time.perf_counter()
         # levels 0..3 (585 bricks) before the first frame: every region has a
         # fallback from then on
@@ -604,7 +604,8 @@ def ooc_workload(args):
         "metric": "out-of-core fly-through: fps, Gsamples/s, streamed GB/s, full-LOD fraction",
         "value": float(a[:, 2].sum() / total / 1e9), "unit": "Gsamples/s", "n_gpus": 1,
         "higher_is_better": True, "vs_baseline": None, "dtype": "u8",
-        "data": "synthetic: 8192^3 u8 value noise (seed 8) from the host DataSource",
+        "data": "synthetic: 8192^3 u8 blobs plus value noise (64 blobs, seed 8) from the "
+                "host DataSource",
         "config": {"workload": "8192^3 uint8 (512 GB logical), 64^3 bricks + 1 halo, depth 8, "
                                "pool %.0f GB (%d slots) with LRU, 3840x2160, %d-frame "
                                "fly-through, samples/unit from the finest rendered level "

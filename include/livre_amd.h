@@ -187,6 +187,34 @@ int lvr_field_brick_host(const lvr_volume_info* info,
 int lvr_field_bricks_device(lvr_ctx* ctx, const lvr_field_params* p,
                             const uint64_t* node_ids, int n);
 
+/* Blobs plus value noise, the out-of-core configuration's source ("generated
+ * per brick from a seed, blobs plus value noise"). A voxel is the integer
+ * blend (blob * w + noise * (256 - w) + 128) >> 8 of two 16-bit values at
+ * its centre, then converted to the volume's type as the noise is:
+ *   - noise: the value-noise source above with `noise`;
+ *   - blob: a smooth field of `blob_count` Gaussian blobs (LVR_FIELD_BLOBS
+ *     with seed `blob_seed` and no uniform noise, scaled to 0..65535),
+ *     tabulated at create time on a lattice of blob_cells + 1 nodes per axis
+ *     (node i at i / blob_cells of the largest axis) and interpolated
+ *     trilinearly in 16.16 fixed point, like the noise octaves.
+ * Host and device bricks are bit-identical. blob_count 1..4096, blob_cells
+ * 1..1024, blob_weight 0..256 (0: the noise source exactly). The handle is
+ * read-only after create and may be shared by source threads. No reference
+ * counterpart (the reference's MemoryDataSource fills bricks with one
+ * constant, MemoryDataSource.cpp:38-61). */
+typedef struct lvr_blobnoise lvr_blobnoise;
+int lvr_blobnoise_create(const lvr_volume_info* info,
+                         const lvr_noise_params* noise, uint64_t blob_seed,
+                         int blob_count, int blob_cells, int blob_weight,
+                         lvr_blobnoise** out);
+void lvr_blobnoise_destroy(lvr_blobnoise* bn);
+int lvr_blobnoise_brick_host(const lvr_blobnoise* bn, uint64_t node_id,
+                             void* out, size_t bytes);
+/* Device generation into the context's pool; the context's volume must be
+ * the one the handle was made for. */
+int lvr_blobnoise_bricks_device(lvr_ctx* ctx, const lvr_blobnoise* bn,
+                                const uint64_t* node_ids, int n);
+
 /* ------------------------------------------------------- raw / NRRD data */
 /* RawDataSource (livre/data/RawDataSource.cpp:62-219): a memory-mapped raw
  * (x fastest) or NRRD volume. uri:
@@ -615,6 +643,8 @@ typedef struct
     lvr_noise_params params;
 } lvr_noise_source_data;
 int lvr_noise_source(void* user, uint64_t node_id, void* out, size_t bytes);
+/* The blobs-plus-noise source as such a callback; user = lvr_blobnoise*. */
+int lvr_blobnoise_source(void* user, uint64_t node_id, void* out, size_t bytes);
 
 /* Asynchronous brick streaming into a context's pool (the reference's async
  * RenderPipeline + DataUploadFilter, RenderPipeline.cpp:160-225,

@@ -28,7 +28,9 @@ EXPORTS = (
     "lvr_brick_upload_many", "lvr_brick_evict", "lvr_brick_resident",
     "lvr_brick_download", "lvr_noise_params_init", "lvr_noise_brick_host",
     "lvr_noise_bricks_device", "lvr_field_brick_host",
-    "lvr_field_bricks_device", "lvr_tf_set", "lvr_tf_set_rgba8",
+    "lvr_field_bricks_device", "lvr_blobnoise_create", "lvr_blobnoise_destroy",
+    "lvr_blobnoise_brick_host", "lvr_blobnoise_bricks_device", "lvr_blobnoise_source",
+    "lvr_tf_set", "lvr_tf_set_rgba8",
     "lvr_select_visibles", "lvr_order", "lvr_render", "lvr_frame_read",
     "lvr_frame_device_ptr", "lvr_histogram", "lvr_histogram_read",
     "lvr_select_visibles_async", "lvr_select_visibles_wait", "lvr_select_visibles_ex",
@@ -229,6 +231,12 @@ def lib():
     L.lvr_field_brick_host.argtypes = [P(VolumeInfo), P(FieldParams), c.c_uint64,
                                        c.c_void_p, c.c_size_t]
     L.lvr_field_bricks_device.argtypes = [c.c_void_p, P(FieldParams), u64p, c.c_int]
+    L.lvr_blobnoise_create.argtypes = [P(VolumeInfo), P(NoiseParams), c.c_uint64, c.c_int,
+                                       c.c_int, c.c_int, P(c.c_void_p)]
+    L.lvr_blobnoise_destroy.argtypes = [c.c_void_p]
+    L.lvr_blobnoise_destroy.restype = None
+    L.lvr_blobnoise_brick_host.argtypes = [c.c_void_p, c.c_uint64, c.c_void_p, c.c_size_t]
+    L.lvr_blobnoise_bricks_device.argtypes = [c.c_void_p, c.c_void_p, u64p, c.c_int]
     L.lvr_raw_open.argtypes = [c.c_char_p, P(c.c_void_p), c.c_char_p, c.c_size_t]
     L.lvr_raw_close.argtypes = [c.c_void_p]
     L.lvr_raw_close.restype = None
@@ -453,6 +461,40 @@ def field_brick_host(vi, params, node_id):
     if rc != LVR_OK:
         raise LivreError("field brick failed (%d)" % rc)
     return out
+
+
+class BlobNoise:
+    """Blobs plus value noise (lvr_blobnoise_*): the value-noise source
+    blended with a smooth field of Gaussian blobs tabulated on a lattice of
+    `cells` cells across the largest axis; `weight` of 256 goes to the blobs.
+    Host and device bricks are bit-identical."""
+
+    def __init__(self, vi, noise, seed, count=64, cells=64, weight=128):
+        self.vi = vi
+        self.noise = noise
+        self._h = ctypes.c_void_p()
+        rc = lib().lvr_blobnoise_create(ctypes.byref(vi), ctypes.byref(noise), seed, count,
+                                        cells, weight, ctypes.byref(self._h))
+        if rc != LVR_OK:
+            raise LivreError("invalid blob-noise parameters (%d)" % rc)
+
+    def brick_host(self, node_id):
+        out = np.zeros(self.vi.brick_shape, self.vi.dtype)
+        rc = lib().lvr_blobnoise_brick_host(self._h, int(node_id), out.ctypes.data, out.nbytes)
+        if rc != LVR_OK:
+            raise LivreError("blob-noise brick failed (%d)" % rc)
+        return out
+
+    def close(self):
+        if self._h:
+            lib().lvr_blobnoise_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def samples_per_ray(vi, ids, requested=0):
@@ -724,6 +766,11 @@ class Context:
         ids, pids = _u64(ids)
         self._check(lib().lvr_noise_bricks_device(self._h, ctypes.byref(params), pids,
                                                   len(ids)), "generate_noise")
+
+    def generate_blobnoise(self, source, ids):
+        ids, pids = _u64(ids)
+        self._check(lib().lvr_blobnoise_bricks_device(self._h, source._h, pids, len(ids)),
+                    "generate_blobnoise")
 
     def generate_field(self, params, ids):
         ids, pids = _u64(ids)
@@ -1176,17 +1223,23 @@ def rendering_set(visible, resident):
 
 
 class Stream:
-    """Out-of-core streaming of a context's bricks from the host value-noise
-    DataSource (lvr_stream_*)."""
+    """Out-of-core streaming of a context's bricks from a host DataSource
+    (lvr_stream_*): the value noise (NoiseParams) or blobs plus value noise
+    (a BlobNoise, which must outlive the stream)."""
 
-    def __init__(self, ctx, noise, threads=4, staging_bricks=64):
+    def __init__(self, ctx, source, threads=4, staging_bricks=64):
         self.ctx = ctx
-        self._data = NoiseSourceData(ctx.vi, noise)
         self._h = ctypes.c_void_p()
-        fn = ctypes.cast(lib().lvr_noise_source, ctypes.c_void_p)
-        rc = lib().lvr_stream_create(ctx._h, fn, ctypes.cast(ctypes.pointer(self._data),
-                                                             ctypes.c_void_p),
-                                     threads, staging_bricks, ctypes.byref(self._h))
+        if isinstance(source, BlobNoise):
+            self._data = source
+            fn = ctypes.cast(lib().lvr_blobnoise_source, ctypes.c_void_p)
+            user = source._h
+        else:
+            self._data = NoiseSourceData(ctx.vi, source)
+            fn = ctypes.cast(lib().lvr_noise_source, ctypes.c_void_p)
+            user = ctypes.cast(ctypes.pointer(self._data), ctypes.c_void_p)
+        rc = lib().lvr_stream_create(ctx._h, fn, user, threads, staging_bricks,
+                                     ctypes.byref(self._h))
         if rc != LVR_OK:
             raise LivreError("lvr_stream_create failed (%d)" % rc)
 

@@ -9,11 +9,18 @@
 // leaves out octaves finer than twice its voxel size (band limit). The sum
 // is remapped through a monotone 4096-segment uint16 table (log-normal
 // shaping, built on the host in double precision).
+//
+// Blobs plus value noise: the noise above blended, in integers, with a
+// smooth field of Gaussian blobs (field_core.h) tabulated once on a lattice
+// and read by trilinear interpolation in the same 16.16 fixed point, so
+// the blend stays a pure integer function of the voxel as well.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <thread>
 #include <vector>
 
+#include "field_core.h"
 #include "lvr_internal.h"
 
 namespace lvr
@@ -94,6 +101,16 @@ __host__ __device__ inline int64_t octaveSum(const NoiseShape& s,
 
 // Voxel of a level-L brick at level-lattice coordinates g (may lie in the
 // halo, outside the volume: clamped to the volume), remapped to 0..65535.
+// Centre of level voxel g along one axis in finest voxels, 16.16 fixed
+// point, clamped to the volume's first and last voxel centres.
+__host__ __device__ inline int64_t voxelCentre16(int64_t g, int64_t S, uint32_t voxels)
+{
+    int64_t c = (2 * g + 1) * S * 32768; // (g + 0.5) * S in 16.16
+    const int64_t lo = 32768;
+    const int64_t hi = (2 * int64_t(voxels) - 1) * 32768;
+    return c < lo ? lo : (c > hi ? hi : c);
+}
+
 __host__ __device__ inline uint32_t noiseVoxel(const NoiseShape& s,
                                                const uint16_t* remap,
                                                uint32_t level, int64_t gx,
@@ -101,16 +118,8 @@ __host__ __device__ inline uint32_t noiseVoxel(const NoiseShape& s,
 {
     const uint32_t shift = s.depth - 1u - level;
     const int64_t S = int64_t(1) << shift; // finest voxels per level voxel
-    const int64_t g[3] = {gx, gy, gz};
-    int64_t u[3];
-    for (int a = 0; a < 3; ++a)
-    {
-        int64_t c = (2 * g[a] + 1) * S * 32768; // (g + 0.5) * S in 16.16
-        const int64_t lo = 32768;
-        const int64_t hi = (2 * int64_t(s.voxels[a]) - 1) * 32768;
-        c = c < lo ? lo : (c > hi ? hi : c);
-        u[a] = c;
-    }
+    const int64_t u[3] = {voxelCentre16(gx, S, s.voxels[0]), voxelCentre16(gy, S, s.voxels[1]),
+                          voxelCentre16(gz, S, s.voxels[2])};
     // band limit: keep octaves whose cell spans at least 2 level voxels
     int octaves = 0;
     for (int o = 0; o < s.octaves; ++o)
@@ -201,8 +210,49 @@ BrickGeo brickGeo(const lvr_volume_info& vi, uint64_t id)
     return g;
 }
 
-template <typename T>
+// The blob field of the blobs-plus-noise source on a lattice of `cells`
+// cells across the largest axis ((cells + 1)^3 uint16 nodes; node i at
+// i / cells of the largest axis), and its share of the blend.
+struct BlobLattice
+{
+    int64_t cellMult; // floor(cells * 2^32 / vmax): cell coordinate scale
+    int32_t cells;
+    int32_t weight; // 0..256
+};
+
+// Trilinear interpolation of the lattice at a 16.16 finest-voxel position
+// (x, then y, then z, as the noise octaves). u < vmax * 65536 on every axis,
+// so the cell index stays below `cells` and its upper node in the table.
+__host__ __device__ inline uint32_t blobValue(const BlobLattice& b, const uint16_t* table,
+                                              const int64_t* u)
+{
+    int64_t ci[3], fr[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        const int64_t c = (u[a] * b.cellMult) >> 32;
+        ci[a] = c >> 16;
+        fr[a] = c & 0xFFFF;
+    }
+    const int64_t n = b.cells + 1;
+    const uint16_t* t0 = table + (ci[2] * n + ci[1]) * n + ci[0];
+    const uint16_t* t1 = t0 + n * n;
+    const int64_t x00 = lerp16(t0[0], t0[1], fr[0]);
+    const int64_t x10 = lerp16(t0[n], t0[n + 1], fr[0]);
+    const int64_t x01 = lerp16(t1[0], t1[1], fr[0]);
+    const int64_t x11 = lerp16(t1[n], t1[n + 1], fr[0]);
+    const int64_t y0 = lerp16(x00, x10, fr[1]);
+    const int64_t y1 = lerp16(x01, x11, fr[1]);
+    return uint32_t(lerp16(y0, y1, fr[2]));
+}
+
+__host__ __device__ inline uint32_t blend(uint32_t blob, uint32_t noise, int32_t w)
+{
+    return (blob * uint32_t(w) + noise * uint32_t(256 - w) + 128u) >> 8;
+}
+
+template <typename T, bool BLOBS>
 __global__ void noiseBricksKernel(NoiseShape s, const uint16_t* __restrict__ remap,
+                                  BlobLattice b, const uint16_t* __restrict__ table,
                                   const BrickGeo* __restrict__ geos,
                                   const int32_t* __restrict__ slots,
                                   uint8_t* __restrict__ pool, size_t slotStride,
@@ -211,14 +261,23 @@ __global__ void noiseBricksKernel(NoiseShape s, const uint16_t* __restrict__ rem
     const BrickGeo g = geos[blockIdx.y];
     T* out = reinterpret_cast<T*>(pool + size_t(slots[blockIdx.y]) * slotStride);
     const uint32_t n = g.dim[0] * g.dim[1] * g.dim[2];
+    const int64_t S = int64_t(1) << (s.depth - 1u - g.level);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += gridDim.x * blockDim.x)
     {
         const uint32_t x = i % g.dim[0];
         const uint32_t y = (i / g.dim[0]) % g.dim[1];
         const uint32_t z = i / (g.dim[0] * g.dim[1]);
-        out[tiledIndex(L, x, y, z)] = convertNoise<T>(noiseVoxel(s, remap, g.level, g.origin[0] + x,
-                                            g.origin[1] + y, g.origin[2] + z));
+        uint32_t v = noiseVoxel(s, remap, g.level, g.origin[0] + x, g.origin[1] + y,
+                                g.origin[2] + z);
+        if (BLOBS)
+        {
+            const int64_t u[3] = {voxelCentre16(g.origin[0] + x, S, s.voxels[0]),
+                                  voxelCentre16(g.origin[1] + y, S, s.voxels[1]),
+                                  voxelCentre16(g.origin[2] + z, S, s.voxels[2])};
+            v = blend(blobValue(b, table, u), v, b.weight);
+        }
+        out[tiledIndex(L, x, y, z)] = convertNoise<T>(v);
     }
 }
 
@@ -342,7 +401,208 @@ void noiseBrickHostDirect(const NoiseShape& s, const uint16_t* remap, const Bric
                 out[i++] = convertNoise<T>(
                     noiseVoxel(s, remap, g.level, g.origin[0] + x, g.origin[1] + y, g.origin[2] + z));
 }
+
+// Blob values of a brick on the host, separable as noiseBrickHost: the
+// lattice nodes the brick touches are interpolated along x into rows, the
+// rows along y into planes, and each voxel takes the lerp along z; the same
+// integer operations in the same order as blobValue.
+void blobBrickHost(const NoiseShape& s, const BlobLattice& b, const uint16_t* table,
+                   const BrickGeo& g, std::vector<int64_t>& out)
+{
+    const int64_t S = int64_t(1) << (s.depth - 1u - g.level);
+    const int n[3] = {int(g.dim[0]), int(g.dim[1]), int(g.dim[2])};
+    std::vector<int64_t> ci[3], fr[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        ci[a].resize(size_t(n[a]));
+        fr[a].resize(size_t(n[a]));
+        for (int i = 0; i < n[a]; ++i)
+        {
+            const int64_t c = (voxelCentre16(g.origin[a] + i, S, s.voxels[a]) * b.cellMult) >> 32;
+            ci[a][size_t(i)] = c >> 16;
+            fr[a][size_t(i)] = c & 0xFFFF;
+        }
+    }
+    const int nx = n[0], ny = n[1], nz = n[2];
+    const int64_t m = b.cells + 1;
+    const int64_t x0 = ci[0][0], y0 = ci[1][0], z0 = ci[2][0];
+    const int lx = int(ci[0][size_t(nx - 1)] - x0) + 2, ly = int(ci[1][size_t(ny - 1)] - y0) + 2,
+              lz = int(ci[2][size_t(nz - 1)] - z0) + 2;
+    std::vector<int64_t> rows(size_t(lz) * ly * nx), planes(size_t(lz) * ny * nx);
+    for (int k = 0; k < lz; ++k)
+        for (int j = 0; j < ly; ++j)
+        {
+            const uint16_t* T = table + ((z0 + k) * m + (y0 + j)) * m + x0;
+            int64_t* R = &rows[(size_t(k) * ly + j) * nx];
+            for (int x = 0; x < nx; ++x)
+            {
+                const int i = int(ci[0][size_t(x)] - x0);
+                R[x] = lerp16(T[i], T[i + 1], fr[0][size_t(x)]);
+            }
+        }
+    for (int k = 0; k < lz; ++k)
+        for (int y = 0; y < ny; ++y)
+        {
+            const int j = int(ci[1][size_t(y)] - y0);
+            const int64_t* R0 = &rows[(size_t(k) * ly + j) * nx];
+            const int64_t* R1 = R0 + nx;
+            int64_t* P = &planes[(size_t(k) * ny + y) * nx];
+            for (int x = 0; x < nx; ++x)
+                P[x] = lerp16(R0[x], R1[x], fr[1][size_t(y)]);
+        }
+    out.resize(size_t(nx) * ny * nz);
+    for (int z = 0; z < nz; ++z)
+    {
+        const int k = int(ci[2][size_t(z)] - z0);
+        for (int y = 0; y < ny; ++y)
+        {
+            const int64_t* P0 = &planes[(size_t(k) * ny + y) * nx];
+            const int64_t* P1 = &planes[(size_t(k + 1) * ny + y) * nx];
+            int64_t* o = &out[(size_t(z) * ny + y) * nx];
+            for (int x = 0; x < nx; ++x)
+                o[x] = lerp16(P0[x], P1[x], fr[2][size_t(z)]);
+        }
+    }
+}
+
+// Host brick of the blobs-plus-noise source: the separable noise brick (as
+// 16-bit values, which convertNoise leaves unchanged) blended with the
+// separable blob values.
+template <typename T>
+void blobNoiseBrickHost(const NoiseShape& s, const uint16_t* remap, const BlobLattice& b,
+                        const uint16_t* table, const BrickGeo& g, T* out)
+{
+    std::vector<uint16_t> noise(size_t(g.dim[0]) * g.dim[1] * g.dim[2]);
+    noiseBrickHost(s, remap, g, noise.data());
+    std::vector<int64_t> blob;
+    blobBrickHost(s, b, table, g, blob);
+    for (size_t i = 0; i < noise.size(); ++i)
+        out[i] = convertNoise<T>(blend(uint32_t(blob[i]), noise[i], b.weight));
+}
+
 } // namespace
+} // namespace lvr
+
+// Blobs plus value noise: the volume it was made for, the noise parameters,
+// and the blob field tabulated on its lattice.
+struct lvr_blobnoise
+{
+    lvr_volume_info vi;
+    lvr_noise_params noise;
+    lvr::BlobLattice lattice;
+    std::vector<uint16_t> table;
+};
+
+namespace lvr
+{
+// Device generation of noise bricks (bn == nullptr) or blobs-plus-noise
+// bricks straight into pool slots, in chunks of 4096.
+int noiseBricksDevice(lvr_ctx* ctx, const lvr_noise_params& np, const lvr_blobnoise* bn,
+                      const uint64_t* node_ids, int n)
+{
+    const lvr_noise_params* p = &np;
+    const int dt = ctx->vi.data_type;
+    if (dt != LVR_DT_UINT8 && dt != LVR_DT_UINT16 && dt != LVR_DT_FLOAT)
+        return fail(ctx, LVR_EINVAL, "noise source supports u8, u16, f32");
+    int rc = ensureDevice(ctx);
+    if (rc)
+        return rc;
+    const NoiseShape s = makeShape(ctx->vi, *p);
+    const int chunk = 4096;
+    const size_t remapBytes = (sizeof p->remap + 255) & ~size_t(255);
+    const size_t geoBytes = (sizeof(BrickGeo) * chunk + 255) & ~size_t(255);
+    const size_t tableBytes = bn ? (sizeof(uint16_t) * bn->table.size() + 255) & ~size_t(255) : 0;
+    uint8_t* scratch = static_cast<uint8_t*>(
+        genScratch(ctx, remapBytes + tableBytes + geoBytes + sizeof(int32_t) * chunk));
+    if (!scratch)
+        return LVR_ENOMEM;
+    uint16_t* d_remap = reinterpret_cast<uint16_t*>(scratch);
+    uint16_t* d_table = reinterpret_cast<uint16_t*>(scratch + remapBytes);
+    BrickGeo* d_geo = reinterpret_cast<BrickGeo*>(scratch + remapBytes + tableBytes);
+    int32_t* d_slots =
+        reinterpret_cast<int32_t*>(scratch + remapBytes + tableBytes + geoBytes);
+    const BlobLattice lat = bn ? bn->lattice : BlobLattice{0, 0, 0};
+    // slots taken here may still be read by batches on other streams
+    rc = waitPoolReaders(ctx, ctx->stream);
+    if (rc)
+        return rc;
+    LVR_HIP(ctx, hipMemcpyAsync(d_remap, p->remap, sizeof p->remap, hipMemcpyHostToDevice,
+                                ctx->stream));
+    if (bn)
+        LVR_HIP(ctx, hipMemcpyAsync(d_table, bn->table.data(), sizeof(uint16_t) * bn->table.size(),
+                                    hipMemcpyHostToDevice, ctx->stream));
+    std::vector<BrickGeo> geo;
+    std::vector<int32_t> slots;
+    for (int base = 0; base < n && rc == LVR_OK; base += chunk)
+    {
+        const int cnt = std::min(chunk, n - base);
+        geo.resize(cnt);
+        slots.resize(cnt);
+        int taken = 0;
+        for (int i = 0; i < cnt && rc == LVR_OK; ++i)
+        {
+            if (decodeNode(node_ids[base + i]).level >= ctx->vi.depth)
+                rc = fail(ctx, LVR_EINVAL, "node id outside the octree");
+            else
+            {
+                geo[i] = brickGeo(ctx->vi, node_ids[base + i]);
+                rc = acquireSlotFor(ctx, node_ids[base + i], &slots[i]);
+                taken += rc == LVR_OK;
+            }
+        }
+        if (rc == LVR_OK &&
+            (hipMemcpyAsync(d_geo, geo.data(), sizeof(BrickGeo) * cnt,
+                            hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+             hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt,
+                            hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
+            rc = fail(ctx, LVR_EHIP, "noise upload");
+        if (rc)
+        {
+            // bricks of this chunk never received their voxels
+            const std::string msg = ctx->error;
+            for (int i = 0; i < taken; ++i)
+                (void)releaseSlot(ctx, node_ids[base + i]);
+            ctx->error = msg;
+            break;
+        }
+        const dim3 grid(64, unsigned(cnt));
+#define LVR_NOISE(T, B)                                                                     \
+    hipLaunchKernelGGL((noiseBricksKernel<T, B>), grid, dim3(256), 0, ctx->stream, s,       \
+                       d_remap, lat, d_table, d_geo, d_slots, ctx->d_pool, ctx->slotStride,     \
+                       ctx->layout)
+        if (dt == LVR_DT_UINT8)
+        {
+            if (bn)
+                LVR_NOISE(uint8_t, true);
+            else
+                LVR_NOISE(uint8_t, false);
+        }
+        else if (dt == LVR_DT_UINT16)
+        {
+            if (bn)
+                LVR_NOISE(uint16_t, true);
+            else
+                LVR_NOISE(uint16_t, false);
+        }
+        else
+        {
+            if (bn)
+                LVR_NOISE(float, true);
+            else
+                LVR_NOISE(float, false);
+        }
+#undef LVR_NOISE
+        if (hipGetLastError() != hipSuccess)
+            rc = fail(ctx, LVR_EHIP, "noise kernel");
+        // each brick's histogram into the cache as it lands in its slot
+        if (rc == LVR_OK)
+            rc = cacheBrickHistograms(ctx, node_ids + base, slots.data(), cnt, ctx->stream);
+        // the host vectors and the scratch are reused by the next chunk
+        if (rc == LVR_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
+            rc = fail(ctx, LVR_EHIP, "noise kernel");
+    }
+    return rc;
+}
 } // namespace lvr
 
 extern "C" {
@@ -439,85 +699,125 @@ int lvr_noise_bricks_device(lvr_ctx* ctx, const lvr_noise_params* p,
 {
     if (!ctx || !p || n < 0 || (n > 0 && !node_ids))
         return LVR_EINVAL;
-    const int dt = ctx->vi.data_type;
+    return lvr::noiseBricksDevice(ctx, *p, nullptr, node_ids, n);
+}
+
+int lvr_blobnoise_create(const lvr_volume_info* info, const lvr_noise_params* noise,
+                         uint64_t blob_seed, int blob_count, int blob_cells, int blob_weight,
+                         lvr_blobnoise** out)
+{
+    if (!info || !noise || !out || blob_count < 1 || blob_count > 4096 || blob_cells < 1 ||
+        blob_cells > 1024 || blob_weight < 0 || blob_weight > 256)
+        return LVR_EINVAL;
+    *out = nullptr;
+    lvr_volume_info vi = *info;
+    if (lvr_fill_regular_volume_info(&vi) != LVR_OK)
+        return LVR_EINVAL;
+    const int dt = vi.data_type;
     if (dt != LVR_DT_UINT8 && dt != LVR_DT_UINT16 && dt != LVR_DT_FLOAT)
-        return lvr::fail(ctx, LVR_EINVAL, "noise source supports u8, u16, f32");
-    int rc = lvr::ensureDevice(ctx);
-    if (rc)
-        return rc;
-    const lvr::NoiseShape s = lvr::makeShape(ctx->vi, *p);
-    const int chunk = 4096;
-    const size_t remapBytes = (sizeof p->remap + 255) & ~size_t(255);
-    const size_t geoBytes = (sizeof(lvr::BrickGeo) * chunk + 255) & ~size_t(255);
-    uint8_t* scratch = static_cast<uint8_t*>(
-        lvr::genScratch(ctx, remapBytes + geoBytes + sizeof(int32_t) * chunk));
-    if (!scratch)
-        return LVR_ENOMEM;
-    uint16_t* d_remap = reinterpret_cast<uint16_t*>(scratch);
-    lvr::BrickGeo* d_geo = reinterpret_cast<lvr::BrickGeo*>(scratch + remapBytes);
-    int32_t* d_slots = reinterpret_cast<int32_t*>(scratch + remapBytes + geoBytes);
-    // slots taken here may still be read by batches on other streams
-    rc = lvr::waitPoolReaders(ctx, ctx->stream);
-    if (rc)
-        return rc;
-    LVR_HIP(ctx, hipMemcpyAsync(d_remap, p->remap, sizeof p->remap, hipMemcpyHostToDevice,
-                                ctx->stream));
-    std::vector<lvr::BrickGeo> geo;
-    std::vector<int32_t> slots;
-    for (int base = 0; base < n && rc == LVR_OK; base += chunk)
+        return LVR_EINVAL;
+    lvr_blobnoise* bn = nullptr;
+    try
     {
-        const int cnt = std::min(chunk, n - base);
-        geo.resize(cnt);
-        slots.resize(cnt);
-        int taken = 0;
-        for (int i = 0; i < cnt && rc == LVR_OK; ++i)
-        {
-            if (lvr::decodeNode(node_ids[base + i]).level >= ctx->vi.depth)
-                rc = lvr::fail(ctx, LVR_EINVAL, "node id outside the octree");
-            else
-            {
-                geo[i] = lvr::brickGeo(ctx->vi, node_ids[base + i]);
-                rc = lvr::acquireSlotFor(ctx, node_ids[base + i], &slots[i]);
-                taken += rc == LVR_OK;
-            }
-        }
-        if (rc == LVR_OK &&
-            (hipMemcpyAsync(d_geo, geo.data(), sizeof(lvr::BrickGeo) * cnt,
-                            hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
-             hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * cnt,
-                            hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
-            rc = lvr::fail(ctx, LVR_EHIP, "noise upload");
-        if (rc)
-        {
-            // bricks of this chunk never received their voxels
-            const std::string msg = ctx->error;
-            for (int i = 0; i < taken; ++i)
-                (void)lvr::releaseSlot(ctx, node_ids[base + i]);
-            ctx->error = msg;
-            break;
-        }
-        const dim3 grid(64, unsigned(cnt));
-        if (dt == LVR_DT_UINT8)
-            hipLaunchKernelGGL(lvr::noiseBricksKernel<uint8_t>, grid, dim3(256), 0,
-                               ctx->stream, s, d_remap, d_geo, d_slots,
-                               ctx->d_pool, ctx->slotStride, ctx->layout);
-        else if (dt == LVR_DT_UINT16)
-            hipLaunchKernelGGL(lvr::noiseBricksKernel<uint16_t>, grid, dim3(256),
-                               0, ctx->stream, s, d_remap, d_geo, d_slots,
-                               ctx->d_pool, ctx->slotStride, ctx->layout);
-        else
-            hipLaunchKernelGGL(lvr::noiseBricksKernel<float>, grid, dim3(256), 0,
-                               ctx->stream, s, d_remap, d_geo, d_slots,
-                               ctx->d_pool, ctx->slotStride, ctx->layout);
-        if (hipGetLastError() != hipSuccess)
-            rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");
-        // each brick's histogram into the cache as it lands in its slot
-        if (rc == LVR_OK)
-            rc = lvr::cacheBrickHistograms(ctx, node_ids + base, slots.data(), cnt, ctx->stream);
-        // the host vectors and the scratch are reused by the next chunk
-        if (rc == LVR_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
-            rc = lvr::fail(ctx, LVR_EHIP, "noise kernel");
+        bn = new lvr_blobnoise;
+        bn->vi = vi;
+        bn->noise = *noise;
+        const uint32_t vmax = std::max(vi.voxels[0], std::max(vi.voxels[1], vi.voxels[2]));
+        bn->lattice.cells = blob_cells;
+        bn->lattice.weight = blob_weight;
+        bn->lattice.cellMult = int64_t((uint64_t(blob_cells) << 32) / vmax);
+        lvr_field_params fp;
+        std::memset(&fp, 0, sizeof fp);
+        fp.kind = LVR_FIELD_BLOBS;
+        fp.count = blob_count;
+        fp.seed = blob_seed;
+        fp.noise = 0.f;
+        const lvr::field::FieldSetup f = lvr::field::makeField(vi, fp);
+        const size_t m = size_t(blob_cells) + 1;
+        bn->table.resize(m * m * m);
+        // node (i, j, k) at (i, j, k) / cells of the largest axis; threads
+        // over z planes
+        const unsigned nt =
+            std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> workers;
+        for (unsigned t = 0; t < nt; ++t)
+            workers.emplace_back([&f, bn, m, t, nt, blob_cells] {
+                for (size_t k = t; k < m; k += nt)
+                    for (size_t j = 0; j < m; ++j)
+                        for (size_t i = 0; i < m; ++i)
+                        {
+                            const float q[3] = {float(i) / float(blob_cells),
+                                                float(j) / float(blob_cells),
+                                                float(k) / float(blob_cells)};
+                            bn->table[(k * m + j) * m + i] = lvr::field::quantise<uint16_t>(
+                                f.shape, lvr::field::fieldValue(f.shape, f.blobs.data(), nullptr,
+                                                                q, 0));
+                        }
+            });
+        for (std::thread& w : workers)
+            w.join();
     }
-    return rc;
+    catch (...)
+    {
+        delete bn;
+        return LVR_ENOMEM;
+    }
+    *out = bn;
+    return LVR_OK;
+}
+
+void lvr_blobnoise_destroy(lvr_blobnoise* bn)
+{
+    delete bn;
+}
+
+int lvr_blobnoise_brick_host(const lvr_blobnoise* bn, uint64_t node_id, void* out,
+                             size_t bytes)
+{
+    if (!bn || !out)
+        return LVR_EINVAL;
+    const lvr_volume_info& vi = bn->vi;
+    const size_t n = size_t(vi.max_block[0]) * vi.max_block[1] * vi.max_block[2];
+    if (bytes != n * lvr::bytesPerVoxel(vi.data_type) || lvr::decodeNode(node_id).level >= vi.depth)
+        return LVR_EINVAL;
+    const lvr::NoiseShape s = lvr::makeShape(vi, bn->noise);
+    const lvr::BrickGeo g = lvr::brickGeo(vi, node_id);
+    const uint16_t* remap = bn->noise.remap;
+    switch (vi.data_type)
+    {
+    case LVR_DT_UINT8:
+        lvr::blobNoiseBrickHost(s, remap, bn->lattice, bn->table.data(), g,
+                                static_cast<uint8_t*>(out));
+        return LVR_OK;
+    case LVR_DT_UINT16:
+        lvr::blobNoiseBrickHost(s, remap, bn->lattice, bn->table.data(), g,
+                                static_cast<uint16_t*>(out));
+        return LVR_OK;
+    default:
+        lvr::blobNoiseBrickHost(s, remap, bn->lattice, bn->table.data(), g,
+                                static_cast<float*>(out));
+        return LVR_OK;
+    }
+}
+
+int lvr_blobnoise_bricks_device(lvr_ctx* ctx, const lvr_blobnoise* bn, const uint64_t* node_ids,
+                                int n)
+{
+    if (!ctx || !bn || n < 0 || (n > 0 && !node_ids))
+        return LVR_EINVAL;
+    const lvr_volume_info &a = ctx->vi, &b = bn->vi;
+    bool same = a.data_type == b.data_type && a.depth == b.depth;
+    for (int i = 0; i < 3; ++i)
+        same = same && a.voxels[i] == b.voxels[i] && a.max_block[i] == b.max_block[i] &&
+               a.overlap[i] == b.overlap[i];
+    if (!same)
+        return lvr::fail(ctx, LVR_EINVAL, "blob-noise source made for another volume");
+    return lvr::noiseBricksDevice(ctx, bn->noise, bn, node_ids, n);
+}
+
+int lvr_blobnoise_source(void* user, uint64_t node_id, void* out, size_t bytes)
+{
+    return lvr_blobnoise_brick_host(static_cast<const lvr_blobnoise*>(user), node_id, out,
+                                    bytes);
 }
 }

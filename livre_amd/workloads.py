@@ -12,13 +12,17 @@ Volumes
   noise2048_u16  2048^3 uint16, 5-octave value noise, seed 3 (the 1080p
                  single-volume workload)
 All use a 1-voxel halo and 64^3 bricks unless stated.
+
+Out-of-core volume
+  ooc8192_u8     8192^3 uint8 (512 GB logical), depth 8: blobs plus value
+                 noise (c4_source), streamed from the host source
 """
 import math
 import time
 
 import numpy as np
 
-from . import (DT_FLOAT, DT_UINT8, DT_UINT16, FIELD_BLOBS, FIELD_SINES, Context,
+from . import (DT_FLOAT, DT_UINT8, DT_UINT16, FIELD_BLOBS, FIELD_SINES, BlobNoise, Context,
                field_params, frustum, noise_params, slot_bytes, volume_info)
 from . import scenes
 
@@ -160,6 +164,20 @@ def poisson_arrivals(rate, duration_s, n_sessions, seed):
         if t > duration_s * 1000.0:
             return out
         out.append((t, int(rng.integers(n_sessions))))
+
+
+def c4_volume():
+    """The out-of-core volume: 8192^3 uint8, 64^3 bricks + 1 halo, depth 8."""
+    return volume_info(8192, 64, 1, DT_UINT8)
+
+
+def c4_source(vi):
+    """Its procedural source, blobs plus value noise: 5-octave value noise
+    (seed 8, log-normal remap) blended half and half with 64 anisotropic
+    Gaussian blobs (seed 8) tabulated on a 64^3-cell lattice (a cell is 128
+    finest voxels)."""
+    return BlobNoise(vi, noise_params(8, octaves=5, base_cells=8, gain=1.5), 8, count=64,
+                     cells=64, weight=128)
 
 
 def flythrough(t, frames=300, seed=8):

@@ -1,5 +1,5 @@
 """The out-of-core configuration at its real size: an 8192^3 uint8
-value-noise volume (512 GB logical, 64^3 bricks, 8 levels) streamed from the
+blobs-plus-value-noise volume (512 GB logical, 64^3 bricks, 8 levels) streamed from the
 host DataSource through pinned memory into a brick pool capped at 6000 slots
 (1.7 GB), rendered at 3840x2160 along the fly-through path. For three
 frames the renderer draws the coarser-LOD rendering set of the bricks that
@@ -19,16 +19,16 @@ W4, H4 = 3840, 2160
 
 
 def test_c4_fallback_frames_at_4k(lvr):
-    vi = livre_amd.volume_info(8192, 64, 1, livre_amd.DT_UINT8)
+    vi = workloads.c4_volume()
     slot = livre_amd.slot_bytes(vi)
     ctx = livre_amd.Context(vi, slot * 6000)
-    noise = livre_amd.noise_params(8, octaves=5, base_cells=8, gain=1.5)
+    source = workloads.c4_source(vi)
     tf = scenes.piecewise_tf(256, seed=8, points=6, max_alpha=0.3)
     rng = (0.0, 255.0)
     params = livre_amd.render_params(0)  # samples per ray from the finest rendered level
     fallbacks = 0
     try:
-        with livre_amd.Stream(ctx, noise, threads=8, staging_bricks=128) as st:
+        with livre_amd.Stream(ctx, source, threads=8, staging_bricks=128) as st:
             coarse = []
             for level in range(4):
                 n = 1 << level
@@ -54,7 +54,7 @@ def test_c4_fallback_frames_at_4k(lvr):
                 ctx.pin(ordered, -1)
                 ref_ctx = livre_amd.Context(vi, slot * (len(ordered) + 16))
                 try:
-                    ref_ctx.generate_noise(noise, ordered)
+                    ref_ctx.generate_blobnoise(source, ordered)
                     ref_ctx.set_tf(tf, rng)
                     ref_ctx.render(fr, W4, H4, ordered, params)
                     ref, ref_smp = ref_ctx.read_frame()
@@ -66,4 +66,5 @@ def test_c4_fallback_frames_at_4k(lvr):
                 assert smp > 0 and smp == ref_smp and np.array_equal(img, ref), t
     finally:
         ctx.close()
+        source.close()
     assert fallbacks >= 1

@@ -231,6 +231,24 @@ def test_device_noise_equals_host_noise(lvr):
             assert np.array_equal(ctx.download(nid), livre_amd.noise_brick_host(vi, p, nid))
 
 
+@pytest.mark.parametrize("dt", [livre_amd.DT_UINT8, livre_amd.DT_UINT16, livre_amd.DT_FLOAT])
+def test_device_blob_noise_equals_host(lvr, dt):
+    vi = livre_amd.volume_info(128, 32, 1, dt)
+    src = livre_amd.BlobNoise(vi, livre_amd.noise_params(4, octaves=5, base_cells=8), 9,
+                              count=24, cells=16, weight=144)
+    ids = scenes.all_nodes(vi)
+    try:
+        with livre_amd.Context(vi, livre_amd.slot_bytes(vi) * len(ids)) as ctx:
+            ctx.generate_blobnoise(src, ids)
+            for nid in ids:
+                assert np.array_equal(ctx.download(nid), src.brick_host(nid)), nid
+            other = livre_amd.volume_info(64, 32, 1, dt)
+            with pytest.raises(livre_amd.LivreError):
+                ctx.generate_blobnoise(livre_amd.BlobNoise(other, src.noise, 9), ids[:1])
+    finally:
+        src.close()
+
+
 @pytest.mark.parametrize("kind,dt", [(livre_amd.FIELD_BLOBS, livre_amd.DT_UINT8),
                                      (livre_amd.FIELD_BLOBS, livre_amd.DT_UINT16),
                                      (livre_amd.FIELD_SINES, livre_amd.DT_FLOAT)])

@@ -142,6 +142,50 @@ def test_field_sources_are_deterministic_and_consistent(lvr):
         lvr.field_brick_host(vi, lvr.field_params(7, 32, 1), 0)
 
 
+def test_blob_noise_source(lvr):
+    """Blobs plus value noise: weight 0 is the value-noise source exactly; a
+    weight w brick is the integer blend (blob * w + noise * (256 - w) + 128)
+    >> 8 of the weight-256 and weight-0 bricks; the blob part is the
+    trilinear interpolation of a lattice of 32-voxel cells, so along x it is
+    affine inside each cell (second differences within the rounding of three
+    fixed-point lerps) and bends only where a voxel centre crosses a cell
+    boundary; 8-bit and float volumes convert the same 16-bit values."""
+    vi = lvr.volume_info(256, 32, 1, lvr.DT_UINT16)
+    noise = lvr.noise_params(5)
+    src = {w: lvr.BlobNoise(vi, noise, 11, count=32, cells=8, weight=w) for w in (0, 96, 256)}
+    for nid in (0, scenes.node_id(1, 1, 0, 1), scenes.node_id(3, 1, 2, 3)):
+        n0, n96, blob = (src[w].brick_host(nid) for w in (0, 96, 256))
+        assert np.array_equal(n0, lvr.noise_brick_host(vi, noise, nid))
+        mix = (blob.astype(np.uint32) * 96 + n0.astype(np.uint32) * 160 + 128) >> 8
+        assert np.array_equal(n96, mix)
+    # finest level: voxel x = 31 .. 64 of the brick, centres (x + 0.5) / 32 cells
+    blob = src[256].brick_host(scenes.node_id(3, 1, 2, 3)).astype(np.int64)
+    x = np.arange(31, 65)
+    cell = (x * 2 + 1) // 64
+    d2 = blob[:, :, 2:] - 2 * blob[:, :, 1:-1] + blob[:, :, :-2]
+    inside = (cell[2:] == cell[:-2])
+    assert np.abs(d2[:, :, inside]).max() <= 8
+    assert np.abs(np.diff(blob, axis=2)).max() > 100  # the field is not flat here
+    # the slope changes from cell to cell (a smoothstep fade of the fraction
+    # would bend inside the cells instead)
+    assert np.abs(d2[:, :, ~inside]).max() > 8
+    for dt in (lvr.DT_UINT8, lvr.DT_FLOAT):
+        v2 = lvr.volume_info(256, 32, 1, dt)
+        b = lvr.BlobNoise(v2, noise, 11, count=32, cells=8, weight=96).brick_host(
+            scenes.node_id(3, 1, 2, 3))
+        ref = src[96].brick_host(scenes.node_id(3, 1, 2, 3))
+        if dt == lvr.DT_UINT8:
+            assert np.array_equal(b, (ref >> 8).astype(np.uint8))
+        else:
+            assert np.array_equal(b, ref.astype(np.float32) * np.float32(2.0 / 65535.0)
+                                  - np.float32(1.0))
+    for bad in (dict(count=0), dict(cells=0), dict(weight=257), dict(cells=2000)):
+        args = dict(count=32, cells=8, weight=96)
+        args.update(bad)
+        with pytest.raises(lvr.LivreError):
+            lvr.BlobNoise(vi, noise, 11, **args)
+
+
 def test_invalid_arguments_are_rejected(lvr):
     with pytest.raises(lvr.LivreError):
         lvr.volume_info(64, 0, 0, lvr.DT_UINT8)

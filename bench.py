@@ -181,16 +181,27 @@ def batching_1080p(livre_amd, scenes, vi, ctx, tf, reps):
             ctx.render(fr, W, H, ordered, params)
         ctx.sync()
     t_plain = time.perf_counter() - t0
-    with livre_amd.Batch() as b:
-        def batched():
+    with livre_amd.Batch() as b0, livre_amd.Batch() as b1:
+        def queue(b):
             for fr, ordered in sess:
                 b.add(ctx, fr, W, H, ordered, params, tf, (0.0, 65535.0))
             b.run()
-            return b.wait()
-        batched()  # warm-up and graph capture
+        for b in (b0, b1):  # warm-up
+            queue(b)
+            solo = b.wait().solo_requests
+        # two batches in turn, as the plain pass queues without waiting: the
+        # host assembles pass k+1 while pass k is on the GPU
         t0 = time.perf_counter()
-        batch_samples = sum(batched().samples for _ in range(reps))
+        batch_samples, prev = 0, None
+        for r in range(reps):
+            b = (b0, b1)[r % 2]
+            queue(b)
+            if prev is not None:
+                batch_samples += prev.wait().samples
+            prev = b
+        batch_samples += prev.wait().samples
         t_batch = time.perf_counter() - t0
+        b = prev
         fr, ordered = sess[5]
         ctx.render(fr, W, H, ordered, params)
         img, smp = ctx.read_frame()
@@ -203,9 +214,13 @@ def batching_1080p(livre_amd, scenes, vi, ctx, tf, reps):
     return {"sessions": 16, "reps": reps, "plain_gsamples_per_s": plain,
             "batched_gsamples_per_s": batched_rate, "ratio": batched_rate / plain,
             "batched_equals_plain": same,
+            "solo_requests": solo,
             "plain": "lvr_render per session, queued back to back, one sync per pass",
-            "batched": "one lvr_batch launch per pass (graph replay)",
-            "shape": "default (8x8 tiles, 2 fetches in flight, 4 waves/SIMD) for both"}
+            "batched": "one lvr_batch per pass, two batches in turn; requests of at least "
+                       "4 tiles per resident wavefront run the single-frame windowed kernel "
+                       "on their own (solo_requests of 16), the rest one shared launch",
+            "shape": "default (8x8 tiles, 2 fetches in flight, 5 waves/SIMD for the "
+                     "windowed kernel) for both"}
 
 
 def batch_workload(args):
@@ -295,6 +310,7 @@ def batch_workload(args):
                 bimg, bsmp = batch.read(i)
                 same = same and smp == bsmp and bool(np.array_equal(img, bimg))
             graph = (st.graph_captures, st.graph_launches)
+            solo = st.solo_requests
         plain = plain_samples / t_plain / 1e9
         batched = batch_samples / t_batch / 1e9
         log("%s: plain %.1f Gsamples/s, batched %.1f Gsamples/s (x%.2f), tuning %.1f s"
@@ -305,6 +321,7 @@ def batch_workload(args):
                 "batched_gsamples_per_s": batched, "batched_ms_per_pass": t_batch / reps * 1e3,
                 "batch_gpu_ms_mean": float(np.mean(gpu_ms)), "speedup": batched / plain,
                 "frames_identical": same, "graph_captures_launches": list(graph),
+                "solo_requests": solo,
                 "tuned_classes": tuner.tuned, "tuning_s": t_tune,
                 "launch_shapes": sorted({(t.tile_w, t.tile_h, t.samples_per_thread)
                                          for t in tuned})}

@@ -522,6 +522,7 @@ typedef struct
     uint64_t tile_entries; /* (tile, brick) pairs binned */
     float gpu_ms;          /* binning + ray casting on the device */
     uint64_t graph_captures, graph_launches;
+    uint64_t solo_requests; /* requests launched on their own (set_solo_tiles) */
 } lvr_batch_stats;
 
 int lvr_batch_create(int device, lvr_batch** out);
@@ -529,6 +530,16 @@ void lvr_batch_destroy(lvr_batch* batch);
 const char* lvr_batch_last_error(const lvr_batch* batch);
 /* 1 (default): capture and replay the launch sequence as a HIP graph. */
 int lvr_batch_set_graph(lvr_batch* batch, int enable);
+/* Requests that the single-frame kernel's two-brick window covers (NEAREST,
+ * 2 samples per thread, classification per sample) and that hold at least
+ * min_tiles ray tiles are launched on their own, one single-frame launch
+ * each after the shared launch, instead of in the batch kernel (which
+ * marches brick by brick); a lone such request always is. 0 (default) = four
+ * tiles per resident wavefront of a launch (2 workgroups of 2 x
+ * waves_per_simd wavefronts per CU), negative = only a lone request. A
+ * batch with such requests is not captured as a graph. Frames and sample
+ * counts do not depend on the split. No reference counterpart. */
+int lvr_batch_set_solo_tiles(lvr_batch* batch, int min_tiles);
 /* Queue one frame of ctx's volume; request receives its index. The TF is
  * n_tf FP32 RGBA entries over [range0, range1] (as lvr_tf_set). Adding after
  * a run starts a new batch. One launch renders all requests, so their

@@ -46,7 +46,7 @@ EXPORTS = (
     "lvr_set_timing", "lvr_frame_stats_get", "lvr_frame_stats_back", "lvr_slot_bytes",
     "lvr_autotune",
     "lvr_batch_create", "lvr_batch_destroy", "lvr_batch_last_error",
-    "lvr_batch_set_graph", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
+    "lvr_batch_set_graph", "lvr_batch_set_solo_tiles", "lvr_batch_add", "lvr_batch_clear", "lvr_batch_run",
     "lvr_batch_wait", "lvr_batch_read", "lvr_batch_query", "lvr_batch_set_output",
     "lvr_batch_reserve_cus",
     "lvr_server_now_ms", "lvr_server_create", "lvr_server_destroy", "lvr_server_last_error",
@@ -64,7 +64,7 @@ class BatchStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_uint64), ("samples", ctypes.c_uint64),
                 ("bricks", ctypes.c_uint64), ("tile_entries", ctypes.c_uint64),
                 ("gpu_ms", ctypes.c_float), ("graph_captures", ctypes.c_uint64),
-                ("graph_launches", ctypes.c_uint64)]
+                ("graph_launches", ctypes.c_uint64), ("solo_requests", ctypes.c_uint64)]
 
 
 class ServerFrame(ctypes.Structure):
@@ -277,6 +277,7 @@ def lib():
     L.lvr_batch_last_error.argtypes = [c.c_void_p]
     L.lvr_batch_last_error.restype = c.c_char_p
     L.lvr_batch_set_graph.argtypes = [c.c_void_p, c.c_int]
+    L.lvr_batch_set_solo_tiles.argtypes = [c.c_void_p, c.c_int]
     L.lvr_batch_add.argtypes = [c.c_void_p, c.c_void_p, P(Frustum), P(c.c_int), u64p,
                                 c.c_int, P(RenderParams), f32p, c.c_int, c.c_float,
                                 c.c_float, P(c.c_int)]
@@ -998,6 +999,12 @@ class Batch:
 
     def set_graph(self, enable):
         self._check(lib().lvr_batch_set_graph(self._h, 1 if enable else 0), "set_graph")
+
+    def set_solo_tiles(self, min_tiles):
+        """Requests the single-frame window kernel covers with at least
+        min_tiles ray tiles get launches of their own (0 = default, negative
+        = only a lone request)."""
+        self._check(lib().lvr_batch_set_solo_tiles(self._h, int(min_tiles)), "set_solo_tiles")
 
     def reserve_cus(self, n):
         self._check(lib().lvr_batch_reserve_cus(self._h, n), "reserve_cus")

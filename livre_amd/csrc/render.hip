@@ -978,6 +978,8 @@ struct lvr_batch
 
     // graph of the last launch sequence and the shape it was captured for
     bool useGraph = true;
+    int soloMinTiles = 0;  // lvr_batch_set_solo_tiles
+    std::vector<int> solo; // requests of the last run launched on their own
     hipGraphExec_t exec = nullptr;
     std::vector<int64_t> graphKey;
     uint64_t graphLaunches = 0, graphCaptures = 0;
@@ -1735,6 +1737,14 @@ int lvr_batch_set_graph(lvr_batch* b, int enable)
     return LVR_OK;
 }
 
+int lvr_batch_set_solo_tiles(lvr_batch* b, int min_tiles)
+{
+    if (!b)
+        return LVR_EINVAL;
+    b->soloMinTiles = min_tiles;
+    return LVR_OK;
+}
+
 int lvr_batch_clear(lvr_batch* b)
 {
     if (!b)
@@ -1839,6 +1849,35 @@ int lvr_batch_run(lvr_batch* b)
     LVR_BHIP(b, hipSetDevice(b->device));
     LVR_BHIP(b, hipStreamSynchronize(b->stream)); // host arrays are reused
     const int nReq = int(b->reqs.size());
+    // Requests the single-frame kernel marches on its own: those its
+    // two-brick window covers (NEAREST, two fetches in flight, classification
+    // per sample) when they are alone or large enough to fill the GPU for
+    // most of their launch; the batch kernel takes the others.
+    const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
+    const int wps = b->reqs.front().wavesPerSimd;
+    const int soloWps = b->wpsDefault ? 5 : wps;
+    const int64_t soloMin = b->soloMinTiles > 0
+                                ? int64_t(b->soloMinTiles)
+                                : (b->soloMinTiles < 0 ? INT64_MAX
+                                                       : 4 * int64_t(slots) * 2 * soloWps);
+    b->solo.clear();
+    std::vector<char> isSolo(size_t(nReq), 0);
+    for (int r = 0; r < nReq; ++r)
+    {
+        const FrameUniforms& U = b->reqs[size_t(r)];
+        const bool window = !U.linear && U.depth == 2 &&
+                            !(U.classify == 1 && (U.dataType == LVR_DT_UINT8 ||
+                                                  U.dataType == LVR_DT_UINT16));
+        if (window && (nReq == 1 || int64_t(U.tilesX) * U.tilesY >= soloMin))
+        {
+            isSolo[size_t(r)] = 1;
+            b->solo.push_back(r);
+        }
+    }
+    const int nSolo = int(b->solo.size());
+    // counters: total and queue of the batch kernel, one per request, then
+    // samples and queue of each solo launch
+    const int nCounters = nReq + 2 + 2 * nSolo;
     bool grew = false;
     int rc = LVR_OK;
     if ((rc = bgrow(b, &b->d_reqs, &b->reqCap, size_t(nReq), &grew)) ||
@@ -1850,7 +1889,7 @@ int lvr_batch_run(lvr_batch* b)
         (rc = bgrow(b, &b->d_lists, &b->listCap, std::max<uint64_t>(1, b->listTotal), &grew)) ||
         (rc = bgrow(b, &b->d_slists, &b->slistCap, std::max<uint64_t>(1, b->superTotal),
                     &grew)) ||
-        (rc = bgrow(b, &b->d_counters, &b->counterCap, size_t(nReq) + 2, &grew)))
+        (rc = bgrow(b, &b->d_counters, &b->counterCap, size_t(nCounters), &grew)))
         return rc;
     if (b->pinnedCap < size_t(nReq) + 2)
     {
@@ -1863,7 +1902,10 @@ int lvr_batch_run(lvr_batch* b)
                                   hipHostMallocDefault));
         b->pinnedCap = size_t(nReq) + 2;
     }
-    // request table: positions in the batch-wide arrays
+    // request table: positions in the batch-wide arrays. A solo request's
+    // work items are not in the batch kernel's queue: its item range is
+    // empty, and the search by item (the last request whose range starts at
+    // or before the item) passes over it.
     int tiles = 0, items = 0, supers = 0;
     for (int r = 0; r < nReq; ++r)
     {
@@ -1873,7 +1915,8 @@ int lvr_batch_run(lvr_batch* b)
         U.superBase = supers;
         tiles += U.tilesX * U.tilesY;
         supers += U.superX * U.superY;
-        items += U.nItems;
+        if (!isSolo[size_t(r)])
+            items += U.nItems;
         U.tf = b->d_tf + b->tfOffset[size_t(r)];
         U.frame = b->outputs[size_t(r)] ? b->outputs[size_t(r)]
                                         : b->d_frames + b->frameOffset[size_t(r)];
@@ -1904,7 +1947,6 @@ int lvr_batch_run(lvr_batch* b)
         depthOf[batchTypeClass(r.dataType)] = batchDepth(r.depth);
     const int depthCode = (depthOf[0] == 3 ? 4 : 0) + (depthOf[1] == 3 ? 2 : 0) +
                           (depthOf[2] == 3 ? 1 : 0);
-    const int wps = b->reqs.front().wavesPerSimd;
     std::vector<int64_t> key = {nReq,
                                 depthCode,
                                 wps,
@@ -1960,73 +2002,70 @@ int lvr_batch_run(lvr_batch* b)
         if ((rc = applyPageTable(c, b->stream)))
             return bfail(b, rc, c->error);
 
-    // A batch of one request with per-sample NEAREST classification at two
-    // fetches in flight runs the single-frame kernel, which marches through
-    // the two-brick window; its uniforms go to the kernel by value, so this
-    // launch sequence is not captured as a graph.
-    const FrameUniforms& U0 = b->reqs.front();
-    const bool single = nReq == 1 && !U0.linear && U0.depth == 2 &&
-                        !(U0.classify == 1 && (U0.dataType == LVR_DT_UINT8 ||
-                                               U0.dataType == LVR_DT_UINT16));
-    // with the single-frame kernel's own default occupancy when the request
-    // left it to the library
-    FrameUniforms U1 = U0;
-    if (single && b->wpsDefault)
-        U1.wavesPerSimd = 5;
-
     auto enqueue = [&](hipStream_t st) -> hipError_t {
         enqueueRecords(st, b->d_reqs, nReq, b->d_keys, int(b->bricks.size()), b->d_bricks);
         enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,
                        b->d_soffsets, b->d_slists, b->d_counts, b->d_offsets, b->d_lists,
-                       b->d_counters, nReq + 2);
+                       b->d_counters, nCounters);
         // two persistent workgroups of wps * 2 wavefronts per CU the stream
         // may use
-        const int slots = std::max(1, 2 * (b->numCUs - b->reservedCUs));
-        const int blocks = std::max(1, std::min((items + 2 * wps - 1) / (2 * wps), slots));
-        if (single)
+        if (items > 0)
         {
-            const int sblocks = std::max(
-                1, std::min((items + 2 * U1.wavesPerSimd - 1) / (2 * U1.wavesPerSimd), slots));
-            const size_t lds = rayLds(U1);
-            switch (U0.dataType)
-            {
-#define LVR_SINGLE(T)                                                                       \
-    launchRaycastAs<T, false, 2, false>(sblocks, lds, st, U1, b->d_bricks, b->d_offsets,     \
-                                        b->d_counts, b->d_lists, b->d_counters)
-            case LVR_DT_UINT8: LVR_SINGLE(uint8_t); break;
-            case LVR_DT_UINT16: LVR_SINGLE(uint16_t); break;
-            case LVR_DT_UINT32: LVR_SINGLE(uint32_t); break;
-            case LVR_DT_INT8: LVR_SINGLE(int8_t); break;
-            case LVR_DT_INT16: LVR_SINGLE(int16_t); break;
-            case LVR_DT_INT32: LVR_SINGLE(int32_t); break;
-            default: LVR_SINGLE(float); break;
-            }
-#undef LVR_SINGLE
-            // the request's composite steps where the batch keeps them
-            return hipMemcpyAsync(b->d_counters + 2, b->d_counters, sizeof(unsigned long long),
-                                  hipMemcpyDeviceToDevice, st);
-        }
-        const dim3 grid(blocks), block(2 * wps * kWave);
+            const int blocks = std::max(1, std::min((items + 2 * wps - 1) / (2 * wps), slots));
+            const dim3 grid(blocks), block(2 * wps * kWave);
 #define LVR_BATCH(a, c, d)                                                                   \
     hipLaunchKernelGGL((batchKernel<a, c, d>), grid, block, 0, st, b->d_reqs, nReq, items,      \
                        b->d_bricks, b->d_offsets, b->d_counts, b->d_lists, b->d_counters)
-        switch (depthCode)
-        {
-        case 0: LVR_BATCH(2, 2, 2); break;
-        case 1: LVR_BATCH(2, 2, 3); break;
-        case 2: LVR_BATCH(2, 3, 2); break;
-        case 3: LVR_BATCH(2, 3, 3); break;
-        case 4: LVR_BATCH(3, 2, 2); break;
-        case 5: LVR_BATCH(3, 2, 3); break;
-        case 6: LVR_BATCH(3, 3, 2); break;
-        default: LVR_BATCH(3, 3, 3); break;
-        }
+            switch (depthCode)
+            {
+            case 0: LVR_BATCH(2, 2, 2); break;
+            case 1: LVR_BATCH(2, 2, 3); break;
+            case 2: LVR_BATCH(2, 3, 2); break;
+            case 3: LVR_BATCH(2, 3, 3); break;
+            case 4: LVR_BATCH(3, 2, 2); break;
+            case 5: LVR_BATCH(3, 2, 3); break;
+            case 6: LVR_BATCH(3, 3, 2); break;
+            default: LVR_BATCH(3, 3, 3); break;
+            }
 #undef LVR_BATCH
+        }
+        // the solo requests through the single-frame kernel (uniforms by
+        // value), each with its own sample counter and tile queue
+        for (int j = 0; j < nSolo; ++j)
+        {
+            const int r = b->solo[size_t(j)];
+            FrameUniforms Us = b->reqs[size_t(r)];
+            Us.wavesPerSimd = soloWps;
+            unsigned long long* c = b->d_counters + nReq + 2 + 2 * j;
+            const int sblocks =
+                std::max(1, std::min((Us.nItems + 2 * soloWps - 1) / (2 * soloWps), slots));
+            const size_t lds = rayLds(Us);
+            switch (Us.dataType)
+            {
+#define LVR_SOLO(T)                                                                         \
+    launchRaycastAs<T, false, 2, false>(sblocks, lds, st, Us, b->d_bricks, b->d_offsets,     \
+                                        b->d_counts, b->d_lists, c)
+            case LVR_DT_UINT8: LVR_SOLO(uint8_t); break;
+            case LVR_DT_UINT16: LVR_SOLO(uint16_t); break;
+            case LVR_DT_UINT32: LVR_SOLO(uint32_t); break;
+            case LVR_DT_INT8: LVR_SOLO(int8_t); break;
+            case LVR_DT_INT16: LVR_SOLO(int16_t); break;
+            case LVR_DT_INT32: LVR_SOLO(int32_t); break;
+            default: LVR_SOLO(float); break;
+            }
+#undef LVR_SOLO
+            // the request's composite steps where the batch keeps them
+            const hipError_t e = hipMemcpyAsync(b->d_counters + 2 + r, c,
+                                                sizeof(unsigned long long),
+                                                hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess)
+                return e;
+        }
         return hipGetLastError();
     };
 
     LVR_BHIP(b, hipEventRecord(b->ev[0], b->stream));
-    if (b->useGraph && !single)
+    if (b->useGraph && nSolo == 0)
     {
         if (!b->exec || key != b->graphKey)
         {
@@ -2140,12 +2179,17 @@ int lvr_batch_wait(lvr_batch* b, lvr_batch_stats* stats)
     if (stats)
     {
         stats->requests = b->reqs.size();
-        stats->samples = b->h_counters[0];
+        // the batch kernel's total, and the solo launches' counts
+        uint64_t samples = b->h_counters[0];
+        for (int r : b->solo)
+            samples += b->h_counters[2 + size_t(r)];
+        stats->samples = samples;
         stats->bricks = b->onScreen;
         stats->tile_entries = b->listTotal;
         stats->gpu_ms = b->kernelMs;
         stats->graph_captures = b->graphCaptures;
         stats->graph_launches = b->graphLaunches;
+        stats->solo_requests = b->solo.size();
     }
     return LVR_OK;
 }

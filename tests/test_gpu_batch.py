@@ -96,6 +96,36 @@ def test_batch_without_graph_and_oracle_tolerance(volumes, orc):
             assert smp == ref_smp
 
 
+@pytest.mark.parametrize("min_tiles", [110, 1, -1])
+def test_solo_requests_in_a_mixed_batch(volumes, min_tiles):
+    """Requests the single-frame window kernel covers (the 16-bit and float
+    ones; the 8-bit ones classify through a table) with at least min_tiles
+    8x8 ray tiles get launches of their own after the shared one: with 110
+    some do and some do not, with 1 all of them, with -1 none (the batch is
+    then captured as a graph). Every frame and sample count equals the frame
+    rendered alone, and the batch total is their sum."""
+    reqs = _requests(volumes, 3)
+    tiles = [((w + 7) // 8) * ((h + 7) // 8) for _s, _c, _f, w, h, _o, _t, _n in reqs]
+    print("tiles", tiles, flush=True)
+    with livre_amd.Batch() as b:
+        b.set_solo_tiles(min_tiles)
+        for s, ctx, fr, w, h, ordered, tf, nsamp in reqs:
+            b.add(ctx, fr, w, h, ordered, livre_amd.render_params(nsamp), tf, s.range)
+        b.run()
+        st = b.wait()
+        assert st.graph_captures == (1 if min_tiles < 0 else 0)
+        eligible = [k for k in range(len(reqs)) if k % 3 != 0]  # 16-bit and float
+        assert st.solo_requests == {110: sum(tiles[k] >= 110 for k in eligible),
+                                    1: len(eligible), -1: 0}[min_tiles]
+        total = 0
+        for i, req in enumerate(reqs):
+            img, smp = b.read(i)
+            ref, ref_smp = _alone(req)
+            assert smp == ref_smp and np.array_equal(img, ref), i
+            total += smp
+        assert st.samples == total
+
+
 @pytest.mark.parametrize("vol,classify", [(1, 0), (2, 0), (0, 2)])
 def test_single_request_batch(volumes, vol, classify):
     """A batch of one request with per-sample classification runs the

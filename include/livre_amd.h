@@ -615,7 +615,10 @@ const char* lvr_server_last_error(const lvr_server* server);
 int lvr_server_set_coalesce(lvr_server* server, int enable);
 /* Serve a volume from ctx's pool. noise or field (at most one) names the
  * procedural source of missing bricks; with neither, the caller keeps the
- * needed bricks resident. The pool's bytes count against the budget. */
+ * needed bricks resident. The pool's bytes count against the budget. The
+ * server queues the selection of a pending request on ctx ahead of its
+ * launch (lvr_select_visibles_async), so between a submit and the return of
+ * lvr_server_drain the caller starts no selection of its own on ctx. */
 int lvr_server_add_volume(lvr_server* server, lvr_ctx* ctx,
                           const lvr_noise_params* noise,
                           const lvr_field_params* field, int* volume);

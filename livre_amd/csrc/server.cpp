@@ -41,6 +41,11 @@ struct Volume
     lvr_noise_params noise;
     lvr_field_params field;
     size_t bytes = 0;
+    // a selection queued ahead on the context for a pending request
+    // (lvr_select_visibles_async), and a finished one held for its request
+    int64_t selQueued = -1;
+    int64_t selReady = -1;
+    std::vector<uint64_t> readyVis;
 };
 
 struct Session
@@ -213,6 +218,61 @@ int retire(lvr_server* s, Flight& fl)
     return LVR_OK;
 }
 
+// The visible-set capacity of a context: every node of its octree.
+uint64_t visCap(lvr_ctx* ctx)
+{
+    return ctx->pt.levelOffset.back();
+}
+
+// Finish a volume's queued selection and hold its result for its request.
+void settle(Volume& v)
+{
+    if (v.selQueued < 0)
+        return;
+    const uint64_t cap = visCap(v.ctx);
+    if (v.readyVis.size() < cap)
+        v.readyVis.resize(cap);
+    int n = 0;
+    v.selReady = lvr_select_visibles_wait(v.ctx, v.readyVis.data(), int(cap), &n) == LVR_OK
+                     ? v.selQueued
+                     : -1;
+    v.readyVis.resize(v.selReady >= 0 ? size_t(n) : 0);
+    v.selQueued = -1;
+}
+
+// Queue the selection of the oldest pending request of every volume whose
+// context has none queued, so that it runs on the GPU while launches are in
+// flight and the request's launch finds its visible set ready. A queued
+// selection whose request is gone (superseded, or its session closed) is
+// finished once the GPU has done it, and its result dropped.
+void preselect(lvr_server* s)
+{
+    for (Volume& v : s->volumes)
+        if (v.selQueued >= 0)
+        {
+            bool live = false;
+            for (const Request& r : s->pending)
+                live = live || r.id == v.selQueued;
+            if (!live && hipEventQuery(v.ctx->selDone) == hipSuccess)
+            {
+                settle(v);
+                v.selReady = -1;
+            }
+        }
+    for (const Request& r : s->pending)
+    {
+        const Session& ss = s->sessions[size_t(r.session)];
+        if (ss.state != 2)
+            continue;
+        Volume& v = s->volumes[size_t(ss.volume)];
+        if (v.selQueued >= 0 || v.selReady == r.id)
+            continue;
+        if (lvr_select_visibles_async(v.ctx, &r.f, uint32_t(ss.height), 1.0f, 0,
+                                      v.ctx->vi.depth - 1) == LVR_OK)
+            v.selQueued = r.id;
+    }
+}
+
 // Visible set, order and residency of one request; pins its bricks into the
 // flight and adds it to the batch. Returns LVR_OK, LVR_ENOMEM when the pool
 // cannot hold the request next to what is pinned (try again later), or an
@@ -222,7 +282,7 @@ int stage(lvr_server* s, Flight& fl, Request& r)
     Session& ss = s->sessions[size_t(r.session)];
     Volume& v = s->volumes[size_t(ss.volume)];
     lvr_ctx* ctx = v.ctx;
-    const uint64_t cap = ctx->pt.levelOffset.back();
+    const uint64_t cap = visCap(ctx);
     if (s->vis.size() < cap)
     {
         s->vis.resize(cap);
@@ -230,8 +290,24 @@ int stage(lvr_server* s, Flight& fl, Request& r)
     }
     int n = 0;
     double t = nowMs();
-    int rc = lvr_select_visibles(ctx, &r.f, uint32_t(ss.height), 1.0f, 0, ctx->vi.depth - 1,
+    int rc = LVR_OK;
+    if (v.selQueued == r.id)
+        settle(v);
+    if (v.selReady == r.id)
+    {
+        // selected ahead (preselect)
+        n = int(v.readyVis.size());
+        std::copy(v.readyVis.begin(), v.readyVis.end(), s->vis.begin());
+        v.selReady = -1;
+    }
+    else
+    {
+        // the context takes one selection at a time: another request's
+        // queued selection is finished and held for it first
+        settle(v);
+        rc = lvr_select_visibles(ctx, &r.f, uint32_t(ss.height), 1.0f, 0, ctx->vi.depth - 1,
                                  s->vis.data(), int(cap), &n);
+    }
     s->stats.select_ms += nowMs() - t;
     if (rc)
         return ctxFail(s, ctx, rc);
@@ -436,7 +512,10 @@ void lvr_server_destroy(lvr_server* s)
     }
     for (Volume& v : s->volumes)
         if (v.ctx)
+        {
+            settle(v); // the context is the caller's: no selection left queued
             v.ctx->pinsGuardReaders = false;
+        }
     for (Session& ss : s->sessions)
         (void)hipFree(ss.d_frames);
     if (s->copyStream)
@@ -572,6 +651,8 @@ int lvr_server_submit(lvr_server* s, int session, const lvr_frustum* f, double a
     s->pending.push_back(r);
     if (request)
         *request = r.id;
+    if (hipSetDevice(s->device) == hipSuccess)
+        preselect(s);
     return LVR_OK;
 }
 
@@ -601,7 +682,10 @@ int lvr_server_step(lvr_server* s, int* launched, int* retired)
                 return rc;
         }
     }
-    return launch(s, launched);
+    const int rc = launch(s, launched);
+    if (rc == LVR_OK)
+        preselect(s);
+    return rc;
 }
 
 int lvr_server_drain(lvr_server* s)
@@ -630,7 +714,12 @@ int lvr_server_drain(lvr_server* s)
         for (const Request& r : s->pending)
             any = any || s->sessions[size_t(r.session)].state == 2;
         if (!any)
+        {
+            // the contexts are the caller's again: no selection left queued
+            for (Volume& v : s->volumes)
+                settle(v);
             return LVR_OK;
+        }
     }
 }
 

@@ -34,6 +34,11 @@
 
 namespace
 {
+// Launches in flight at once: while one is retired and the next staged on
+// the host, the others keep the GPU busy. Each session holds one
+// framebuffer more than this: its finished frame is never one being written.
+constexpr int kFlights = 3;
+
 struct Volume
 {
     lvr_ctx* ctx = nullptr;
@@ -42,10 +47,11 @@ struct Volume
     lvr_field_params field;
     size_t bytes = 0;
     // a selection queued ahead on the context for a pending request
-    // (lvr_select_visibles_async), and a finished one held for its request
+    // (lvr_select_visibles_async), and the finished ones held for their
+    // requests
     int64_t selQueued = -1;
-    int64_t selReady = -1;
-    std::vector<uint64_t> readyVis;
+    std::vector<std::pair<int64_t, std::vector<uint64_t>>> ready;
+    std::vector<uint64_t> selBuf; // what lvr_select_visibles_wait writes
 };
 
 struct Session
@@ -57,7 +63,7 @@ struct Session
     float r0 = 0.f, r1 = 0.f;
     lvr_render_params params{};
     size_t bytes = 0;
-    uint32_t* d_frames = nullptr; // three frames: finished + up to two in flight
+    uint32_t* d_frames = nullptr; // kFlights + 1 frames: finished + those in flight
     int nextBuf = 0, shownBuf = -1;
     int64_t shownRequest = -1; // launches on two streams may finish out of order
     bool hasFrame = false;
@@ -101,7 +107,7 @@ struct lvr_server
     std::vector<Session> sessions;
     std::deque<int> waiting;
     std::deque<Request> pending;
-    Flight flights[2];
+    Flight flights[kFlights];
     hipStream_t copyStream = nullptr;
     std::vector<lvr_server_frame> done;
     std::vector<uint64_t> vis, ord;
@@ -131,7 +137,7 @@ size_t frameBytes(int w, int h)
 size_t sessionBytes(int w, int h)
 {
     // the finished frame + one per batch slot in flight
-    return frameBytes(w, h) * 3u;
+    return frameBytes(w, h) * size_t(kFlights + 1);
 }
 
 // Kept free of the batch kernel for selection and brick-source kernels.
@@ -230,46 +236,58 @@ void settle(Volume& v)
     if (v.selQueued < 0)
         return;
     const uint64_t cap = visCap(v.ctx);
-    if (v.readyVis.size() < cap)
-        v.readyVis.resize(cap);
+    if (v.selBuf.size() < cap)
+        v.selBuf.resize(cap);
     int n = 0;
-    v.selReady = lvr_select_visibles_wait(v.ctx, v.readyVis.data(), int(cap), &n) == LVR_OK
-                     ? v.selQueued
-                     : -1;
-    v.readyVis.resize(v.selReady >= 0 ? size_t(n) : 0);
+    if (lvr_select_visibles_wait(v.ctx, v.selBuf.data(), int(cap), &n) == LVR_OK)
+        v.ready.emplace_back(v.selQueued,
+                             std::vector<uint64_t>(v.selBuf.begin(), v.selBuf.begin() + n));
     v.selQueued = -1;
 }
 
-// Queue the selection of the oldest pending request of every volume whose
-// context has none queued, so that it runs on the GPU while launches are in
-// flight and the request's launch finds its visible set ready. A queued
-// selection whose request is gone (superseded, or its session closed) is
-// finished once the GPU has done it, and its result dropped.
+bool isPending(const lvr_server* s, int64_t id)
+{
+    for (const Request& r : s->pending)
+        if (r.id == id)
+            return true;
+    return false;
+}
+
+// Keep each volume's context selecting ahead: a finished selection is held
+// for its request, and the oldest pending request of the volume without a
+// visible set is queued next (lvr_select_visibles_async), so that it runs
+// on the GPU while launches are in flight and the request's launch finds
+// its visible set ready. Results of requests that are gone (superseded, or
+// their session closed) are dropped.
 void preselect(lvr_server* s)
 {
-    for (Volume& v : s->volumes)
-        if (v.selQueued >= 0)
-        {
-            bool live = false;
-            for (const Request& r : s->pending)
-                live = live || r.id == v.selQueued;
-            if (!live && hipEventQuery(v.ctx->selDone) == hipSuccess)
-            {
-                settle(v);
-                v.selReady = -1;
-            }
-        }
-    for (const Request& r : s->pending)
+    for (size_t vi = 0; vi < s->volumes.size(); ++vi)
     {
-        const Session& ss = s->sessions[size_t(r.session)];
-        if (ss.state != 2)
+        Volume& v = s->volumes[vi];
+        if (v.selQueued >= 0 && hipEventQuery(v.ctx->selDone) == hipSuccess)
+            settle(v);
+        v.ready.erase(std::remove_if(v.ready.begin(), v.ready.end(),
+                                     [s](const std::pair<int64_t, std::vector<uint64_t>>& e) {
+                                         return !isPending(s, e.first);
+                                     }),
+                      v.ready.end());
+        if (v.selQueued >= 0)
             continue;
-        Volume& v = s->volumes[size_t(ss.volume)];
-        if (v.selQueued >= 0 || v.selReady == r.id)
-            continue;
-        if (lvr_select_visibles_async(v.ctx, &r.f, uint32_t(ss.height), 1.0f, 0,
-                                      v.ctx->vi.depth - 1) == LVR_OK)
-            v.selQueued = r.id;
+        for (const Request& r : s->pending)
+        {
+            const Session& ss = s->sessions[size_t(r.session)];
+            if (ss.state != 2 || size_t(ss.volume) != vi)
+                continue;
+            bool have = false;
+            for (const auto& e : v.ready)
+                have = have || e.first == r.id;
+            if (have)
+                continue;
+            if (lvr_select_visibles_async(v.ctx, &r.f, uint32_t(ss.height), 1.0f, 0,
+                                          v.ctx->vi.depth - 1) == LVR_OK)
+                v.selQueued = r.id;
+            break;
+        }
     }
 }
 
@@ -293,12 +311,16 @@ int stage(lvr_server* s, Flight& fl, Request& r)
     int rc = LVR_OK;
     if (v.selQueued == r.id)
         settle(v);
-    if (v.selReady == r.id)
+    auto held = std::find_if(v.ready.begin(), v.ready.end(),
+                             [&r](const std::pair<int64_t, std::vector<uint64_t>>& e) {
+                                 return e.first == r.id;
+                             });
+    if (held != v.ready.end())
     {
         // selected ahead (preselect)
-        n = int(v.readyVis.size());
-        std::copy(v.readyVis.begin(), v.readyVis.end(), s->vis.begin());
-        v.selReady = -1;
+        n = int(held->second.size());
+        std::copy(held->second.begin(), held->second.end(), s->vis.begin());
+        v.ready.erase(held);
     }
     else
     {
@@ -374,9 +396,9 @@ int stage(lvr_server* s, Flight& fl, Request& r)
                        int(ss.tf.size() / 4), ss.r0, ss.r1, &req);
     if (rc)
         return sfail(s, rc, lvr_batch_last_error(fl.batch));
-    // the next of the three framebuffers that is neither shown nor in flight
+    // the next of the kFlights + 1 framebuffers, neither shown nor in flight
     r.buf = ss.nextBuf;
-    ss.nextBuf = (ss.nextBuf + 1) % 3;
+    ss.nextBuf = (ss.nextBuf + 1) % (kFlights + 1);
     rc = lvr_batch_set_output(fl.batch, req, ss.d_frames + size_t(r.buf) * ss.width * ss.height);
     if (rc)
         return sfail(s, rc, lvr_batch_last_error(fl.batch));
@@ -395,7 +417,9 @@ int launch(lvr_server* s, int* launched)
         }
     if (!fl || s->pending.empty())
         return LVR_OK;
-    const bool otherBusy = s->flights[0].busy || s->flights[1].busy;
+    bool otherBusy = false;
+    for (const Flight& f : s->flights)
+        otherBusy = otherBusy || f.busy;
     int rc = lvr_batch_clear(fl->batch);
     if (rc)
         return sfail(s, rc, lvr_batch_last_error(fl->batch));
@@ -814,8 +838,9 @@ int lvr_server_get_stats(lvr_server* s, lvr_server_stats* out)
     out->used_bytes = s->used;
     out->pending = s->pending.size();
     out->waiting_sessions = s->waiting.size();
-    out->in_flight = (s->flights[0].busy ? s->flights[0].reqs.size() : 0) +
-                     (s->flights[1].busy ? s->flights[1].reqs.size() : 0);
+    out->in_flight = 0;
+    for (const Flight& f : s->flights)
+        out->in_flight += f.busy ? f.reqs.size() : 0;
     return LVR_OK;
 }
 }

@@ -279,6 +279,8 @@ def batch_workload(args):
         t_plain = time.perf_counter() - t0
         with livre_amd.Batch() as b0, livre_amd.Batch() as b1:
             pair = (b0, b1)
+            for batch in pair:
+                batch.set_solo_tiles(args.solo_tiles)
             for batch in pair:  # warm-up and graph capture of both
                 batch_pass(batch, items, tuned)
             # two batches in turn: the host assembles pass k+1 while pass k
@@ -1096,6 +1098,10 @@ def main():
                     help="requests per serving launch; one request per launch runs the "
                          "single-frame kernel (two-brick window)")
     ap.add_argument("--pool-frac", type=float, default=1.0)
+    ap.add_argument("--solo-tiles", type=int, default=0,
+                    help="batch workload: requests of at least this many ray tiles run "
+                         "single-frame launches of their own (0: the library's default, "
+                         "negative: only a lone request)")
     ap.add_argument("--pool-gb", type=float, default=200.0)
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--raw-size", type=int, default=1024)

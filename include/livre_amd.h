@@ -533,12 +533,15 @@ int lvr_batch_set_graph(lvr_batch* batch, int enable);
 /* Requests that the single-frame kernel's two-brick window covers (NEAREST,
  * 2 samples per thread, classification per sample) and that hold at least
  * min_tiles ray tiles are launched on their own, one single-frame launch
- * each after the shared launch, instead of in the batch kernel (which
- * marches brick by brick); a lone such request always is. 0 (default) = four
- * tiles per resident wavefront of a launch (2 workgroups of 2 x
- * waves_per_simd wavefronts per CU), negative = only a lone request. A
- * batch with such requests is not captured as a graph. Frames and sample
- * counts do not depend on the split. No reference counterpart. */
+ * each, instead of in the batch kernel (which marches brick by brick); a
+ * lone such request always is. Without reserved CUs these launches are
+ * dealt, largest first, to the batch's stream and up to three more that
+ * start after the binning, so that one launch's tail overlaps the next and
+ * the shared launch. 0 (default) = four tiles per resident wavefront of a
+ * launch (2 workgroups of 2 x waves_per_simd wavefronts per CU), negative =
+ * only a lone request. A batch with such requests is not captured as a
+ * graph. Frames and sample counts do not depend on the split. No reference
+ * counterpart. */
 int lvr_batch_set_solo_tiles(lvr_batch* batch, int min_tiles);
 /* Queue one frame of ctx's volume; request receives its index. The TF is
  * n_tf FP32 RGBA entries over [range0, range1] (as lvr_tf_set). Adding after

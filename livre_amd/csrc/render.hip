@@ -980,6 +980,14 @@ struct lvr_batch
     bool useGraph = true;
     int soloMinTiles = 0;  // lvr_batch_set_solo_tiles
     std::vector<int> solo; // requests of the last run launched on their own
+    // streams beside `stream` on which solo launches run at once, so that
+    // one launch's tail overlaps the next (created on first use; used while
+    // no CUs are reserved), the event that starts them after the binning,
+    // and the ones that join them back
+    static constexpr int kSoloStreams = 3;
+    hipStream_t side[kSoloStreams] = {};
+    hipEvent_t sideDone[kSoloStreams] = {};
+    hipEvent_t prepDone = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<int64_t> graphKey;
     uint64_t graphLaunches = 0, graphCaptures = 0;
@@ -1719,6 +1727,15 @@ void lvr_batch_destroy(lvr_batch* b)
     for (auto& e : b->ev)
         if (e)
             (void)hipEventDestroy(e);
+    for (int i = 0; i < lvr_batch::kSoloStreams; ++i)
+    {
+        if (b->side[i])
+            (void)hipStreamDestroy(b->side[i]);
+        if (b->sideDone[i])
+            (void)hipEventDestroy(b->sideDone[i]);
+    }
+    if (b->prepDone)
+        (void)hipEventDestroy(b->prepDone);
     if (b->stream)
         (void)hipStreamDestroy(b->stream);
     delete b;
@@ -1875,6 +1892,38 @@ int lvr_batch_run(lvr_batch* b)
         }
     }
     const int nSolo = int(b->solo.size());
+    // solo launches on `stream` and up to kSoloStreams more, the largest
+    // first, each to the stream with the fewest tiles so far
+    const int nSide = b->reservedCUs == 0
+                          ? std::min(lvr_batch::kSoloStreams, std::max(0, nSolo - 1))
+                          : 0;
+    for (int i = 0; i < nSide; ++i)
+    {
+        if (!b->side[i])
+            LVR_BHIP(b, hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+        if (!b->sideDone[i])
+            LVR_BHIP(b, hipEventCreateWithFlags(&b->sideDone[i], hipEventDisableTiming));
+    }
+    if (nSide > 0 && !b->prepDone)
+        LVR_BHIP(b, hipEventCreateWithFlags(&b->prepDone, hipEventDisableTiming));
+    std::vector<int> soloOrder(static_cast<size_t>(nSolo)), soloLane(static_cast<size_t>(nSolo), 0);
+    for (int j = 0; j < nSolo; ++j)
+        soloOrder[size_t(j)] = j;
+    auto tilesOf = [&](int j) {
+        const FrameUniforms& U = b->reqs[size_t(b->solo[size_t(j)])];
+        return int64_t(U.tilesX) * U.tilesY;
+    };
+    std::stable_sort(soloOrder.begin(), soloOrder.end(),
+                     [&](int x, int y) { return tilesOf(x) > tilesOf(y); });
+    {
+        std::vector<int64_t> load(size_t(nSide + 1), 0);
+        for (int j : soloOrder)
+        {
+            const int lane = int(std::min_element(load.begin(), load.end()) - load.begin());
+            soloLane[size_t(j)] = lane;
+            load[size_t(lane)] += tilesOf(j);
+        }
+    }
     // counters: total and queue of the batch kernel, one per request, then
     // samples and queue of each solo launch
     const int nCounters = nReq + 2 + 2 * nSolo;
@@ -2007,6 +2056,15 @@ int lvr_batch_run(lvr_batch* b)
         enqueueBinning(st, b->d_reqs, nReq, b->d_ranges, supers, tiles, b->d_scounts,
                        b->d_soffsets, b->d_slists, b->d_counts, b->d_offsets, b->d_lists,
                        b->d_counters, nCounters);
+        // the side streams start after the binning (beside the shared launch)
+        if (nSide > 0)
+        {
+            hipError_t e = hipEventRecord(b->prepDone, st);
+            for (int i = 0; i < nSide && e == hipSuccess; ++i)
+                e = hipStreamWaitEvent(b->side[i], b->prepDone, 0);
+            if (e != hipSuccess)
+                return e;
+        }
         // two persistent workgroups of wps * 2 wavefronts per CU the stream
         // may use
         if (items > 0)
@@ -2030,10 +2088,13 @@ int lvr_batch_run(lvr_batch* b)
 #undef LVR_BATCH
         }
         // the solo requests through the single-frame kernel (uniforms by
-        // value), each with its own sample counter and tile queue
-        for (int j = 0; j < nSolo; ++j)
+        // value), each with its own sample counter and tile queue, on the
+        // streams they were dealt to; the side streams rejoin `stream` at
+        // the end
+        for (int j : soloOrder)
         {
             const int r = b->solo[size_t(j)];
+            const hipStream_t sj = soloLane[size_t(j)] == 0 ? st : b->side[soloLane[size_t(j)] - 1];
             FrameUniforms Us = b->reqs[size_t(r)];
             Us.wavesPerSimd = soloWps;
             unsigned long long* c = b->d_counters + nReq + 2 + 2 * j;
@@ -2043,7 +2104,7 @@ int lvr_batch_run(lvr_batch* b)
             switch (Us.dataType)
             {
 #define LVR_SOLO(T)                                                                         \
-    launchRaycastAs<T, false, 2, false>(sblocks, lds, st, Us, b->d_bricks, b->d_offsets,     \
+    launchRaycastAs<T, false, 2, false>(sblocks, lds, sj, Us, b->d_bricks, b->d_offsets,     \
                                         b->d_counts, b->d_lists, c)
             case LVR_DT_UINT8: LVR_SOLO(uint8_t); break;
             case LVR_DT_UINT16: LVR_SOLO(uint16_t); break;
@@ -2057,7 +2118,15 @@ int lvr_batch_run(lvr_batch* b)
             // the request's composite steps where the batch keeps them
             const hipError_t e = hipMemcpyAsync(b->d_counters + 2 + r, c,
                                                 sizeof(unsigned long long),
-                                                hipMemcpyDeviceToDevice, st);
+                                                hipMemcpyDeviceToDevice, sj);
+            if (e != hipSuccess)
+                return e;
+        }
+        for (int i = 0; i < nSide; ++i)
+        {
+            hipError_t e = hipEventRecord(b->sideDone[i], b->side[i]);
+            if (e == hipSuccess)
+                e = hipStreamWaitEvent(st, b->sideDone[i], 0);
             if (e != hipSuccess)
                 return e;
         }

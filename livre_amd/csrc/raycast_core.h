@@ -77,6 +77,36 @@ __device__ inline uint32_t tiledOffsetDev(const FrameUniforms& U, uint32_t x, ui
     return a;
 }
 
+// The byte offset of a voxel in a slot is a sum of one term per axis
+// (tiledOffsetDev); with TAB the three terms come from per-axis tables in LDS
+// (offsetTables) instead of being computed per sample.
+template <typename T, bool TAB>
+__device__ __forceinline__ uint32_t slotOffset(const FrameUniforms& U, const uint32_t* tab,
+                                               uint32_t ix, uint32_t iy, uint32_t iz)
+{
+    if (TAB)
+        return tab[ix] + tab[uint32_t(U.texDim[0]) + iy] +
+               tab[uint32_t(U.texDim[0] + U.texDim[1]) + iz];
+    return tiledOffsetDev<T>(U, ix, iy, iz);
+}
+
+// The per-axis terms of tiledOffsetDev for every texel index of the brick:
+// texDim[0] + texDim[1] + texDim[2] words, entry i written by the thread
+// with i = first (mod stride).
+template <typename T>
+__device__ inline void offsetTables(const FrameUniforms& U, uint32_t* tab, int first,
+                                    int stride)
+{
+    constexpr uint32_t lb = sizeof(T) == 1 ? 0u : (sizeof(T) == 2 ? 1u : 2u);
+    const int n0 = U.texDim[0], n1 = U.texDim[1], n2 = U.texDim[2];
+    for (int i = first; i < n0 + n1 + n2; i += stride)
+    {
+        const int a = i < n0 ? 0 : (i < n0 + n1 ? 1 : 2);
+        const uint32_t x = uint32_t(i - (a == 0 ? 0 : (a == 1 ? n0 : n0 + n1)));
+        tab[i] = __umul24(x >> 2, U.tileK[a]) + (x << (lb + 2u * uint32_t(a)));
+    }
+}
+
 // Voxel loads through a buffer resource over one pool slot: a 32-bit byte
 // offset per lane instead of a 64-bit address, and the hardware returns 0
 // for any offset past the slot.
@@ -202,17 +232,19 @@ __device__ __forceinline__ void texturePosition(const FrameUniforms& U, const Br
     texturePositionOf<P2>(U, b.bmin, b.rcp, b.ext, texExt, pos, tp);
 }
 
-// Byte offset in the slot of the GL_NEAREST texel of a sample position.
-template <typename T, bool P2>
+// Byte offset in the slot of the GL_NEAREST texel of a sample position
+// (with TAB from the per-axis offset tables `otab`).
+template <typename T, bool P2, bool TAB = false>
 __device__ __forceinline__ uint32_t nearestOffset(const FrameUniforms& U, const BrickEntry& b,
-                                                  const float texExt[3], const float pos[3])
+                                                  const float texExt[3], const float pos[3],
+                                                  const uint32_t* otab = nullptr)
 {
     float tp[3];
     texturePosition<P2>(U, b, texExt, pos, tp);
     const uint32_t ix = nearestTexel(tp[0] * U.texDimF[0], U.texLastF[0]);
     const uint32_t iy = nearestTexel(tp[1] * U.texDimF[1], U.texLastF[1]);
     const uint32_t iz = nearestTexel(tp[2] * U.texDimF[2], U.texLastF[2]);
-    return tiledOffsetDev<T>(U, ix, iy, iz);
+    return slotOffset<T, TAB>(U, otab, ix, iy, iz);
 }
 
 // Classification of one sample value (fragRayCast.glsl:184-204 before the
@@ -316,25 +348,25 @@ __device__ inline int itemToTile(const FrameUniforms& U, int item)
 // on a load. Fetches past the last composited sample (end of segment or early
 // exit) are discarded; their texel indices are clamped into the brick like
 // every other.
-template <typename T, int D, bool CLS, bool P2>
+template <typename T, int D, bool CLS, bool P2, bool TAB = false>
 __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
                                              const BrickEntry& b, const float texExt[3],
                                              float pos[3], const float step[3], float travel,
                                              float stepSize, Composite& C, const float4* tf,
-                                             uint32_t& nsamp)
+                                             uint32_t& nsamp, const uint32_t* otab = nullptr)
 {
     constexpr int S = D > 1 ? D - 1 : 1;
     if (!(travel > 0.0f))
         return;
     T raw[S];
-    raw[0] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
+    raw[0] = loadSlot<T>(vox, nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab));
 #pragma unroll
     for (int i = 1; i < S; ++i)
     {
         pos[0] = pos[0] + step[0];
         pos[1] = pos[1] + step[1];
         pos[2] = pos[2] + step[2];
-        raw[i] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
+        raw[i] = loadSlot<T>(vox, nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab));
     }
     for (bool more = true; more;)
     {
@@ -358,7 +390,7 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                const uint32_t next = nearestOffset<T, P2>(U, b, texExt, pos);
+                const uint32_t next = nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab);
                 // the oldest voxel has been used before its register takes
                 // the next load; the load stays above this sample's blend
                 __builtin_amdgcn_sched_barrier(0);
@@ -377,7 +409,7 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
                 pos[0] = pos[0] + step[0];
                 pos[1] = pos[1] + step[1];
                 pos[2] = pos[2] + step[2];
-                raw[0] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
+                raw[0] = loadSlot<T>(vox, nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab));
             }
         }
     }
@@ -400,17 +432,19 @@ __device__ __forceinline__ void marchNearest(__amdgpu_buffer_rsrc_t vox, const F
 // of the pass. Its extra fetches are clamped into the brick like every other.
 // Per sample this leaves the position step, the texel address and the blend;
 // the table holds exactly what the per-sample classification computes.
-template <typename T, int D, bool P2>
+template <typename T, int D, bool P2, bool TAB = false>
 __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
                                            const BrickEntry& b, const float texExt[3],
                                            float pos[3], const float step[3], float travel,
                                            float stepSize, Composite& C,
-                                           const float4* __restrict__ tab, uint32_t& nsamp)
+                                           const float4* __restrict__ tab, uint32_t& nsamp,
+                                           const uint32_t* otab = nullptr)
 {
     constexpr int S = D + 1;
     if (!(travel > 0.0f))
         return;
-    const uint32_t first = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
+    const uint32_t first =
+        loadSlot<T>(vox, nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab));
     // the voxel of sample m (m >= 1) of the segment is raw[(m - 1) % S]; one
     // full register each (two 16-bit voxels packed in one register would
     // have to wait for both loads to merge them)
@@ -421,7 +455,7 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
         pos[0] = pos[0] + step[0];
         pos[1] = pos[1] + step[1];
         pos[2] = pos[2] + step[2];
-        raw[i] = loadSlot<T>(vox, nearestOffset<T, P2>(U, b, texExt, pos));
+        raw[i] = loadSlot<T>(vox, nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab));
     }
     float4 cls = tab[first];
     bool alive = true;
@@ -446,7 +480,7 @@ __device__ __forceinline__ void marchTable(__amdgpu_buffer_rsrc_t vox, const Fra
             pos[0] = pos[0] + step[0];
             pos[1] = pos[1] + step[1];
             pos[2] = pos[2] + step[2];
-            const uint32_t next = nearestOffset<T, P2>(U, b, texExt, pos);
+            const uint32_t next = nearestOffset<T, P2, TAB>(U, b, texExt, pos, otab);
             __builtin_amdgcn_sched_barrier(0);
             raw[(u + S - 1) % S] = loadSlot<T>(vox, next);
             __builtin_amdgcn_sched_barrier(0);
@@ -603,13 +637,14 @@ __device__ __forceinline__ void storePixel(const FrameUniforms& U, int px, int p
 // transfer function (LDS or global, resolved at the call site), or with CLS
 // the classified table indexed by the raw voxel value. The wavefront marches
 // one brick at a time; castTileWindow below lets lanes run one brick ahead.
-template <typename T, bool LINEAR, int D, bool CLS = false>
+template <typename T, bool LINEAR, int D, bool CLS = false, bool TAB = false>
 __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int lane,
                                          const BrickEntry* __restrict__ bricks,
                                          const int32_t* __restrict__ offsets,
                                          const int32_t* __restrict__ counts,
                                          const int32_t* __restrict__ lists,
-                                         const float4* tf, uint32_t& nsamp)
+                                         const float4* tf, uint32_t& nsamp,
+                                         const uint32_t* otab = nullptr)
 {
     const int px = (tile % U.tilesX) * U.tileW + lane % U.tileW;
     const int py = (tile / U.tilesX) * U.tileH + lane / U.tileW;
@@ -656,22 +691,24 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         if (!LINEAR && CLS)
         {
             if (p2)
-                marchTable<T, D, true>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt,
-                                       pos, step, segLen, stepSize, C, tf, nsamp);
+                marchTable<T, D, true, TAB>(slotResource(vox, uint32_t(U.slotStride)), U, b,
+                                            texExt, pos, step, segLen, stepSize, C, tf, nsamp,
+                                            otab);
             else
-                marchTable<T, D, false>(slotResource(vox, uint32_t(U.slotStride)), U, b, texExt,
-                                        pos, step, segLen, stepSize, C, tf, nsamp);
+                marchTable<T, D, false, TAB>(slotResource(vox, uint32_t(U.slotStride)), U, b,
+                                             texExt, pos, step, segLen, stepSize, C, tf, nsamp,
+                                             otab);
         }
         else if (!LINEAR)
         {
             if (p2)
-                marchNearest<T, D, false, true>(slotResource(vox, uint32_t(U.slotStride)), U, b,
-                                                texExt, pos, step, segLen, stepSize, C, tf,
-                                                nsamp);
+                marchNearest<T, D, false, true, TAB>(slotResource(vox, uint32_t(U.slotStride)),
+                                                     U, b, texExt, pos, step, segLen, stepSize,
+                                                     C, tf, nsamp, otab);
             else
-                marchNearest<T, D, false, false>(slotResource(vox, uint32_t(U.slotStride)), U, b,
-                                                 texExt, pos, step, segLen, stepSize, C, tf,
-                                                 nsamp);
+                marchNearest<T, D, false, false, TAB>(slotResource(vox, uint32_t(U.slotStride)),
+                                                      U, b, texExt, pos, step, segLen, stepSize,
+                                                      C, tf, nsamp, otab);
         }
         else
         {
@@ -732,34 +769,6 @@ __device__ __forceinline__ void laneBrick(const BrickEntry& b, const float rcpG[
         L.rcpP[a] = b.rcp[a];
     }
     L.vox = vox;
-}
-
-// The byte offset of a voxel in a slot is a sum of one term per axis
-// (tiledOffsetDev); with TAB the three terms come from per-axis tables in LDS
-// (offsetTables) instead of being computed per sample.
-template <typename T, bool TAB>
-__device__ __forceinline__ uint32_t slotOffset(const FrameUniforms& U, const uint32_t* tab,
-                                               uint32_t ix, uint32_t iy, uint32_t iz)
-{
-    if (TAB)
-        return tab[ix] + tab[uint32_t(U.texDim[0]) + iy] +
-               tab[uint32_t(U.texDim[0] + U.texDim[1]) + iz];
-    return tiledOffsetDev<T>(U, ix, iy, iz);
-}
-
-// The per-axis terms of tiledOffsetDev for every texel index of the brick:
-// texDim[0] + texDim[1] + texDim[2] words, filled by the workgroup.
-template <typename T>
-__device__ inline void offsetTables(const FrameUniforms& U, uint32_t* tab)
-{
-    constexpr uint32_t lb = sizeof(T) == 1 ? 0u : (sizeof(T) == 2 ? 1u : 2u);
-    const int n0 = U.texDim[0], n1 = U.texDim[1], n2 = U.texDim[2];
-    for (int i = int(threadIdx.x); i < n0 + n1 + n2; i += int(blockDim.x))
-    {
-        const int a = i < n0 ? 0 : (i < n0 + n1 ? 1 : 2);
-        const uint32_t x = uint32_t(i - (a == 0 ? 0 : (a == 1 ? n0 : n0 + n1)));
-        tab[i] = __umul24(x >> 2, U.tileK[a]) + (x << (lb + 2u * uint32_t(a)));
-    }
 }
 
 template <typename T, bool P2, bool TAB>
@@ -977,7 +986,8 @@ __device__ __forceinline__ void castNearestTile(const FrameUniforms& U, int tile
     if (D == 2)
         castTileWindow<T, TAB>(U, tile, lane, bricks, offsets, counts, lists, tf, tab, nsamp);
     else
-        castTile<T, false, D>(U, tile, lane, bricks, offsets, counts, lists, tf, nsamp);
+        castTile<T, false, D, false, TAB>(U, tile, lane, bricks, offsets, counts, lists, tf,
+                                          nsamp, tab);
 }
 
 // The per-brick record of a frame from its key (the device half of frame

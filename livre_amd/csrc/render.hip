@@ -296,16 +296,16 @@ __global__ __launch_bounds__(1024) __attribute__((
     // (16-bit tables stay in global memory)
     extern __shared__ float4 sTf[];
     const float4* table = sTf;
-    // the windowed march reads the slot offsets of its texels from per-axis
-    // tables placed after the transfer function (rayLds)
-    constexpr bool kTab = !LINEAR && !CLS && D == 2;
-    uint32_t* offTab = reinterpret_cast<uint32_t*>(sTf + (U.nTf + 2));
+    // the GL_NEAREST marches read the slot offsets of their texels from
+    // per-axis tables placed after the transfer function or the 8-bit
+    // classified table (rayLds)
+    constexpr bool kTab = !LINEAR;
+    uint32_t* offTab = reinterpret_cast<uint32_t*>(
+        sTf + (!CLS ? U.nTf + 2 : (sizeof(T) == 1 ? 256 : 0)));
     if (!CLS)
     {
         for (int i = threadIdx.x; i < U.nTf + 2; i += blockDim.x)
             sTf[i] = U.tf[i];
-        if (kTab)
-            offsetTables<T>(U, offTab);
     }
     else if (sizeof(T) == 1)
     {
@@ -314,6 +314,8 @@ __global__ __launch_bounds__(1024) __attribute__((
     }
     else
         table = U.cls;
+    if (kTab)
+        offsetTables<T>(U, offTab, int(threadIdx.x), int(blockDim.x));
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     uint32_t nsamp = 0;
@@ -329,21 +331,25 @@ __global__ __launch_bounds__(1024) __attribute__((
             castNearestTile<T, D, kTab>(U, tile, lane, bricks, offsets, counts, lists, table,
                                         nsamp, offTab);
         else
-            castTile<T, LINEAR, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, table,
-                                        nsamp);
+            castTile<T, LINEAR, D, CLS, kTab>(U, tile, lane, bricks, offsets, counts, lists,
+                                              table, nsamp, offTab);
     }
     const unsigned long long s = waveSum(nsamp);
     if (lane == 0 && s)
         atomicAdd(&counters[0], s);
 }
 
-// Dynamic LDS of a single-frame ray cast without a classified table: the
-// transfer function with its edge copies, then the per-axis slot-offset
-// tables of the windowed march.
+// Dynamic LDS of a single-frame ray cast: the transfer function with its
+// edge copies (or, with a classified table, the 8-bit table; a 16-bit one
+// stays in global memory), then the per-axis slot-offset tables of the
+// GL_NEAREST marches.
+inline size_t offsetTablesLds(const FrameUniforms& U)
+{
+    return sizeof(uint32_t) * size_t(U.texDim[0] + U.texDim[1] + U.texDim[2]);
+}
 inline size_t rayLds(const FrameUniforms& U)
 {
-    return size_t(U.nTf + 2) * sizeof(float4) +
-           sizeof(uint32_t) * size_t(U.texDim[0] + U.texDim[1] + U.texDim[2]);
+    return size_t(U.nTf + 2) * sizeof(float4) + offsetTablesLds(U);
 }
 
 // LDS entries per wavefront for a batched request's transfer function: a
@@ -358,6 +364,26 @@ inline int batchTypeClass(int dataType)
 }
 inline int batchDepth(int depth) { return depth >= 3 ? 3 : 2; }
 
+
+// One tile of a batched request, brick by brick: its transfer function (or
+// 8-bit classified table) from this wavefront's LDS slice or, when it did not
+// fit there, from global memory. The two are separate instantiations so that
+// LDS reads stay ds_read. (Per-wavefront slot-offset tables, as the
+// single-frame kernel stages per workgroup, made this kernel 4% slower on the
+// batched uneven-request workload.)
+template <typename T, int D, bool CLS>
+__device__ __forceinline__ void batchCast(const FrameUniforms& U, int tile, int lane,
+                                          const BrickEntry* __restrict__ bricks,
+                                          const int32_t* __restrict__ offsets,
+                                          const int32_t* __restrict__ counts,
+                                          const int32_t* __restrict__ lists, const float4* myTf,
+                                          bool lds, uint32_t& nsamp)
+{
+    if (lds)
+        castTile<T, false, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+    else
+        castTile<T, false, D, CLS>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+}
 
 // Persistent batch kernel: the work items of all frames of a batch form one
 // queue. A wavefront looks up the frame of each item in the request table,
@@ -427,24 +453,19 @@ __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void 
         {
         case LVR_DT_UINT8:
             if (U.classify == 1)
-                castTile<uint8_t, false, D8, true>(U, tile, lane, bricks, offsets, counts, lists,
-                                                  myTf, nsamp);
-            else if (lds)
-                castTile<uint8_t, false, D8>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
+                batchCast<uint8_t, D8, true>(U, tile, lane, bricks, offsets, counts, lists, myTf,
+                                             true, nsamp);
             else
-                castTile<uint8_t, false, D8>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+                batchCast<uint8_t, D8, false>(U, tile, lane, bricks, offsets, counts, lists,
+                                              myTf, lds, nsamp);
             break;
         case LVR_DT_UINT16:
-            if (lds)
-                castTile<uint16_t, false, D16>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
-            else
-                castTile<uint16_t, false, D16>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+            batchCast<uint16_t, D16, false>(U, tile, lane, bricks, offsets, counts, lists, myTf,
+                                            lds, nsamp);
             break;
         default:
-            if (lds)
-                castTile<float, false, DF>(U, tile, lane, bricks, offsets, counts, lists, myTf, nsamp);
-            else
-                castTile<float, false, DF>(U, tile, lane, bricks, offsets, counts, lists, U.tf, nsamp);
+            batchCast<float, DF, false>(U, tile, lane, bricks, offsets, counts, lists, myTf, lds,
+                                        nsamp);
             break;
         }
     }
@@ -1271,7 +1292,7 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
         if (rc)
             return rc;
         U.cls = ctx->d_cls;
-        lds = dt == LVR_DT_UINT8 ? 256 * sizeof(float4) : 0;
+        lds = (dt == LVR_DT_UINT8 ? 256 * sizeof(float4) : 0) + offsetTablesLds(U);
         if (dt == LVR_DT_UINT8)
             launchNearest<uint8_t, true>(blocks, lds, ctx->stream, U, ctx->d_bricks,
                                          ctx->d_tileOffset, ctx->d_tileCount, ctx->d_tileList,
@@ -1434,7 +1455,7 @@ int lvr_render_noalloc(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
         Classify c = makeClassifyHost(U);
         hipLaunchKernelGGL(classifyKernel, dim3((nValues + 255) / 256), dim3(256), 0,
                            ctx->stream, c, U.tf, nValues, dCls);
-        lds = dt == LVR_DT_UINT8 ? 256 * sizeof(float4) : 0;
+        lds = (dt == LVR_DT_UINT8 ? 256 * sizeof(float4) : 0) + offsetTablesLds(U);
         if (dt == LVR_DT_UINT8)
             launchNearest<uint8_t, true>(blocks, lds, ctx->stream, U, dBricks, dTileOffset,
                                          dTileCount, dTileList, dCounters);

@@ -246,6 +246,9 @@ struct FrameUniforms
     uint32_t* frame;   // RGBA8 output, row 0 = bottom
     float4* layer;     // or, when set, the unconverted RGBA32F accumulation
                        // (a sort-last partial), row 0 = bottom
+    // the order in which the work queue hands out the kSuper x kSuper
+    // super-tiles of work items, or null for screen order
+    const int32_t* superOrder;
 };
 
 // Pinned staging of the per-frame host records (brick entries, screen
@@ -360,6 +363,8 @@ struct lvr_ctx
         size_t tileCountCap = 0, tileOffsetCap = 0, tileListCap = 0;
         int32_t *superCount = nullptr, *superOffset = nullptr, *superList = nullptr;
         size_t superCountCap = 0, superOffsetCap = 0, superListCap = 0;
+        int32_t* superOrder = nullptr; // work super-tiles: the heaviest first
+        size_t superOrderCap = 0;
         unsigned long long* counters = nullptr; // [0] samples, [1] work queue
         size_t countersCap = 0;
         hipEvent_t ready = nullptr; // binned (recStream)

@@ -589,7 +589,7 @@ void lvr_ctx_destroy(lvr_ctx* ctx)
     {
         void* bufs[] = {rs.req,        rs.bricks,     rs.keys,      rs.ranges,    rs.tileCount,
                         rs.tileOffset, rs.tileList,   rs.superCount, rs.superOffset,
-                        rs.superList,  rs.counters};
+                        rs.superList,  rs.counters,   rs.superOrder};
         for (void* p : bufs)
             (void)hipFree(p);
         if (rs.ready)

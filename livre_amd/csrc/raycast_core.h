@@ -329,7 +329,10 @@ struct Composite
 __device__ inline int itemToTile(const FrameUniforms& U, int item)
 {
     const int superX = (U.tilesX + kSuper - 1) / kSuper;
-    const int s = item / (kSuper * kSuper), w = item % (kSuper * kSuper);
+    int s = item / (kSuper * kSuper);
+    const int w = item % (kSuper * kSuper);
+    if (U.superOrder)
+        s = U.superOrder[s];
     const int tx = (s % superX) * kSuper + w % kSuper;
     const int ty = (s / superX) * kSuper + w / kSuper;
     return (tx < U.tilesX && ty < U.tilesY) ? ty * U.tilesX + tx : -1;

@@ -228,6 +228,86 @@ static void enqueueBinning(hipStream_t st, const FrameUniforms* reqs, int nReq,
                        tiles, sOffset, sCount, sList, tOffset, tList);
 }
 
+// The order of a frame's work super-tiles (kSuper x kSuper ray tiles) for
+// its work queue: the heaviest quarter first, then the rest, each part in
+// screen order. The weight of a super-tile is the (tile, brick) pairs binned
+// to its tiles; a persistent launch then ends on light tiles, and tiles
+// marched at the same time stay neighbours that share voxel lines in L2. The
+// order changes when a tile is marched, never what it renders.
+// One workgroup: a 1024-bucket weight histogram gives the threshold, then a
+// stable partition by a block-wide scan.
+__global__ __launch_bounds__(1024) void superOrderKernel(const int32_t* __restrict__ tCount,
+                                                         int tilesX, int tilesY,
+                                                         int32_t* __restrict__ order)
+{
+    constexpr int kBuckets = 1024;
+    __shared__ int32_t hist[kBuckets];
+    __shared__ int32_t waveSums[1024 / kWave];
+    __shared__ int32_t shared[2];
+    const int superX = (tilesX + kSuper - 1) / kSuper;
+    const int nSupers = superX * ((tilesY + kSuper - 1) / kSuper);
+    const int t = int(threadIdx.x), lane = t & (kWave - 1), wave = t / kWave;
+    auto weight = [&](int sIdx) {
+        const int tx0 = (sIdx % superX) * kSuper, ty0 = (sIdx / superX) * kSuper;
+        int w = 0;
+        for (int ty = ty0; ty < min(ty0 + kSuper, tilesY); ++ty)
+            for (int tx = tx0; tx < min(tx0 + kSuper, tilesX); ++tx)
+                w += tCount[ty * tilesX + tx];
+        return min(w, kBuckets - 1);
+    };
+    hist[t] = 0;
+    __syncthreads();
+    for (int sIdx = t; sIdx < nSupers; sIdx += 1024)
+        atomicAdd(&hist[weight(sIdx)], 1);
+    __syncthreads();
+    if (t == 0)
+    {
+        // the lowest weight whose super-tiles and all heavier ones make up
+        // at least a quarter
+        int acc = 0, th = kBuckets - 1;
+        for (; th > 0; --th)
+        {
+            acc += hist[th];
+            if (4 * acc >= nSupers)
+                break;
+        }
+        shared[0] = th;
+        shared[1] = 0; // heavy super-tiles placed so far
+    }
+    __syncthreads();
+    const int th = shared[0];
+    // heavy ones first, in screen order: a block-wide exclusive scan per
+    // pass of 1024 super-tiles, carried across passes
+    for (int pass = 0; pass < 2; ++pass)
+    {
+        int base = pass == 0 ? 0 : shared[1];
+        for (int s0 = 0; s0 < nSupers; s0 += 1024)
+        {
+            const int sIdx = s0 + t;
+            const bool heavy = sIdx < nSupers && weight(sIdx) >= th;
+            const bool mine = sIdx < nSupers && (pass == 0 ? heavy : !heavy);
+            const unsigned long long m = __ballot(mine);
+            const int before = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0)
+                waveSums[wave] = __popcll(m);
+            __syncthreads();
+            int off = 0;
+            for (int w = 0; w < wave; ++w)
+                off += waveSums[w];
+            int total = 0;
+            for (int w = 0; w < 1024 / kWave; ++w)
+                total += waveSums[w];
+            if (mine)
+                order[base + off + before] = sIdx;
+            base += total;
+            __syncthreads();
+        }
+        if (t == 0 && pass == 0)
+            shared[1] = base;
+        __syncthreads();
+    }
+}
+
 __device__ inline int findByBrick(const FrameUniforms* __restrict__ reqs, int nReq, int brick)
 {
     int lo = 0, hi = nReq - 1;
@@ -1209,8 +1289,11 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
         (rc = growBuffer(ctx, &rs.superOffset, &rs.superOffsetCap, size_t(nSupers))) ||
         (rc = growBuffer(ctx, &rs.superList, &rs.superListCap,
                          size_t(std::max<uint64_t>(superTotal, 1)))) ||
-        (rc = growBuffer(ctx, &rs.counters, &rs.countersCap, 4)))
+        (rc = growBuffer(ctx, &rs.counters, &rs.countersCap, 4)) ||
+        (rc = growBuffer(ctx, &rs.superOrder, &rs.superOrderCap,
+                         size_t(U.nItems / (kSuper * kSuper)))))
         return rc;
+    U.superOrder = rs.superOrder;
     ctx->recNext ^= 1;
     ctx->d_bricks = rs.bricks;
     ctx->d_ranges = rs.ranges;
@@ -1268,6 +1351,8 @@ static int renderFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4]
     enqueueBinning(ctx->recStream, ctx->d_req, 1, ctx->d_ranges, nSupers, nTiles,
                    ctx->d_superCount, ctx->d_superOffset, ctx->d_superList, ctx->d_tileCount,
                    ctx->d_tileOffset, ctx->d_tileList, ctx->curCounters, 2);
+    hipLaunchKernelGGL(superOrderKernel, dim3(1), dim3(1024), 0, ctx->recStream,
+                       ctx->d_tileCount, U.tilesX, U.tilesY, rs.superOrder);
     LVR_HIP(ctx, hipGetLastError());
     if (rec.timed)
         LVR_HIP(ctx, hipEventRecord(rec.ev[5], ctx->recStream));

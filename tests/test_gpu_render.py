@@ -84,6 +84,35 @@ def test_tile_shapes_agree(blobs_u8):
         assert np.array_equal(img, imgs[0][0]) and smp == imgs[0][1]
 
 
+def test_degenerate_inputs(blobs_u8):
+    """An empty brick list renders the background (all zero, no composite
+    steps); one-pixel-wide and one-pixel-high viewports (a partial ray tile
+    at every shape) match the oracle; invalid viewports, launch shapes and
+    more than 2^20 samples per ray are rejected before anything is queued,
+    and the context renders on afterwards."""
+    s, ctx = blobs_u8
+    tf = scenes.piecewise_tf(256, seed=11)
+    ctx.set_tf(tf, s.range)
+    mv, proj, fr = s.camera((1.3, 0.7, 1.2), 48, 40)
+    ctx.render(fr, 48, 40, [], livre_amd.render_params(64))
+    img, smp = ctx.read_frame()
+    assert smp == 0 and img.shape == (40, 48, 4) and not img.any()
+    for w, h in ((1, 1), (1, 45), (45, 1)):
+        _check_frame(s, ctx, (1.3, 0.7, 1.2), w, h, tf, 64)
+    ordered = s.visible_ordered(mv, proj, 40)
+    bad = [((0, 40), {}), ((48, 0), {}), ((16385, 40), {}),
+           ((48, 40), dict(tile=(7, 9))), ((48, 40), dict(samples_per_thread=4)),
+           ((48, 40), dict(waves_per_simd=9)), ((48, 40), dict(classify=3))]
+    for (w, h), kw in bad:
+        with pytest.raises(livre_amd.LivreError):
+            ctx.render(fr, w, h, ordered, livre_amd.render_params(64, **kw))
+    with pytest.raises(livre_amd.LivreError):
+        ctx.render(fr, 48, 40, ordered, livre_amd.render_params((1 << 20) + 1))
+    ctx.render(fr, 48, 40, ordered, livre_amd.render_params(1 << 20))  # the bound itself
+    assert ctx.read_frame()[1] > 0
+    _check_frame(s, ctx, (1.3, 0.7, 1.2), 48, 40, tf, 64)
+
+
 def test_frame_stats_of_frames_in_flight(blobs_u8):
     """Frames queued back to back keep their own statistics: the stats read
     `back` frames later equal those of the same frame rendered alone."""

@@ -253,7 +253,8 @@ __device__ __forceinline__ uint32_t nearestOffset(const FrameUniforms& U, const 
 // filtered colour and the corrected alpha the blend weighs it with.
 struct Classify
 {
-    float mult, add, nTfF, nTfMax, alphaCorrection, maxA;
+    float mult, add, nTfF, alphaCorrection, maxA;
+    int32_t nTfLast;
 
     __device__ __forceinline__ float4 operator()(const float4* tf, float value) const
     {
@@ -265,9 +266,17 @@ struct Classify
         // both with one index: below the range both reads give entry 0,
         // above it both give entry nTf - 1, exactly as clamping each index.
         const float u = density * nTfF - 0.5f;
-        const float fl = floorf(u);
-        const float w = u - fl;
-        const int i = int(__builtin_amdgcn_fmed3f(fl, -1.0f, nTfMax));
+        // floor and conversion in one instruction (saturating, so the
+        // integer clamp keeps every input, NaN and infinities included,
+        // inside the table); the weight u - floor(u) as v_fract_f32, which
+        // equals it except for u in (-2^-25, 0), where it returns the largest
+        // float below 1 instead of 1, and there both entries are the first
+        // one. For infinities and NaN both give NaN.
+        int fl;
+        asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(fl) : "v"(u));
+        int i;
+        asm("v_med3_i32 %0, %1, -1, %2" : "=v"(i) : "v"(fl), "s"(nTfLast));
+        const float w = __builtin_amdgcn_fractf(u);
         const float4 ca = tf[i + 1];
         const float4 cb = tf[i + 2];
         // the linear filter as a + w (b - a), one FMA per channel (a GLSL
@@ -292,9 +301,9 @@ __device__ __forceinline__ Classify makeClassify(const FrameUniforms& U)
     c.mult = U.mult;
     c.add = U.add;
     c.nTfF = U.nTfF;
-    c.nTfMax = U.nTfLastF;
     c.alphaCorrection = U.alphaCorrection;
     c.maxA = 1.0f - 1.0f / 256.0f;
+    c.nTfLast = U.nTf - 1;
     return c;
 }
 

@@ -361,9 +361,12 @@ __device__ inline int nextItem(unsigned long long* counter, int lane)
 // to the end. Up to 8 wavefronts per SIMD fit in <= 64 VGPRs; fewer keep
 // fewer voxel lines in flight per CU, which the 32 KB L1 holds better.
 // The windowed march (castTileWindow) keeps two bricks' segments per lane and
-// needs about 80 VGPRs, at most 6 wavefronts per SIMD; the others fit 8.
+// is held to 80 VGPRs, 6 wavefronts per SIMD: a workgroup of 2 * 5
+// wavefronts spreads 3, 3, 2, 2 over the SIMDs, so two of them need room
+// for 6 on a SIMD (at 81 VGPRs, allocated as 88, only one fits per CU and
+// the kernel runs 25% slower). The others fit 8.
 template <bool LINEAR, int D, bool CLS>
-constexpr int kRaycastWavesPerEu = (!LINEAR && !CLS && D == 2) ? 4 : 8;
+constexpr int kRaycastWavesPerEu = (!LINEAR && !CLS && D == 2) ? 6 : 8;
 
 template <typename T, bool LINEAR, int D, bool CLS>
 __global__ __launch_bounds__(1024) __attribute__((
@@ -629,9 +632,9 @@ inline Classify makeClassifyHost(const FrameUniforms& U)
     c.mult = U.mult;
     c.add = U.add;
     c.nTfF = U.nTfF;
-    c.nTfMax = U.nTfLastF;
     c.alphaCorrection = U.alphaCorrection;
     c.maxA = 1.0f - 1.0f / 256.0f;
+    c.nTfLast = U.nTf - 1;
     return c;
 }
 

@@ -1219,8 +1219,11 @@ def main():
     # HBM bytes per algorithmic byte, measured over the same sampled path
     ratio = profile_value(HBM_PROFILE, "traffic_per_algorithmic_byte")
     traffic = ratio * float(np.mean(alg_bytes)) if ratio is not None else None
-    valu_frac = profile_value(COUNTER_PROFILE, "valu_issue_fraction")
+    valu_busy = profile_value(COUNTER_PROFILE, "valu_busy")
     lanes = profile_value(COUNTER_PROFILE, "active_lanes_per_valu")
+    valu_ceiling = profile_value(COUNTER_PROFILE, "valu_issue_ceiling_gsamples_per_s")
+    l1_lines = profile_value(COUNTER_PROFILE, "l1_lines_per_voxel_load")
+    ta_busy = profile_value(COUNTER_PROFILE, "texture_addresser_busy")
     # samples per second if every algorithmic byte moved at the HBM peak
     ceiling = samples / (float(np.sum(alg_bytes)) / HBM_PEAK) / 1e9
     result = {
@@ -1279,11 +1282,15 @@ def main():
                      "kernel": "raycastKernel<unsigned short, NEAREST, 2 fetches in flight>",
                      "bytes_per_launch": "rendered bricks x 574992 B + 1920x1080x4 + "
                                          "4096x16",
-                     "limiter": "VALU issue: %s of the SIMDs' VALU issue cycles, %s of 64 "
-                                "lanes active (profiles/%s)"
-                                % ("%.2f" % valu_frac if valu_frac is not None else "?",
-                                   "%.1f" % lanes if lanes is not None else "?",
-                                   COUNTER_PROFILE)},
+                     "limiter": "VALU issue with the vector memory path beside it: VALU "
+                                "busy %s of cycles at %s of 64 lanes active (a VALU-issue "
+                                "ceiling of %s Gsamples/s at the measured clock); each "
+                                "voxel load touches %s L1 lines, texture addresser busy "
+                                "%s (profiles/%s)"
+                                % tuple("%.2f" % x if x is not None and x < 1 else
+                                        ("%.0f" % x if x is not None else "?")
+                                        for x in (valu_busy, lanes, valu_ceiling, l1_lines,
+                                                  ta_busy)) + (COUNTER_PROFILE,)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:

@@ -32,8 +32,11 @@ def test_schedule_of_twenty_steps():
 def test_profile_values_present():
     """The committed summaries carry the lines the bench line reads."""
     for name, key in ((bench.HBM_PROFILE, "traffic_per_algorithmic_byte"),
-                      (bench.COUNTER_PROFILE, "valu_issue_fraction"),
+                      (bench.COUNTER_PROFILE, "valu_busy"),
                       (bench.COUNTER_PROFILE, "active_lanes_per_valu"),
+                      (bench.COUNTER_PROFILE, "valu_issue_ceiling_gsamples_per_s"),
+                      (bench.COUNTER_PROFILE, "l1_lines_per_voxel_load"),
+                      (bench.COUNTER_PROFILE, "texture_addresser_busy"),
                       (bench.COUNTER_PROFILE, "l2_hit_rate"),
                       ("composite_8layer_1080p_hbm.txt", "traffic_bytes_per_launch")):
         v = bench.profile_value(name, key)

@@ -238,6 +238,43 @@ def test_data_range_narrower_than_the_volume(lvr, orc, dt, linear):
         ctx.close()
 
 
+def test_non_finite_voxels_stay_inside_the_table(lvr):
+    """Float voxels that are NaN or infinite make the TF coordinate NaN or
+    infinite. The TF index is a saturating floor-convert clamped in integers
+    to [-1, nTF - 1], so every march reads inside the table: the brick-by-brick
+    marches at 1 and 3 fetches in flight and the two-brick window at 2, with
+    two ray-tile shapes, all render the same frame and composite the same
+    number of samples. (The reference shader leaves non-finite texels to the
+    GL implementation, so there is no oracle frame to compare with.)"""
+    s = Scene(livre_amd.DT_FLOAT, n=64, block=16, overlap=1, source="noise", seed=15)
+    rng = np.random.default_rng(15)
+    for i in list(s.bricks):
+        b = np.array(s.bricks[i], dtype=np.float32, copy=True)
+        flat = b.reshape(-1)
+        idx = rng.choice(flat.size, 60, replace=False)
+        flat[idx[:20]] = np.nan
+        flat[idx[20:40]] = np.inf
+        flat[idx[40:]] = -np.inf
+        s.bricks[i] = b
+    ctx = s.context()
+    try:
+        ctx.set_tf(scenes.piecewise_tf(256, seed=16, max_alpha=0.4), s.range)
+        mv, proj, fr = s.camera((1.5, 0.6, 1.2), 64, 48)
+        ordered = s.visible_ordered(mv, proj, 48)
+        frames = []
+        for depth in (1, 2, 3):
+            for tile in ((8, 8), (16, 4)):
+                ctx.render(fr, 64, 48, ordered,
+                           livre_amd.render_params(64, samples_per_thread=depth, tile=tile))
+                frames.append(ctx.read_frame())
+        img0, smp0 = frames[0]
+        assert smp0 > 0
+        for img, smp in frames[1:]:
+            assert smp == smp0 and np.array_equal(img, img0)
+    finally:
+        ctx.close()
+
+
 def test_trilinear_against_fp32_oracle(lvr, orc):
     s = Scene(livre_amd.DT_UINT16, n=64, block=16, overlap=1, source="noise", seed=13)
     ctx = s.context()

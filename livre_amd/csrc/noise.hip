@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <thread>
 #include <vector>
 
@@ -513,9 +514,12 @@ __attribute__((always_inline)) inline void blobNoiseBrickHostBody(
 #define LVR_HOST_AVX2 __attribute__((target("avx2")))
 #endif
 
+// LIVRE_AMD_HOST_BASELINE=1 selects the baseline build on any CPU (tests
+// compare the two).
 inline bool hostHasAvx2()
 {
-    static const bool has = __builtin_cpu_supports("avx2");
+    static const bool has =
+        __builtin_cpu_supports("avx2") && !std::getenv("LIVRE_AMD_HOST_BASELINE");
     return has;
 }
 

@@ -300,3 +300,34 @@ def test_samples_per_ray_rule_host(lvr):
     assert livre_amd.samples_per_ray(small, [nid(top, 0, 0, 0)]) == 512
     with pytest.raises(livre_amd.LivreError):
         livre_amd.samples_per_ray(vi, [nid(6, 0, 0, 0)])
+
+
+def test_host_source_builds_agree(lvr):
+    """The host brick source is built for the x86-64 baseline and for AVX2,
+    picked at run time; LIVRE_AMD_HOST_BASELINE=1 forces the baseline. Both
+    give the same bricks (value noise and blobs plus value noise, three
+    voxel types, several levels)."""
+    import subprocess
+    import sys
+    code = r'''
+import sys, numpy as np
+sys.path.insert(0, %r)
+import livre_amd as lvr
+from livre_amd import scenes
+out = []
+for dt in (lvr.DT_UINT8, lvr.DT_UINT16, lvr.DT_FLOAT):
+    vi = lvr.volume_info(512, 64, 1, dt)
+    p = lvr.noise_params(7, octaves=5, base_cells=8, gain=1.5)
+    bn = lvr.BlobNoise(vi, p, 3, count=16, cells=8, weight=128)
+    for nid in (0, scenes.node_id(2, 1, 3, 2), scenes.node_id(3, 5, 0, 7)):
+        out.append(lvr.noise_brick_host(vi, p, nid).tobytes())
+        out.append(bn.brick_host(nid).tobytes())
+sys.stdout.buffer.write(b"".join(out))
+''' % ROOT
+    runs = []
+    for env in ({}, {"LIVRE_AMD_HOST_BASELINE": "1"}):
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, timeout=300,
+                           env=dict(os.environ, **env))
+        assert r.returncode == 0, r.stderr.decode()[-2000:]
+        runs.append(r.stdout)
+    assert len(runs[0]) > 0 and runs[0] == runs[1]

@@ -55,6 +55,18 @@ HBM_PROFILE = "raycast_2048u16_1080p_hbm.txt"
 COUNTER_PROFILE = "raycast_2048u16_1080p_counters.txt"
 
 
+def roofline_limiter():
+    """What bounds the headline ray cast, from the committed counter summary."""
+    vals = [profile_value(COUNTER_PROFILE, k) for k in (
+        "valu_busy", "active_lanes_per_valu", "valu_issue_ceiling_gsamples_per_s",
+        "l1_lines_per_voxel_load", "texture_addresser_busy")]
+    txt = ["?" if x is None else ("%.2f" % x if x < 1 else "%.1f" % x) for x in vals]
+    return ("VALU issue with the vector memory path beside it: VALU busy %s of cycles at %s "
+            "of 64 lanes active (a VALU-issue ceiling of %s Gsamples/s at the measured "
+            "clock); each voxel load touches %s L1 lines, texture addresser busy %s "
+            "(profiles/%s)" % tuple(txt + [COUNTER_PROFILE]))
+
+
 def schedule(steps):
     """Frame indices of the timed steps: the 120-frame orbit-zoom path
     sampled evenly (stride 120 / steps), so any --steps covers the whole
@@ -1102,6 +1114,10 @@ def main():
                     help="batch workload: requests of at least this many ray tiles run "
                          "single-frame launches of their own (0: the library's default, "
                          "negative: only a lone request)")
+    ap.add_argument("--linear", action="store_true",
+                    help="render workload with GL_LINEAR (trilinear) sampling instead of "
+                         "the GL_NEAREST parity filter; no serving or batching legs, no "
+                         "CPU baseline")
     ap.add_argument("--pool-gb", type=float, default=200.0)
     ap.add_argument("--frames", type=int, default=300)
     ap.add_argument("--raw-size", type=int, default=1024)
@@ -1145,7 +1161,10 @@ def main():
     from livre_amd import scenes
     livre_amd.lib()
     vi, ctx, _ids, tf = build_scene(livre_amd, scenes)
-    params = livre_amd.render_params(N_SAMPLES, early_exit=0.999)
+    params = livre_amd.render_params(N_SAMPLES, early_exit=0.999, linear=args.linear)
+    if args.linear:
+        args.no_extra_legs = True
+        args.no_cpu_baseline = True
     ctx.set_timing(True)
 
     frames = schedule(args.steps)
@@ -1219,11 +1238,6 @@ def main():
     # HBM bytes per algorithmic byte, measured over the same sampled path
     ratio = profile_value(HBM_PROFILE, "traffic_per_algorithmic_byte")
     traffic = ratio * float(np.mean(alg_bytes)) if ratio is not None else None
-    valu_busy = profile_value(COUNTER_PROFILE, "valu_busy")
-    lanes = profile_value(COUNTER_PROFILE, "active_lanes_per_valu")
-    valu_ceiling = profile_value(COUNTER_PROFILE, "valu_issue_ceiling_gsamples_per_s")
-    l1_lines = profile_value(COUNTER_PROFILE, "l1_lines_per_voxel_load")
-    ta_busy = profile_value(COUNTER_PROFILE, "texture_addresser_busy")
     # samples per second if every algorithmic byte moved at the HBM peak
     ceiling = samples / (float(np.sum(alg_bytes)) / HBM_PEAK) / 1e9
     result = {
@@ -1243,10 +1257,11 @@ def main():
         "config": {"workload": "2048^3 uint16, 64^3 bricks + 1 halo, depth 6, all 37449 "
                                "bricks resident; 1920x1080, %d frames sampled evenly from "
                                "the 120-frame orbit-zoom (frames %s), 2048 samples/unit, "
-                               "FP32 4096-entry TF, sse 1, 65536-bin histogram"
+                               "FP32 4096-entry TF, sse 1, 65536-bin histogram%s"
                                % (args.steps, "0..119" if args.steps >= FRAMES else
                                   ", ".join(map(str, frames[:3])) +
-                                  (", ..., %d" % frames[-1] if len(frames) > 3 else "")),
+                                  (", ..., %d" % frames[-1] if len(frames) > 3 else ""),
+                                  "; GL_LINEAR (trilinear) sampling" if args.linear else ""),
                    "frame": "select (device) + order + bin + raycast + histogram; the "
                             "next frame is ordered and queued while the current one "
                             "ray-casts, its selection queued one frame ahead"},
@@ -1279,20 +1294,22 @@ def main():
                                        + HBM_PROFILE + " (rocprofv3 FETCH_SIZE and "
                                        "WRITE_SIZE per dispatch over the same sampled path, "
                                        "each scaled by the same counter on a 1 GiB copy)",
-                     "kernel": "raycastKernel<unsigned short, NEAREST, 2 fetches in flight>",
+                     "kernel": ("raycastKernel<unsigned short, LINEAR, the next sample's 8 "
+                                "texels in flight>" if args.linear else
+                                "raycastKernel<unsigned short, NEAREST, 2 fetches in flight>"),
                      "bytes_per_launch": "rendered bricks x 574992 B + 1920x1080x4 + "
                                          "4096x16",
-                     "limiter": "VALU issue with the vector memory path beside it: VALU "
-                                "busy %s of cycles at %s of 64 lanes active (a VALU-issue "
-                                "ceiling of %s Gsamples/s at the measured clock); each "
-                                "voxel load touches %s L1 lines, texture addresser busy "
-                                "%s (profiles/%s)"
-                                % tuple("%.2f" % x if x is not None and x < 1 else
-                                        ("%.0f" % x if x is not None else "?")
-                                        for x in (valu_busy, lanes, valu_ceiling, l1_lines,
-                                                  ta_busy)) + (COUNTER_PROFILE,)},
+                     "limiter": roofline_limiter()},
         "cpu_baseline": None,
     }
+    if args.linear:
+        # the committed counter and traffic summaries are of the GL_NEAREST
+        # kernel; none is claimed for this variant
+        result["roofline"].update(traffic=None, traffic_source=None,
+                                  limiter="not profiled (the counter summaries under "
+                                          "profiles/ are of the GL_NEAREST kernel)")
+        result["hbm_gb_per_s"] = None
+        result["l2_hit_rate"] = None
     if not args.no_cpu_baseline:
         try:
             result["cpu_baseline"] = cpu_baseline(livre_amd, scenes, vi, ctx, tf,

@@ -623,6 +623,122 @@ __device__ __forceinline__ bool brickSegment(const FrameUniforms& U, const Pixel
     return travel > 0.0f; // false for an empty segment (and a NaN one)
 }
 
+// One axis's term of a voxel's byte offset in its slot (tiledOffsetDev is
+// the sum of the three): from the per-axis LDS table (offsetTables), or
+// computed as offsetTables computes it.
+template <typename T, bool TAB>
+__device__ __forceinline__ uint32_t axisOffset(const FrameUniforms& U, const uint32_t* tab,
+                                               int axis, uint32_t base, uint32_t x)
+{
+    constexpr uint32_t lb = sizeof(T) == 1 ? 0u : (sizeof(T) == 2 ? 1u : 2u);
+    if (TAB)
+        return tab[base + x];
+    return __umul24(x >> 2, U.tileK[axis]) + (x << (lb + 2u * uint32_t(axis)));
+}
+
+// The eight texels of a GL_LINEAR sample and its three blend weights:
+// sampleBrick<T, true>'s indices (u = tp N - 0.5, floor(u) and floor(u) + 1
+// clamped to the brick) and weights (u - floor(u)), the texels read through
+// the slot's buffer resource. v[x + 2 y + 4 z] is the texel at (i_x, j_y, k_z).
+template <typename T>
+struct LinearTexels
+{
+    T v[8];
+    float w[3];
+};
+
+template <typename T, bool P2, bool TAB>
+__device__ __forceinline__ void linearTexels(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
+                                             const BrickEntry& b, const float texExt[3],
+                                             const float pos[3], const uint32_t* tab,
+                                             LinearTexels<T>& t)
+{
+    float tp[3];
+    texturePosition<P2>(U, b, texExt, pos, tp);
+    uint32_t o0[3], o1[3];
+    uint32_t base = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+    {
+        const float u = tp[a] * U.texDimF[a] - 0.5f;
+        const float fl = floorf(u);
+        t.w[a] = u - fl;
+        const int k = int(fl);
+        const int last = U.texDim[a] - 1;
+        o0[a] = axisOffset<T, TAB>(U, tab, a, base, uint32_t(clampi(k, 0, last)));
+        o1[a] = axisOffset<T, TAB>(U, tab, a, base, uint32_t(clampi(k + 1, 0, last)));
+        base += uint32_t(U.texDim[a]);
+    }
+    const uint32_t xy[4] = {o0[0] + o0[1], o1[0] + o0[1], o0[0] + o1[1], o1[0] + o1[1]};
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+    {
+        t.v[c] = loadSlot<T>(vox, xy[c] + o0[2]);
+        t.v[c + 4] = loadSlot<T>(vox, xy[c] + o1[2]);
+    }
+}
+
+// sampleBrick<T, true>'s blend of the eight texels, expression by expression
+template <typename T>
+__device__ __forceinline__ float linearBlend(const LinearTexels<T>& t)
+{
+    const float w0 = t.w[0], w1 = t.w[1], w2 = t.w[2];
+    const float c00 = float(t.v[0]) * (1.f - w0) + float(t.v[1]) * w0;
+    const float c10 = float(t.v[2]) * (1.f - w0) + float(t.v[3]) * w0;
+    const float c01 = float(t.v[4]) * (1.f - w0) + float(t.v[5]) * w0;
+    const float c11 = float(t.v[6]) * (1.f - w0) + float(t.v[7]) * w0;
+    const float c0 = c00 * (1.f - w1) + c10 * w1;
+    const float c1 = c01 * (1.f - w1) + c11 * w1;
+    return c0 * (1.f - w2) + c1 * w2;
+}
+
+// The GL_LINEAR march of one ray through one brick segment: the texels of
+// sample k + 1 are fetched before sample k is blended, classified and
+// composited, so eight gathers are always in flight per lane. The two texel
+// sets alternate (the loop is unrolled twice), so no register copy waits on
+// a gather. Sample positions advance by the shader's `pos += step` and
+// `travel` as in castTile's loop; the texels fetched past the last sample
+// are discarded (their indices are clamped into the brick like every other).
+template <typename T, bool P2, bool TAB>
+__device__ __forceinline__ bool linearStep(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
+                                           const BrickEntry& b, const float texExt[3],
+                                           float pos[3], const float step[3], float& travel,
+                                           float stepSize, Composite& C, const float4* tf,
+                                           uint32_t& nsamp, const uint32_t* tab,
+                                           const LinearTexels<T>& cur, LinearTexels<T>& next)
+{
+    pos[0] = pos[0] + step[0];
+    pos[1] = pos[1] + step[1];
+    pos[2] = pos[2] + step[2];
+    linearTexels<T, P2, TAB>(vox, U, b, texExt, pos, tab, next);
+    // the next sample's gathers stay above this sample's blend
+    __builtin_amdgcn_sched_barrier(0);
+    ++nsamp;
+    if (C.add(U, tf, linearBlend(cur)))
+        return false;
+    travel -= stepSize;
+    return travel > 0.0f;
+}
+
+template <typename T, bool P2, bool TAB>
+__device__ __forceinline__ void marchLinear(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
+                                            const BrickEntry& b, const float texExt[3],
+                                            float pos[3], const float step[3], float travel,
+                                            float stepSize, Composite& C, const float4* tf,
+                                            uint32_t& nsamp, const uint32_t* tab)
+{
+    if (!(travel > 0.0f))
+        return;
+    LinearTexels<T> t0, t1;
+    linearTexels<T, P2, TAB>(vox, U, b, texExt, pos, tab, t0);
+    while (linearStep<T, P2, TAB>(vox, U, b, texExt, pos, step, travel, stepSize, C, tf, nsamp,
+                                  tab, t0, t1) &&
+           linearStep<T, P2, TAB>(vox, U, b, texExt, pos, step, travel, stepSize, C, tf, nsamp,
+                                  tab, t1, t0))
+    {
+    }
+}
+
 // fragTexCopy.glsl into the RGBA8 framebuffer, round(clamp(c) * 255), or the
 // unconverted RGBA32F accumulation into a sort-last layer.
 __device__ __forceinline__ void storePixel(const FrameUniforms& U, int px, int py,
@@ -724,20 +840,14 @@ __device__ __forceinline__ void castTile(const FrameUniforms& U, int tile, int l
         }
         else
         {
-            for (float travel = segLen; travel > 0.0f; travel -= stepSize)
-            {
-                float tp[3];
-                if (p2)
-                    texturePosition<true>(U, b, texExt, pos, tp);
-                else
-                    texturePosition<false>(U, b, texExt, pos, tp);
-                ++nsamp;
-                if (C.add(U, tf, sampleBrick<T, true>(vox, U, tp)))
-                    break;
-                pos[0] = pos[0] + step[0];
-                pos[1] = pos[1] + step[1];
-                pos[2] = pos[2] + step[2];
-            }
+            if (p2)
+                marchLinear<T, true, TAB>(slotResource(vox, uint32_t(U.slotStride)), U, b,
+                                          texExt, pos, step, segLen, stepSize, C, tf, nsamp,
+                                          otab);
+            else
+                marchLinear<T, false, TAB>(slotResource(vox, uint32_t(U.slotStride)), U, b,
+                                           texExt, pos, step, segLen, stepSize, C, tf, nsamp,
+                                           otab);
         }
     }
     if (inside)

@@ -364,9 +364,11 @@ __device__ inline int nextItem(unsigned long long* counter, int lane)
 // is held to 80 VGPRs, 6 wavefronts per SIMD: a workgroup of 2 * 5
 // wavefronts spreads 3, 3, 2, 2 over the SIMDs, so two of them need room
 // for 6 on a SIMD (at 81 VGPRs, allocated as 88, only one fits per CU and
-// the kernel runs 25% slower). The others fit 8.
+// the kernel runs 25% slower). The GL_LINEAR march keeps two samples' eight
+// texels in flight and may take up to 128 VGPRs: its workgroups of 2 * w
+// wavefronts put at most 4 on a SIMD for every w up to 8. The others fit 8.
 template <bool LINEAR, int D, bool CLS>
-constexpr int kRaycastWavesPerEu = (!LINEAR && !CLS && D == 2) ? 6 : 8;
+constexpr int kRaycastWavesPerEu = LINEAR ? 4 : ((!CLS && D == 2) ? 6 : 8);
 
 template <typename T, bool LINEAR, int D, bool CLS>
 __global__ __launch_bounds__(1024) __attribute__((
@@ -379,10 +381,10 @@ __global__ __launch_bounds__(1024) __attribute__((
     // (16-bit tables stay in global memory)
     extern __shared__ float4 sTf[];
     const float4* table = sTf;
-    // the GL_NEAREST marches read the slot offsets of their texels from
-    // per-axis tables placed after the transfer function or the 8-bit
-    // classified table (rayLds)
-    constexpr bool kTab = !LINEAR;
+    // the marches read the slot offsets of their texels from per-axis
+    // tables placed after the transfer function or the 8-bit classified
+    // table (rayLds)
+    constexpr bool kTab = true;
     uint32_t* offTab = reinterpret_cast<uint32_t*>(
         sTf + (!CLS ? U.nTf + 2 : (sizeof(T) == 1 ? 256 : 0)));
     if (!CLS)

@@ -44,6 +44,12 @@ def test_profile_values_present():
     assert bench.profile_value("no_such_file.txt", "x") is None
 
 
+def test_roofline_limiter_reads_the_counter_summary():
+    txt = bench.roofline_limiter()
+    assert "?" not in txt and bench.COUNTER_PROFILE in txt
+    assert "%.1f" % bench.profile_value(bench.COUNTER_PROFILE, "active_lanes_per_valu") in txt
+
+
 def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)

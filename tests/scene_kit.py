@@ -13,9 +13,11 @@ ORC_DT = {livre_amd.DT_UINT8: oracle.DT_UINT8, livre_amd.DT_UINT16: oracle.DT_UI
 class Scene:
     def __init__(self, dt=livre_amd.DT_UINT8, n=64, block=16, overlap=1, source="blobs",
                  seed=1):
+        # n: voxels per axis, or one count for a cube (the blob source is cubic)
         self.dt = dt
         self.vi = livre_amd.volume_info(n, block, overlap, dt)
-        self.ovi = oracle.volume_info([n] * 3, [block] * 3, [overlap] * 3, ORC_DT[dt])
+        self.ovi = oracle.volume_info(list(n) if not np.isscalar(n) else [n] * 3, [block] * 3,
+                                      [overlap] * 3, ORC_DT[dt])
         self.ids = scenes.all_nodes(self.vi)
         if source == "blobs":
             vol = scenes.blob_volume(n, seed=seed, blobs=16, dtype=livre_amd.NP_DTYPE[dt])

@@ -286,6 +286,25 @@ def test_trilinear_against_fp32_oracle(lvr, orc):
         ctx.close()
 
 
+@pytest.mark.parametrize("dt", [livre_amd.DT_UINT8, livre_amd.DT_UINT16, livre_amd.DT_FLOAT])
+def test_trilinear_non_cubic_volume(lvr, orc, dt):
+    """A 72 x 56 x 40 volume: brick extents along y and z are not powers of two,
+    so the trilinear march takes the corrected texture coordinate
+    (texturePosition<false>) beside the power-of-two one along x, and texel
+    indices clamp at the short axes' edges."""
+    s = Scene(dt, n=(72, 56, 40), block=16, overlap=1, source="noise", seed=21)
+    assert any(float(e) != 2.0 ** round(np.log2(float(e))) for e in s.vi.world_size)
+    ctx = s.context()
+    try:
+        tf = scenes.piecewise_tf(256, seed=22, max_alpha=0.5)
+        _check_frame(s, ctx, (1.2, 0.9, 1.3), 80, 64, tf, 96, linear=True,
+                     rgba8_tf=dt == livre_amd.DT_UINT8)
+        _check_frame(s, ctx, (-0.9, -1.1, 1.0), 64, 72, tf, 160, linear=True,
+                     rgba8_tf=dt == livre_amd.DT_UINT8)
+    finally:
+        ctx.close()
+
+
 def test_device_noise_equals_host_noise(lvr):
     vi = livre_amd.volume_info(128, 32, 1, livre_amd.DT_UINT16)
     p = livre_amd.noise_params(3, octaves=5, base_cells=8)

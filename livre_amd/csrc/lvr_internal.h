@@ -204,9 +204,9 @@ struct FrameUniforms
     int32_t texDim[3];
     // host-computed per-frame constants (scalar operands in the kernels):
     // texMax - texMin, float(texDim), float(texDim - 1), 1 / nSamples,
-    // float(nTf), float(nTf - 1), maxSamples / nSamples
+    // float(nTf), maxSamples / nSamples
     float texExt[3], texDimF[3], texLastF[3];
-    float stepSize, nTfF, nTfLastF, alphaCorrection;
+    float stepSize, nTfF, alphaCorrection;
     int32_t nSamples, maxSamples;
     float exitA;
     float mult, add;

@@ -416,19 +416,6 @@ __attribute__((always_inline)) inline void noiseBrickHostBody(const NoiseShape& 
     }
 }
 
-// Per-voxel host evaluation (noiseVoxel at every voxel): the definition the
-// separable path is checked against.
-template <typename T>
-void noiseBrickHostDirect(const NoiseShape& s, const uint16_t* remap, const BrickGeo& g, T* out)
-{
-    size_t i = 0;
-    for (uint32_t z = 0; z < g.dim[2]; ++z)
-        for (uint32_t y = 0; y < g.dim[1]; ++y)
-            for (uint32_t x = 0; x < g.dim[0]; ++x)
-                out[i++] = convertNoise<T>(
-                    noiseVoxel(s, remap, g.level, g.origin[0] + x, g.origin[1] + y, g.origin[2] + z));
-}
-
 // Blob values of a brick on the host, separable as noiseBrickHost: the
 // lattice nodes the brick touches are interpolated along x into rows, the
 // rows along y into planes, and each voxel takes the lerp along z; the same

@@ -841,7 +841,6 @@ static int prepareFrame(lvr_ctx* ctx, const lvr_frustum* f, const int viewport[4
     U->add = -r0 / (r1 - r0);
     U->nTf = nTf;
     U->nTfF = float(nTf);
-    U->nTfLastF = float(nTf - 1);
     U->tileW = tw;
     U->tileH = th;
     U->tilesX = (W + tw - 1) / tw;

@@ -67,6 +67,17 @@ def roofline_limiter():
             "(profiles/%s)" % tuple(txt + [COUNTER_PROFILE]))
 
 
+def valu_ceiling_entry(kernel_gsamples_per_s):
+    """The kernel's rate against its VALU-issue ceiling (1024 SIMDs x the
+    measured clock / 4 cycles per wave64 instruction / VALU instructions per
+    voxel fetch x active lanes), as profiles/ measured it."""
+    ceil = profile_value(COUNTER_PROFILE, "valu_issue_ceiling_gsamples_per_s")
+    return {"bound": "valu", "achieved_gsamples_per_s": kernel_gsamples_per_s,
+            "ceiling_gsamples_per_s": ceil,
+            "frac": kernel_gsamples_per_s / ceil if ceil else None,
+            "source": "profiles/" + COUNTER_PROFILE}
+
+
 def schedule(steps):
     """Frame indices of the timed steps: the 120-frame orbit-zoom path
     sampled evenly (stride 120 / steps), so any --steps covers the whole
@@ -1278,6 +1289,9 @@ def main():
         "render_call_host_ms_mean": float(np.mean(host_ms)),
         "kernel_gsamples_per_s": samples / (rc.sum() * 1e-3) / 1e9,
         "bandwidth_ceiling_gsamples_per_s": ceiling,
+        # the ray cast's binding ceiling: VALU issue at the measured clock,
+        # from the committed counter summary (same kernel, same sampled path)
+        "valu_ceiling": valu_ceiling_entry(samples / (rc.sum() * 1e-3) / 1e9),
         # HBM read + write bytes per second of the ray cast: the measured
         # traffic per launch over the mean launch time
         "hbm_gb_per_s": traffic / (float(rc.mean()) * 1e-3) / 1e9 if traffic else None,
@@ -1310,6 +1324,7 @@ def main():
                                           "profiles/ are of the GL_NEAREST kernel)")
         result["hbm_gb_per_s"] = None
         result["l2_hit_rate"] = None
+        result["valu_ceiling"] = None
     if not args.no_cpu_baseline:
         try:
             result["cpu_baseline"] = cpu_baseline(livre_amd, scenes, vi, ctx, tf,

@@ -50,6 +50,12 @@ def test_roofline_limiter_reads_the_counter_summary():
     assert "%.1f" % bench.profile_value(bench.COUNTER_PROFILE, "active_lanes_per_valu") in txt
 
 
+def test_valu_ceiling_entry():
+    e = bench.valu_ceiling_entry(400.0)
+    assert e["bound"] == "valu" and 0.5 < e["frac"] < 1.0
+    assert abs(e["frac"] * e["ceiling_gsamples_per_s"] - 400.0) < 1e-9
+
+
 def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True, timeout=120)

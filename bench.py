@@ -788,7 +788,10 @@ def c1_workload(args):
     lut = scenes.lut_rgba8(tf)
     ctx.set_tf_rgba8(lut, (0.0, 255.0))
     ctx.set_timing(True)
-    params = livre_amd.render_params(256, early_exit=0.99)
+    # the launch shape the shipped tuning table holds for this class
+    from livre_amd import autotune
+    tuner = autotune.Tuner()
+    params = tuner.cached(vi, w, h, livre_amd.render_params(256, early_exit=0.99))
     cams = []
     for k in range(8):
         mv, proj = scenes.camera(scenes.orbit(k, 8, radius=2.0), w, h, 60.0)
@@ -818,7 +821,7 @@ def c1_workload(args):
     rc = np.array(rc_ms)
     achieved = float(np.mean(np.array(alg) / (rc * 1e-3))) / 1e9
     # the reference-parity variant (early exit 0.999) over the 8 cameras
-    params = livre_amd.render_params(256, early_exit=0.999)
+    params = tuner.cached(vi, w, h, livre_amd.render_params(256, early_exit=0.999))
     one(0)
     p_samples, p_rc, t1 = 0, [], time.perf_counter()
     for k in range(8):
@@ -834,7 +837,11 @@ def c1_workload(args):
         "config": {"workload": "256^3 uint8, 32^3 bricks + 1 halo, 4 levels, 256-entry uint8 "
                                "LUT, 1024x1024, 8 orbit cameras (radius 2), 256 samples/unit, "
                                "exit 0.99, sse 1, 256-bin histogram; one frame at a time "
-                               "(select + order + ray cast + histogram, synchronised)"},
+                               "(select + order + ray cast + histogram, synchronised)",
+                   "launch": "tile %dx%d, %d fetches in flight, classify %d, %d waves/SIMD "
+                             "(livre_amd/tune_gfx950.json)"
+                             % (params.tile_w, params.tile_h, params.samples_per_thread,
+                                params.classify, params.waves_per_simd)},
         "fps": frames / total, "frame_p50_ms": float(np.percentile(lat, 50)),
         "frame_p99_ms": float(np.percentile(lat, 99)), "raycast_ms_mean": float(rc.mean()),
         "kernel_gsamples_per_s": samples / (rc.sum() * 1e-3) / 1e9,

@@ -100,6 +100,18 @@ class Tuner:
         out.waves_per_simd = c.get("waves_per_simd", 0)
         return out
 
+    def cached(self, vi, width, height, params):
+        """params with the cached choice of the frame's class applied, without
+        tuning; params unchanged when the class has no entry."""
+        c = self.table.get(request_class(vi, width, height, params))
+        out = RenderParams.from_buffer_copy(params)
+        if c is not None:
+            out.tile_w, out.tile_h = c["tile"]
+            out.samples_per_thread = c["samples_per_thread"]
+            out.classify = c.get("classify", 0)
+            out.waves_per_simd = c.get("waves_per_simd", 0)
+        return out
+
     def tune_batch(self, requests):
         """requests: list of (ctx, fr, width, height, ordered, params).
         Returns one RenderParams per request: the class's tile, samples per

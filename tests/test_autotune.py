@@ -104,3 +104,20 @@ def test_autotune_candidates_render_the_same_frame(lvr):
             assert smp == ref_smp and np.array_equal(img, ref), (tw, th, d, m, w)
     finally:
         ctx.close()
+
+
+def test_cached_choice_is_applied_without_tuning(lvr, tmp_path):
+    """Tuner.cached applies the table's choice of the frame's class and leaves
+    the parameters of a class without an entry unchanged."""
+    vi = livre_amd.volume_info(256, 32, 1, livre_amd.DT_UINT8)
+    t = autotune.Tuner(path=str(tmp_path / "none.json"))
+    key = autotune.request_class(vi, 1024, 1024, livre_amd.render_params(256))
+    t.table = {key: {"tile": [16, 4], "samples_per_thread": 3, "classify": 1,
+                     "waves_per_simd": 6}}
+    p = t.cached(vi, 1024, 1024, livre_amd.render_params(256, early_exit=0.99))
+    assert (p.tile_w, p.tile_h, p.samples_per_thread, p.classify, p.waves_per_simd) == \
+        (16, 4, 3, 1, 6)
+    assert abs(p.early_exit - 0.99) < 1e-6 and p.n_samples_per_ray == 256
+    q = t.cached(vi, 640, 480, livre_amd.render_params(256))
+    assert (q.tile_w, q.tile_h, q.samples_per_thread) == (0, 0, 0)
+    assert t.tuned == 0

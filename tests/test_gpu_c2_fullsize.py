@@ -68,3 +68,32 @@ def test_c2_frame_band(c2, t):
         assert np.array_equal(got, want)
         scale = [8 ** (vi.depth - 1 - (int(i) & 0xF)) for i in ordered]
         assert int(got.sum()) == 64 ** 3 * sum(scale)
+
+
+@pytest.mark.parametrize("t", [30, 60])
+def test_c2_trilinear_band(c2, t):
+    """The same volume and frames sampled with GL_LINEAR (trilinear, halo
+    voxels at the brick faces): a 16-row band within 2/255 max and 0.5/255
+    mean of the FP32 oracle's trilinear march, and its error against the FP64
+    transcription within the FP64 bound (lines starting FP64ERR)."""
+    vi, ovi, ctx = c2
+    tf = scenes.piecewise_tf(4096, seed=4, points=8, max_alpha=0.35)
+    ctx.set_tf(tf, (0.0, 65535.0))
+    mv, proj = scenes.camera(scenes.orbit_zoom(t, 120), W, H, 60.0)
+    fr = livre_amd.frustum(mv, proj)
+    ordered = livre_amd.order(vi, fr, ctx.select_visibles(fr, H, 1.0, 0, vi.depth - 1))
+    ctx.render(fr, W, H, ordered, livre_amd.render_params(2048, linear=True))
+    img, _samples = ctx.read_frame()
+    bricks = [ctx.download(i) for i in ordered]
+    cam = oracle.camera(list(fr.inv_proj), list(fr.inv_mv), list(fr.eye), fr.near_plane,
+                        fr.far_plane, W, H)
+    y0, y1 = H // 2 - 8, H // 2 + 8
+    band = oracle.render_params(2048, 0.999, (0.0, 65535.0), linear=True, window=(0, y0, W, y1))
+    ref, ref_samples = oracle.render(ovi, cam, ordered, bricks, tf, band, "f32")
+    mx, mean = diff_stats(img[y0:y1], oracle.to_rgba8(ref)[y0:y1])
+    print("trilinear frame", t, "bricks", len(ordered), "band samples", ref_samples, "max", mx,
+          "mean", mean, flush=True)
+    assert ref_samples > 0 and mx <= 2 and mean <= 0.5
+    ref64, samples64 = oracle.render(ovi, cam, ordered, bricks, tf, band, "f64")
+    fp64_error(img[y0:y1], ref[y0:y1], ref64[y0:y1],
+               "c2 trilinear frame %d rows %d-%d" % (t, y0, y1), (-1, ref_samples, samples64))

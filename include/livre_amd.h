@@ -346,7 +346,10 @@ typedef struct
     int32_t samples_per_thread;  /* voxel fetches in flight per ray, 1..3 (NEAREST);
                                     0 = 2. With 2 and per-sample classification a
                                     ray may run one brick ahead of the others of
-                                    its wavefront tile; identical images always */
+                                    its wavefront tile. GL_LINEAR: 1 brick by brick,
+                                    2 (or 3) with the one-brick-ahead window, the
+                                    next sample's 8 texels in flight either way;
+                                    identical images always */
     int32_t classify;            /* 8/16-bit NEAREST: 1 = classification tabulated
                                     per raw value (one table read per sample),
                                     2 = computed per sample from the TF,
@@ -486,9 +489,9 @@ int lvr_frame_stats_back(lvr_ctx* ctx, int back, lvr_frame_stats* out);
 
 /* Autotuner of the launch shape (no reference counterpart: the GL path has
  * no tunable launch). Renders the given frame with every ray tile in
- * {8x8, 16x4, 4x16, 32x2, 2x32}, every waves_per_simd in {4, 5, 6, 8}, for
- * NEAREST every samples_per_thread in {1, 2, 3}, and for 8/16-bit data both
- * classify modes; times the ray-cast kernel with HIP events (median of reps)
+ * {8x8, 16x4, 4x16, 32x2, 2x32}, every waves_per_simd in {4, 5, 6, 8}, every
+ * samples_per_thread in {1, 2, 3} (NEAREST) or {1, 2} (GL_LINEAR), and for
+ * 8/16-bit NEAREST both classify modes; times the ray-cast kernel with HIP events (median of reps)
  * and writes the fastest tile_w, tile_h, samples_per_thread, classify and
  * waves_per_simd into *params.
  * All candidates produce the same image. The timings of the candidates are

@@ -30,7 +30,9 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
     int rc = lvr_set_timing(ctx, 1);
     if (rc)
         return rc;
-    const int depths = params->linear_filter ? 1 : 3; // depth applies to NEAREST
+    // NEAREST: 1 to 3 fetches in flight; GL_LINEAR: brick by brick (1) or
+    // through the two-brick window (2)
+    const int depths = params->linear_filter ? 2 : 3;
     // 8/16-bit NEAREST: tabulated or per-sample classification
     const int dt = ctx->vi.data_type;
     const bool tabulable =
@@ -48,7 +50,7 @@ extern "C" int lvr_autotune(lvr_ctx* ctx, const lvr_frustum* f, const int viewpo
         lvr_render_params p = *params;
         p.tile_w = t[0];
         p.tile_h = t[1];
-        p.samples_per_thread = params->linear_filter ? 0 : d;
+        p.samples_per_thread = d;
         p.classify = tabulable ? m : 0;
         p.waves_per_simd = w;
         std::vector<float> ms;

@@ -647,14 +647,12 @@ struct LinearTexels
     float w[3];
 };
 
-template <typename T, bool P2, bool TAB>
-__device__ __forceinline__ void linearTexels(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
-                                             const BrickEntry& b, const float texExt[3],
-                                             const float pos[3], const uint32_t* tab,
-                                             LinearTexels<T>& t)
+// The byte offsets of the eight texels of a GL_LINEAR sample at texture
+// position tp (off[x + 2 y + 4 z]) and its blend weights.
+template <typename T, bool TAB>
+__device__ __forceinline__ void linearOffsets(const FrameUniforms& U, const uint32_t* tab,
+                                              const float tp[3], uint32_t off[8], float w[3])
 {
-    float tp[3];
-    texturePosition<P2>(U, b, texExt, pos, tp);
     uint32_t o0[3], o1[3];
     uint32_t base = 0;
 #pragma unroll
@@ -662,7 +660,7 @@ __device__ __forceinline__ void linearTexels(__amdgpu_buffer_rsrc_t vox, const F
     {
         const float u = tp[a] * U.texDimF[a] - 0.5f;
         const float fl = floorf(u);
-        t.w[a] = u - fl;
+        w[a] = u - fl;
         const int k = int(fl);
         const int last = U.texDim[a] - 1;
         o0[a] = axisOffset<T, TAB>(U, tab, a, base, uint32_t(clampi(k, 0, last)));
@@ -673,9 +671,24 @@ __device__ __forceinline__ void linearTexels(__amdgpu_buffer_rsrc_t vox, const F
 #pragma unroll
     for (int c = 0; c < 4; ++c)
     {
-        t.v[c] = loadSlot<T>(vox, xy[c] + o0[2]);
-        t.v[c + 4] = loadSlot<T>(vox, xy[c] + o1[2]);
+        off[c] = xy[c] + o0[2];
+        off[c + 4] = xy[c] + o1[2];
     }
+}
+
+template <typename T, bool P2, bool TAB>
+__device__ __forceinline__ void linearTexels(__amdgpu_buffer_rsrc_t vox, const FrameUniforms& U,
+                                             const BrickEntry& b, const float texExt[3],
+                                             const float pos[3], const uint32_t* tab,
+                                             LinearTexels<T>& t)
+{
+    float tp[3];
+    texturePosition<P2>(U, b, texExt, pos, tp);
+    uint32_t off[8];
+    linearOffsets<T, TAB>(U, tab, tp, off, t.w);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        t.v[c] = loadSlot<T>(vox, off[c]);
 }
 
 // sampleBrick<T, true>'s blend of the eight texels, expression by expression
@@ -907,6 +920,22 @@ __device__ __forceinline__ T laneFetch(const FrameUniforms& U, const LaneBrick& 
     return *reinterpret_cast<const T*>(L.vox + slotOffset<T, TAB>(U, tab, ix, iy, iz));
 }
 
+// laneFetch for GL_LINEAR: the eight texels and weights of the sample at pos
+// in the lane's brick (linearTexels through the lane's slot pointer)
+template <typename T, bool P2, bool TAB>
+__device__ __forceinline__ void laneLinear(const FrameUniforms& U, const LaneBrick& L,
+                                           const uint32_t* tab, const float pos[3],
+                                           LinearTexels<T>& t)
+{
+    float tp[3];
+    texturePositionOf<P2>(U, L.bmin, P2 ? L.rcpP : L.rcpG, L.ext, U.texExt, pos, tp);
+    uint32_t off[8];
+    linearOffsets<T, TAB>(U, tab, tp, off, t.w);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        t.v[c] = *reinterpret_cast<const T*>(L.vox + off[c]);
+}
+
 // A lane of castTileWindow. The wavefront holds two bricks of the tile's
 // list: A, the oldest one a lane still marches, and B, the next one with a
 // segment for some lane (list positions kA < kB; `seg` is the position of
@@ -926,15 +955,17 @@ struct WindowLane
     LaneBrick L;
 };
 
-// The march of the lanes that are marching, until none marches A:
-// GL_NEAREST fetches with one in flight per lane (the voxel of the next
-// sample is fetched before the current one is classified and blended).
-template <typename T, bool P2, bool TAB>
+// The march of the lanes that are marching, until none marches A, with one
+// sample's fetch in flight per lane: the voxel of the next sample (GL_NEAREST)
+// or its eight texels (LIN, GL_LINEAR) are fetched before the current sample
+// is classified and blended. For LIN the current sample's texels are blended
+// first and the next sample's are loaded into the same registers.
+template <typename T, bool P2, bool TAB, bool LIN>
 __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4* tf,
                                             const uint32_t* tab, Composite& C, WindowLane& W,
-                                            T& raw, int kA, int kB, const BrickEntry& bB,
-                                            const float rcpGB[3], const uint8_t* voxB,
-                                            uint32_t& nsamp)
+                                            T& raw, LinearTexels<T>& tex, int kA, int kB,
+                                            const BrickEntry& bB, const float rcpGB[3],
+                                            const uint8_t* voxB, uint32_t& nsamp)
 {
     const float stepSize = U.stepSize;
     for (;;)
@@ -942,7 +973,7 @@ __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4
         // uniform exit: the lanes still here march A or B
         if (__ballot(W.seg == kA) == 0)
             break;
-        const float value = float(raw);
+        const float value = LIN ? linearBlend(tex) : float(raw);
         W.travel -= stepSize;
         W.pos[0] = W.pos[0] + W.step[0];
         W.pos[1] = W.pos[1] + W.step[1];
@@ -965,9 +996,17 @@ __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4
             laneBrick(bB, rcpGB, voxB, W.L);
             cont = true;
         }
-        const T next = laneFetch<T, P2, TAB>(U, W.L, tab, W.pos);
-        __builtin_amdgcn_sched_barrier(0);
-        raw = next;
+        if (LIN)
+        {
+            laneLinear<T, P2, TAB>(U, W.L, tab, W.pos, tex);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        else
+        {
+            const T next = laneFetch<T, P2, TAB>(U, W.L, tab, W.pos);
+            __builtin_amdgcn_sched_barrier(0);
+            raw = next;
+        }
         ++nsamp;
         const bool opaque = C.add(U, tf, value);
         if (opaque || !cont)
@@ -976,9 +1015,9 @@ __device__ __forceinline__ void marchWindow(const FrameUniforms& U, const float4
 }
 
 // castTile with the two-brick window (WindowLane) for the GL_NEAREST march
-// with per-sample classification: identical samples and frames, fewer idle
-// lanes per wavefront step.
-template <typename T, bool TAB>
+// with per-sample classification, or (LIN) the GL_LINEAR march: identical
+// samples and frames, fewer idle lanes per wavefront step.
+template <typename T, bool TAB, bool LIN = false>
 __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile, int lane,
                                                const BrickEntry* __restrict__ bricks,
                                                const int32_t* __restrict__ offsets,
@@ -1007,6 +1046,11 @@ __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile,
     W.seg = -1;
     W.L.vox = U.pool;
     T raw = T(0);
+    LinearTexels<T> tex;
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        tex.v[c] = T(0);
+    tex.w[0] = tex.w[1] = tex.w[2] = 0.f;
 
     const int gtile = U.tileBase + tile;
     const int beg = __builtin_amdgcn_readfirstlane(offsets[gtile]);
@@ -1077,17 +1121,25 @@ __device__ __forceinline__ void castTileWindow(const FrameUniforms& U, int tile,
             W.travelB = 0.0f;
             W.seg = kB;
             laneBrick(bB, rcpGB, voxB, W.L);
-            raw = p2 ? laneFetch<T, true, TAB>(U, W.L, tab, W.pos)
-                     : laneFetch<T, false, TAB>(U, W.L, tab, W.pos);
+            if (LIN)
+            {
+                if (p2)
+                    laneLinear<T, true, TAB>(U, W.L, tab, W.pos, tex);
+                else
+                    laneLinear<T, false, TAB>(U, W.L, tab, W.pos, tex);
+            }
+            else
+                raw = p2 ? laneFetch<T, true, TAB>(U, W.L, tab, W.pos)
+                         : laneFetch<T, false, TAB>(U, W.L, tab, W.pos);
         }
         if (alive && W.travel > 0.0f)
         {
             if (p2)
-                marchWindow<T, true, TAB>(U, tf, tab, C, W, raw, kA, kB, bB, rcpGB, voxB,
-                                          nsamp);
+                marchWindow<T, true, TAB, LIN>(U, tf, tab, C, W, raw, tex, kA, kB, bB, rcpGB,
+                                               voxB, nsamp);
             else
-                marchWindow<T, false, TAB>(U, tf, tab, C, W, raw, kA, kB, bB, rcpGB, voxB,
-                                           nsamp);
+                marchWindow<T, false, TAB, LIN>(U, tf, tab, C, W, raw, tex, kA, kB, bB,
+                                                rcpGB, voxB, nsamp);
         }
     }
     if (inside)

@@ -415,6 +415,9 @@ __global__ __launch_bounds__(1024) __attribute__((
         if (!LINEAR && !CLS)
             castNearestTile<T, D, kTab>(U, tile, lane, bricks, offsets, counts, lists, table,
                                         nsamp, offTab);
+        else if (LINEAR && D == 2)
+            castTileWindow<T, kTab, true>(U, tile, lane, bricks, offsets, counts, lists, table,
+                                          offTab, nsamp);
         else
             castTile<T, LINEAR, D, CLS, kTab>(U, tile, lane, bricks, offsets, counts, lists,
                                               table, nsamp, offTab);
@@ -675,8 +678,13 @@ void launchRaycast(int blocks, size_t lds, hipStream_t st, const FrameUniforms& 
                    const BrickEntry* bricks, const int32_t* offsets, const int32_t* counts,
                    const int32_t* lists, unsigned long long* counters)
 {
-    if (U.linear)
+    // GL_LINEAR: through the two-brick window at the default depth of 2,
+    // brick by brick at 1
+    if (U.linear && U.depth == 1)
         launchRaycastAs<T, true, 1, false>(blocks, lds, st, U, bricks, offsets, counts, lists,
+                                           counters);
+    else if (U.linear)
+        launchRaycastAs<T, true, 2, false>(blocks, lds, st, U, bricks, offsets, counts, lists,
                                            counters);
     else
         launchNearest<T, false>(blocks, lds, st, U, bricks, offsets, counts, lists, counters);

@@ -503,10 +503,19 @@ __attribute__((always_inline)) inline void blobNoiseBrickHostBody(
 
 // LIVRE_AMD_HOST_BASELINE=1 selects the baseline build on any CPU (tests
 // compare the two).
+inline bool detectHostAvx2()
+{
+#if !defined(__HIP_DEVICE_COMPILE__)
+    // the CPU model may not be set up yet in a library opened at run time
+    // (host pass only: the device pass parses this host function too)
+    __builtin_cpu_init();
+#endif
+    return __builtin_cpu_supports("avx2") && !std::getenv("LIVRE_AMD_HOST_BASELINE");
+}
+
 inline bool hostHasAvx2()
 {
-    static const bool has =
-        __builtin_cpu_supports("avx2") && !std::getenv("LIVRE_AMD_HOST_BASELINE");
+    static const bool has = detectHostAvx2();
     return has;
 }
 

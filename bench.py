@@ -422,7 +422,7 @@ def serve_measure(args, rates, duration):
                                       generate=False)
     sessions = workloads.serve_sessions(32, seed=9)
     pools = sum(v.pool_bytes for v in vols.values())
-    frames_bytes = sum(4 * 4 * s["width"] * s["height"] for s in sessions)  # 4 frames each
+    frames_bytes = sum(5 * 4 * s["width"] * s["height"] for s in sessions)  # 5 frames each
     budget = pools + frames_bytes
     result_rates = []
     with livre_amd.Server(budget, max_batch=args.max_batch) as srv:
@@ -527,7 +527,7 @@ def serve_measure(args, rates, duration):
                                "256^3 u8 / 512^3 u8 / 1024^3 u16 / 1536^3 f32 / 2048^3 u16, "
                                "random-walk cameras, 256-entry TFs, 1 sample/voxel, sse 1; "
                                "Poisson arrivals, %.0f s per rate" % duration,
-                   "server": "continuous batching, 3 launches in flight, max %d requests "
+                   "server": "continuous batching, 4 launches in flight, max %d requests "
                              "per launch, newest request per session wins" % args.max_batch,
                    "budget_bytes": budget, "pool_bytes": pools,
                    "pool_fraction_of_pyramid": args.pool_frac},

@@ -584,12 +584,12 @@ int lvr_batch_reserve_cus(lvr_batch* batch, int n);
  * until a session closes. Submitted frame requests queue; lvr_server_step
  * retires finished launches and puts every queued request of an admitted
  * session (up to max_batch) into the next persistent batched launch while at
- * most two other launches are on the GPU. A newer request of a session
+ * most three other launches are on the GPU. A newer request of a session
  * supersedes its older one still queued (status 1). Missing bricks are
  * produced on the device by the volume's procedural source, evicting
  * least-recently-used bricks; bricks read by queued or running launches are
  * pinned. Frames render straight into per-session device framebuffers
- * (four per session, so the finished one is never one being written);
+ * (five per session, so the finished one is never one being written);
  * launches leave a few CUs to the selection and source kernels. Not
  * thread-safe: one host thread drives a server. While a context belongs to a
  * server, other batches must not read its pool. */

@@ -37,7 +37,7 @@ namespace
 // Launches in flight at once: while one is retired and the next staged on
 // the host, the others keep the GPU busy. Each session holds one
 // framebuffer more than this: its finished frame is never one being written.
-constexpr int kFlights = 3;
+constexpr int kFlights = 4;
 
 struct Volume
 {

@@ -125,7 +125,7 @@ def test_admission_by_budget_and_coalescing():
     tf = scenes.piecewise_tf(256, seed=9, max_alpha=0.5)
     rng = scenes.type_range(vi.data_type)
     params = livre_amd.render_params(64)
-    per = 64 * 64 * 4 * 4  # four framebuffers per session
+    per = 64 * 64 * 4 * 5  # five framebuffers per session
     with livre_amd.Server(budget_bytes=pool + 2 * per, max_batch=8) as srv:
         v = srv.add_volume(ctx, **src)
         a, adm_a = srv.open(v, 64, 64, tf, rng, params)
